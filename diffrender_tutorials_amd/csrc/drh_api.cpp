@@ -1,0 +1,966 @@
+// drh_api.cpp -- host runtime behind the C ABI (include/drh.h).
+//
+// Owns one HIP stream and the device buffers of a committed scene; orders the commit
+// (prepare + LBVH build), render (+ fused loss), interior and edge kernels the way
+// DiffRender<SHADING_MODEL> orders its loops (src/drender.h:74-434), and the optimiser
+// step of OptSimple (src/optimizer_simple.cpp:188-309).
+//
+// Host float math (camera inverses, transforms) is built with -ffp-contract=off and keeps
+// LiteMath's term order, so the matrices the kernels receive are the ones the reference
+// computes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/drh.h"
+#include "drh_kernels.h"
+
+using namespace drh;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// LiteMath matrices on the host (common_src/LiteMath.h:1342-1634)
+// ---------------------------------------------------------------------------
+struct M4 {
+  float m[16];  // column-major: m[c*4 + r]
+  float operator()(int r, int c) const { return m[c * 4 + r]; }
+  float& at(int r, int c) { return m[c * 4 + r]; }
+};
+M4 m4_load(const float* p) { M4 r; std::memcpy(r.m, p, sizeof r.m); return r; }
+M4 m4_identity() { M4 r{}; r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.0f; return r; }
+void m4_vec(const M4& B, const float V[4], float R[4]) {
+  for (int r = 0; r < 4; r++) R[r] = V[0] * B.m[r] + V[1] * B.m[4 + r] + V[2] * B.m[8 + r] + V[3] * B.m[12 + r];
+}
+M4 m4_mul(const M4& a, const M4& b) {
+  M4 r;
+  for (int c = 0; c < 4; c++) m4_vec(a, b.m + 4 * c, r.m + 4 * c);
+  return r;
+}
+M4 m4_transpose(const M4& a) { M4 r; for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) r.m[j * 4 + i] = a.m[i * 4 + j]; return r; }
+
+// Cofactor inverse with pair products, term for term as inverse4x4 (LiteMath.h:1551-1634).
+M4 m4_inverse(const M4& a) {
+  float p[12];
+  M4 o;
+  auto A = [&](int r, int c) { return a(r, c); };
+  // cofactors of columns 0 and 1 from 2x2 products of rows 2, 3
+  p[0] = A(2,2) * A(3,3); p[1] = A(2,3) * A(3,2); p[2] = A(2,1) * A(3,3); p[3] = A(2,3) * A(3,1);
+  p[4] = A(2,1) * A(3,2); p[5] = A(2,2) * A(3,1); p[6] = A(2,0) * A(3,3); p[7] = A(2,3) * A(3,0);
+  p[8] = A(2,0) * A(3,2); p[9] = A(2,2) * A(3,0); p[10] = A(2,0) * A(3,1); p[11] = A(2,1) * A(3,0);
+  o.at(0,0) = p[0] * A(1,1) + p[3] * A(1,2) + p[4] * A(1,3);  o.at(0,0) -= p[1] * A(1,1) + p[2] * A(1,2) + p[5] * A(1,3);
+  o.at(1,0) = p[1] * A(1,0) + p[6] * A(1,2) + p[9] * A(1,3);  o.at(1,0) -= p[0] * A(1,0) + p[7] * A(1,2) + p[8] * A(1,3);
+  o.at(2,0) = p[2] * A(1,0) + p[7] * A(1,1) + p[10] * A(1,3); o.at(2,0) -= p[3] * A(1,0) + p[6] * A(1,1) + p[11] * A(1,3);
+  o.at(3,0) = p[5] * A(1,0) + p[8] * A(1,1) + p[11] * A(1,2); o.at(3,0) -= p[4] * A(1,0) + p[9] * A(1,1) + p[10] * A(1,2);
+  o.at(0,1) = p[1] * A(0,1) + p[2] * A(0,2) + p[5] * A(0,3);  o.at(0,1) -= p[0] * A(0,1) + p[3] * A(0,2) + p[4] * A(0,3);
+  o.at(1,1) = p[0] * A(0,0) + p[7] * A(0,2) + p[8] * A(0,3);  o.at(1,1) -= p[1] * A(0,0) + p[6] * A(0,2) + p[9] * A(0,3);
+  o.at(2,1) = p[3] * A(0,0) + p[6] * A(0,1) + p[11] * A(0,3); o.at(2,1) -= p[2] * A(0,0) + p[7] * A(0,1) + p[10] * A(0,3);
+  o.at(3,1) = p[4] * A(0,0) + p[9] * A(0,1) + p[10] * A(0,2); o.at(3,1) -= p[5] * A(0,0) + p[8] * A(0,1) + p[11] * A(0,2);
+  // columns 2 and 3 from 2x2 products of rows 0, 1
+  p[0] = A(0,2) * A(1,3); p[1] = A(0,3) * A(1,2); p[2] = A(0,1) * A(1,3); p[3] = A(0,3) * A(1,1);
+  p[4] = A(0,1) * A(1,2); p[5] = A(0,2) * A(1,1); p[6] = A(0,0) * A(1,3); p[7] = A(0,3) * A(1,0);
+  p[8] = A(0,0) * A(1,2); p[9] = A(0,2) * A(1,0); p[10] = A(0,0) * A(1,1); p[11] = A(0,1) * A(1,0);
+  o.at(0,2) = p[0] * A(3,1) + p[3] * A(3,2) + p[4] * A(3,3);  o.at(0,2) -= p[1] * A(3,1) + p[2] * A(3,2) + p[5] * A(3,3);
+  o.at(1,2) = p[1] * A(3,0) + p[6] * A(3,2) + p[9] * A(3,3);  o.at(1,2) -= p[0] * A(3,0) + p[7] * A(3,2) + p[8] * A(3,3);
+  o.at(2,2) = p[2] * A(3,0) + p[7] * A(3,1) + p[10] * A(3,3); o.at(2,2) -= p[3] * A(3,0) + p[6] * A(3,1) + p[11] * A(3,3);
+  o.at(3,2) = p[5] * A(3,0) + p[8] * A(3,1) + p[11] * A(3,2); o.at(3,2) -= p[4] * A(3,0) + p[9] * A(3,1) + p[10] * A(3,2);
+  o.at(0,3) = p[2] * A(2,2) + p[5] * A(2,3) + p[1] * A(2,1);  o.at(0,3) -= p[4] * A(2,3) + p[0] * A(2,1) + p[3] * A(2,2);
+  o.at(1,3) = p[8] * A(2,3) + p[0] * A(2,0) + p[7] * A(2,2);  o.at(1,3) -= p[6] * A(2,2) + p[9] * A(2,3) + p[1] * A(2,0);
+  o.at(2,3) = p[6] * A(2,1) + p[11] * A(2,3) + p[3] * A(2,0); o.at(2,3) -= p[10] * A(2,3) + p[2] * A(2,0) + p[7] * A(2,1);
+  o.at(3,3) = p[10] * A(2,2) + p[4] * A(2,0) + p[9] * A(2,1); o.at(3,3) -= p[8] * A(2,1) + p[11] * A(2,2) + p[5] * A(2,0);
+  const float k = 1.0f / (A(0,0) * o(0,0) + A(0,1) * o(1,0) + A(0,2) * o(2,0) + A(0,3) * o(3,0));
+  for (int i = 0; i < 16; i++) o.m[i] = o.m[i] * k;
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// sample sequences on the host
+// ---------------------------------------------------------------------------
+// The reference's edge-sample streams: `omp parallel for num_threads(14)` with a static
+// schedule gives thread t the contiguous range [t*q + min(t, r), +q + (t < r)), q = N/14,
+// r = N%14; each thread draws (rnd0, rnd1) per sample from RandomGenInit(7777 + t*t + 1)
+// (src/drender.h:306-331; common_src/qmc.h:81-87, common_src/qmc.cpp:250-268).
+void prng14_table(long long N, std::vector<float>& out) {
+  out.resize(2 * (size_t)N);
+  const long long T = 14, q = N / T, r = N % T;
+  for (long long t = 0; t < T; t++) {
+    const uint32_t seed = (uint32_t)(7777 + t * t + 1);
+    uint32_t sx = seed * (seed * seed * 15731u + 74323u) + 871483u;
+    uint32_t sy = seed * (seed * seed * 13734u + 37828u) + 234234u;
+    auto next = [&]() { const uint32_t x = sx * 17u + sy * 13123u; sx = (x << 13) ^ x; sy ^= (x << 7); return x; };
+    for (uint32_t i = 0; i < seed % 7; i++) next();
+    auto rnd = [&]() { const uint32_t x = next(); const uint32_t v = x * (x * x * 15731u + 74323u) + 871483u; return (float)v * (1.0f / 4294967296.0f); };
+    const long long start = t * q + std::min(t, r), cnt = q + (t < r ? 1 : 0);
+    for (long long k = 0; k < cnt; k++) {
+      const float a = rnd();
+      const float b = rnd();
+      out[2 * (start + k)] = a;
+      out[2 * (start + k) + 1] = b;
+    }
+  }
+}
+
+// Niederreiter base-2 generator (Bratley, Fox & Niederreiter, ACM TOMACS 2(3), 1992), the
+// construction of qmc::init (common_src/qmc.cpp:14-187): dimension d uses the d-th
+// irreducible GF(2) polynomial in numeric order; each block of deg(p) output bits comes
+// from the linear recurrence with characteristic polynomial p^k.  table[d][bit] keeps
+// bits 32..62 of the 63-bit generator column.
+void niederreiter_table(uint32_t table[11 * 31]) {
+  std::vector<unsigned> irr;
+  for (unsigned p = 2; irr.size() < 11; p++) {
+    const int dp = 31 - __builtin_clz(p);
+    bool ok = true;
+    for (unsigned q : irr) {
+      const int dq = 31 - __builtin_clz(q);
+      if (2 * dq > dp) break;
+      unsigned rem = p;
+      for (int k = dp; k >= dq; k--) if (rem >> k & 1u) rem ^= q << (k - dq);
+      if (!rem) { ok = false; break; }
+    }
+    if (ok) irr.push_back(p);
+  }
+  for (int d = 0; d < 11; d++) {
+    const unsigned P = irr[d];
+    const int e = 31 - __builtin_clz(P);
+    std::vector<uint8_t> b{1};             // b(x) = p(x)^k, coefficients highest degree first
+    std::vector<uint8_t> v(63 + e, 0);
+    uint64_t col[63] = {0};
+    for (int r = 0; r < 63; r++) {
+      const int u = r % e;
+      if (u == 0) {
+        const size_t m1 = b.size() - 1;
+        std::vector<uint8_t> nb(m1 + e + 1, 0);
+        for (size_t i = 0; i <= m1; i++)
+          if (b[i]) for (int k = 0; k <= e; k++) nb[i + k] ^= (uint8_t)((P >> (e - k)) & 1u);
+        b.swap(nb);
+        const size_t m = b.size() - 1;
+        for (size_t i = 0; i < 63 + (size_t)e - 1; i++) {
+          if (i < m1) v[i] = 0;
+          else if (i < m) v[i] = 1;
+          else { uint8_t s = 0; for (size_t k = 1; k <= m; k++) s ^= v[i - k] & b[k]; v[i] = s; }
+        }
+      }
+      for (int i = 0; i < 63; i++) col[i] |= (uint64_t)v[i + u] << (62 - r);
+    }
+    for (int bit = 0; bit < 31; bit++) table[d * 31 + bit] = (uint32_t)((col[bit] >> 32) & 0x7FFFFFFFu);
+  }
+}
+
+// planeHammersley (common_src/qmc.cpp:227-243)
+void hammersley(int n, std::vector<float>& out) {
+  out.resize(2 * (size_t)std::max(n, 1));
+  for (int k = 0; k < n; k++) {
+    float u = 0;
+    int kk = k;
+    for (float p = 0.5f; kk; p *= 0.5f, kk >>= 1) if (kk & 1) u += p;
+    out[2 * k] = u;
+    out[2 * k + 1] = (k + 0.5f) / n;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  hipError_t reserve(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+struct ProfEvent { std::string name; hipEvent_t a, b; };
+
+}  // namespace
+
+struct drh_renderer {
+  drh_settings st{};
+  int sq = 1, nsamp = 1;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Light light{};
+  bool committed = false;
+
+  // scene (host view)
+  int nm = 0, V = 0, F = 0, G = 0, n_diff = 0;
+  std::vector<int> vo, to, tex_w, tex_h, tex_off, diff, layout;  // layout: 4 per mesh
+  std::vector<float> mats;      // 48 per mesh
+  std::vector<float> tr_inv0;   // 16
+  std::vector<uint32_t> idx_local_cache;
+  std::vector<int> diff_cache;
+  int E = 0;
+  bool has_nrm = false, has_col = false;
+
+  // device scene
+  DBuf pos_orig, pos, col, tc, nrm_in, nrm, nrm2d, texels, idx, tri_mesh, mesh_vo, mesh_to, dtex_w, dtex_h,
+      dtex_off, dmats, dlayout, dtr_inv0;
+  // LBVH
+  DBuf bf_a, bf_e1, bf_e2, box_lo, box_hi, cen, bounds, keys0, vals0, keys1, vals1, hist, child, node_parent,
+      leaf_parent, counter, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
+  // edges
+  DBuf edges, edge_mesh, pmf_raw, pmf, cdf, proj;
+  // sampling
+  DBuf rnd, qmc, hamm;
+  long long rnd_n = -1;
+  // per call
+  DBuf cams, img, adj, tgt, loss_part, loss, grad;
+  std::vector<DevCam> cam_cache;
+  // optimiser
+  int opt_kind = -1;
+  float pos_lr = 0.1f, col_lr = 0.1f;
+  int opt_iter = 0;
+  DBuf opt_m, opt_v;
+  // profiling
+  bool prof = false;
+  std::vector<ProfEvent> pending;
+  std::map<std::string, std::pair<double, int>> prof_acc;
+  std::vector<hipEvent_t> event_pool;
+
+  bool deep() const { return F > DEEP_F; }
+
+  drh_status fail(drh_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return s;
+  }
+  drh_status hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return DRH_OK;
+    return fail(e == hipErrorOutOfMemory ? DRH_E_OOM : DRH_E_HIP, "%s: %s", what, hipGetErrorString(e));
+  }
+
+  hipEvent_t ev() {
+    if (!event_pool.empty()) { hipEvent_t e = event_pool.back(); event_pool.pop_back(); return e; }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  template <class F> void timed(const char* name, F&& launch) {
+    if (!prof) { launch(); return; }
+    ProfEvent pe{name, ev(), ev()};
+    (void)hipEventRecord(pe.a, stream);
+    launch();
+    (void)hipEventRecord(pe.b, stream);
+    pending.push_back(pe);
+  }
+  void drain_prof() {
+    if (pending.empty()) return;
+    (void)hipStreamSynchronize(stream);
+    for (auto& p : pending) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, p.a, p.b);
+      auto& acc = prof_acc[p.name];
+      acc.first += ms;
+      acc.second += 1;
+      event_pool.push_back(p.a);
+      event_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+
+  Tracer tracer() const {
+    Tracer t{};
+    DevScene& s = t.s;
+    s.tri_a = tri_a.as<float4>(); s.tri_e1 = tri_e1.as<float4>(); s.tri_e2 = tri_e2.as<float4>();
+    s.nodes = nodes.as<float4>();
+    s.F = F; s.V = V;
+    s.idx = idx.as<uint32_t>(); s.tri_mesh = tri_mesh.as<int>(); s.mesh_vo = mesh_vo.as<int>(); s.mesh_to = mesh_to.as<int>();
+    s.pos = pos.as<float>(); s.pos_orig = pos_orig.as<float>(); s.col = col.as<float>(); s.tc = tc.as<float>();
+    s.nrm = nrm.as<float>(); s.nrm2d = nrm2d.as<float>(); s.texels = texels.as<float>();
+    s.tex_w = dtex_w.as<int>(); s.tex_h = dtex_h.as<int>(); s.tex_off = dtex_off.as<int>();
+    s.n_meshes = nm;
+    t.bf_a = bf_a.as<float4>(); t.bf_e1 = bf_e1.as<float4>(); t.bf_e2 = bf_e2.as<float4>();
+    t.use_bf = st.tracer == DRH_TRACER_BF;
+    return t;
+  }
+
+  drh_status build_bvh();
+  drh_status upload_scene(const drh_scene_desc* d);
+  drh_status set_cams(const drh_camera* cams, int views, long long* total_floats);
+  drh_status run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss);
+  drh_status run_d_render(int views, const float* d_adj, long long N, float* d_grad, bool accumulate);
+  drh_status ensure_rnd(long long N);
+};
+
+// ---------------------------------------------------------------------------
+// commit
+// ---------------------------------------------------------------------------
+drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
+  if (!d || d->n_meshes <= 0 || !d->vcount || !d->fcount || !d->transforms || !d->pos || !d->idx)
+    return fail(DRH_E_INVALID, "commit: scene arrays missing");
+  const int n = d->n_meshes;
+  std::vector<int> nvo(n + 1, 0), nto(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    if (d->vcount[i] < 0 || d->fcount[i] < 0) return fail(DRH_E_INVALID, "commit: negative size in mesh %d", i);
+    nvo[i + 1] = nvo[i] + d->vcount[i];
+    nto[i + 1] = nto[i] + d->fcount[i];
+  }
+  const int nV = nvo[n], nF = nto[n];
+  for (int i = 0; i < n; i++)
+    for (int f = nto[i]; f < nto[i + 1]; f++)
+      for (int k = 0; k < 3; k++)
+        if (d->idx[3 * f + k] >= (uint32_t)d->vcount[i])
+          return fail(DRH_E_INVALID, "commit: index %u out of range in mesh %d", d->idx[3 * f + k], i);
+  std::vector<int> tw(n), th(n), toff(n), df(n);
+  int tsum = 0;
+  for (int i = 0; i < n; i++) {
+    tw[i] = d->tex_w ? d->tex_w[i] : 0;
+    th[i] = d->tex_h ? d->tex_h[i] : 0;
+    toff[i] = tsum;
+    tsum += 3 * tw[i] * th[i];
+    df[i] = d->differentiable ? (d->differentiable[i] != 0) : 1;
+    if ((st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) && tw[i] * th[i] == 0)
+      return fail(DRH_E_UNSUPPORTED, "commit: mesh %d has no texture (TEXTURE_COLOR / LAMBERT read texture 0)", i);
+  }
+  if ((st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) && !d->tc)
+    return fail(DRH_E_UNSUPPORTED, "commit: texture coordinates required for this shading mode");
+  nm = n; V = nV; F = nF; vo = nvo; to = nto; tex_w = tw; tex_h = th; tex_off = toff; diff = df;
+  has_nrm = d->nrm != nullptr;
+  has_col = d->col != nullptr;
+  // gradient layout (DMesh, src/dmesh.h:35-85)
+  layout.assign(4 * n, -1);
+  G = 0; n_diff = 0;
+  for (int i = 0; i < n; i++) {
+    if (!df[i]) continue;
+    n_diff++;
+    const int nv = vo[i + 1] - vo[i];
+    int off = 3 * nv;
+    int colo = -1, texo = -1;
+    if (st.mode == DRH_VERTEX_COLOR) { colo = off; off += 3 * nv; }
+    if (st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) { texo = off; off += 3 * tw[i] * th[i]; }
+    layout[4 * i] = G; layout[4 * i + 1] = colo; layout[4 * i + 2] = texo; layout[4 * i + 3] = off;
+    G += off + 12;
+  }
+  // matrices: T, transpose(inverse(T)), identity
+  mats.assign(48 * (size_t)n, 0.0f);
+  for (int i = 0; i < n; i++) {
+    const M4 T = m4_load(d->transforms + 16 * i);
+    const M4 NT = m4_transpose(m4_inverse(T));
+    const M4 I = m4_identity();
+    std::memcpy(&mats[48 * i], T.m, 64);
+    std::memcpy(&mats[48 * i + 16], NT.m, 64);
+    std::memcpy(&mats[48 * i + 32], I.m, 64);
+  }
+  {
+    const M4 T0inv = m4_inverse(m4_load(d->transforms));
+    tr_inv0.assign(T0inv.m, T0inv.m + 16);
+  }
+  // edge set: std::set<Edge> of differentiable meshes ordered by (v0, v1), first kept
+  // (src/drender.h:26-52,274-285); depends on topology only, so cached.
+  std::vector<uint32_t> idx_local(d->idx, d->idx + 3 * (size_t)nF);
+  const bool topo_same = idx_local == idx_local_cache && df == diff_cache && E >= 0 && edges.p;
+  std::vector<int2> hedges;
+  std::vector<int> hemesh;
+  if (!topo_same) {
+    std::vector<std::pair<uint64_t, int>> keyed;
+    keyed.reserve(3 * (size_t)nF);
+    for (int i = 0; i < n; i++) {
+      if (!df[i]) continue;
+      for (int f = nto[i]; f < nto[i + 1]; f++)
+        for (int k = 0; k < 3; k++) {
+          const uint32_t a = nvo[i] + idx_local[3 * f + k], b = nvo[i] + idx_local[3 * f + (k + 1) % 3];
+          const uint32_t v0 = std::min(a, b), v1 = std::max(a, b);
+          keyed.push_back({((uint64_t)v0 << 32) | v1, i});
+        }
+    }
+    std::stable_sort(keyed.begin(), keyed.end(), [](const std::pair<uint64_t, int>& x, const std::pair<uint64_t, int>& y) { return x.first < y.first; });
+    for (size_t k = 0; k < keyed.size(); k++)
+      if (k == 0 || keyed[k].first != keyed[k - 1].first) {
+        hedges.push_back(make_int2((int)(keyed[k].first >> 32), (int)(keyed[k].first & 0xFFFFFFFFu)));
+        hemesh.push_back(keyed[k].second);
+      }
+  }
+  std::vector<uint32_t> gidx(3 * (size_t)nF);
+  std::vector<int> tmesh(nF);
+  for (int i = 0; i < n; i++)
+    for (int f = nto[i]; f < nto[i + 1]; f++) {
+      tmesh[f] = i;
+      for (int k = 0; k < 3; k++) gidx[3 * f + k] = (uint32_t)nvo[i] + idx_local[3 * f + k];
+    }
+
+  auto R = [&](DBuf& b, size_t bytes, const char* what) -> drh_status { return hip(b.reserve(std::max<size_t>(bytes, 16)), what); };
+  auto up = [&](DBuf& b, const void* src, size_t bytes) -> drh_status {
+    if (!bytes) return DRH_OK;
+    return hip(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream), "commit upload");
+  };
+  drh_status s;
+  const size_t v3 = 12 * (size_t)V;
+  if ((s = R(pos_orig, v3, "alloc")) || (s = R(pos, v3, "alloc")) || (s = R(col, v3, "alloc")) ||
+      (s = R(tc, 8 * (size_t)V, "alloc")) || (s = R(nrm_in, v3, "alloc")) || (s = R(nrm, v3, "alloc")) ||
+      (s = R(nrm2d, v3, "alloc")) || (s = R(texels, 4 * (size_t)tsum, "alloc")) || (s = R(idx, 12 * (size_t)F, "alloc")) ||
+      (s = R(tri_mesh, 4 * (size_t)F, "alloc")) || (s = R(mesh_vo, 4 * (size_t)(n + 1), "alloc")) ||
+      (s = R(mesh_to, 4 * (size_t)(n + 1), "alloc")) || (s = R(dtex_w, 4 * (size_t)n, "alloc")) ||
+      (s = R(dtex_h, 4 * (size_t)n, "alloc")) || (s = R(dtex_off, 4 * (size_t)n, "alloc")) ||
+      (s = R(dmats, 4 * mats.size(), "alloc")) || (s = R(dlayout, 4 * layout.size(), "alloc")) ||
+      (s = R(dtr_inv0, 64, "alloc")))
+    return s;
+  if ((s = up(pos_orig, d->pos, v3))) return s;
+  if (d->col) { if ((s = up(col, d->col, v3))) return s; }
+  else { launch_fill_f32(col.as<float>(), 3ll * V, 0.0f, stream); }
+  if (d->tc) { if ((s = up(tc, d->tc, 8 * (size_t)V))) return s; }
+  else { launch_fill_f32(tc.as<float>(), 2ll * V, 0.0f, stream); }
+  if (d->nrm) { if ((s = up(nrm_in, d->nrm, v3))) return s; }
+  else {
+    std::vector<float> def(3 * (size_t)V, 0.0f);
+    for (int k = 0; k < V; k++) def[3 * k] = 1.0f;  // prepared default normal (1,0,0)
+    if ((s = up(nrm, def.data(), v3)) || (s = up(nrm2d, def.data(), v3))) return s;
+    if ((s = hip(hipStreamSynchronize(stream), "commit upload"))) return s;
+  }
+  if (tsum && d->texels) { if ((s = up(texels, d->texels, 4 * (size_t)tsum))) return s; }
+  if ((s = up(idx, gidx.data(), 12 * (size_t)F)) || (s = up(tri_mesh, tmesh.data(), 4 * (size_t)F)) ||
+      (s = up(mesh_vo, vo.data(), 4 * (size_t)(n + 1))) || (s = up(mesh_to, to.data(), 4 * (size_t)(n + 1))) ||
+      (s = up(dtex_w, tw.data(), 4 * (size_t)n)) || (s = up(dtex_h, th.data(), 4 * (size_t)n)) ||
+      (s = up(dtex_off, toff.data(), 4 * (size_t)n)) || (s = up(dmats, mats.data(), 4 * mats.size())) ||
+      (s = up(dlayout, layout.data(), 4 * layout.size())) || (s = up(dtr_inv0, tr_inv0.data(), 64)))
+    return s;
+  if (!topo_same) {
+    E = (int)hedges.size();
+    if ((s = R(edges, 8 * (size_t)E, "alloc")) || (s = R(edge_mesh, 4 * (size_t)E, "alloc")) ||
+        (s = R(pmf_raw, 4 * (size_t)E, "alloc")) || (s = R(pmf, 4 * (size_t)E, "alloc")) ||
+        (s = R(cdf, 4 * (size_t)(E + 1), "alloc")))
+      return s;
+    if ((s = up(edges, hedges.data(), 8 * (size_t)E)) || (s = up(edge_mesh, hemesh.data(), 4 * (size_t)E))) return s;
+    idx_local_cache = idx_local;
+    diff_cache = df;
+  }
+  if ((s = R(proj, 16 * (size_t)V, "alloc"))) return s;
+  // everything uploaded from this call's host arrays must land before they may change
+  if ((s = hip(hipStreamSynchronize(stream), "commit sync"))) return s;
+  return DRH_OK;
+}
+
+drh_status drh_renderer::build_bvh() {
+  PrepareArgs pa{V, nm, has_nrm ? 1 : 0, mesh_vo.as<int>(), dmats.as<float>(), pos_orig.as<float>(),
+                 nrm_in.as<float>(), pos.as<float>(), nrm.as<float>(), nrm2d.as<float>()};
+  if (V > 0) timed("k_prepare", [&] { launch_prepare(pa, stream); });
+  if (F == 0) return hip(hipGetLastError(), "commit kernels");
+  auto R = [&](DBuf& b, size_t bytes) -> drh_status { return hip(b.reserve(std::max<size_t>(bytes, 16)), "alloc"); };
+  const size_t f16 = 16 * (size_t)F, f4 = 4 * (size_t)F;
+  const int nb = (F + RADIX_TILE - 1) / RADIX_TILE;
+  drh_status s;
+  if ((s = R(bf_a, f16)) || (s = R(bf_e1, f16)) || (s = R(bf_e2, f16)) || (s = R(box_lo, f16)) || (s = R(box_hi, f16)) ||
+      (s = R(cen, f16)) || (s = R(bounds, 32)) || (s = R(keys0, f4)) || (s = R(vals0, f4)) || (s = R(keys1, f4)) ||
+      (s = R(vals1, f4)) || (s = R(hist, 4 * 256 * (size_t)nb)) || (s = R(child, 8 * (size_t)F)) ||
+      (s = R(node_parent, f4)) || (s = R(leaf_parent, f4)) || (s = R(counter, f4)) || (s = R(leaf_lo, f16)) ||
+      (s = R(leaf_hi, f16)) || (s = R(node_lo, f16)) || (s = R(node_hi, f16)) || (s = R(nodes, 4 * f16)) ||
+      (s = R(tri_a, f16)) || (s = R(tri_e1, f16)) || (s = R(tri_e2, f16)))
+    return s;
+  timed("k_fill_u32", [&] {
+    launch_fill_u32(bounds.as<uint32_t>(), 3, 0xFFFFFFFFu, stream);
+    launch_fill_u32(bounds.as<uint32_t>() + 3, 3, 0u, stream);
+  });
+  TriSetupArgs ta{F, tri_mesh.as<int>(), dmats.as<float>(), idx.as<uint32_t>(), pos.as<float>(), bf_a.as<float4>(),
+                  bf_e1.as<float4>(), bf_e2.as<float4>(), box_lo.as<float4>(), box_hi.as<float4>(), cen.as<float4>(),
+                  bounds.as<uint32_t>()};
+  timed("k_tri_setup", [&] { launch_tri_setup(ta, stream); });
+  MortonArgs ma{F, cen.as<float4>(), bounds.as<uint32_t>(), keys0.as<uint32_t>(), vals0.as<uint32_t>()};
+  timed("k_morton", [&] { launch_morton(ma, stream); });
+  for (int pass = 0; pass < 4; pass++) {
+    const bool even = (pass & 1) == 0;
+    const uint32_t* kin = even ? keys0.as<uint32_t>() : keys1.as<uint32_t>();
+    const uint32_t* vin = even ? vals0.as<uint32_t>() : vals1.as<uint32_t>();
+    uint32_t* kout = even ? keys1.as<uint32_t>() : keys0.as<uint32_t>();
+    uint32_t* vout = even ? vals1.as<uint32_t>() : vals0.as<uint32_t>();
+    timed("k_radix_sort", [&] { launch_radix_pass(kin, vin, kout, vout, F, 8 * pass, hist.as<uint32_t>(), stream); });
+  }
+  // after 4 passes the sorted keys / triangle order are back in keys0 / vals0
+  if (F > 1) {
+    KarrasArgs ka{F, keys0.as<uint32_t>(), child.as<int>(), node_parent.as<int>(), leaf_parent.as<int>()};
+    timed("k_karras", [&] { launch_karras(ka, stream); });
+    timed("k_fill_u32", [&] { launch_fill_u32(counter.as<uint32_t>(), F, 0u, stream); });
+    RefitArgs ra{F, vals0.as<uint32_t>(), box_lo.as<float4>(), box_hi.as<float4>(), child.as<int>(),
+                 node_parent.as<int>(), leaf_parent.as<int>(), counter.as<uint32_t>(), leaf_lo.as<float4>(),
+                 leaf_hi.as<float4>(), node_lo.as<float4>(), node_hi.as<float4>()};
+    timed("k_refit", [&] { launch_refit(ra, stream); });
+    PackArgs pk{F, child.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
+                node_hi.as<float4>(), nodes.as<float4>()};
+    timed("k_pack_nodes", [&] { launch_pack(pk, stream); });
+  }
+  ReorderArgs ro{F, vals0.as<uint32_t>(), bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
+                 tri_a.as<float4>(), tri_e1.as<float4>(), tri_e2.as<float4>()};
+  timed("k_reorder", [&] { launch_reorder(ro, stream); });
+  return hip(hipGetLastError(), "commit kernels");
+}
+
+drh_status drh_renderer::set_cams(const drh_camera* c, int views, long long* total) {
+  if (!c || views <= 0) return fail(DRH_E_INVALID, "no cameras");
+  std::vector<DevCam> dc(views);
+  long long off = 0;
+  for (int v = 0; v < views; v++) {
+    if (!(c[v].width >= 1.0f) || !(c[v].height >= 1.0f) || c[v].width > 65536.0f || c[v].height > 65536.0f)
+      return fail(DRH_E_INVALID, "camera %d: bad size %g x %g", v, c[v].width, c[v].height);
+    const M4 wv = m4_load(c[v].world_view), pr = m4_load(c[v].proj);
+    const M4 pi = m4_inverse(pr), wi = m4_inverse(wv), wvp = m4_mul(pr, wv);
+    std::memcpy(dc[v].proj_inv, pi.m, 64);
+    std::memcpy(dc[v].wv_inv, wi.m, 64);
+    std::memcpy(dc[v].wvp, wvp.m, 64);
+    dc[v].W = c[v].width; dc[v].H = c[v].height;
+    dc[v].w = (int)c[v].width; dc[v].h = (int)c[v].height;
+    dc[v].img_off = off;
+    off += 3ll * dc[v].w * dc[v].h;
+  }
+  *total = off;
+  const bool same = cam_cache.size() == dc.size() && std::memcmp(cam_cache.data(), dc.data(), sizeof(DevCam) * dc.size()) == 0;
+  if (!same) {
+    drh_status s;
+    if ((s = hip(cams.reserve(sizeof(DevCam) * views), "alloc"))) return s;
+    if ((s = hip(hipMemcpyAsync(cams.p, dc.data(), sizeof(DevCam) * views, hipMemcpyHostToDevice, stream), "camera upload"))) return s;
+    if ((s = hip(hipStreamSynchronize(stream), "camera upload"))) return s;
+    cam_cache = dc;
+  }
+  return DRH_OK;
+}
+
+drh_status drh_renderer::run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss) {
+  const Tracer T = tracer();
+  int part = 0;
+  std::vector<int> tiles(views);
+  for (int v = 0; v < views; v++) { tiles[v] = ((cam_cache[v].w + 7) / 8) * ((cam_cache[v].h + 7) / 8); part += tiles[v]; }
+  if (d_target) {
+    drh_status s;
+    if ((s = hip(loss_part.reserve(8 * (size_t)std::max(part, 1)), "alloc"))) return s;
+  }
+  int part_off = 0;
+  for (int v = 0; v < views; v++) {
+    RenderArgs ra{T, light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off};
+    timed("k_render", [&] { launch_render(st.mode, deep(), ra, tiles[v], stream); });
+    part_off += tiles[v];
+  }
+  if (d_target && d_loss) {
+    double denom = 0;
+    for (int v = 0; v < views; v++) denom += (double)cam_cache[0].w * cam_cache[0].h;
+    timed("k_loss_final", [&] { launch_loss_final(loss_part.as<double>(), part, d_loss, denom, stream); });
+  }
+  return hip(hipGetLastError(), "render kernels");
+}
+
+drh_status drh_renderer::ensure_rnd(long long N) {
+  if (st.rng_mode != DRH_RNG_REF_PRNG14 || N == rnd_n) return DRH_OK;
+  std::vector<float> t;
+  prng14_table(N, t);
+  drh_status s;
+  if ((s = hip(rnd.reserve(std::max<size_t>(8 * (size_t)N, 16)), "alloc"))) return s;
+  if (N > 0 && (s = hip(hipMemcpy(rnd.p, t.data(), 8 * (size_t)N, hipMemcpyHostToDevice), "rng upload"))) return s;
+  rnd_n = N;
+  return DRH_OK;
+}
+
+drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N, float* d_grad, bool accumulate) {
+  drh_status s;
+  if ((s = ensure_rnd(N))) return s;
+  if (!accumulate) timed("k_fill_f32", [&] { launch_fill_f32(d_grad, G, 0.0f, stream); });
+  const Tracer T = tracer();
+  // single differentiable mesh: position / transform gradient offsets
+  int single = -1;
+  for (int i = 0; i < nm; i++) if (diff[i]) { single = (single == -1) ? i : -2; }
+  for (int v = 0; v < views; v++) {
+    const DevCam& c = cam_cache[v];
+    const int tiles = ((c.w + 7) / 8) * ((c.h + 7) / 8);
+    if (st.mode != DRH_SILHOUETTE && G > 0) {
+      InteriorArgs ia{};
+      ia.tr = T; ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.hamm = hamm.as<float>();
+      ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
+      ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
+      int blocks = tiles;
+      if ((st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) && nm == 1 && diff[0] && tex_w[0] * tex_h[0] == 1) {
+        ia.uniform = 1; ia.uniform_off = layout[2];
+        blocks = std::min(tiles, 4096);
+      } else if (G <= INTERIOR_LDS_FLOATS) {
+        ia.lds_base = 0; ia.lds_n = G;
+        blocks = std::min(tiles, 1024);
+      }
+      timed("k_interior", [&] { launch_interior(st.mode, deep(), ia, blocks, stream); });
+    }
+    if (E > 0 && N > 0) {
+      ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>()};
+      timed("k_project", [&] { launch_project(pa, stream); });
+      timed("k_edge_len", [&] { launch_edge_len(edges.as<int2>(), E, proj.as<float4>(), pmf_raw.as<float>(), stream); });
+      timed("k_cdf", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), stream); });
+      EdgeArgs ea{};
+      ea.tr = T; ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
+      ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
+      ea.proj = proj.as<float4>();
+      ea.rnd = st.rng_mode == DRH_RNG_REF_PRNG14 ? rnd.as<float2>() : nullptr;
+      ea.qmc = qmc.as<uint32_t>(); ea.adj = d_adj; ea.grad = d_grad; ea.layout = dlayout.as<int>();
+      ea.tr_inv0 = dtr_inv0.as<float>();
+      ea.lds_base = 0; ea.lds_n = 0;
+      if (single >= 0 && 3 * (vo[single + 1] - vo[single]) <= EDGE_LDS_FLOATS) {
+        ea.lds_base = layout[4 * single]; ea.lds_n = 3 * (vo[single + 1] - vo[single]);
+      }
+      ea.single_mesh = single >= 0 ? 1 : 0;
+      ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
+      const int blocks = (int)std::max<long long>(1, std::min<long long>((N + 255) / 256, 1024));
+      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, deep(), ea, blocks, stream); });
+    }
+  }
+  return hip(hipGetLastError(), "d_render kernels");
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+drh_status drh_create(const drh_settings* settings, drh_handle* out) {
+  if (!settings || !out) return DRH_E_INVALID;
+  *out = nullptr;
+  if (settings->mode < DRH_SILHOUETTE || settings->mode > DRH_LAMBERT || settings->spp < 1 ||
+      (settings->rng_mode != DRH_RNG_REF_PRNG14 && settings->rng_mode != DRH_RNG_QMC) ||
+      (settings->tracer != DRH_TRACER_BVH && settings->tracer != DRH_TRACER_BF))
+    return DRH_E_INVALID;
+  std::unique_ptr<drh_renderer> r(new drh_renderer());
+  r->st = *settings;
+  if (hipSetDevice(settings->device) != hipSuccess) return DRH_E_HIP;
+  if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return DRH_E_HIP;
+  r->sq = (int)std::sqrt((float)settings->spp);
+  r->nsamp = r->sq * r->sq;
+  // light (src/shade_lambert.cpp:10-12 unless a custom one is given)
+  if (settings->light_custom) {
+    r->light.L = {settings->light_dir[0], settings->light_dir[1], settings->light_dir[2]};
+    r->light.Lc = {settings->light_color[0], settings->light_color[1], settings->light_color[2]};
+    r->light.amb = {settings->ambient[0], settings->ambient[1], settings->ambient[2]};
+  } else {
+    const float x = -1, y = -1, z = 0;
+    const float li = 1.0f / std::sqrt(x * x + y * y + z * z);
+    r->light.L = {x * li, y * li, z * li};
+    r->light.Lc = {1.0f, 0.5f, 0.2f};
+    r->light.amb = {0.1f * 1.0f, 0.1f * 1.0f, 0.1f * 1.0f};
+  }
+  std::vector<float> hm;
+  hammersley(settings->spp, hm);
+  uint32_t tab[11 * 31];
+  niederreiter_table(tab);
+  if (r->hamm.reserve(4 * hm.size()) != hipSuccess || r->qmc.reserve(sizeof tab) != hipSuccess) return DRH_E_OOM;
+  if (hipMemcpy(r->hamm.p, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(r->qmc.p, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess || r->loss.reserve(16) != hipSuccess)
+    return DRH_E_HIP;
+  *out = r.release();
+  return DRH_OK;
+}
+
+void drh_destroy(drh_handle h) {
+  if (!h) return;
+  (void)hipSetDevice(h->st.device);
+  (void)hipStreamSynchronize(h->stream);
+  for (auto& p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : h->event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char* drh_last_error(drh_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+drh_status drh_commit(drh_handle h, const drh_scene_desc* scene) {
+  if (!h) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  h->committed = false;
+  drh_status s = h->upload_scene(scene);
+  if (s) return s;
+  if ((s = h->build_bvh())) return s;
+  h->committed = true;
+  h->opt_kind = -1;  // optimiser state belongs to the previous scene
+  return DRH_OK;
+}
+
+int drh_grad_size(drh_handle h) { return h ? h->G : -1; }
+
+drh_status drh_render_dev(drh_handle h, const drh_camera* cams, int views, float* d_images) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "render: renderer was not prepared for this scene (commit first)");
+  if (!d_images) return h->fail(DRH_E_INVALID, "render: images NULL");
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cams, views, &total))) return s;
+  return h->run_render(views, d_images, nullptr, nullptr, nullptr);
+}
+
+drh_status drh_render(drh_handle h, const drh_camera* cams, int views, float* images) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "render: renderer was not prepared for this scene (commit first)");
+  if (!images) return h->fail(DRH_E_INVALID, "render: images NULL");
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cams, views, &total))) return s;
+  if ((s = h->hip(h->img.reserve(4 * (size_t)total), "alloc"))) return s;
+  if ((s = h->run_render(views, h->img.as<float>(), nullptr, nullptr, nullptr))) return s;
+  if ((s = h->hip(hipMemcpyAsync(images, h->img.p, 4 * (size_t)total, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  return h->hip(hipStreamSynchronize(h->stream), "render");
+}
+
+drh_status drh_d_render_dev(drh_handle h, const drh_camera* cams, int views, const float* d_adj, int edge_samples,
+                            float* d_grad, int accumulate) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "d_render: renderer was not prepared for this scene (commit first)");
+  if (!d_adj || !d_grad || edge_samples < 0) return h->fail(DRH_E_INVALID, "d_render: bad arguments");
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cams, views, &total))) return s;
+  return h->run_d_render(views, d_adj, edge_samples, d_grad, accumulate != 0);
+}
+
+drh_status drh_d_render(drh_handle h, const drh_camera* cams, int views, const float* adjoints, int edge_samples,
+                        float* grad, int accumulate) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "d_render: renderer was not prepared for this scene (commit first)");
+  if (!adjoints || !grad || edge_samples < 0) return h->fail(DRH_E_INVALID, "d_render: bad arguments");
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cams, views, &total))) return s;
+  if ((s = h->hip(h->adj.reserve(4 * (size_t)total), "alloc")) || (s = h->hip(h->grad.reserve(4 * (size_t)std::max(h->G, 1)), "alloc")))
+    return s;
+  if ((s = h->hip(hipMemcpyAsync(h->adj.p, adjoints, 4 * (size_t)total, hipMemcpyHostToDevice, h->stream), "upload"))) return s;
+  if (accumulate && h->G > 0 &&
+      (s = h->hip(hipMemcpyAsync(h->grad.p, grad, 4 * (size_t)h->G, hipMemcpyHostToDevice, h->stream), "upload")))
+    return s;
+  if ((s = h->run_d_render(views, h->adj.as<float>(), edge_samples, h->grad.as<float>(), accumulate != 0))) return s;
+  if (h->G > 0 && (s = h->hip(hipMemcpyAsync(grad, h->grad.p, 4 * (size_t)h->G, hipMemcpyDeviceToHost, h->stream), "download")))
+    return s;
+  return h->hip(hipStreamSynchronize(h->stream), "d_render");
+}
+
+drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, int views, const float* d_targets,
+                                        int edge_samples, float* d_grad, float* d_images, float* d_loss) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "d_render_and_compare: commit first");
+  if (!d_targets || !d_grad || edge_samples < 0) return h->fail(DRH_E_INVALID, "d_render_and_compare: bad arguments");
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cams, views, &total))) return s;
+  for (int v = 1; v < views; v++)
+    if (h->cam_cache[v].w != h->cam_cache[0].w || h->cam_cache[v].h != h->cam_cache[0].h)
+      return h->fail(DRH_E_INVALID, "d_render_and_compare: views must share one resolution");
+  if ((s = h->hip(h->adj.reserve(4 * (size_t)total), "alloc"))) return s;
+  float* img = d_images;
+  if (!img) {
+    if ((s = h->hip(h->img.reserve(4 * (size_t)total), "alloc"))) return s;
+    img = h->img.as<float>();
+  }
+  float* loss = d_loss ? d_loss : h->loss.as<float>();
+  if ((s = h->run_render(views, img, d_targets, h->adj.as<float>(), loss))) return s;
+  return h->run_d_render(views, h->adj.as<float>(), edge_samples, d_grad, false);
+}
+
+drh_status drh_d_render_and_compare(drh_handle h, const drh_camera* cams, int views, const float* targets,
+                                    int edge_samples, float* grad, float* out_images, float* loss) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "d_render_and_compare: commit first");
+  if (!targets || !grad || edge_samples < 0) return h->fail(DRH_E_INVALID, "d_render_and_compare: bad arguments");
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cams, views, &total))) return s;
+  if ((s = h->hip(h->tgt.reserve(4 * (size_t)total), "alloc")) || (s = h->hip(h->img.reserve(4 * (size_t)total), "alloc")) ||
+      (s = h->hip(h->grad.reserve(4 * (size_t)std::max(h->G, 1)), "alloc")))
+    return s;
+  if ((s = h->hip(hipMemcpyAsync(h->tgt.p, targets, 4 * (size_t)total, hipMemcpyHostToDevice, h->stream), "upload"))) return s;
+  if ((s = drh_d_render_and_compare_dev(h, cams, views, h->tgt.as<float>(), edge_samples, h->grad.as<float>(),
+                                        h->img.as<float>(), h->loss.as<float>())))
+    return s;
+  if (h->G > 0 && (s = h->hip(hipMemcpyAsync(grad, h->grad.p, 4 * (size_t)h->G, hipMemcpyDeviceToHost, h->stream), "download")))
+    return s;
+  if (out_images && (s = h->hip(hipMemcpyAsync(out_images, h->img.p, 4 * (size_t)total, hipMemcpyDeviceToHost, h->stream), "download")))
+    return s;
+  float l = 0.0f;
+  if ((s = h->hip(hipMemcpyAsync(&l, h->loss.p, 4, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if ((s = h->hip(hipStreamSynchronize(h->stream), "d_render_and_compare"))) return s;
+  if (loss) *loss = l;
+  return DRH_OK;
+}
+
+drh_status drh_opt_init(drh_handle h, int kind, float pos_lr, float col_lr) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "opt_init: commit first");
+  if (kind != DRH_OPT_SIMPLE && kind != DRH_OPT_ADAM) return h->fail(DRH_E_INVALID, "opt_init: kind");
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  const size_t n = 4 * (size_t)std::max(h->G, 1);
+  if ((s = h->hip(h->opt_m.reserve(n), "alloc")) || (s = h->hip(h->opt_v.reserve(n), "alloc")) ||
+      (s = h->hip(h->grad.reserve(n), "alloc")))
+    return s;
+  launch_fill_f32(h->opt_m.as<float>(), h->G, 0.0f, h->stream);
+  launch_fill_f32(h->opt_v.as<float>(), h->G, 0.0f, h->stream);
+  h->opt_kind = kind;
+  h->pos_lr = pos_lr;
+  h->col_lr = col_lr;
+  h->opt_iter = 0;
+  return h->hip(hipGetLastError(), "opt_init");
+}
+
+drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const float* d_targets, int edge_samples,
+                        float* d_loss) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "opt_step: commit first");
+  if (h->opt_kind < 0) return h->fail(DRH_E_INVALID, "opt_step: call drh_opt_init first");
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  // re-commit from the device parameters (the optimiser moved them)
+  if (h->opt_iter > 0 && (s = h->build_bvh())) return s;
+  if ((s = drh_d_render_and_compare_dev(h, cams, views, d_targets, edge_samples, h->grad.as<float>(), nullptr, d_loss)))
+    return s;
+  // StepDecay (src/optimizer_simple.cpp:82-90): decayPeriod 100, decay_mult 0.75 for GD_Adam,
+  // applied after iterations 100, 200, ...
+  float plr = h->pos_lr, clr = h->col_lr;
+  if (h->opt_kind == DRH_OPT_SIMPLE)
+    for (int k = 1; k < h->opt_iter; k++)
+      if (k % 100 == 0) { plr *= 0.75f; clr *= 0.75f; }
+  for (int i = 0; i < h->nm; i++) {
+    if (!h->diff[i]) continue;
+    const int off = h->layout[4 * i];
+    const int nv = h->vo[i + 1] - h->vo[i];
+    const int colo = h->layout[4 * i + 1];
+    const int n = h->layout[4 * i + 3] + 12;
+    int seg_start[MAX_SEG] = {0};
+    float seg_lr[MAX_SEG] = {0};
+    float* seg_param[MAX_SEG] = {nullptr};
+    int nseg = 0;
+    // OptSimple::GetLR (src/optimizer_simple.cpp:61-80) blocks; positions and colours are
+    // updated (OptUpdateScene with _OPT_POS | _OPT_COL), texture / transform blocks are not
+    seg_start[nseg] = 0; seg_lr[nseg] = plr; seg_param[nseg] = h->pos_orig.as<float>() + 3ll * h->vo[i]; nseg++;
+    if (colo >= 0) {
+      seg_start[nseg] = colo; seg_lr[nseg] = clr; seg_param[nseg] = h->col.as<float>() + 3ll * h->vo[i]; nseg++;
+    }
+    const int texo = h->layout[4 * i + 2];
+    if (texo >= 0) { seg_start[nseg] = texo; seg_lr[nseg] = 0.0f; seg_param[nseg] = nullptr; nseg++; }
+    seg_start[nseg] = h->layout[4 * i + 3]; seg_lr[nseg] = 0.0f; seg_param[nseg] = nullptr; nseg++;
+    (void)nv;
+    if (h->opt_kind == DRH_OPT_SIMPLE) {
+      OptApplyArgs a{};
+      a.n = n; a.off = off; a.nseg = nseg; a.m = h->opt_m.as<float>(); a.G = h->opt_v.as<float>();
+      for (int k = 0; k < nseg; k++) { a.seg_start[k] = seg_start[k]; a.seg_lr[k] = seg_lr[k]; a.seg_param[k] = seg_param[k]; }
+      h->timed("k_optsimple", [&] { launch_optsimple(h->grad.as<float>(), h->opt_m.as<float>(), h->opt_v.as<float>(), off, a, h->stream); });
+    } else {
+      AdamArgs a{};
+      a.n = n; a.off = off; a.nseg = nseg; a.grad = h->grad.as<float>(); a.m = h->opt_m.as<float>(); a.v = h->opt_v.as<float>();
+      a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-8f;
+      a.bc1 = (float)(1.0 - std::pow(0.9, h->opt_iter + 1));
+      a.bc2 = (float)(1.0 - std::pow(0.999, h->opt_iter + 1));
+      for (int k = 0; k < nseg; k++) { a.seg_start[k] = seg_start[k]; a.seg_lr[k] = seg_lr[k]; a.seg_param[k] = seg_param[k]; }
+      h->timed("k_adam", [&] { launch_adam(a, h->stream); });
+    }
+  }
+  h->opt_iter++;
+  return h->hip(hipGetLastError(), "opt_step");
+}
+
+drh_status drh_get_params(drh_handle h, float* pos, float* col) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "get_params: commit first");
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  if (pos && (s = h->hip(hipMemcpyAsync(pos, h->pos_orig.p, 12 * (size_t)h->V, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if (col && (s = h->hip(hipMemcpyAsync(col, h->col.p, 12 * (size_t)h->V, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  return h->hip(hipStreamSynchronize(h->stream), "get_params");
+}
+
+void* drh_stream(drh_handle h) { return h ? (void*)h->stream : nullptr; }
+
+drh_status drh_sync(drh_handle h) {
+  if (!h) return DRH_E_INVALID;
+  return h->hip(hipStreamSynchronize(h->stream), "sync");
+}
+
+drh_status drh_profile_enable(drh_handle h, int on) {
+  if (!h) return DRH_E_INVALID;
+  h->drain_prof();
+  h->prof_acc.clear();
+  h->prof = on != 0;
+  return DRH_OK;
+}
+
+drh_status drh_profile_read(drh_handle h, const char* name, double* total_ms, int* launches) {
+  if (!h || !name) return DRH_E_INVALID;
+  h->drain_prof();
+  auto it = h->prof_acc.find(name);
+  if (total_ms) *total_ms = it == h->prof_acc.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == h->prof_acc.end() ? 0 : it->second.second;
+  return DRH_OK;
+}
+
+drh_status drh_debug_hits(drh_handle h, const drh_camera* cam, uint32_t* prim, uint32_t* geom, float* t, float* u,
+                          float* v) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "debug_hits: commit first");
+  if (!prim || !geom || !t || !u || !v) return h->fail(DRH_E_INVALID, "debug_hits: NULL output");
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cam, 1, &total))) return s;
+  const int n = h->cam_cache[0].w * h->cam_cache[0].h;
+  DBuf b;
+  if ((s = h->hip(b.reserve(20 * (size_t)n), "alloc"))) return s;
+  HitsArgs a{h->tracer(), h->cams.as<DevCam>(), b.as<uint32_t>(), b.as<uint32_t>() + n, (float*)(b.as<uint32_t>() + 2 * n),
+             (float*)(b.as<uint32_t>() + 3 * n), (float*)(b.as<uint32_t>() + 4 * n)};
+  const int tiles = ((h->cam_cache[0].w + 7) / 8) * ((h->cam_cache[0].h + 7) / 8);
+  launch_debug_hits(h->deep(), a, tiles, h->stream);
+  if ((s = h->hip(hipGetLastError(), "debug_hits"))) return s;
+  std::vector<uint32_t> hb(5 * (size_t)n);
+  if ((s = h->hip(hipMemcpyAsync(hb.data(), b.p, 20 * (size_t)n, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if ((s = h->hip(hipStreamSynchronize(h->stream), "debug_hits"))) return s;
+  std::memcpy(prim, hb.data(), 4 * (size_t)n);
+  std::memcpy(geom, hb.data() + n, 4 * (size_t)n);
+  std::memcpy(t, hb.data() + 2 * n, 4 * (size_t)n);
+  std::memcpy(u, hb.data() + 3 * n, 4 * (size_t)n);
+  std::memcpy(v, hb.data() + 4 * n, 4 * (size_t)n);
+  return DRH_OK;
+}
+
+drh_status drh_debug_bvh(drh_handle h, uint32_t* morton_sorted, uint32_t* tri_order, float* scene_bounds) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "debug_bvh: commit first");
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  if (h->F == 0) return DRH_OK;
+  if (morton_sorted && (s = h->hip(hipMemcpyAsync(morton_sorted, h->keys0.p, 4 * (size_t)h->F, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if (tri_order && (s = h->hip(hipMemcpyAsync(tri_order, h->vals0.p, 4 * (size_t)h->F, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  uint32_t b[6];
+  if ((s = h->hip(hipMemcpyAsync(b, h->bounds.p, 24, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if ((s = h->hip(hipStreamSynchronize(h->stream), "debug_bvh"))) return s;
+  if (scene_bounds)
+    for (int i = 0; i < 6; i++) {
+      const uint32_t k = b[i];
+      const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+      std::memcpy(&scene_bounds[i], &u, 4);
+    }
+  return DRH_OK;
+}
+
+drh_status drh_debug_edges(drh_handle h, int* n_edges, int* v0, int* v1, float* pmf, float* cdf) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "debug_edges: commit first");
+  (void)hipSetDevice(h->st.device);
+  if (n_edges) *n_edges = h->E;
+  if (!v0 || h->E == 0) return DRH_OK;
+  std::vector<int2> e(h->E);
+  drh_status s;
+  if ((s = h->hip(hipMemcpyAsync(e.data(), h->edges.p, 8 * (size_t)h->E, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if (pmf && (s = h->hip(hipMemcpyAsync(pmf, h->pmf.p, 4 * (size_t)h->E, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if (cdf && (s = h->hip(hipMemcpyAsync(cdf, h->cdf.p, 4 * (size_t)(h->E + 1), hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  if ((s = h->hip(hipStreamSynchronize(h->stream), "debug_edges"))) return s;
+  for (int i = 0; i < h->E; i++) { v0[i] = e[i].x; if (v1) v1[i] = e[i].y; }
+  return DRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,321 @@
+// drh_device.h -- device math, ray generation, LBVH / brute-force closest hit and the
+// shading models of the render / d_render path, for gfx950.
+//
+// Bit-exactness: every translation unit is built with -ffp-contract=off and HIP's
+// default correctly rounded f32 divide and sqrt, and each expression keeps the
+// reference's evaluation order (LiteMath, common_src/LiteMath.h), so a ray, its
+// Moller-Trumbore test and the shaded colour are the same IEEE operations the
+// reference's brute-force tracer performs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace drh {
+
+constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
+
+struct F2 { float x, y; };
+struct F3 { float x, y, z; };
+
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 mul(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ F3 mul(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ F3 mulv(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ F3 divs(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+// a.x*b.x + a.y*b.y + a.z*b.z evaluated left to right (LiteMath.h:792)
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(F3 a) { return sqrtf(dot(a, a)); }
+// a * (1/|a|) (LiteMath.h:795)
+__device__ __forceinline__ F3 normalize(F3 a) { const float li = 1.0f / length(a); return mul(a, li); }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// LiteMath sign (LiteMath.h:109-114)
+__device__ __forceinline__ float sgn(float x) { return x == 0.0f ? 0.0f : (x < 0.0f ? -1.0f : 1.0f); }
+// std::max / std::min (NaN in the first argument propagates differently from fmaxf)
+__device__ __forceinline__ float max_std(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float min_std(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ int clampi(int u, int a, int b) { int t = (a < u) ? u : a; return (b < t) ? b : t; }
+__device__ __forceinline__ float clampf(float u, float a, float b) { return min_std(max_std(a, u), b); }
+
+__device__ __forceinline__ F3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ F3 f4xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+// float4x4 (column-major memory) times (x, y, z, w) with the reference's term order
+// (mat4_colmajor_mul_vec4, LiteMath.h:1342-1348)
+__device__ __forceinline__ void mv4(const float* B, float x, float y, float z, float w, float r[4]) {
+  r[0] = x * B[0] + y * B[4] + z * B[8] + w * B[12];
+  r[1] = x * B[1] + y * B[5] + z * B[9] + w * B[13];
+  r[2] = x * B[2] + y * B[6] + z * B[10] + w * B[14];
+  r[3] = x * B[3] + y * B[7] + z * B[11] + w * B[15];
+}
+__device__ __forceinline__ F3 mp(const float* B, F3 v) { float r[4]; mv4(B, v.x, v.y, v.z, 1.0f, r); return {r[0], r[1], r[2]}; }
+
+// Per-view camera constants (host computes inverse4x4 and mProj*mWorldView exactly as
+// BruteForce3D::SetCamera and CamInfo::commit do, raytrace3d_bf.cpp:93-100, camera.h:39-43).
+struct DevCam {
+  float proj_inv[16];
+  float wv_inv[16];
+  float wvp[16];
+  float W, H;          // CamInfo::width / height
+  int   w, h;          // integer image size
+  long long img_off;   // float offset of this view's image in a views x H x W x 3 buffer
+};
+
+// CastSingleRay: EyeRayDirNormalized + transform_ray3f (raytrace.h:49-76, raytrace3d_bf.cpp:158-174)
+__device__ __forceinline__ void eye_ray(const DevCam& c, float x, float y, F3& o, F3& d) {
+  const float xn = x / c.W, yn = y / c.H;
+  float p[4];
+  mv4(c.proj_inv, 2.0f * xn - 1.0f, -2.0f * yn + 1.0f, 0.0f, 1.0f, p);
+  const float w = p[3];
+  const F3 dir = normalize(F3{p[0] / w, p[1] / w, p[2] / w});
+  float a[4], b[4];
+  mv4(c.wv_inv, 0.0f, 0.0f, 0.0f, 1.0f, a);
+  mv4(c.wv_inv, 0.0f + 100.0f * dir.x, 0.0f + 100.0f * dir.y, 0.0f + 100.0f * dir.z, 1.0f, b);
+  o = {a[0], a[1], a[2]};
+  d = normalize(F3{b[0] - a[0], b[1] - a[1], b[2] - a[2]});
+}
+
+// ---------------------------------------------------------------------------
+// Scene data on the device
+// ---------------------------------------------------------------------------
+// Triangles in LBVH leaf order, SoA: A (w = original global triangle id as bits),
+// e1 = B - A, e2 = C - A -- the same subtractions BruteForce3D::GetNearestHit does
+// (raytrace3d_bf.cpp:123-129), done once per commit.
+// LBVH node: both child boxes in the parent (n0..n2) and child links in n3.x / n3.y
+// (>= 0 internal node, < 0 leaf: ~slot in the triangle arrays).
+struct DevScene {
+  const float4* __restrict__ tri_a;
+  const float4* __restrict__ tri_e1;
+  const float4* __restrict__ tri_e2;
+  const float4* __restrict__ nodes;   // 4 float4 per internal node
+  int F;                              // triangles
+  int V;                              // vertices
+  const uint32_t* __restrict__ idx;   // 3F global vertex indices (get_index)
+  const int* __restrict__ tri_mesh;   // mesh of each global triangle
+  const int* __restrict__ mesh_vo;    // vertex offset per mesh (n_meshes+1)
+  const int* __restrict__ mesh_to;    // triangle offset per mesh (n_meshes+1)
+  const float* __restrict__ pos;      // 3V prepared positions (transform applied)
+  const float* __restrict__ pos_orig; // 3V mesh-local positions
+  const float* __restrict__ col;      // 3V
+  const float* __restrict__ tc;       // 2V
+  const float* __restrict__ nrm;      // 3V normals as the 3D scene prepares them
+  const float* __restrict__ nrm2d;    // 3V normals as the projected (edge) scene prepares them
+  const float* __restrict__ texels;
+  const int* __restrict__ tex_w;
+  const int* __restrict__ tex_h;
+  const int* __restrict__ tex_off;
+  int n_meshes;
+};
+
+struct Hit { uint32_t gtri; float t, u, v; };
+
+// Moller-Trumbore exactly as BruteForce3D::GetNearestHit (raytrace3d_bf.cpp:123-150).
+__device__ __forceinline__ bool mt_test(float4 A4, float4 E14, float4 E24, F3 o, F3 d, float tNear,
+                                        float& t, float& u, float& v) {
+  const F3 A = f4xyz(A4), e1 = f4xyz(E14), e2 = f4xyz(E24);
+  const F3 pvec = cross(d, e2);
+  const F3 tvec = sub(o, A);
+  const F3 qvec = cross(tvec, e1);
+  const float det = dot(e1, pvec);
+  const float sv = sgn(det);
+  const float invDet = sv / max_std(sv * det, 1e-6f);
+  v = dot(tvec, pvec) * invDet;
+  u = dot(qvec, d) * invDet;
+  t = dot(e2, qvec) * invDet;
+  return v > 0.0f && u > 0.0f && (u + v < 1.0f) && t > tNear;
+}
+
+// Slab test against a padded child box; NaN terms (0 * inf) are dropped by fminf/fmaxf,
+// which keeps the test conservative.
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
+                                     F3 o, F3 id, float tmax, float& tn) {
+  const float t0x = (lx - o.x) * id.x, t1x = (hx - o.x) * id.x;
+  const float t0y = (ly - o.y) * id.y, t1y = (hy - o.y) * id.y;
+  const float t0z = (lz - o.z) * id.z, t1z = (hz - o.z) * id.z;
+  const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+  const float tmx = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+  tn = tmin;
+  return tmin <= tmx;
+}
+
+constexpr int STACK_DEPTH = 64;  // > LBVH depth bound (30 Morton bits + 32 index bits)
+
+// Closest hit with the brute-force tracer's answer: the same MT test, and on equal t the
+// triangle that comes first in (mesh, triangle) order wins (BF's strict `t < hit.t`).
+// `stack` is this lane's column of an LDS stack laid out [depth][64 lanes].
+template <bool ANY>
+__device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar,
+                                         int* stack, int lane_stride) {
+  Hit h{NO_HIT, tFar, 0.0f, 0.0f};
+  if (s.F == 0) return h;
+  if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
+  const F3 id = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  auto leaf = [&](int slot) -> bool {
+    float t, u, v;
+    const float4 A = s.tri_a[slot];
+    if (mt_test(A, s.tri_e1[slot], s.tri_e2[slot], o, d, tNear, t, u, v)) {
+      const uint32_t g = __float_as_uint(A.w);
+      if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; return true; } }
+      else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) { h.gtri = g; h.t = t; h.u = u; h.v = v; }
+    }
+    return false;
+  };
+  if (s.F == 1) { leaf(0); return h; }
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    const float4* n = s.nodes + 4 * node;
+    const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
+    const int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
+    const float lim = h.t * 1.00002f + 1e-6f;
+    float tl, tr;
+    const bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, id, lim, tl);
+    const bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, id, lim, tr);
+    bool goL = false, goR = false;
+    if (hl) { if (cl < 0) { if (leaf(~cl) && ANY) return h; } else goL = true; }
+    if (hr) { if (cr < 0) { if (leaf(~cr) && ANY) return h; } else goR = true; }
+    if (goL && goR) {
+      const bool lfirst = tl <= tr;
+      const int nearc = lfirst ? cl : cr, farc = lfirst ? cr : cl;
+      if (sp < STACK_DEPTH) { stack[sp * lane_stride] = farc; sp++; }
+      node = nearc;
+    } else if (goL) {
+      node = cl;
+    } else if (goR) {
+      node = cr;
+    } else {
+      if (sp == 0) break;
+      sp--;
+      node = stack[sp * lane_stride];
+    }
+  }
+  return h;
+}
+
+// Brute force over every triangle in the reference's order (raytrace3d_bf.cpp:113-153).
+template <bool ANY>
+__device__ __forceinline__ Hit trace_bf(const DevScene& s, const float4* tri_a_orig, const float4* tri_e1_orig,
+                                        const float4* tri_e2_orig, F3 o, F3 d, float tNear, float tFar) {
+  Hit h{NO_HIT, tFar, 0.0f, 0.0f};
+  if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
+  for (int g = 0; g < s.F; g++) {
+    float t, u, v;
+    if (mt_test(tri_a_orig[g], tri_e1_orig[g], tri_e2_orig[g], o, d, tNear, t, u, v) && t < h.t) {
+      h.gtri = (uint32_t)g; h.t = t; h.u = u; h.v = v;
+      if (ANY && t < tFar) return h;
+    }
+  }
+  return h;
+}
+
+// Tracer selector: BVH or BF.  BF reads triangles in original order (bf_* arrays).
+struct Tracer {
+  DevScene s;
+  const float4* bf_a;
+  const float4* bf_e1;
+  const float4* bf_e2;
+  int use_bf;
+};
+
+template <bool ANY>
+__device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* stack, int ls) {
+  if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
+  return trace_bvh<ANY>(T.s, o, d, tNear, tFar, stack, ls);
+}
+
+// ---------------------------------------------------------------------------
+// Shading (src/shade_simple.cpp, src/shade_lambert.cpp, src/shade_common.cpp)
+// ---------------------------------------------------------------------------
+struct Light { F3 L, Lc, amb; };
+
+__device__ __forceinline__ F3 tex_fetch(const DevScene& s, int m, int x, int y) {
+  return ld3(s.texels + s.tex_off[m] + 3 * (y * s.tex_w[m] + x));
+}
+// sample_bilinear_clamp_3f (shade_common.cpp:3-21)
+__device__ __forceinline__ F3 bilinear(const DevScene& s, int m, F2 tc) {
+  const int w = s.tex_w[m], h = s.tex_h[m];
+  if (w == 0 || h == 0) return {0, 0, 0};
+  tc.x = clampf(tc.x, 0.0f, 1.0f); tc.y = clampf(tc.y, 0.0f, 1.0f);
+  tc.x *= (float)w; tc.y *= (float)h;
+  const int ix = (int)tc.x, iy = (int)tc.y;
+  const int x0 = clampi(ix, 0, w - 1), y0 = clampi(iy, 0, h - 1);
+  const int x1 = clampi(ix + 1, 0, w - 1), y1 = clampi(iy + 1, 0, h - 1);
+  const float dx = tc.x - (float)x0, dy = tc.y - (float)y0;
+  const F3 p00 = tex_fetch(s, m, x0, y0), p01 = tex_fetch(s, m, x0, y1);
+  const F3 p10 = tex_fetch(s, m, x1, y0), p11 = tex_fetch(s, m, x1, y1);
+  const float a = (1 - dx) * (1 - dy), b = (1 - dx) * dy, c = dx * (1 - dy), e = dx * dy;
+  return {a * p00.x + b * p01.x + c * p10.x + e * p11.x,
+          a * p00.y + b * p01.y + c * p10.y + e * p11.y,
+          a * p00.z + b * p01.z + c * p10.z + e * p11.z};
+}
+
+// a*(1-u-v) + b*v + u*c (shade_simple.cpp:36)
+__device__ __forceinline__ F3 blend3(F3 a, F3 b, F3 c, float u, float v) {
+  const float w = 1.0f - u - v;
+  return add(add(mul(a, w), mul(b, v)), mul(u, c));
+}
+__device__ __forceinline__ F2 blend2(const float* tc, uint32_t A, uint32_t B, uint32_t C, float u, float v) {
+  const float w = 1.0f - u - v;
+  return {tc[2 * A] * w + tc[2 * B] * v + u * tc[2 * C], tc[2 * A + 1] * w + tc[2 * B + 1] * v + u * tc[2 * C + 1]};
+}
+
+// shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT.  `nrm` is the 3D scene's normals
+// for render and the projected scene's for edge samples (src/drender.h:353-354).
+template <int MODE>
+__device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
+                                    const float* nrm, int* stack, int ls) {
+  F3 o, d;
+  eye_ray(c, sx, sy, o, d);
+  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, stack, ls);
+  if (h.gtri == NO_HIT) return {0, 0, 0};
+  if (MODE == 1) return {1, 1, 1};
+  const DevScene& s = T.s;
+  const uint32_t A = s.idx[3 * h.gtri], B = s.idx[3 * h.gtri + 1], C = s.idx[3 * h.gtri + 2];
+  if (MODE == 2) return blend3(ld3(s.col + 3 * A), ld3(s.col + 3 * B), ld3(s.col + 3 * C), h.u, h.v);
+  const int m = s.tri_mesh[h.gtri];
+  const F2 tcv = blend2(s.tc, A, B, C, h.u, h.v);
+  const F3 diffuse = bilinear(s, m, tcv);
+  if (MODE == 3) return diffuse;
+  const F3 n = blend3(ld3(nrm + 3 * A), ld3(nrm + 3 * B), ld3(nrm + 3 * C), h.u, h.v);
+  const float BIAS = 1e-5f;
+  const F3 surf = add(o, mul(h.t - BIAS, d));
+  const F3 negL = mul(-1.0f, L.L);
+  const Hit sh = trace<true>(T, surf, negL, 0.0f, 1e9f, stack, ls);
+  const float shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
+  const F3 lit = mul(mul(shadow, L.Lc), max_std(0.0f, dot(n, negL)));
+  return mulv(add(lit, L.amb), diffuse);
+}
+
+// ---------------------------------------------------------------------------
+// Projection and its adjoint (src/functions.h:5-13, src/functions.cpp:21-163)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void vertex_shader(const DevCam& c, float vx, float vy, float vz, float& sx, float& sy) {
+  const float* M = c.wvp;
+  const float W = vx * M[3] + vy * M[7] + vz * M[11] + M[15];
+  const float xN = (vx * M[0] + vy * M[4] + vz * M[8] + M[12]) / W;
+  const float yN = -(vx * M[1] + vy * M[5] + vz * M[9] + M[13]) / W;
+  sx = (xN * 0.5f + 0.5f) * c.W;
+  sy = (yN * 0.5f + 0.5f) * c.H;
+}
+// d(screen x)/dV (axis 0) or d(screen y)/dV (axis 1): the reverse-mode chain with the
+// reference's term order.
+__device__ __forceinline__ F3 vs_grad(const DevCam& c, F3 V, int axis) {
+  const float* M = c.wvp;
+  const float W = V.x * M[3] + V.y * M[7] + V.z * M[11] + M[15];
+  const int r = axis;
+  float num = V.x * M[r] + V.y * M[4 + r] + V.z * M[8 + r] + M[12 + r];
+  if (axis == 1) num = -num;
+  const float size = axis == 0 ? c.W : c.H;
+  const float dN = (1 * size) * 0.5f;
+  const float q = dN / W;
+  const float dW = dN * -num / (W * W);
+  F3 g;
+  if (axis == 0) { g = {0 + q * M[0], 0 + q * M[4], 0 + q * M[8]}; }
+  else { g = {0 + -q * M[1], 0 + -q * M[5], 0 + -q * M[9]}; }
+  g.x += dW * M[3]; g.y += dW * M[7]; g.z += dW * M[11];
+  return g;
+}
+
+}  // namespace drh

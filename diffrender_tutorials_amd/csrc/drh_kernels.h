@@ -1,0 +1,206 @@
+// drh_kernels.h -- kernel argument blocks and launchers shared by the host runtime
+// (drh_api.cpp) and the kernels (drh_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "drh_device.h"
+
+namespace drh {
+
+constexpr int RADIX_TILE = 2048;           // keys per radix-sort block (8 rounds of 256)
+constexpr int CDF_CHUNK = 8192;            // pmf floats staged in LDS per CDF round
+constexpr int INTERIOR_LDS_FLOATS = 4096;  // largest gradient target accumulated in LDS (interior)
+constexpr int EDGE_LDS_FLOATS = 4096;      // largest position gradient accumulated in LDS (edges)
+constexpr int DEEP_F = 1 << 18;            // above this many triangles use the 64-deep traversal stack
+
+struct PrepareArgs {
+  int V, n_meshes, has_nrm;
+  const int* mesh_vo;
+  const float* mats;      // per mesh: T, transpose(inverse(T)), identity (48 floats)
+  const float* pos_orig;
+  const float* nrm_in;
+  float* pos;
+  float* nrm;
+  float* nrm2d;
+};
+
+struct TriSetupArgs {
+  int F;
+  const int* tri_mesh;
+  const float* mats;
+  const uint32_t* idx;
+  const float* pos;
+  float4* bf_a;
+  float4* bf_e1;
+  float4* bf_e2;
+  float4* box_lo;
+  float4* box_hi;
+  float4* cen;
+  uint32_t* bounds;       // 6 ordered float keys: min xyz, max xyz
+};
+
+struct MortonArgs {
+  int F;
+  const float4* cen;
+  const uint32_t* bounds;
+  uint32_t* keys;
+  uint32_t* vals;
+};
+
+struct KarrasArgs {
+  int F;
+  const uint32_t* keys;   // sorted
+  int* child;             // 2 per internal node
+  int* node_parent;
+  int* leaf_parent;
+};
+
+struct RefitArgs {
+  int F;
+  const uint32_t* order;  // sorted slot -> original triangle
+  const float4* box_lo;
+  const float4* box_hi;
+  const int* child;
+  const int* node_parent;
+  const int* leaf_parent;
+  uint32_t* counter;
+  float4* leaf_lo;
+  float4* leaf_hi;
+  float4* node_lo;
+  float4* node_hi;
+};
+
+struct PackArgs {
+  int F;
+  const int* child;
+  const float4* leaf_lo;
+  const float4* leaf_hi;
+  const float4* node_lo;
+  const float4* node_hi;
+  float4* nodes;
+};
+
+struct ReorderArgs {
+  int F;
+  const uint32_t* order;
+  const float4* bf_a;
+  const float4* bf_e1;
+  const float4* bf_e2;
+  float4* tri_a;
+  float4* tri_e1;
+  float4* tri_e2;
+};
+
+struct RenderArgs {
+  Tracer tr;
+  Light light;
+  const DevCam* cams;
+  int view;
+  int sq;                 // (int)sqrt(spp)
+  float* img;
+  const float* target;    // NULL: no loss
+  float* adj;
+  double* loss_part;
+  int part_off;
+};
+
+struct InteriorArgs {
+  Tracer tr;
+  const DevCam* cams;
+  int view;
+  int nsamp;
+  const float* hamm;      // 2*nsamp Hammersley offsets
+  const float* adj;
+  float* grad;
+  const int* layout;      // per mesh: grad offset (-1: not differentiable), colour, texture, transform offsets
+  int lds_base, lds_n;    // gradient range accumulated in LDS (lds_n = 0: global atomics)
+  int uniform, uniform_off;  // one 1x1 texture: wave-reduce into grad[uniform_off..+3)
+};
+
+struct ProjectArgs {
+  int V;
+  const DevCam* cams;
+  int view;
+  const float* pos;
+  float4* proj;
+};
+
+struct EdgeArgs {
+  Tracer tr;
+  Light light;
+  const DevCam* cams;
+  int view;
+  long long N;
+  int E;
+  const int2* edges;
+  const int* edge_mesh;
+  const float* pmf;
+  const float* cdf;
+  const float4* proj;
+  const float2* rnd;      // REF_PRNG14 table, or NULL for QMC
+  const uint32_t* qmc;
+  const float* adj;
+  float* grad;
+  const int* layout;
+  const float* tr_inv0;   // 16 floats
+  int lds_base, lds_n;
+  int single_mesh, single_tr_off;
+};
+
+constexpr int MAX_SEG = 8;
+struct OptApplyArgs {
+  int n, off, nseg;
+  const float* m;
+  const float* G;
+  int seg_start[MAX_SEG];
+  float seg_lr[MAX_SEG];
+  float* seg_param[MAX_SEG];
+};
+
+struct AdamArgs {
+  int n, off, nseg;
+  const float* grad;
+  float* m;
+  float* v;
+  float b1, b2, eps, bc1, bc2;
+  int seg_start[MAX_SEG];
+  float seg_lr[MAX_SEG];
+  float* seg_param[MAX_SEG];
+};
+
+struct HitsArgs {
+  Tracer tr;
+  const DevCam* cams;
+  uint32_t* prim;
+  uint32_t* geom;
+  float* t;
+  float* u;
+  float* v;
+};
+
+void launch_fill_f32(float* p, long long n, float v, hipStream_t s);
+void launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t s);
+void launch_prepare(const PrepareArgs& a, hipStream_t s);
+void launch_tri_setup(const TriSetupArgs& a, hipStream_t s);
+void launch_morton(const MortonArgs& a, hipStream_t s);
+void launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, int n, int shift,
+                       uint32_t* hist, hipStream_t s);
+void launch_karras(const KarrasArgs& a, hipStream_t s);
+void launch_refit(const RefitArgs& a, hipStream_t s);
+void launch_pack(const PackArgs& a, hipStream_t s);
+void launch_reorder(const ReorderArgs& a, hipStream_t s);
+void launch_render(int mode, bool deep, const RenderArgs& a, int tiles, hipStream_t s);
+void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s);
+void launch_interior(int mode, bool deep, const InteriorArgs& a, int blocks, hipStream_t s);
+void launch_project(const ProjectArgs& a, hipStream_t s);
+void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s);
+void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s);
+void launch_edge_sample(int mode, bool deep, const EdgeArgs& a, int blocks, hipStream_t s);
+void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);
+void launch_adam(const AdamArgs& a, hipStream_t s);
+void launch_debug_hits(bool deep, const HitsArgs& a, int tiles, hipStream_t s);
+
+}  // namespace drh

@@ -1,0 +1,780 @@
+// drh_kernels.hip -- HIP kernels of the render / d_render path for gfx950 (MI355X).
+//
+//   commit   k_prepare, k_tri_setup, k_morton, k_radix_{hist,scan,scatter},
+//            k_karras, k_refit, k_pack_nodes, k_reorder      (replaces IRayTracer::Init)
+//   render   k_render<MODE> (+ fused L2 loss / adjoint), k_loss_final
+//   interior k_interior<MODE>                                 (interior_derivatives)
+//   edges    k_project, k_edge_len, k_cdf, k_edge_sample<MODE> (edge_derivatives)
+//   optimise k_optsimple_{moments,apply}, k_adam
+//
+// Wave64 everywhere: one 8x8 pixel tile per wavefront, LDS-resident traversal stacks
+// laid out [depth][lane] so pushes and pops of a wave hit 64 different banks.
+#include <hip/hip_runtime.h>
+
+#include "drh_device.h"
+#include "drh_kernels.h"
+
+namespace drh {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f2key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void k_fill_f32(float* p, long long n, float v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// commit: prepare_for_render (common_src/scene_utils.cpp:98-173)
+// ---------------------------------------------------------------------------
+// pos = T*v, nrm = transpose(inverse(T))*(n,1) for the 3D scene; the projected scene's
+// normals are I*(T*(n,0), 1) (transform() then an identity prepare, src/drender.h:260-268).
+__global__ void k_prepare(PrepareArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.V) return;
+  int m = 0;
+  while (m + 1 < a.n_meshes && a.mesh_vo[m + 1] <= k) m++;
+  const float* T = a.mats + 48 * m;        // T, transpose(inverse(T)), identity
+  const float* NT = T + 16;
+  const float* I = T + 32;
+  const F3 v = ld3(a.pos_orig + 3 * k);
+  const F3 p = mp(T, v);
+  a.pos[3 * k] = p.x; a.pos[3 * k + 1] = p.y; a.pos[3 * k + 2] = p.z;
+  if (a.has_nrm) {
+    const F3 n = ld3(a.nrm_in + 3 * k);
+    const F3 n3 = mp(NT, n);
+    a.nrm[3 * k] = n3.x; a.nrm[3 * k + 1] = n3.y; a.nrm[3 * k + 2] = n3.z;
+    float r[4];
+    mv4(T, n.x, n.y, n.z, 0.0f, r);
+    const F3 n2 = mp(I, F3{r[0], r[1], r[2]});
+    a.nrm2d[3 * k] = n2.x; a.nrm2d[3 * k + 1] = n2.y; a.nrm2d[3 * k + 2] = n2.z;
+  }
+}
+
+// Per triangle: BF's transformed corners T*pos (raytrace3d_bf.cpp:123-125), its edges, AABB
+// and centroid; scene centroid bounds by ordered-key atomics.
+__global__ void k_tri_setup(TriSetupArgs a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ uint32_t smin[3], smax[3];
+  if (threadIdx.x < 3) { smin[threadIdx.x] = 0xFFFFFFFFu; smax[threadIdx.x] = 0u; }
+  __syncthreads();
+  if (g < a.F) {
+    const int m = a.tri_mesh[g];
+    const float* T = a.mats + 48 * m;
+    const uint32_t ia = a.idx[3 * g], ib = a.idx[3 * g + 1], ic = a.idx[3 * g + 2];
+    const F3 A = mp(T, ld3(a.pos + 3 * ia));
+    const F3 B = mp(T, ld3(a.pos + 3 * ib));
+    const F3 C = mp(T, ld3(a.pos + 3 * ic));
+    const F3 e1 = sub(B, A), e2 = sub(C, A);
+    a.bf_a[g] = make_float4(A.x, A.y, A.z, __uint_as_float((uint32_t)g));
+    a.bf_e1[g] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+    a.bf_e2[g] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    const float lx = fminf(A.x, fminf(B.x, C.x)), ly = fminf(A.y, fminf(B.y, C.y)), lz = fminf(A.z, fminf(B.z, C.z));
+    const float hx = fmaxf(A.x, fmaxf(B.x, C.x)), hy = fmaxf(A.y, fmaxf(B.y, C.y)), hz = fmaxf(A.z, fmaxf(B.z, C.z));
+    a.box_lo[g] = make_float4(lx, ly, lz, 0.0f);
+    a.box_hi[g] = make_float4(hx, hy, hz, 0.0f);
+    const float cx = (lx + hx) * 0.5f, cy = (ly + hy) * 0.5f, cz = (lz + hz) * 0.5f;
+    a.cen[g] = make_float4(cx, cy, cz, 0.0f);
+    if (cx == cx && cy == cy && cz == cz) {
+      atomicMin(&smin[0], f2key(cx)); atomicMin(&smin[1], f2key(cy)); atomicMin(&smin[2], f2key(cz));
+      atomicMax(&smax[0], f2key(cx)); atomicMax(&smax[1], f2key(cy)); atomicMax(&smax[2], f2key(cz));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    atomicMin(&a.bounds[threadIdx.x], smin[threadIdx.x]);
+    atomicMax(&a.bounds[3 + threadIdx.x], smax[threadIdx.x]);
+  }
+}
+
+// 30-bit Morton code of the centroid in the scene's centroid box.
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+  x &= 0x3FFu;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__device__ __forceinline__ uint32_t quant10(float c, float lo, float hi) {
+  const float ext = hi - lo;
+  float q = ext > 0.0f ? (c - lo) / ext : 0.0f;
+  q = q * 1024.0f;
+  if (!(q > 0.0f)) q = 0.0f;   // also NaN
+  if (q > 1023.0f) q = 1023.0f;
+  return (uint32_t)q;
+}
+__global__ void k_morton(MortonArgs a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.F) return;
+  const float lx = key2f(a.bounds[0]), ly = key2f(a.bounds[1]), lz = key2f(a.bounds[2]);
+  const float hx = key2f(a.bounds[3]), hy = key2f(a.bounds[4]), hz = key2f(a.bounds[5]);
+  const float4 c = a.cen[g];
+  const uint32_t code = (spread10(quant10(c.x, lx, hx)) << 2) | (spread10(quant10(c.y, ly, hy)) << 1) |
+                        spread10(quant10(c.z, lz, hz));
+  a.keys[g] = code;
+  a.vals[g] = (uint32_t)g;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits, RADIX_TILE keys per block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, int n, int shift, uint32_t* hist, int nblocks) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RADIX_TILE;
+  for (int i = threadIdx.x; i < RADIX_TILE; i += 256) {
+    const int k = base + i;
+    if (k < n) atomicAdd(&cnt[(keys[k] >> shift) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of hist[256*nblocks] (digit-major) in one block.
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int total) {
+  __shared__ uint32_t part[1024];
+  const int per = (total + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  uint32_t s = 0;
+  for (int i = 0; i < per; i++) if (b + i < total) s += hist[b + i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (int i = 0; i < per; i++)
+    if (b + i < total) { const uint32_t h = hist[b + i]; hist[b + i] = run; run += h; }
+}
+
+// Stable scatter: the tile is taken in rounds of 256 keys in index order; the rank of a key
+// among equal digits of its round comes from wave ballots over the 8 digit bits.
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                       uint32_t* vout, int n, int shift, const uint32_t* offs,
+                                                       int nblocks) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[4][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  base[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < RADIX_TILE / 256; r++) {
+    const int k = blockIdx.x * RADIX_TILE + r * 256 + threadIdx.x;
+    const bool valid = k < n;
+    uint32_t key = 0, val = 0, dig = 0;
+    if (valid) { key = kin[k]; val = vin[k]; dig = (key >> shift) & 0xFFu; }
+    for (int i = threadIdx.x; i < 4 * 256; i += 256) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < 8; b++) {
+      const uint64_t bb = __ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[wave][dig] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t before = 0;
+      for (int w = 0; w < wave; w++) before += wcnt[w][dig];
+      const uint32_t dst = base[dig] + before + rank;
+      kout[dst] = key;
+      vout[dst] = val;
+    }
+    __syncthreads();
+    {  // advance the running base of each digit by this round's count
+      const int d = threadIdx.x;
+      base[d] += wcnt[0][d] + wcnt[1][d] + wcnt[2][d] + wcnt[3][d];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Karras (2012) hierarchy over sorted (key, index) and bottom-up refit
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int kdelta(const uint32_t* k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = k[i], b = k[j];
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+__global__ void k_karras(KarrasArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.F;
+  if (i >= n - 1) return;
+  const uint32_t* k = a.keys;
+  const int d = (kdelta(k, n, i, i + 1) - kdelta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = kdelta(k, n, i, i - d);
+  int lmax = 2;
+  while (kdelta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta(k, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = kdelta(k, n, i, j);
+  int s = 0, t = l;
+  while (true) {
+    t = (t + 1) >> 1;
+    if (kdelta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+  }
+  const int gamma = i + s * d + (d < 0 ? d : 0);
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  const int left = (lo == gamma) ? ~gamma : gamma;
+  const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  a.child[2 * i] = left;
+  a.child[2 * i + 1] = right;
+  if (left < 0) a.leaf_parent[~left] = i; else a.node_parent[left] = i;
+  if (right < 0) a.leaf_parent[~right] = i; else a.node_parent[right] = i;
+  if (i == 0) a.node_parent[0] = -1;
+}
+
+__global__ void k_refit(RefitArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.F) return;
+  const int g = (int)a.order[s];
+  a.leaf_lo[s] = a.box_lo[g];
+  a.leaf_hi[s] = a.box_hi[g];
+  int p = a.leaf_parent[s];
+  while (p >= 0) {
+    __threadfence();
+    if (atomicAdd(&a.counter[p], 1u) == 0u) return;  // the other child is still on its way
+    __threadfence();
+    const int c0 = a.child[2 * p], c1 = a.child[2 * p + 1];
+    // the sibling's box was written by another thread: read it past the non-coherent caches
+    const volatile float* lo0 = (const volatile float*)(c0 < 0 ? a.leaf_lo + ~c0 : a.node_lo + c0);
+    const volatile float* hi0 = (const volatile float*)(c0 < 0 ? a.leaf_hi + ~c0 : a.node_hi + c0);
+    const volatile float* lo1 = (const volatile float*)(c1 < 0 ? a.leaf_lo + ~c1 : a.node_lo + c1);
+    const volatile float* hi1 = (const volatile float*)(c1 < 0 ? a.leaf_hi + ~c1 : a.node_hi + c1);
+    const float4 L = make_float4(fminf(lo0[0], lo1[0]), fminf(lo0[1], lo1[1]), fminf(lo0[2], lo1[2]), 0.0f);
+    const float4 H = make_float4(fmaxf(hi0[0], hi1[0]), fmaxf(hi0[1], hi1[1]), fmaxf(hi0[2], hi1[2]), 0.0f);
+    a.node_lo[p] = L;
+    a.node_hi[p] = H;
+    p = a.node_parent[p];
+  }
+}
+
+// Box padding: the MT test can accept a point a few ulps outside its triangle, and the slab
+// test rounds; widen every box so traversal never culls a triangle BF would accept.
+__device__ __forceinline__ void pad_box(float4& lo, float4& hi) {
+  const float px = 1e-5f * (fabsf(lo.x) + fabsf(hi.x) + (hi.x - lo.x)) + 1e-30f;
+  const float py = 1e-5f * (fabsf(lo.y) + fabsf(hi.y) + (hi.y - lo.y)) + 1e-30f;
+  const float pz = 1e-5f * (fabsf(lo.z) + fabsf(hi.z) + (hi.z - lo.z)) + 1e-30f;
+  lo.x -= px; lo.y -= py; lo.z -= pz;
+  hi.x += px; hi.y += py; hi.z += pz;
+}
+__global__ void k_pack_nodes(PackArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.F - 1) return;
+  const int c0 = a.child[2 * i], c1 = a.child[2 * i + 1];
+  float4 l0 = c0 < 0 ? a.leaf_lo[~c0] : a.node_lo[c0];
+  float4 h0 = c0 < 0 ? a.leaf_hi[~c0] : a.node_hi[c0];
+  float4 l1 = c1 < 0 ? a.leaf_lo[~c1] : a.node_lo[c1];
+  float4 h1 = c1 < 0 ? a.leaf_hi[~c1] : a.node_hi[c1];
+  pad_box(l0, h0);
+  pad_box(l1, h1);
+  float4* n = a.nodes + 4 * i;
+  n[0] = make_float4(l0.x, l0.y, l0.z, h0.x);
+  n[1] = make_float4(h0.y, h0.z, l1.x, l1.y);
+  n[2] = make_float4(l1.z, h1.x, h1.y, h1.z);
+  n[3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.0f, 0.0f);
+}
+
+__global__ void k_reorder(ReorderArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.F) return;
+  const int g = (int)a.order[s];
+  a.tri_a[s] = a.bf_a[g];
+  a.tri_e1[s] = a.bf_e1[g];
+  a.tri_e2[s] = a.bf_e2[g];
+}
+
+// ---------------------------------------------------------------------------
+// render (src/drender.h:98-141) with the L2 loss fused (src/utils.cpp:15-27)
+// ---------------------------------------------------------------------------
+template <int MODE, int DEPTH>
+__global__ __launch_bounds__(64) void k_render(RenderArgs a) {
+  __shared__ int stack[DEPTH * 64];
+  const DevCam& c = a.cams[a.view];
+  const int lane = threadIdx.x;
+  const int tiles_x = (c.w + 7) >> 3;
+  const int tile = blockIdx.x;
+  const int x = (tile % tiles_x) * 8 + (lane & 7);
+  const int y = (tile / tiles_x) * 8 + (lane >> 3);
+  const bool inside = x < c.w && y < c.h;
+  double l2 = 0.0;
+  if (inside) {
+    const int sq = a.sq;
+    const float ns = (float)(sq * sq);
+    F3 acc = {0, 0, 0};
+    for (int dy = 0; dy < sq; dy++)
+      for (int dx = 0; dx < sq; dx++) {
+        const float xo = (dx + 0.5f) / float(sq);
+        const float yo = (dy + 0.5f) / float(sq);
+        const F3 col = shade<MODE>(a.tr, a.light, c, x + xo, y + yo, a.tr.s.nrm, stack + lane, 64);
+        acc = add(acc, divs(col, ns));
+      }
+    const long long p = c.img_off + 3ll * ((long long)y * c.w + x);
+    if (a.img) { a.img[p] = acc.x; a.img[p + 1] = acc.y; a.img[p + 2] = acc.z; }
+    if (a.target) {
+      const F3 t = ld3(a.target + p);
+      const F3 dv = sub(acc, t);
+      const F3 ad = mul(2.0f, dv);
+      a.adj[p] = ad.x; a.adj[p + 1] = ad.y; a.adj[p + 2] = ad.z;
+      l2 = double(dot(dv, dv));
+    }
+  }
+  if (a.target) {
+    l2 = wave_sum_d(l2);
+    if (lane == 0) a.loss_part[a.part_off + tile] = l2;
+  }
+}
+
+// Deterministic final sum of the per-tile loss partials: mse / (views*W*H) as a float.
+__global__ __launch_bounds__(1024) void k_loss_final(const double* part, int n, float* out, double denom) {
+  __shared__ double s[1024];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) v += part[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = float(float(s[0]) / float(denom));
+}
+
+// ---------------------------------------------------------------------------
+// interior_derivatives (src/drender.h:197-233): Hammersley subpixel offsets,
+// shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74) / <TEXTURE_COLOR> (:132-162, LAMBERT too)
+// ---------------------------------------------------------------------------
+template <int MODE, int DEPTH>
+__global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
+  __shared__ int stack[DEPTH * 64];
+  extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
+  const DevCam& c = a.cams[a.view];
+  const int lane = threadIdx.x;
+  const bool use_lds = a.lds_n > 0;
+  for (int i = lane; i < a.lds_n; i += 64) lgrad[i] = 0.0f;
+  __syncthreads();
+  const int tiles_x = (c.w + 7) >> 3;
+  const int ntiles = tiles_x * ((c.h + 7) >> 3);
+  F3 uni = {0, 0, 0};  // uniform-target accumulator (single 1x1 texture)
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int x = (tile % tiles_x) * 8 + (lane & 7);
+    const int y = (tile / tiles_x) * 8 + (lane >> 3);
+    if (x >= c.w || y >= c.h) continue;
+    const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)y * c.w + x));
+    for (int si = 0; si < a.nsamp; si++) {
+      const float xo = a.hamm[2 * si], yo = a.hamm[2 * si + 1];
+      const F3 val = divs(adj, (float)a.nsamp);
+      F3 o, d;
+      eye_ray(c, x + xo, y + yo, o, d);
+      const Hit h = trace<false>(a.tr, o, d, 0.0f, 1e9f, stack + lane, 64);
+      if (h.gtri == NO_HIT) continue;
+      const int m = a.tr.s.tri_mesh[h.gtri];
+      const int goff = a.layout[4 * m + 0];
+      if (goff < 0) continue;
+      const int vo = a.tr.s.mesh_vo[m];
+      if (MODE == 2) {
+        const float w = 1.0f - h.u - h.v;
+        const F3 cA = mul(w, val), cB = mul(h.v, val), cC = mul(h.u, val);
+        const int base = goff + a.layout[4 * m + 1];
+        const uint32_t* I = a.tr.s.idx + 3 * h.gtri;
+        const int A = (int)I[0] - vo, B = (int)I[1] - vo, C = (int)I[2] - vo;
+        float* G = use_lds ? lgrad : a.grad;
+        const int gb = use_lds ? base - a.lds_base : base;
+        atomicAdd(G + gb + 3 * A, cA.x); atomicAdd(G + gb + 3 * A + 1, cA.y); atomicAdd(G + gb + 3 * A + 2, cA.z);
+        atomicAdd(G + gb + 3 * B, cB.x); atomicAdd(G + gb + 3 * B + 1, cB.y); atomicAdd(G + gb + 3 * B + 2, cB.z);
+        atomicAdd(G + gb + 3 * C, cC.x); atomicAdd(G + gb + 3 * C + 1, cC.y); atomicAdd(G + gb + 3 * C + 2, cC.z);
+      } else {
+        const int w = a.tr.s.tex_w[m], hh = a.tr.s.tex_h[m];
+        if (w == 0 || hh == 0) continue;
+        const uint32_t* I = a.tr.s.idx + 3 * h.gtri;
+        const F2 tcv = blend2(a.tr.s.tc, I[0], I[1], I[2], h.u, h.v);
+        // sample_bilinear_clamp_3f_grad (shade_common.cpp:23-45): tc not clamped
+        const float px = tcv.x * (float)w, py = tcv.y * (float)hh;
+        const int ix = (int)px, iy = (int)py;
+        const int x0 = clampi(ix, 0, w - 1), y0 = clampi(iy, 0, hh - 1);
+        const int x1 = clampi(ix + 1, 0, w - 1), y1 = clampi(iy + 1, 0, hh - 1);
+        const float dx = px - (float)x0, dy = py - (float)y0;
+        const float w00 = (1 - dx) * (1 - dy), w01 = (1 - dx) * dy, w10 = dx * (1 - dy), w11 = dx * dy;
+        if (a.uniform) {
+          uni.x += w00 * val.x + w01 * val.x + w10 * val.x + w11 * val.x;
+          uni.y += w00 * val.y + w01 * val.y + w10 * val.y + w11 * val.y;
+          uni.z += w00 * val.z + w01 * val.z + w10 * val.z + w11 * val.z;
+          continue;
+        }
+        const int base = goff + a.layout[4 * m + 2];
+        float* G = use_lds ? lgrad : a.grad;
+        const int gb = use_lds ? base - a.lds_base : base;
+        float* d00 = G + gb + 3 * (y0 * w + x0);
+        float* d01 = G + gb + 3 * (y1 * w + x0);
+        float* d10 = G + gb + 3 * (y0 * w + x1);
+        float* d11 = G + gb + 3 * (y1 * w + x1);
+        atomicAdd(d00, w00 * val.x); atomicAdd(d00 + 1, w00 * val.y); atomicAdd(d00 + 2, w00 * val.z);
+        atomicAdd(d01, w01 * val.x); atomicAdd(d01 + 1, w01 * val.y); atomicAdd(d01 + 2, w01 * val.z);
+        atomicAdd(d10, w10 * val.x); atomicAdd(d10 + 1, w10 * val.y); atomicAdd(d10 + 2, w10 * val.z);
+        atomicAdd(d11, w11 * val.x); atomicAdd(d11 + 1, w11 * val.y); atomicAdd(d11 + 2, w11 * val.z);
+      }
+    }
+  }
+  if (a.uniform) {
+    uni.x = wave_sum(uni.x); uni.y = wave_sum(uni.y); uni.z = wave_sum(uni.z);
+    if (lane == 0) {
+      atomicAdd(a.grad + a.uniform_off, uni.x);
+      atomicAdd(a.grad + a.uniform_off + 1, uni.y);
+      atomicAdd(a.grad + a.uniform_off + 2, uni.z);
+    }
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int i = lane; i < a.lds_n; i += 64)
+      if (lgrad[i] != 0.0f) atomicAdd(a.grad + a.lds_base + i, lgrad[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// edge_derivatives (src/drender.h:235-425)
+// ---------------------------------------------------------------------------
+// VertexShader of every vertex (transform() already applied: pos is T*v), then the
+// projected scene's identity prepare: I*(sx, sy, z, 1).
+__global__ void k_project(ProjectArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.V) return;
+  const DevCam& c = a.cams[a.view];
+  const F3 v = ld3(a.pos + 3 * k);
+  float sx, sy;
+  vertex_shader(c, v.x, v.y, v.z, sx, sy);
+  const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  const F3 p = mp(I, F3{sx, sy, v.z});
+  a.proj[k] = make_float4(p.x, p.y, p.z, 0.0f);
+}
+
+// pmf before normalisation: float3 length in (screen x, screen y, world z)
+// (build_edge_sampler, src/drender.cpp:13-15)
+__global__ void k_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int2 v = edges[e];
+  const float4 p0 = proj[v.x], p1 = proj[v.y];
+  pmf_raw[e] = length(F3{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z});
+}
+
+// The reference's CDF is a sequential float running sum (src/drender.cpp:12-17); its
+// rounding decides which edge a sample lands on, so it is reproduced exactly: chunks of
+// pmf are staged in LDS by the whole block and one lane carries the running sum.  Then
+// pmf and cdf are divided by the total (:18-20).
+__global__ __launch_bounds__(1024) void k_cdf(const float* pmf_raw, int E, float* pmf, float* cdf) {
+  __shared__ float buf[CDF_CHUNK];
+  __shared__ float total_s;
+  float run = 0.0f;
+  for (int start = 0; start < E; start += CDF_CHUNK) {
+    const int n = min(CDF_CHUNK, E - start);
+    for (int i = threadIdx.x; i < n; i += 1024) buf[i] = pmf_raw[start + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < n; i++) { run = buf[i] + run; buf[i] = run; }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) cdf[start + 1 + i] = buf[i];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total_s = run;
+  __syncthreads();
+  const float total = total_s;
+  for (int i = threadIdx.x; i < E; i += 1024) pmf[i] = pmf_raw[i] / total;
+  for (int i = threadIdx.x; i <= E; i += 1024) cdf[i] = (i == 0 ? 0.0f : cdf[i]) / total;
+}
+
+// qmc::rndFloat (common_src/qmc.cpp:190-197)
+__device__ __forceinline__ float qmc_rnd(uint32_t pos, int dim, const uint32_t* table) {
+  uint32_t r = 0, d = pos;
+  for (int bit = 0; bit < 31; bit++, d >>= 1)
+    if (d & 1) r ^= table[bit + dim * 31];
+  return (float)(r + 1) * (1.0f / 2147483648.0f);
+}
+
+// sample(): clamp(upper_bound(cdf, u) - 1, 0, E - 1) (src/drender.cpp:25-29)
+__device__ __forceinline__ int sample_edge(const float* cdf, int E, float u) {
+  int first = 0, count = E + 1;
+  while (count > 0) {
+    const int step = count >> 1;
+    const int it = first + step;
+    if (!(u < cdf[it])) { first = it + 1; count -= step + 1; }
+    else count = step;
+  }
+  return clampi(first - 1, 0, E - 1);
+}
+
+template <int MODE, int DEPTH>
+__global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
+  __shared__ int stack[4 * DEPTH * 64];
+  extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
+  __shared__ float ltr[4][12];
+  const DevCam& c = a.cams[a.view];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int* stk = stack + wave * DEPTH * 64 + lane;
+  const bool use_lds = a.lds_n > 0;
+  for (int i = threadIdx.x; i < a.lds_n; i += 256) lpos[i] = 0.0f;
+  float tr_acc[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) tr_acc[q] = 0.0f;
+  __syncthreads();
+  const float* Ti = a.tr_inv0;  // mesh 0's inverse transform for every mesh (drender.cpp:60)
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.N; i += stride) {
+    float rnd0, rnd1;
+    if (a.rnd) { const float2 r = a.rnd[i]; rnd0 = r.x; rnd1 = r.y; }
+    else { rnd0 = qmc_rnd((uint32_t)i, 0, a.qmc); rnd1 = qmc_rnd((uint32_t)i, 1, a.qmc); }
+    const int eid = sample_edge(a.cdf, a.E, rnd0);
+    const int2 ev = a.edges[eid];
+    const float pmf = a.pmf[eid];
+    const float4 P0 = a.proj[ev.x], P1 = a.proj[ev.y];
+    const float t = rnd1;
+    const float px = P0.x + t * (P1.x - P0.x), py = P0.y + t * (P1.y - P0.y);
+    // int() truncation; NaN and out-of-range values are off-screen (as x86 cvttss2si's INT_MIN)
+    if (!(px > -1.0f) || !(py > -1.0f) || !(px < (float)c.w) || !(py < (float)c.h)) continue;
+    const int xi = (int)px, yi = (int)py;
+    if (xi < 0 || yi < 0 || xi >= c.w || yi >= c.h) continue;
+    const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)yi * c.w + xi));
+    if ((double)length(adj) < 1e-3) continue;
+    const float dvx = P1.x - P0.x, dvy = P1.y - P0.y;
+    const float len = sqrtf(dvx * dvx + dvy * dvy);
+    const float ndx = dvx / len, ndy = dvy / len;
+    const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
+    const F3 cin = shade<MODE>(a.tr, a.light, c, px - 1e-3f * nx, py - 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
+    const F3 cout = shade<MODE>(a.tr, a.light, c, px + 1e-3f * nx, py + 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
+    const float pdf = pmf / len;
+    const float weight = 1.0f / (pdf * float(a.N));
+    const float adjv = dot(sub(cin, cout), adj);
+    if (!(adjv * adjv > 0.0f)) continue;
+    const float d0x = ((1 - t) * nx) * adjv * weight, d0y = ((1 - t) * ny) * adjv * weight;
+    const float d1x = (t * nx) * adjv * weight, d1y = (t * ny) * adjv * weight;
+    // edge_grad (src/drender.cpp:31-87): diagonal screen Jacobian, as the reference
+    const F3 V0 = ld3(a.tr.s.pos + 3 * ev.x), V1 = ld3(a.tr.s.pos + 3 * ev.y);
+    const F3 g0x = vs_grad(c, V0, 0), g0y = vs_grad(c, V0, 1);
+    const F3 g1x = vs_grad(c, V1, 0), g1y = vs_grad(c, V1, 1);
+    const float dv0_dx = g0x.x * d0x, dv0_dy = g0y.y * d0y, dv0_dz = (g0x.z * d0x + g0y.z * d0y);
+    const float dv1_dx = g1x.x * d1x, dv1_dy = g1y.y * d1y, dv1_dz = (g1x.z * d1x + g1y.z * d1y);
+    const int m = a.edge_mesh[eid];
+    const int goff = a.layout[4 * m + 0];
+    if (goff < 0) continue;
+    const int vo = a.tr.s.mesh_vo[m];
+    const int vn0 = ev.x - vo, vn1 = ev.y - vo;
+    const float g0[3] = {Ti[0] * dv0_dx + Ti[4] * dv0_dy + Ti[8] * dv0_dz,
+                         Ti[1] * dv0_dx + Ti[5] * dv0_dy + Ti[9] * dv0_dz,
+                         Ti[2] * dv0_dx + Ti[6] * dv0_dy + Ti[10] * dv0_dz};
+    const float g1[3] = {Ti[0] * dv1_dx + Ti[4] * dv1_dy + Ti[8] * dv1_dz,
+                         Ti[1] * dv1_dx + Ti[5] * dv1_dy + Ti[9] * dv1_dz,
+                         Ti[2] * dv1_dx + Ti[6] * dv1_dy + Ti[10] * dv1_dz};
+    float* G = use_lds ? lpos : a.grad;
+    const int gb = use_lds ? goff - a.lds_base : goff;
+    atomicAdd(G + gb + 3 * vn0, g0[0]); atomicAdd(G + gb + 3 * vn0 + 1, g0[1]); atomicAdd(G + gb + 3 * vn0 + 2, g0[2]);
+    atomicAdd(G + gb + 3 * vn1, g1[0]); atomicAdd(G + gb + 3 * vn1 + 1, g1[1]); atomicAdd(G + gb + 3 * vn1 + 2, g1[2]);
+    const F3 o0 = ld3(a.tr.s.pos_orig + 3 * ev.x), o1 = ld3(a.tr.s.pos_orig + 3 * ev.y);
+    const float tv[12] = {o0.x * dv0_dx + o1.x * dv1_dx, o0.y * dv0_dx + o1.y * dv1_dx,
+                          o0.z * dv0_dx + o1.z * dv1_dx, 1 * dv0_dx + 1 * dv1_dx,
+                          o0.x * dv0_dy + o1.x * dv1_dy, o0.y * dv0_dy + o1.y * dv1_dy,
+                          o0.z * dv0_dy + o1.z * dv1_dy, 1 * dv0_dy + 1 * dv1_dy,
+                          o0.x * dv0_dz + o1.x * dv1_dz, o0.y * dv0_dz + o1.y * dv1_dz,
+                          o0.z * dv0_dz + o1.z * dv1_dz, 1 * dv0_dz + 1 * dv1_dz};
+    if (a.single_mesh) {
+#pragma unroll
+      for (int q = 0; q < 12; q++) tr_acc[q] += tv[q];
+    } else {
+      float* gt = a.grad + goff + a.layout[4 * m + 3];
+#pragma unroll
+      for (int q = 0; q < 12; q++) atomicAdd(gt + q, tv[q]);
+    }
+  }
+  if (a.single_mesh) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) tr_acc[q] = wave_sum(tr_acc[q]);
+    if (lane == 0)
+      for (int q = 0; q < 12; q++) ltr[wave][q] = tr_acc[q];
+  }
+  __syncthreads();
+  if (a.single_mesh && threadIdx.x < 12) {
+    const float v = ltr[0][threadIdx.x] + ltr[1][threadIdx.x] + ltr[2][threadIdx.x] + ltr[3][threadIdx.x];
+    if (v != 0.0f) atomicAdd(a.grad + a.single_tr_off + threadIdx.x, v);
+  }
+  if (use_lds)
+    for (int i = threadIdx.x; i < a.lds_n; i += 256)
+      if (lpos[i] != 0.0f) atomicAdd(a.grad + a.lds_base + i, lpos[i]);
+}
+
+// ---------------------------------------------------------------------------
+// optimisers
+// ---------------------------------------------------------------------------
+// OptSimple::OptStep, GD_Adam (src/optimizer_simple.cpp:208-216) on one mesh block.
+__global__ void k_optsimple_moments(const float* grad, float* m, float* G, int off, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = grad[off + i];
+  m[off + i] = m[off + i] * 0.25f + g * (1.0f - 0.25f);
+  G[off + i] = 2.0f * (G[off + i] * 0.5f + (g * g) * (1.0f - 0.5f));
+}
+// Normalisation (:223-224; the reference indexes m_GSquare without the mesh offset),
+// per-block learning rate (:227-232), then OptUpdateScene's p -= step (:92-186).
+__global__ void k_optsimple_apply(OptApplyArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float step = a.m[a.off + i] / sqrtf(a.G[i] + float(1e-8f));
+  int seg = 0;
+  for (int s = 1; s < a.nseg; s++) if (i >= a.seg_start[s]) seg = s;
+  step = step * a.seg_lr[seg];
+  float* p = a.seg_param[seg];
+  if (p) p[i - a.seg_start[seg]] -= step;
+}
+
+// Standard Adam (Kingma & Ba) with bias correction.
+__global__ void k_adam(AdamArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int seg = 0;
+  for (int s = 1; s < a.nseg; s++) if (i >= a.seg_start[s]) seg = s;
+  float* p = a.seg_param[seg];
+  if (!p) return;
+  const float g = a.grad[a.off + i];
+  const float m = a.b1 * a.m[a.off + i] + (1.0f - a.b1) * g;
+  const float v = a.b2 * a.v[a.off + i] + (1.0f - a.b2) * g * g;
+  a.m[a.off + i] = m;
+  a.v[a.off + i] = v;
+  const float mh = m / a.bc1, vh = v / a.bc2;
+  p[i - a.seg_start[seg]] -= a.seg_lr[seg] * mh / (sqrtf(vh) + a.eps);
+}
+
+// ---------------------------------------------------------------------------
+// debug: closest hit at pixel centres
+// ---------------------------------------------------------------------------
+template <int DEPTH>
+__global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
+  __shared__ int stack[DEPTH * 64];
+  const DevCam& c = a.cams[0];
+  const int lane = threadIdx.x;
+  const int tiles_x = (c.w + 7) >> 3;
+  const int x = (blockIdx.x % tiles_x) * 8 + (lane & 7);
+  const int y = (blockIdx.x / tiles_x) * 8 + (lane >> 3);
+  if (x >= c.w || y >= c.h) return;
+  F3 o, d;
+  eye_ray(c, x + 0.5f, y + 0.5f, o, d);
+  const Hit h = trace<false>(a.tr, o, d, 0.0f, 1e9f, stack + lane, 64);
+  const int p = y * c.w + x;
+  uint32_t prim = NO_HIT, geom = NO_HIT;
+  if (h.gtri != NO_HIT) { geom = (uint32_t)a.tr.s.tri_mesh[h.gtri]; prim = h.gtri - (uint32_t)a.tr.s.mesh_to[geom]; }
+  a.prim[p] = prim; a.geom[p] = geom; a.t[p] = h.t; a.u[p] = h.u; a.v[p] = h.v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define DRH_DISPATCH_MODE(MODE_VAR, BODY)          \
+  switch (MODE_VAR) {                              \
+    case 1: { constexpr int M = 1; BODY; } break;  \
+    case 2: { constexpr int M = 2; BODY; } break;  \
+    case 3: { constexpr int M = 3; BODY; } break;  \
+    default: { constexpr int M = 4; BODY; } break; \
+  }
+#define DRH_DISPATCH_DEPTH(DEEP, BODY)                 \
+  if (DEEP) { constexpr int D = 64; BODY; }             \
+  else { constexpr int D = 48; BODY; }
+
+void launch_fill_f32(float* p, long long n, float v, hipStream_t s) {
+  if (n <= 0) return;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t s) {
+  if (n <= 0) return;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+void launch_prepare(const PrepareArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_prepare, dim3((a.V + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_tri_setup(const TriSetupArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_tri_setup, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_morton(const MortonArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_morton, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, int n, int shift,
+                       uint32_t* hist, hipStream_t s) {
+  const int nb = (n + RADIX_TILE - 1) / RADIX_TILE;
+  hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, kin, n, shift, hist, nb);
+  hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, hist, 256 * nb);
+  hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, kin, vin, kout, vout, n, shift, hist, nb);
+}
+void launch_karras(const KarrasArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_karras, dim3((a.F + 254) / 256), dim3(256), 0, s, a);
+}
+void launch_refit(const RefitArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_refit, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_pack(const PackArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_nodes, dim3((a.F + 254) / 256), dim3(256), 0, s, a);
+}
+void launch_reorder(const ReorderArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_render(int mode, bool deep, const RenderArgs& a, int tiles, hipStream_t s) {
+  DRH_DISPATCH_MODE(mode, DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_render<M, D>), dim3(tiles), dim3(64), 0, s, a)))
+}
+void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s) {
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1024), 0, s, part, n, out, denom);
+}
+void launch_interior(int mode, bool deep, const InteriorArgs& a, int blocks, hipStream_t s) {
+  if (mode == 2) {
+    DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_interior<2, D>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a))
+  } else {
+    DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_interior<3, D>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a))
+  }
+}
+void launch_project(const ProjectArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_project, dim3((a.V + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s) {
+  hipLaunchKernelGGL(k_edge_len, dim3((E + 255) / 256), dim3(256), 0, s, edges, E, proj, pmf_raw);
+}
+void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s) {
+  hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
+}
+void launch_edge_sample(int mode, bool deep, const EdgeArgs& a, int blocks, hipStream_t s) {
+  DRH_DISPATCH_MODE(mode, DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_edge_sample<M, D>), dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a)))
+}
+void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s) {
+  const int b = (a.n + 255) / 256;
+  hipLaunchKernelGGL(k_optsimple_moments, dim3(b), dim3(256), 0, s, grad, m, G, off, a.n);
+  hipLaunchKernelGGL(k_optsimple_apply, dim3(b), dim3(256), 0, s, a);
+}
+void launch_adam(const AdamArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_debug_hits(bool deep, const HitsArgs& a, int tiles, hipStream_t s) {
+  DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_debug_hits<D>), dim3(tiles), dim3(64), 0, s, a))
+}
+
+}  // namespace drh

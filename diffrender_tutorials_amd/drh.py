@@ -1,0 +1,262 @@
+"""Python binding of libdrh.so (ctypes over the C ABI in include/drh.h).
+
+Mirrors the reference's plugin interface ``IDiffRender`` (src/virtual_drender.h:14-26):
+``MakeDifferentialRenderer(settings)`` returns a renderer with ``commit``, ``render``,
+``d_render`` and ``d_render_and_compare``; scenes are lists of
+:class:`diffrender_tutorials_amd.scenes.Mesh`, cameras :class:`scenes.Camera`, images
+``(H, W, 3)`` float32 arrays, gradients the flat ``DScene`` vector.
+
+The shared library is the product path.  There is no fallback: a missing library or
+a missing GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdrh.so")
+
+SILHOUETTE, VERTEX_COLOR, TEXTURE_COLOR, LAMBERT = 1, 2, 3, 4
+RNG_REF_PRNG14, RNG_QMC = 0, 1
+TRACER_BVH, TRACER_BF = 0, 1
+OPT_SIMPLE, OPT_ADAM = 0, 1
+
+STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_E_HIP", -4: "DRH_E_OOM",
+          -5: "DRH_E_UNSUPPORTED"}
+
+# every symbol include/drh.h declares
+EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
+           "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
+           "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params", "drh_stream",
+           "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
+           "drh_debug_edges"]
+
+
+class DrhError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Settings(ctypes.Structure):
+    """DiffRenderSettings (src/virtual_drender.h:8-12) plus this path's options."""
+    _fields_ = [("mode", ctypes.c_int), ("spp", ctypes.c_int), ("rng_mode", ctypes.c_int), ("tracer", ctypes.c_int),
+                ("device", ctypes.c_int), ("light_custom", ctypes.c_int), ("light_dir", ctypes.c_float * 3),
+                ("light_color", ctypes.c_float * 3), ("ambient", ctypes.c_float * 3)]
+
+
+class CameraC(ctypes.Structure):
+    _fields_ = [("world_view", ctypes.c_float * 16), ("proj", ctypes.c_float * 16),
+                ("width", ctypes.c_float), ("height", ctypes.c_float)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("n_meshes", ctypes.c_int), ("vcount", ctypes.POINTER(ctypes.c_int)),
+                ("fcount", ctypes.POINTER(ctypes.c_int)), ("transforms", ctypes.POINTER(ctypes.c_float)),
+                ("pos", ctypes.POINTER(ctypes.c_float)), ("col", ctypes.POINTER(ctypes.c_float)),
+                ("tc", ctypes.POINTER(ctypes.c_float)), ("nrm", ctypes.POINTER(ctypes.c_float)),
+                ("idx", ctypes.POINTER(ctypes.c_uint32)), ("tex_w", ctypes.POINTER(ctypes.c_int)),
+                ("tex_h", ctypes.POINTER(ctypes.c_int)), ("texels", ctypes.POINTER(ctypes.c_float)),
+                ("differentiable", ctypes.POINTER(ctypes.c_int))]
+
+
+_lib = None
+
+
+def load():
+    """Load libdrh.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m diffrender_tutorials_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.drh_last_error.restype = ctypes.c_char_p
+        L.drh_stream.restype = ctypes.c_void_p
+        L.drh_create.argtypes = [ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_void_p)]
+        for n in EXPORTS:
+            f = getattr(L, n)
+            if n not in ("drh_last_error", "drh_stream", "drh_destroy"):
+                f.restype = ctypes.c_int
+        L.drh_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def camera_struct(cam) -> CameraC:
+    c = CameraC()
+    c.world_view[:] = [float(x) for x in np.asarray(cam.world_view, np.float32).reshape(16)]
+    c.proj[:] = [float(x) for x in np.asarray(cam.proj, np.float32).reshape(16)]
+    c.width = float(cam.width)
+    c.height = float(cam.height)
+    return c
+
+
+def cameras_array(cams):
+    return (CameraC * len(cams))(*[camera_struct(c) for c in cams])
+
+
+class DiffRender:
+    """IDiffRender over libdrh (one handle = one HIP stream + its device buffers)."""
+
+    def __init__(self, mode=VERTEX_COLOR, spp=1, rng_mode=RNG_REF_PRNG14, tracer=TRACER_BVH, device=0, light=None):
+        L = load()
+        s = Settings()
+        s.mode, s.spp, s.rng_mode, s.tracer, s.device = mode, spp, rng_mode, tracer, device
+        if light is not None and light.get("light_dir") is not None:
+            s.light_custom = 1
+            s.light_dir[:] = [float(x) for x in light["light_dir"]]
+            s.light_color[:] = [float(x) for x in light["light_color"]]
+            s.ambient[:] = [float(x) for x in light["ambient"]]
+        h = ctypes.c_void_p()
+        st = L.drh_create(ctypes.byref(s), ctypes.byref(h))
+        if st != 0:
+            raise DrhError(st, "drh_create failed (no HIP device?)")
+        self.h = h
+        self.mode = mode
+        self.spp = spp
+        self._keep = None
+        self.V = 0
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.drh_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, st):
+        if st != 0:
+            raise DrhError(st, load().drh_last_error(self.h).decode())
+
+    # -- IDiffRender -------------------------------------------------------
+    def commit(self, meshes):
+        """IDiffRender::commit (src/drender.h:91-96)."""
+        L = load()
+        n = len(meshes)
+        vc = np.array([m.V for m in meshes], np.int32)
+        fc = np.array([m.F for m in meshes], np.int32)
+        tr = np.ascontiguousarray(np.concatenate([np.asarray(m.transform, np.float32).reshape(16) for m in meshes]))
+        pos = np.ascontiguousarray(np.concatenate([m.pos for m in meshes]).astype(np.float32))
+        has = lambda a: all(getattr(m, a) is not None for m in meshes)
+        cat = lambda a: np.ascontiguousarray(np.concatenate([getattr(m, a) for m in meshes]).astype(np.float32)) if has(a) else None
+        col, tc, nrm = cat("col"), cat("tc"), cat("nrm")
+        idx = np.ascontiguousarray(np.concatenate([m.idx for m in meshes]).astype(np.uint32))
+        tw = np.array([0 if m.tex is None else m.tex.shape[1] for m in meshes], np.int32)
+        th = np.array([0 if m.tex is None else m.tex.shape[0] for m in meshes], np.int32)
+        texs = [np.asarray(m.tex, np.float32).reshape(-1) for m in meshes if m.tex is not None]
+        tex = np.ascontiguousarray(np.concatenate(texs)) if texs else None
+        df = np.array([1 if m.differentiable else 0 for m in meshes], np.int32)
+        d = SceneDesc(n, _ip(vc), _ip(fc), _fp(tr), _fp(pos), _fp(col), _fp(tc), _fp(nrm),
+                      idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ip(tw), _ip(th), _fp(tex), _ip(df))
+        self._check(L.drh_commit(self.h, ctypes.byref(d)))
+        self.V = int(vc.sum())
+        self.has_col = col is not None
+
+    def grad_size(self):
+        return load().drh_grad_size(self.h)
+
+    def render(self, cams):
+        """IDiffRender::render (src/drender.h:98-141); returns one (H, W, 3) image per camera."""
+        cs = cameras_array(cams)
+        out = np.empty(sum(c.width * c.height * 3 for c in cams), np.float32)
+        self._check(load().drh_render(self.h, cs, len(cams), _fp(out)))
+        return _split(out, cams)
+
+    def d_render(self, cams, adjoints, edge_samples, grad=None):
+        """IDiffRender::d_render (src/drender.h:144-167): accumulates into grad if given."""
+        cs = cameras_array(cams)
+        adj = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1) for a in adjoints]))
+        acc = grad is not None
+        if grad is None:
+            grad = np.zeros(self.grad_size(), np.float32)
+        self._check(load().drh_d_render(self.h, cs, len(cams), _fp(adj), int(edge_samples), _fp(grad), int(acc)))
+        return grad
+
+    def d_render_and_compare(self, cams, targets, edge_samples):
+        """IDiffRender::d_render_and_compare (src/drender.h:169-191): returns
+        (loss, grad, images)."""
+        cs = cameras_array(cams)
+        tgt = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1) for a in targets]))
+        grad = np.zeros(self.grad_size(), np.float32)
+        imgs = np.empty_like(tgt)
+        loss = ctypes.c_float()
+        self._check(load().drh_d_render_and_compare(self.h, cs, len(cams), _fp(tgt), int(edge_samples), _fp(grad),
+                                                    _fp(imgs), ctypes.byref(loss)))
+        return float(loss.value), grad, _split(imgs, cams)
+
+    # -- device-resident optimisation --------------------------------------
+    def opt_init(self, kind=OPT_SIMPLE, pos_lr=0.1, col_lr=0.1):
+        self._check(load().drh_opt_init(self.h, kind, ctypes.c_float(pos_lr), ctypes.c_float(col_lr)))
+
+    def opt_step_dev(self, cams_c, views, d_targets_ptr, edge_samples, d_loss_ptr=None):
+        self._check(load().drh_opt_step(self.h, cams_c, views, ctypes.c_void_p(d_targets_ptr), int(edge_samples),
+                                        ctypes.c_void_p(d_loss_ptr) if d_loss_ptr else None))
+
+    def get_params(self):
+        pos = np.empty(3 * self.V, np.float32)
+        col = np.empty(3 * self.V, np.float32)
+        self._check(load().drh_get_params(self.h, _fp(pos), _fp(col)))
+        return pos.reshape(-1, 3), col.reshape(-1, 3)
+
+    # -- diagnostics ---------------------------------------------------------
+    def hits(self, cam):
+        n = cam.width * cam.height
+        prim = np.empty(n, np.uint32); geom = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32); u = np.empty(n, np.float32); v = np.empty(n, np.float32)
+        c = camera_struct(cam)
+        u32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self._check(load().drh_debug_hits(self.h, ctypes.byref(c), u32(prim), u32(geom), _fp(t), _fp(u), _fp(v)))
+        return prim, geom, t, u, v
+
+    def bvh(self, F):
+        keys = np.empty(F, np.uint32); order = np.empty(F, np.uint32); b = np.empty(6, np.float32)
+        u32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self._check(load().drh_debug_bvh(self.h, u32(keys), u32(order), _fp(b)))
+        return keys, order, b
+
+    def edges(self):
+        n = ctypes.c_int()
+        self._check(load().drh_debug_edges(self.h, ctypes.byref(n), None, None, None, None))
+        v0 = np.empty(n.value, np.int32); v1 = np.empty(n.value, np.int32)
+        pmf = np.empty(n.value, np.float32); cdf = np.empty(n.value + 1, np.float32)
+        self._check(load().drh_debug_edges(self.h, ctypes.byref(n), _ip(v0), _ip(v1), _fp(pmf), _fp(cdf)))
+        return v0, v1, pmf, cdf
+
+    def profile(self, on=True):
+        self._check(load().drh_profile_enable(self.h, int(on)))
+
+    def profile_read(self, name):
+        ms = ctypes.c_double(); cnt = ctypes.c_int()
+        self._check(load().drh_profile_read(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def sync(self):
+        self._check(load().drh_sync(self.h))
+
+    def stream(self):
+        return load().drh_stream(self.h)
+
+
+def _split(flat, cams):
+    res, off = [], 0
+    for c in cams:
+        n = c.width * c.height * 3
+        res.append(flat[off:off + n].reshape(c.height, c.width, 3))
+        off += n
+    return res
+
+
+def MakeDifferentialRenderer(mode=VERTEX_COLOR, spp=1, **kw) -> DiffRender:
+    """MakeDifferentialRenderer (src/drender.cpp:89-147)."""
+    return DiffRender(mode=mode, spp=spp, **kw)

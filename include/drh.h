@@ -1,0 +1,159 @@
+/*
+ * drh.h -- C ABI of the MI355X differentiable-rendering path (libdrh.so).
+ *
+ * Drop-in for the reference's renderer plugin interface IDiffRender
+ * (FROL256/diffrender_tutorials, src/virtual_drender.h:14-26) and its factory
+ * MakeDifferentialRenderer (src/drender.cpp:89-147).  Plain pointers and sizes only;
+ * host arrays are read or written only during a call.  Every call returns a
+ * drh_status (the reference prints and returns, src/drender.h:103-107; here the
+ * message is kept in drh_last_error()).  One handle per host thread: a handle owns
+ * its own HIP stream and device buffers and is not re-entrant (neither is the
+ * reference's DiffRender, whose m_aux / tracer camera change on every call).
+ *
+ * Layouts follow the reference:
+ *   images   views x H x W x 3 float32, row-major, y = 0 is the top row
+ *            (Img = LiteImage::Image2D<float3>, src/utils.h:20)
+ *   gradient DScene::full_data of every differentiable mesh, in mesh order:
+ *            per mesh [pos 3V][colour 3V if VERTEX_COLOR][texture w*h*3 if TEXTURE_COLOR
+ *            or LAMBERT][transform 12] (DMesh, src/dmesh.h:35-85)
+ *   cameras  CamInfo::mWorldView and mProj as float4x4 memory (column-major),
+ *            width, height (common_src/camera.h:6-44)
+ */
+#ifndef DRH_H
+#define DRH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct drh_renderer* drh_handle;
+
+typedef enum {
+  DRH_OK = 0,
+  DRH_E_INVALID = -1,        /* bad argument (sizes, mode, NULL pointers) */
+  DRH_E_NOT_COMMITTED = -2,  /* render/d_render before commit (reference: "renderer was not prepared") */
+  DRH_E_HIP = -3,            /* HIP runtime error; see drh_last_error */
+  DRH_E_OOM = -4,            /* device allocation failed */
+  DRH_E_UNSUPPORTED = -5,    /* a scene feature outside this path (see drh_last_error) */
+} drh_status;
+
+/* SHADING_MODEL numbering (src/dmesh.h:16-23); PHONG / GGX / PATH_TEST are not on this path. */
+enum { DRH_SILHOUETTE = 1, DRH_VERTEX_COLOR = 2, DRH_TEXTURE_COLOR = 3, DRH_LAMBERT = 4 };
+/* Edge-sample random numbers: the reference's 14 OpenMP PRNG streams (src/drender.h:306-331),
+ * or index-addressed Niederreiter QMC (common_src/qmc.cpp:190-197). */
+enum { DRH_RNG_REF_PRNG14 = 0, DRH_RNG_QMC = 1 };
+/* Ray tracer: GPU LBVH (default), or brute force in the reference's triangle order. */
+enum { DRH_TRACER_BVH = 0, DRH_TRACER_BF = 1 };
+/* Optimiser for drh_opt_step: the reference OptSimple "Adam" (src/optimizer_simple.cpp:188-236)
+ * or standard Adam. */
+enum { DRH_OPT_SIMPLE = 0, DRH_OPT_ADAM = 1 };
+
+typedef struct {
+  int mode;             /* DRH_SILHOUETTE ... DRH_LAMBERT           (DiffRenderSettings::mode) */
+  int spp;              /* samples per pixel; sqrt(spp)^2 are used   (DiffRenderSettings::spp)  */
+  int rng_mode;         /* DRH_RNG_*                                                             */
+  int tracer;           /* DRH_TRACER_*                                                          */
+  int device;           /* HIP device ordinal                                                    */
+  int light_custom;     /* 0: the reference's hard-coded light (src/shade_lambert.cpp:10-12)     */
+  float light_dir[3];   /* direction the light travels (used when light_custom != 0)             */
+  float light_color[3];
+  float ambient[3];
+} drh_settings;
+
+typedef struct {
+  float world_view[16];
+  float proj[16];
+  float width, height;
+} drh_camera;
+
+/* A scene: meshes concatenated (Scene::add_mesh, common_src/scene.h:130-138), one instance per
+ * mesh with a 4x4 transform.  col / tc / nrm may be NULL (reference defaults (0,0,0), (0,0),
+ * (1,0,0)); textures are tex_w[i] x tex_h[i] RGB float32 texels, concatenated (0 x 0 = none). */
+typedef struct {
+  int n_meshes;
+  const int* vcount;
+  const int* fcount;
+  const float* transforms;   /* 16 per mesh, column-major */
+  const float* pos;          /* 3 per vertex */
+  const float* col;          /* 3 per vertex or NULL */
+  const float* tc;           /* 2 per vertex or NULL */
+  const float* nrm;          /* 3 per vertex or NULL */
+  const uint32_t* idx;       /* 3 per triangle, mesh-local vertex indices */
+  const int* tex_w;
+  const int* tex_h;
+  const float* texels;
+  const int* differentiable; /* per mesh 0/1 (the DScene's differentiable mesh ids) */
+} drh_scene_desc;
+
+/* MakeDifferentialRenderer + IDiffRender::init (src/drender.cpp:89-147, src/drender.h:79-88). */
+drh_status drh_create(const drh_settings* settings, drh_handle* out);
+void       drh_destroy(drh_handle h);
+const char* drh_last_error(drh_handle h);
+
+/* IDiffRender::commit (src/drender.h:91-96): flattens the scene (Scene::prepare_for_render,
+ * common_src/scene_utils.cpp:98-173) into device buffers and builds the LBVH. */
+drh_status drh_commit(drh_handle h, const drh_scene_desc* scene);
+/* Number of floats of the flat gradient (DScene::full_size, src/dmesh.h:231-237). */
+int        drh_grad_size(drh_handle h);
+
+/* IDiffRender::render (src/drender.h:98-141).  images: host, views x H x W x 3. */
+drh_status drh_render(drh_handle h, const drh_camera* cams, int views, float* images);
+/* IDiffRender::d_render (src/drender.h:144-167): ACCUMULATES into grad when accumulate != 0
+ * (the reference always accumulates; the caller clears), else overwrites. */
+drh_status drh_d_render(drh_handle h, const drh_camera* cams, int views, const float* adjoints,
+                        int edge_samples, float* grad, int accumulate);
+/* IDiffRender::d_render_and_compare (src/drender.h:169-191): commit must have been called
+ * (the reference calls commit itself; a caller does drh_commit then this).  Renders, writes
+ * out_images (may be NULL), computes the L2 loss and its adjoint 2*(I - T)
+ * (LossAndDiffLoss, src/utils.cpp:15-27), clears grad and runs d_render.  *loss =
+ * sum of squared errors / (views*W*H).  Views must share one resolution (as in the reference,
+ * src/drender.h:180-182). */
+drh_status drh_d_render_and_compare(drh_handle h, const drh_camera* cams, int views, const float* targets,
+                                    int edge_samples, float* grad, float* out_images, float* loss);
+
+/* Device-resident forms: every array argument is a device pointer (hipMalloc'd, on the
+ * handle's device); work is ordered on the handle's stream; no host synchronisation except
+ * where a host output (loss) is requested. */
+drh_status drh_render_dev(drh_handle h, const drh_camera* cams, int views, float* d_images);
+drh_status drh_d_render_dev(drh_handle h, const drh_camera* cams, int views, const float* d_adjoints,
+                            int edge_samples, float* d_grad, int accumulate);
+drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, int views,
+                                        const float* d_targets, int edge_samples, float* d_grad,
+                                        float* d_images, float* d_loss);
+
+/* One optimisation step on the device-resident scene (OptSimple::Run's loop body,
+ * src/optimizer_simple.cpp:293-305): re-commit from the device parameters (LBVH rebuild),
+ * d_render_and_compare against d_targets, then the optimiser update of positions
+ * (and colours for DRH_VERTEX_COLOR).  kind DRH_OPT_SIMPLE follows OptStep + OptUpdateScene
+ * + StepDecay with OptimizerParameters(GD_Adam) learning rates (pos_lr, col_lr, decay every
+ * 100 steps by 0.75); DRH_OPT_ADAM is standard Adam(lr, 0.9, 0.999, 1e-8) on the same
+ * parameters.  d_loss (device float, may be NULL) receives the loss of this step. */
+drh_status drh_opt_init(drh_handle h, int kind, float pos_lr, float col_lr);
+drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const float* d_targets,
+                        int edge_samples, float* d_loss);
+/* Download the current (optimised) vertex positions (3V) and colours (3V, may be NULL). */
+drh_status drh_get_params(drh_handle h, float* pos, float* col);
+
+/* The handle's HIP stream (a hipStream_t) and device synchronisation of it. */
+void*      drh_stream(drh_handle h);
+drh_status drh_sync(drh_handle h);
+
+/* Per-kernel timing with HIP events recorded on the handle's stream around each launch.
+ * drh_profile_read returns the summed milliseconds and launch count of kernel `name`
+ * (names as in the rocprof trace) since the last drh_profile_enable. */
+drh_status drh_profile_enable(drh_handle h, int on);
+drh_status drh_profile_read(drh_handle h, const char* name, double* total_ms, int* launches);
+
+/* Diagnostics for parity tests: per-pixel closest hit at pixel centres (spp 1) for one camera,
+ * and the LBVH's sorted Morton keys / triangle order and the edge sampler of the last
+ * d_render view. */
+drh_status drh_debug_hits(drh_handle h, const drh_camera* cam, uint32_t* prim, uint32_t* geom,
+                          float* t, float* u, float* v);
+drh_status drh_debug_bvh(drh_handle h, uint32_t* morton_sorted, uint32_t* tri_order, float* scene_bounds);
+drh_status drh_debug_edges(drh_handle h, int* n_edges, int* v0, int* v1, float* pmf, float* cdf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DRH_H */

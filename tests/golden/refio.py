@@ -1,0 +1,62 @@
+"""Run the reference (compiled from its own sources by ``make -C oracle ref``) on a scene
+and read its outputs.  Used only by ``make_golden.py`` in this container, to write the
+golden vectors under tests/golden/; the GPU tests never run it."""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HARNESS = os.path.join(HERE, "..", "..", "oracle", "_ref", "ref_harness")
+
+OP_RENDER, OP_HITS, OP_DRENDER, OP_COMPARE, OP_SEQ, OP_EDGES, OP_OPTIMIZE = 1, 2, 4, 8, 16, 32, 64
+TAGS = {1: "images", 2: "hit_prim", 3: "hit_geom", 4: "hit_t", 5: "hit_u", 6: "hit_v", 7: "grad", 8: "loss",
+        9: "qmc", 10: "prng", 11: "hamm", 12: "edge_v0", 13: "edge_v1", 14: "edge_pmf", 15: "edge_cdf",
+        16: "compare_grad", 17: "compare_images", 18: "proj", 19: "opt_pos", 20: "opt_col", 21: "opt_err",
+        22: "proj_inv", 23: "wv_inv", 24: "wvp"}
+INT_TAGS = {"hit_prim", "hit_geom", "qmc", "edge_v0", "edge_v1"}
+
+
+def run(meshes, cams, mode, spp, ops, edge_samples=0, images=None, opt_iters=0):
+    buf = bytearray()
+    buf += struct.pack("<8i", 0x44524831, mode, spp, len(meshes), len(cams), edge_samples, ops, opt_iters)
+    for m in meshes:
+        tw = 0 if m.tex is None else m.tex.shape[1]
+        th = 0 if m.tex is None else m.tex.shape[0]
+        buf += struct.pack("<8i", m.V, m.F, m.col is not None, m.tc is not None, m.nrm is not None, tw, th,
+                           1 if m.differentiable else 0)
+        buf += np.asarray(m.transform, np.float32).tobytes()
+        buf += np.asarray(m.pos, np.float32).tobytes()
+        for a in (m.col, m.tc, m.nrm):
+            if a is not None:
+                buf += np.asarray(a, np.float32).tobytes()
+        buf += np.asarray(m.idx, np.uint32).tobytes()
+        if m.tex is not None:
+            buf += np.asarray(m.tex, np.float32).tobytes()
+    for c in cams:
+        buf += np.asarray(c.world_view, np.float32).tobytes() + np.asarray(c.proj, np.float32).tobytes()
+        buf += struct.pack("<2f", float(c.width), float(c.height))
+    if images is not None:
+        for im in images:
+            buf += np.asarray(im, np.float32).tobytes()
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fi, "wb") as f:
+            f.write(buf)
+        subprocess.run([HARNESS, fi, fo], check=True)
+        data = open(fo, "rb").read()
+    out, off = {}, 0
+    while off < len(data):
+        tag, n = struct.unpack_from("<2i", data, off)
+        off += 8
+        name = TAGS[tag]
+        dt = np.uint32 if name in INT_TAGS else np.float32
+        if name in ("edge_v0", "edge_v1"):
+            dt = np.int32
+        out[name] = np.frombuffer(data[off:off + n], dtype=dt).copy()
+        off += n
+    return out

@@ -289,6 +289,7 @@ struct drh_renderer {
     s.n_meshes = nm;
     t.bf_a = bf_a.as<float4>(); t.bf_e1 = bf_e1.as<float4>(); t.bf_e2 = bf_e2.as<float4>();
     t.use_bf = st.tracer == DRH_TRACER_BF;
+    t.depth = deep() ? 64 : 48;  // matches the kernel's LDS stack (DEPTH template argument)
     return t;
   }
 

@@ -140,14 +140,15 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
   return tmin <= tmx;
 }
 
-constexpr int STACK_DEPTH = 64;  // > LBVH depth bound (30 Morton bits + 32 index bits)
+// LDS traversal stack depth: the Karras tree over (30-bit Morton key, triangle index) is at
+// most 30 + ceil(log2 F) levels deep, so 48 entries cover F <= 2^18 and 64 any F < 2^32.
 
 // Closest hit with the brute-force tracer's answer: the same MT test, and on equal t the
 // triangle that comes first in (mesh, triangle) order wins (BF's strict `t < hit.t`).
 // `stack` is this lane's column of an LDS stack laid out [depth][64 lanes].
 template <bool ANY>
 __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar,
-                                         int* stack, int lane_stride) {
+                                         int* stack, int lane_stride, int depth) {
   Hit h{NO_HIT, tFar, 0.0f, 0.0f};
   if (s.F == 0) return h;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
@@ -165,7 +166,8 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
   if (s.F == 1) { leaf(0); return h; }
   int sp = 0;
   int node = 0;
-  while (true) {
+  // a tree over F leaves has F-1 internal nodes, each visited at most once per ray
+  for (int visits = 0; visits < s.F; visits++) {
     const float4* n = s.nodes + 4 * node;
     const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
     const int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
@@ -179,7 +181,7 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
     if (goL && goR) {
       const bool lfirst = tl <= tr;
       const int nearc = lfirst ? cl : cr, farc = lfirst ? cr : cl;
-      if (sp < STACK_DEPTH) { stack[sp * lane_stride] = farc; sp++; }
+      if (sp < depth) { stack[sp * lane_stride] = farc; sp++; }
       node = nearc;
     } else if (goL) {
       node = cl;
@@ -217,12 +219,13 @@ struct Tracer {
   const float4* bf_e1;
   const float4* bf_e2;
   int use_bf;
+  int depth;   // entries of the caller's LDS stack column
 };
 
 template <bool ANY>
 __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* stack, int ls) {
   if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
-  return trace_bvh<ANY>(T.s, o, d, tNear, tFar, stack, ls);
+  return trace_bvh<ANY>(T.s, o, d, tNear, tFar, stack, ls, T.depth);
 }
 
 // ---------------------------------------------------------------------------

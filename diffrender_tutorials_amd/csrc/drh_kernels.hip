@@ -258,7 +258,7 @@ __global__ void k_refit(RefitArgs a) {
   a.leaf_lo[s] = a.box_lo[g];
   a.leaf_hi[s] = a.box_hi[g];
   int p = a.leaf_parent[s];
-  while (p >= 0) {
+  for (int level = 0; p >= 0 && level < 64; level++) {  // tree depth <= 62
     __threadfence();
     if (atomicAdd(&a.counter[p], 1u) == 0u) return;  // the other child is still on its way
     __threadfence();
@@ -317,6 +317,8 @@ __global__ void k_reorder(ReorderArgs a) {
 template <int MODE, int DEPTH>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   __shared__ int stack[DEPTH * 64];
+  Tracer T = a.tr;
+  T.depth = DEPTH;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
   const int tiles_x = (c.w + 7) >> 3;
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
       for (int dx = 0; dx < sq; dx++) {
         const float xo = (dx + 0.5f) / float(sq);
         const float yo = (dy + 0.5f) / float(sq);
-        const F3 col = shade<MODE>(a.tr, a.light, c, x + xo, y + yo, a.tr.s.nrm, stack + lane, 64);
+        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, stack + lane, 64);
         acc = add(acc, divs(col, ns));
       }
     const long long p = c.img_off + 3ll * ((long long)y * c.w + x);
@@ -374,6 +376,8 @@ template <int MODE, int DEPTH>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   __shared__ int stack[DEPTH * 64];
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
+  Tracer T = a.tr;
+  T.depth = DEPTH;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
   const bool use_lds = a.lds_n > 0;
@@ -392,7 +396,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
       const F3 val = divs(adj, (float)a.nsamp);
       F3 o, d;
       eye_ray(c, x + xo, y + yo, o, d);
-      const Hit h = trace<false>(a.tr, o, d, 0.0f, 1e9f, stack + lane, 64);
+      const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, stack + lane, 64);
       if (h.gtri == NO_HIT) continue;
       const int m = a.tr.s.tri_mesh[h.gtri];
       const int goff = a.layout[4 * m + 0];
@@ -534,6 +538,8 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
   __shared__ int stack[4 * DEPTH * 64];
   extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
   __shared__ float ltr[4][12];
+  Tracer T = a.tr;
+  T.depth = DEPTH;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int* stk = stack + wave * DEPTH * 64 + lane;
@@ -565,8 +571,8 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     const float len = sqrtf(dvx * dvx + dvy * dvy);
     const float ndx = dvx / len, ndy = dvy / len;
     const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
-    const F3 cin = shade<MODE>(a.tr, a.light, c, px - 1e-3f * nx, py - 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
-    const F3 cout = shade<MODE>(a.tr, a.light, c, px + 1e-3f * nx, py + 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
+    const F3 cin = shade<MODE>(T, a.light, c, px - 1e-3f * nx, py - 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
+    const F3 cout = shade<MODE>(T, a.light, c, px + 1e-3f * nx, py + 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
     const float pdf = pmf / len;
     const float weight = 1.0f / (pdf * float(a.N));
     const float adjv = dot(sub(cin, cout), adj);
@@ -673,6 +679,8 @@ __global__ void k_adam(AdamArgs a) {
 template <int DEPTH>
 __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
   __shared__ int stack[DEPTH * 64];
+  Tracer T = a.tr;
+  T.depth = DEPTH;
   const DevCam& c = a.cams[0];
   const int lane = threadIdx.x;
   const int tiles_x = (c.w + 7) >> 3;
@@ -681,7 +689,7 @@ __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
   if (x >= c.w || y >= c.h) return;
   F3 o, d;
   eye_ray(c, x + 0.5f, y + 0.5f, o, d);
-  const Hit h = trace<false>(a.tr, o, d, 0.0f, 1e9f, stack + lane, 64);
+  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, stack + lane, 64);
   const int p = y * c.w + x;
   uint32_t prim = NO_HIT, geom = NO_HIT;
   if (h.gtri != NO_HIT) { geom = (uint32_t)a.tr.s.tri_mesh[h.gtri]; prim = h.gtri - (uint32_t)a.tr.s.mesh_to[geom]; }

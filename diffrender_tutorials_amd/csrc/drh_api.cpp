@@ -212,6 +212,9 @@ struct drh_renderer {
   // LBVH
   DBuf bf_a, bf_e1, bf_e2, box_lo, box_hi, cen, bounds, keys0, vals0, keys1, vals1, hist, child, node_parent,
       leaf_parent, counter, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
+  // clamped-determinant suspects (per view for camera rays, per light for shadow rays)
+  DBuf sus_view, sus_view_cnt, sus_light, sus_light_cnt;
+  bool light_sus = false;
   // edges
   DBuf edges, edge_mesh, pmf_raw, pmf, cdf, proj;
   // sampling
@@ -290,7 +293,35 @@ struct drh_renderer {
     t.bf_a = bf_a.as<float4>(); t.bf_e1 = bf_e1.as<float4>(); t.bf_e2 = bf_e2.as<float4>();
     t.use_bf = st.tracer == DRH_TRACER_BF;
     t.depth = deep() ? 64 : 48;  // matches the kernel's LDS stack (DEPTH template argument)
+    t.cam = {nullptr, nullptr};
+    t.light = {light_sus ? sus_light.as<float4>() : nullptr, light_sus ? sus_light_cnt.as<int>() : nullptr};
     return t;
+  }
+  // Tracer for the camera rays of view v (its suspect list must have been built).
+  Tracer tracer_view(int v) const {
+    Tracer t = tracer();
+    if (!t.use_bf && F > 0) t.cam = {sus_view.as<float4>() + (size_t)v * F, sus_view_cnt.as<int>() + v};
+    return t;
+  }
+  // Suspect triangles of every view's camera origin (CastSingleRay's origin,
+  // WorldViewInv * (0,0,0,1), raytrace.h:67-76).
+  drh_status build_view_suspects(int views) {
+    if (st.tracer == DRH_TRACER_BF || F == 0) return DRH_OK;
+    drh_status s;
+    if ((s = hip(sus_view.reserve(16 * (size_t)F * views), "alloc")) ||
+        (s = hip(sus_view_cnt.reserve(4 * (size_t)views), "alloc")))
+      return s;
+    timed("k_fill_u32", [&] { launch_fill_u32(sus_view_cnt.as<uint32_t>(), views, 0u, stream); });
+    for (int v = 0; v < views; v++) {
+      const M4 wi = m4_load(cam_cache[v].wv_inv);
+      const float z4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+      float o[4];
+      m4_vec(wi, z4, o);
+      SuspectArgs a{F, 0, o[0], o[1], o[2], bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
+                    sus_view.as<float4>() + (size_t)v * F, sus_view_cnt.as<int>() + v};
+      timed("k_suspects", [&] { launch_suspects(a, stream); });
+    }
+    return hip(hipGetLastError(), "suspects");
   }
 
   drh_status build_bvh();
@@ -472,6 +503,15 @@ drh_status drh_renderer::build_bvh() {
                   bf_e1.as<float4>(), bf_e2.as<float4>(), box_lo.as<float4>(), box_hi.as<float4>(), cen.as<float4>(),
                   bounds.as<uint32_t>()};
   timed("k_tri_setup", [&] { launch_tri_setup(ta, stream); });
+  light_sus = false;
+  if (st.mode == DRH_LAMBERT && st.tracer != DRH_TRACER_BF) {
+    if ((s = R(sus_light, f16)) || (s = R(sus_light_cnt, 16))) return s;
+    timed("k_fill_u32", [&] { launch_fill_u32(sus_light_cnt.as<uint32_t>(), 1, 0u, stream); });
+    SuspectArgs la{F, 1, light.L.x, light.L.y, light.L.z, bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
+                   sus_light.as<float4>(), sus_light_cnt.as<int>()};
+    timed("k_suspects", [&] { launch_suspects(la, stream); });
+    light_sus = true;
+  }
   MortonArgs ma{F, cen.as<float4>(), bounds.as<uint32_t>(), keys0.as<uint32_t>(), vals0.as<uint32_t>()};
   timed("k_morton", [&] { launch_morton(ma, stream); });
   for (int pass = 0; pass < 4; pass++) {
@@ -531,7 +571,8 @@ drh_status drh_renderer::set_cams(const drh_camera* c, int views, long long* tot
 }
 
 drh_status drh_renderer::run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss) {
-  const Tracer T = tracer();
+  drh_status s0;
+  if ((s0 = build_view_suspects(views))) return s0;
   int part = 0;
   std::vector<int> tiles(views);
   for (int v = 0; v < views; v++) { tiles[v] = ((cam_cache[v].w + 7) / 8) * ((cam_cache[v].h + 7) / 8); part += tiles[v]; }
@@ -541,7 +582,7 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   }
   int part_off = 0;
   for (int v = 0; v < views; v++) {
-    RenderArgs ra{T, light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off};
+    RenderArgs ra{tracer_view(v), light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off};
     timed("k_render", [&] { launch_render(st.mode, deep(), ra, tiles[v], stream); });
     part_off += tiles[v];
   }
@@ -567,8 +608,8 @@ drh_status drh_renderer::ensure_rnd(long long N) {
 drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N, float* d_grad, bool accumulate) {
   drh_status s;
   if ((s = ensure_rnd(N))) return s;
+  if ((s = build_view_suspects(views))) return s;
   if (!accumulate) timed("k_fill_f32", [&] { launch_fill_f32(d_grad, G, 0.0f, stream); });
-  const Tracer T = tracer();
   // single differentiable mesh: position / transform gradient offsets
   int single = -1;
   for (int i = 0; i < nm; i++) if (diff[i]) { single = (single == -1) ? i : -2; }
@@ -577,7 +618,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
     const int tiles = ((c.w + 7) / 8) * ((c.h + 7) / 8);
     if (st.mode != DRH_SILHOUETTE && G > 0) {
       InteriorArgs ia{};
-      ia.tr = T; ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.hamm = hamm.as<float>();
+      ia.tr = tracer_view(v); ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.hamm = hamm.as<float>();
       ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
       ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
       int blocks = tiles;
@@ -596,7 +637,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       timed("k_edge_len", [&] { launch_edge_len(edges.as<int2>(), E, proj.as<float4>(), pmf_raw.as<float>(), stream); });
       timed("k_cdf", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), stream); });
       EdgeArgs ea{};
-      ea.tr = T; ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
+      ea.tr = tracer_view(v); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
       ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
       ea.proj = proj.as<float4>();
       ea.rnd = st.rng_mode == DRH_RNG_REF_PRNG14 ? rnd.as<float2>() : nullptr;
@@ -912,7 +953,8 @@ drh_status drh_debug_hits(drh_handle h, const drh_camera* cam, uint32_t* prim, u
   const int n = h->cam_cache[0].w * h->cam_cache[0].h;
   DBuf b;
   if ((s = h->hip(b.reserve(20 * (size_t)n), "alloc"))) return s;
-  HitsArgs a{h->tracer(), h->cams.as<DevCam>(), b.as<uint32_t>(), b.as<uint32_t>() + n, (float*)(b.as<uint32_t>() + 2 * n),
+  if ((s = h->build_view_suspects(1))) return s;
+  HitsArgs a{h->tracer_view(0), h->cams.as<DevCam>(), b.as<uint32_t>(), b.as<uint32_t>() + n, (float*)(b.as<uint32_t>() + 2 * n),
              (float*)(b.as<uint32_t>() + 3 * n), (float*)(b.as<uint32_t>() + 4 * n)};
   const int tiles = ((h->cam_cache[0].w + 7) / 8) * ((h->cam_cache[0].h + 7) / 8);
   launch_debug_hits(h->deep(), a, tiles, h->stream);

@@ -213,19 +213,59 @@ __device__ __forceinline__ Hit trace_bf(const DevScene& s, const float4* tri_a_o
 }
 
 // Tracer selector: BVH or BF.  BF reads triangles in original order (bf_* arrays).
+//
+// Clamped determinants: BF's invDet = sign(det)/max(|det|, 1e-6) scales u, v, t by
+// |det|/1e-6 when |det| < 1e-6, so BF can accept a triangle a nearly parallel ray never
+// touches (reported t shrinks by the same factor).  Writing d = a*e1 + b*e2 + c*(o-A),
+// acceptance needs |d.(e2 x (o-A))|, |d.((o-A) x e1)|, |d.N| < 1e-6, i.e. |a|, |b|, |c| <
+// 1e-6/|V| with V = (o-A).N, hence |V| < 1e-6 (|e1| + |e2| + |o-A|).  Camera rays share o,
+// so only the triangles meeting that bound for the view's origin can be accepted this way;
+// shadow rays share d = -L, so only triangles with |L.N| < 1e-6 can.  Those "suspect"
+// triangles are found per view / per light (k_suspects) and tested by brute force after the
+// LBVH traversal, which makes the LBVH answer the BF answer.  Suspect entries are
+// (N.xyz, triangle id); rays with |d.N| >= 1.5e-6 skip an entry (their det is not clamped).
+struct Suspects {
+  const float4* list;  // (N.x, N.y, N.z, id bits) in no particular order
+  const int* count;
+};
+
 struct Tracer {
   DevScene s;
   const float4* bf_a;
   const float4* bf_e1;
   const float4* bf_e2;
   int use_bf;
-  int depth;   // entries of the caller's LDS stack column
+  int depth;        // entries of the caller's LDS stack column
+  Suspects cam;     // for closest-hit (camera) rays of the current view
+  Suspects light;   // for any-hit (shadow) rays
 };
+
+template <bool ANY>
+__device__ __forceinline__ void test_suspects(const Tracer& T, const Suspects& S, F3 o, F3 d, float tNear, float tFar,
+                                              Hit& h) {
+  if (!S.count) return;
+  const int n = *S.count;
+  for (int k = 0; k < n; k++) {
+    const float4 e = S.list[k];
+    const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
+    if (!(dn < 1.5e-6f)) continue;
+    const uint32_t g = __float_as_uint(e.w);
+    float t, u, v;
+    if (mt_test(T.bf_a[g], T.bf_e1[g], T.bf_e2[g], o, d, tNear, t, u, v)) {
+      if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; return; } }
+      else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) { h.gtri = g; h.t = t; h.u = u; h.v = v; }
+    }
+  }
+}
 
 template <bool ANY>
 __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* stack, int ls) {
   if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
-  return trace_bvh<ANY>(T.s, o, d, tNear, tFar, stack, ls, T.depth);
+  Hit h = trace_bvh<ANY>(T.s, o, d, tNear, tFar, stack, ls, T.depth);
+  if (ANY && h.gtri != NO_HIT) return h;
+  if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
+  test_suspects<ANY>(T, ANY ? T.light : T.cam, o, d, tNear, tFar, h);
+  return h;
 }
 
 // ---------------------------------------------------------------------------

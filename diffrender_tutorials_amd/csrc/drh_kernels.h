@@ -94,6 +94,17 @@ struct ReorderArgs {
   float4* tri_e2;
 };
 
+struct SuspectArgs {
+  int F;
+  int light;              // 0: view origin (ox, oy, oz); 1: light direction (ox, oy, oz)
+  float ox, oy, oz;
+  const float4* bf_a;
+  const float4* bf_e1;
+  const float4* bf_e2;
+  float4* list;           // capacity F
+  int* count;             // zeroed by the caller
+};
+
 struct RenderArgs {
   Tracer tr;
   Light light;
@@ -192,6 +203,7 @@ void launch_karras(const KarrasArgs& a, hipStream_t s);
 void launch_refit(const RefitArgs& a, hipStream_t s);
 void launch_pack(const PackArgs& a, hipStream_t s);
 void launch_reorder(const ReorderArgs& a, hipStream_t s);
+void launch_suspects(const SuspectArgs& a, hipStream_t s);
 void launch_render(int mode, bool deep, const RenderArgs& a, int tiles, hipStream_t s);
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s);
 void launch_interior(int mode, bool deep, const InteriorArgs& a, int blocks, hipStream_t s);

@@ -302,6 +302,31 @@ __global__ void k_pack_nodes(PackArgs a) {
   n[3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.0f, 0.0f);
 }
 
+// Suspect triangles (see Tracer in drh_device.h): light == 0 tests the view origin bound
+// |(o-A).N| <= 2e-6 (|e1| + |e2| + |o-A|), light == 1 tests |L.N| <= 2e-6; both in double
+// with a 2x margin over the exact bound.
+__global__ void k_suspects(SuspectArgs a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.F) return;
+  const float4 A = a.bf_a[g], E1 = a.bf_e1[g], E2 = a.bf_e2[g];
+  const double e1x = E1.x, e1y = E1.y, e1z = E1.z, e2x = E2.x, e2y = E2.y, e2z = E2.z;
+  const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+  bool hit;
+  if (a.light) {
+    hit = fabs(a.ox * nx + a.oy * ny + a.oz * nz) <= 2e-6;
+  } else {
+    const double tx = (double)a.ox - A.x, ty = (double)a.oy - A.y, tz = (double)a.oz - A.z;
+    const double V = fabs(tx * nx + ty * ny + tz * nz);
+    const double bound = sqrt(e1x * e1x + e1y * e1y + e1z * e1z) + sqrt(e2x * e2x + e2y * e2y + e2z * e2z) +
+                         sqrt(tx * tx + ty * ty + tz * tz);
+    hit = V <= 2e-6 * bound + 1e-12;
+  }
+  if (hit || !(nx == nx && ny == ny && nz == nz)) {
+    const int k = atomicAdd(a.count, 1);
+    a.list[k] = make_float4((float)nx, (float)ny, (float)nz, __uint_as_float((uint32_t)g));
+  }
+}
+
 __global__ void k_reorder(ReorderArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.F) return;
@@ -744,6 +769,9 @@ void launch_refit(const RefitArgs& a, hipStream_t s) {
 }
 void launch_pack(const PackArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_pack_nodes, dim3((a.F + 254) / 256), dim3(256), 0, s, a);
+}
+void launch_suspects(const SuspectArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_suspects, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_reorder(const ReorderArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);

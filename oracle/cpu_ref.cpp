@@ -286,7 +286,42 @@ struct Scene {
   std::vector<BvhNode> nodes;
   std::vector<int> order;
   bool bvh_built = false;
+  // triangles BF may accept with a clamped determinant (|det| < 1e-6), tested by brute force
+  // after BVH traversal: for camera rays of the current view (shared origin) and for shadow
+  // rays (shared direction).  Set before the parallel loops.
+  std::vector<int> sus_cam, sus_light;
 };
+
+// Clamped-determinant suspects.  BF's invDet = sign/max(|det|, 1e-6) lets a nearly parallel
+// ray accept a triangle it never touches.  With d = a*e1 + b*e2 + c*(o-A), acceptance needs
+// |a|, |b|, |c| < 1e-6/|V|, V = (o-A).N, so |V| < 1e-6 (|e1| + |e2| + |o-A|) for a shared
+// origin o; for a shared direction d it needs |d.N| < 1e-6.  Bounds checked in double with a
+// 2x margin.
+void suspects_origin(Scene& s, f3 o) {
+  s.sus_cam.clear();
+  for (int g = 0; g < s.F; g++) {
+    const double ax = s.ta[g].x, ay = s.ta[g].y, az = s.ta[g].z;
+    const double e1x = (double)s.tb[g].x - ax, e1y = (double)s.tb[g].y - ay, e1z = (double)s.tb[g].z - az;
+    const double e2x = (double)s.tcv[g].x - ax, e2y = (double)s.tcv[g].y - ay, e2z = (double)s.tcv[g].z - az;
+    const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const double tx = o.x - ax, ty = o.y - ay, tz = o.z - az;
+    const double V = std::fabs(tx * nx + ty * ny + tz * nz);
+    const double b = std::sqrt(e1x * e1x + e1y * e1y + e1z * e1z) + std::sqrt(e2x * e2x + e2y * e2y + e2z * e2z) +
+                     std::sqrt(tx * tx + ty * ty + tz * tz);
+    if (V <= 2e-6 * b + 1e-12 || V != V) s.sus_cam.push_back(g);
+  }
+}
+void suspects_direction(Scene& s, f3 d) {
+  s.sus_light.clear();
+  for (int g = 0; g < s.F; g++) {
+    const double ax = s.ta[g].x, ay = s.ta[g].y, az = s.ta[g].z;
+    const double e1x = (double)s.tb[g].x - ax, e1y = (double)s.tb[g].y - ay, e1z = (double)s.tb[g].z - az;
+    const double e2x = (double)s.tcv[g].x - ax, e2y = (double)s.tcv[g].y - ay, e2z = (double)s.tcv[g].z - az;
+    const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const double dn = std::fabs(d.x * nx + d.y * ny + d.z * nz);
+    if (dn <= 2e-6 || dn != dn) s.sus_light.push_back(g);
+  }
+}
 
 int grad_layout(const Scene& s, int mode, std::vector<int>* mesh_off) {
   int off = 0;
@@ -443,8 +478,19 @@ Hit trace_bvh(const Scene& s, f3 o, f3 d, float tNear, float tFar) {
   return h;
 }
 
-inline Hit trace(const Scene& s, f3 o, f3 d, float tNear = 0.0f, float tFar = 1e9f) {
-  return s.tracer == ORC_TRACER_BVH ? trace_bvh(s, o, d, tNear, tFar) : trace_bf(s, o, d, tNear, tFar);
+// Closest hit (camera rays) or, with shadow = true, a shadow ray's hit.  The BVH path adds
+// the suspect triangles of the ray family, so both tracers return the BF answer.
+inline Hit trace(const Scene& s, f3 o, f3 d, float tNear = 0.0f, float tFar = 1e9f, bool shadow = false) {
+  if (s.tracer != ORC_TRACER_BVH) return trace_bf(s, o, d, tNear, tFar);
+  Hit h = trace_bvh(s, o, d, tNear, tFar);
+  for (int g : (shadow ? s.sus_light : s.sus_cam)) {
+    float t, u, v;
+    if (mt_test(s, g, o, d, tNear, t, u, v) && (t < h.t || (t == h.t && h.gtri != 0xFFFFFFFFu && (uint32_t)g < h.gtri))) {
+      h.t = t; h.u = u; h.v = v; h.gtri = g;
+    }
+  }
+  if (h.gtri != 0xFFFFFFFFu) { h.geom = s.tri_mesh[h.gtri]; h.inst = 0; h.prim = h.gtri - s.to[h.geom]; }
+  return h;
 }
 
 struct Cam {
@@ -462,6 +508,9 @@ Cam make_cam(const orc_camera& c) {
   std::memcpy(r.wvp, wvp.m, sizeof r.wvp);
   return r;
 }
+
+// The camera origin every CastSingleRay of this view starts from: WorldViewInv*(0,0,0,1).
+inline f3 cam_origin(const Cam& c) { return mp(c.wv_inv, f3{0, 0, 0}); }
 
 // CastSingleRay (raytrace3d_bf.cpp:158-174) + EyeRayDirNormalized / transform_ray3f (raytrace.h:49-76)
 inline void eye_ray(const Cam& c, float x, float y, f3& o, f3& d) {
@@ -551,7 +600,7 @@ f3 shade(const Scene& s, int mode, const Lam& lam, const Cam& c, f2 sp, const st
   const float BIAS = 1e-5;
   const f3 surf = add(o, mul(h.t - BIAS, d));
   const f3 negL = mul(-1.0f, lam.L);
-  const float shadow = trace(s, surf, negL).prim == 0xFFFFFFFFu ? 1.0f : 0.0f;
+  const float shadow = trace(s, surf, negL, 0.0f, 1e9f, true).prim == 0xFFFFFFFFu ? 1.0f : 0.0f;
   const f3 lit = mul(mul(shadow, lam.Lc), fmax_std(0.0f, dot(n, negL)));
   return mulv(add(lit, lam.amb), diffuse);
 }
@@ -721,8 +770,9 @@ void orc_scene_set_tracer(void* sp, int kind) {
 int orc_grad_size(void* s, int mode) { return grad_layout(*(Scene*)s, mode, nullptr); }
 
 void orc_hits(void* sp, const orc_camera* cam, uint32_t* prim, uint32_t* geom, float* t, float* u, float* v) {
-  const Scene& s = *(Scene*)sp;
+  Scene& s = *(Scene*)sp;
   const Cam c = make_cam(*cam);
+  if (s.tracer == ORC_TRACER_BVH) suspects_origin(s, cam_origin(c));
   const int W = (int)cam->width, H = (int)cam->height;
 #pragma omp parallel for schedule(dynamic, 4)
   for (int y = 0; y < H; y++)
@@ -750,13 +800,15 @@ static Lam make_lam(const orc_lambert* l) {
 }
 
 void orc_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc_camera* cams, int views, float* images) {
-  const Scene& s = *(Scene*)sp;
+  Scene& s = *(Scene*)sp;
   const Lam L = make_lam(lam);
+  if (s.tracer == ORC_TRACER_BVH) suspects_direction(s, mul(-1.0f, L.L));
   const int sq = (int)std::sqrt((float)spp);
   const int nsamp = sq * sq;
   size_t img_off = 0;
   for (int vi = 0; vi < views; vi++) {
     const Cam c = make_cam(cams[vi]);
+    if (s.tracer == ORC_TRACER_BVH) suspects_origin(s, cam_origin(c));
     const int W = (int)cams[vi].width, H = (int)cams[vi].height;
     float* img = images + img_off;
 #pragma omp parallel for schedule(dynamic, 2)
@@ -792,8 +844,9 @@ float orc_loss(const float* image, const float* target, int npix, float* adjoint
 
 void orc_d_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc_camera* cams, int views,
                   const float* adjoints, int N, int rng_mode, float* grad) {
-  const Scene& s = *(Scene*)sp;
+  Scene& s = *(Scene*)sp;
   const Lam L = make_lam(lam);
+  if (s.tracer == ORC_TRACER_BVH) suspects_direction(s, mul(-1.0f, L.L));
   std::vector<int> moff;
   const int G = grad_layout(s, mode, &moff);
   const int sq = (int)std::sqrt((float)spp);
@@ -815,6 +868,7 @@ void orc_d_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc
   size_t img_off = 0;
   for (int vi = 0; vi < views; vi++) {
     const Cam c = make_cam(cams[vi]);
+    if (s.tracer == ORC_TRACER_BVH) suspects_origin(s, cam_origin(c));
     const int W = (int)cams[vi].width, H = (int)cams[vi].height;
     const float* adj = adjoints + img_off;
     auto adj_at = [&](int x, int y) { const float* p = adj + 3 * ((size_t)y * W + x); return f3{p[0], p[1], p[2]}; };

@@ -229,3 +229,20 @@ def test_edge_sampler_closed_form():
     L = np.linalg.norm(proj[v1].astype(np.float64) - proj[v0], axis=1)
     assert np.allclose(pmf, L / L.sum(), rtol=1e-6)
     assert np.allclose(cdf[1:], np.cumsum(L) / L.sum(), rtol=1e-6) and cdf[0] == 0
+
+
+def test_config2_bvh_equals_reference_bf():
+    """Config 2 at full size (1024^2, 20480 triangles): the oracle's BVH tracer, with the
+    clamped-determinant suspect triangles, reproduces the reference BF hit map and image."""
+    fx = load("cfg2_ico5_lambert")
+    meshes, _, cam = scene("cfg2_ico5_lambert")
+    assert mesh_hash(meshes[0]) == str(fx["mesh0_sha"])
+    sc = O.Scene(meshes, O.TRACER_BVH)
+    prim, _, t, u, v = sc.hits(cam)
+    assert sha(prim) == str(fx["hit_prim_sha"]) and sha(t) == str(fx["hit_t_sha"])
+    assert sha(u) == str(fx["hit_u_sha"]) and sha(v) == str(fx["hit_v_sha"])
+    img = sc.render(O.LAMBERT, 4, [cam])[0]
+    assert sha(img) == str(fx["image_sha"])
+    v0, v1, pmf, cdf, proj = sc.edge_sampler(cam)
+    assert sha(cdf) == str(fx["edge_cdf_sha"]) and sha(pmf) == str(fx["edge_pmf_sha"])
+    assert sha(v0) == str(fx["edge_v0_sha"]) and int(fx["n_edges"]) == 30720

@@ -50,8 +50,6 @@ def main():
               "per GPU; see DESIGN.md for how it shards)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     from diffrender_tutorials_amd import drh, scenes as S
 
     c = S.config2()
@@ -72,18 +70,17 @@ def main():
     r.commit(meshes)
     P = r.grad_size()
     r.opt_init(drh.OPT_ADAM, 1e-3, 1e-3)
-    d_target = torch.tensor(target.reshape(-1), device="cuda")
-    d_loss = torch.zeros(1, device="cuda")
-    torch.cuda.synchronize()
+    d_target = r.device_array(target.reshape(-1))
+    d_loss = r.device_array(np.zeros(1, np.float32))
     cams = drh.cameras_array([cam])
 
     def step():
-        r.opt_step_dev(cams, 1, d_target.data_ptr(), N, d_loss.data_ptr())
+        r.opt_step_dev(cams, 1, d_target.ptr, N, d_loss.ptr)
 
     for _ in range(args.warmup):
         step()
     r.sync()
-    loss0 = float(d_loss.cpu()[0])
+    loss0 = float(d_loss.download(1)[0])
 
     r.profile(True)
     r.sync()
@@ -92,7 +89,7 @@ def main():
         step()
     r.sync()
     t1 = time.perf_counter()
-    loss1 = float(d_loss.cpu()[0])
+    loss1 = float(d_loss.download(1)[0])
     dt = t1 - t0
     ms = 1e3 * dt / args.steps
 

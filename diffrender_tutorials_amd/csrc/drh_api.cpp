@@ -1007,3 +1007,38 @@ drh_status drh_debug_edges(drh_handle h, int* n_edges, int* v0, int* v1, float* 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+drh_status drh_malloc(drh_handle h, size_t bytes, void** d_ptr) {
+  if (!h || !d_ptr) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  *d_ptr = nullptr;
+  return h->hip(hipMalloc(d_ptr, std::max<size_t>(bytes, 4)), "drh_malloc");
+}
+
+drh_status drh_free(drh_handle h, void* d_ptr) {
+  if (!h) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  drh_status s = h->hip(hipStreamSynchronize(h->stream), "drh_free");
+  if (s) return s;
+  return d_ptr ? h->hip(hipFree(d_ptr), "drh_free") : DRH_OK;
+}
+
+drh_status drh_memcpy_htod(drh_handle h, void* d_dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!d_dst || !src))) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  drh_status s = h->hip(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, h->stream), "drh_memcpy_htod");
+  if (s) return s;
+  return h->hip(hipStreamSynchronize(h->stream), "drh_memcpy_htod");
+}
+
+drh_status drh_memcpy_dtoh(drh_handle h, void* dst, const void* d_src, size_t bytes) {
+  if (!h || (bytes && (!dst || !d_src))) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  drh_status s = h->hip(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, h->stream), "drh_memcpy_dtoh");
+  if (s) return s;
+  return h->hip(hipStreamSynchronize(h->stream), "drh_memcpy_dtoh");
+}
+
+}  // extern "C"

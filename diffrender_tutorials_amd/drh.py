@@ -32,7 +32,7 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params", "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
-           "drh_debug_edges"]
+           "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh"]
 
 
 class DrhError(RuntimeError):
@@ -81,6 +81,10 @@ def load():
             if n not in ("drh_last_error", "drh_stream", "drh_destroy"):
                 f.restype = ctypes.c_int
         L.drh_destroy.restype = None
+        L.drh_malloc.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+        L.drh_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.drh_memcpy_htod.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.drh_memcpy_dtoh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -241,11 +245,48 @@ class DiffRender:
         self._check(load().drh_profile_read(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
 
+    # -- device memory plumbing ------------------------------------------------
+    def device_array(self, a):
+        """Upload a float32 numpy array; returns a DeviceBuffer (freed with the buffer)."""
+        a = np.ascontiguousarray(a, np.float32)
+        b = DeviceBuffer(self, a.nbytes)
+        b.upload(a)
+        return b
+
     def sync(self):
         self._check(load().drh_sync(self.h))
 
     def stream(self):
         return load().drh_stream(self.h)
+
+
+class DeviceBuffer:
+    """HBM allocation on a renderer's device (drh_malloc / drh_free)."""
+
+    def __init__(self, r: DiffRender, nbytes: int):
+        self.r = r
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        r._check(load().drh_malloc(r.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.r._check(load().drh_memcpy_htod(self.r.h, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        self.r._check(load().drh_memcpy_dtoh(self.r.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and getattr(self.r, "h", None) is not None:
+            load().drh_free(self.r.h, self.ptr)
+        self.ptr = None
+
+    __del__ = free
 
 
 def _split(flat, cams):

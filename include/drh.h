@@ -21,6 +21,7 @@
  */
 #ifndef DRH_H
 #define DRH_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -134,6 +135,14 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
                         int edge_samples, float* d_loss);
 /* Download the current (optimised) vertex positions (3V) and colours (3V, may be NULL). */
 drh_status drh_get_params(drh_handle h, float* pos, float* col);
+
+/* Device memory plumbing (no counterpart in the reference): for callers that hold no HIP
+ * allocator of their own, e.g. Python tests and the benchmark.  Allocations are on the
+ * handle's device; copies are ordered on its stream and complete before return. */
+drh_status drh_malloc(drh_handle h, size_t bytes, void** d_ptr);
+drh_status drh_free(drh_handle h, void* d_ptr);
+drh_status drh_memcpy_htod(drh_handle h, void* d_dst, const void* src, size_t bytes);
+drh_status drh_memcpy_dtoh(drh_handle h, void* dst, const void* d_src, size_t bytes);
 
 /* The handle's HIP stream (a hipStream_t) and device synchronisation of it. */
 void*      drh_stream(drh_handle h);

@@ -177,25 +177,23 @@ def test_config2_full_size_gradient_matches_oracle():
 def test_optimizer_step_matches_reference():
     """One device-resident OptSimple step (commit, d_render_and_compare, OptStep,
     OptUpdateScene) against the reference's OptSimple run."""
-    import torch
     fx = load("cfg1_cube_vcol")
     meshes, targets, cam = scene("cfg1_cube_vcol")
     tgt = O.Scene(targets).render(2, 1, [cam])[0]
     r = renderer(2, 1)
     r.commit(meshes)
     r.opt_init(drh.OPT_SIMPLE, 0.1, 0.1)
-    dt = torch.tensor(tgt.reshape(-1), device="cuda")
-    dl = torch.zeros(1, device="cuda")
-    torch.cuda.synchronize()
+    dt = r.device_array(tgt.reshape(-1))
+    dl = r.device_array(np.zeros(1, np.float32))
     cams = drh.cameras_array([cam])
-    r.opt_step_dev(cams, 1, dt.data_ptr(), cam.width * cam.height, dl.data_ptr())
+    r.opt_step_dev(cams, 1, dt.ptr, cam.width * cam.height, dl.ptr)
     r.sync()
     pos, col = r.get_params()
     assert np.allclose(pos.reshape(-1), fx["opt_pos"], rtol=0, atol=1e-6)
     assert np.allclose(col.reshape(-1), fx["opt_col"], rtol=0, atol=1e-6)
     img = O.Scene(meshes).render(2, 1, [cam])[0]
     l, _ = O.loss(img, tgt)
-    assert np.isclose(float(dl.cpu()[0]), l / (cam.width * cam.height), rtol=1e-5)
+    assert np.isclose(float(dl.download(1)[0]), l / (cam.width * cam.height), rtol=1e-5)
 
 
 def test_multiview_matches_oracle():

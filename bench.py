@@ -44,7 +44,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 3 times (k_copy_f32), the known-size reference "
+                         "for FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from diffrender_tutorials_amd import drh
+        r = drh.DiffRender(mode=drh.VERTEX_COLOR, spp=1)
+        n = 64 * 1024 * 1024
+        a, b = drh.DeviceBuffer(r, 4 * n), drh.DeviceBuffer(r, 4 * n)
+        for _ in range(3):
+            r._check(drh.load().drh_debug_copy(r.h, b.ptr, a.ptr, n))
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path runs one job "
               "per GPU; see DESIGN.md for how it shards)", file=sys.stderr)

@@ -1042,3 +1042,10 @@ drh_status drh_memcpy_dtoh(drh_handle h, void* dst, const void* d_src, size_t by
 }
 
 }  // extern "C"
+
+extern "C" drh_status drh_debug_copy(drh_handle h, float* d_dst, const float* d_src, long long n) {
+  if (!h || n < 0 || (n && (!d_dst || !d_src))) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  h->timed("k_copy_f32", [&] { launch_copy_f32(d_dst, d_src, n, h->stream); });
+  return h->hip(hipStreamSynchronize(h->stream), "debug_copy");
+}

@@ -194,6 +194,7 @@ struct HitsArgs {
 
 void launch_fill_f32(float* p, long long n, float v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t s);
+void launch_copy_f32(float* d, const float* src, long long n, hipStream_t s);
 void launch_prepare(const PrepareArgs& a, hipStream_t s);
 void launch_tri_setup(const TriSetupArgs& a, hipStream_t s);
 void launch_morton(const MortonArgs& a, hipStream_t s);

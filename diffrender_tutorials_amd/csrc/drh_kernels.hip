@@ -38,6 +38,9 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 __global__ void k_fill_f32(float* p, long long n, float v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_copy_f32(float4* __restrict__ d, const float4* __restrict__ s, long long n4) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x) d[i] = s[i];
+}
 __global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -739,6 +742,13 @@ void launch_fill_f32(float* p, long long n, float v, hipStream_t s) {
   if (n <= 0) return;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+void launch_copy_f32(float* d, const float* src, long long n, hipStream_t s) {
+  // n is a multiple of 4 floats (16 B) for the calibration buffers
+  const long long n4 = n / 4;
+  if (n4 <= 0) return;
+  const int blocks = (int)std::min<long long>((n4 + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_copy_f32, dim3(blocks), dim3(256), 0, s, (float4*)d, (const float4*)src, n4);
 }
 void launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t s) {
   if (n <= 0) return;

@@ -32,7 +32,8 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params", "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
-           "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh"]
+           "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
+           "drh_debug_copy"]
 
 
 class DrhError(RuntimeError):
@@ -85,6 +86,7 @@ def load():
         L.drh_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.drh_memcpy_htod.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.drh_memcpy_dtoh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.drh_debug_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
         _lib = L
     return _lib
 

@@ -37,6 +37,16 @@ def algorithmic_bytes(kernel, F, V, E, W, H, spp_sq, N, P, tiles):
     return None
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/traffic.json, copy-calibrated), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)["bytes_per_launch"].get(kernel)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,7 +154,7 @@ def main():
         "loss_first_last": [loss0, loss1],
         "kernels": {k: round(v["ms_per_launch"], 5) for k, v in prof.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom),
                      "algorithmic_bytes": bytes_dom},
         "cpu_baseline": None,
     }

@@ -115,7 +115,8 @@ def main():
     dt = t1 - t0
     ms = 1e3 * dt / args.steps
 
-    kernels = ["k_render", "k_interior", "k_edge_sample", "k_cdf", "k_radix_sort", "k_refit", "k_karras",
+    kernels = ["k_render", "k_interior", "k_edge_sample", "k_cdf_par", "k_edge_bin", "k_edge_scan", "k_edge_scatter",
+               "k_suspects", "k_radix_sort", "k_refit", "k_karras",
                "k_tri_setup", "k_morton", "k_reorder", "k_pack_nodes", "k_prepare", "k_project", "k_edge_len",
                "k_loss_final", "k_fill_f32", "k_fill_u32", "k_adam"]
     prof = {}

@@ -217,6 +217,7 @@ struct drh_renderer {
   bool light_sus = false;
   // edges
   DBuf edges, edge_mesh, pmf_raw, pmf, cdf, proj;
+  DBuf ekey, ebin, eorder;   // edge-sample binning (sort by sampled edge)
   // sampling
   DBuf rnd, qmc, hamm;
   long long rnd_n = -1;
@@ -635,7 +636,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>()};
       timed("k_project", [&] { launch_project(pa, stream); });
       timed("k_edge_len", [&] { launch_edge_len(edges.as<int2>(), E, proj.as<float4>(), pmf_raw.as<float>(), stream); });
-      timed("k_cdf", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), stream); });
+      timed("k_cdf_par", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), stream, true); });
       EdgeArgs ea{};
       ea.tr = tracer_view(v); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
       ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
@@ -649,8 +650,16 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       }
       ea.single_mesh = single >= 0 ? 1 : 0;
       ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
-      const int blocks = (int)std::max<long long>(1, std::min<long long>((N + 255) / 256, 1024));
-      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, deep(), ea, blocks, stream); });
+      if ((s = hip(ekey.reserve(4 * (size_t)N), "alloc")) || (s = hip(eorder.reserve(4 * (size_t)N), "alloc")) ||
+          (s = hip(ebin.reserve(4 * (size_t)(E + 1)), "alloc")))
+        return s;
+      ea.ekey = ekey.as<int>(); ea.ebin = ebin.as<int>(); ea.order = eorder.as<int>();
+      const int blocks = (int)std::max<long long>(1, std::min<long long>((N + 255) / 256, 2048));
+      timed("k_fill_u32", [&] { launch_fill_u32(ebin.as<uint32_t>(), E + 1, 0u, stream); });
+      timed("k_edge_bin", [&] { launch_edge_bin(ea, blocks, stream); });
+      timed("k_edge_scan", [&] { launch_scan_u32(ebin.as<uint32_t>(), E + 1, stream); });
+      timed("k_edge_scatter", [&] { launch_edge_scatter(ea, blocks, stream); });
+      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, deep(), ea, std::min(blocks, 1024), stream); });
     }
   }
   return hip(hipGetLastError(), "d_render kernels");
@@ -1048,4 +1057,22 @@ extern "C" drh_status drh_debug_copy(drh_handle h, float* d_dst, const float* d_
   (void)hipSetDevice(h->st.device);
   h->timed("k_copy_f32", [&] { launch_copy_f32(d_dst, d_src, n, h->stream); });
   return h->hip(hipStreamSynchronize(h->stream), "debug_copy");
+}
+
+extern "C" drh_status drh_debug_cdf(drh_handle h, const float* pmf_raw, int E, float* pmf_out, float* cdf_out,
+                                    int parallel) {
+  if (!h || E <= 0 || !pmf_raw || !pmf_out || !cdf_out) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  DBuf a, p, c;
+  drh_status s;
+  if ((s = h->hip(a.reserve(4 * (size_t)E), "alloc")) || (s = h->hip(p.reserve(4 * (size_t)E), "alloc")) ||
+      (s = h->hip(c.reserve(4 * (size_t)(E + 1)), "alloc")))
+    return s;
+  if ((s = h->hip(hipMemcpyAsync(a.p, pmf_raw, 4 * (size_t)E, hipMemcpyHostToDevice, h->stream), "upload"))) return s;
+  h->timed(parallel ? "k_cdf_par" : "k_cdf", [&] { launch_cdf(a.as<float>(), E, p.as<float>(), c.as<float>(), h->stream, parallel != 0); });
+  if ((s = h->hip(hipGetLastError(), "debug_cdf"))) return s;
+  if ((s = h->hip(hipMemcpyAsync(pmf_out, p.p, 4 * (size_t)E, hipMemcpyDeviceToHost, h->stream), "download")) ||
+      (s = h->hip(hipMemcpyAsync(cdf_out, c.p, 4 * (size_t)(E + 1), hipMemcpyDeviceToHost, h->stream), "download")))
+    return s;
+  return h->hip(hipStreamSynchronize(h->stream), "debug_cdf");
 }

@@ -11,7 +11,8 @@
 namespace drh {
 
 constexpr int RADIX_TILE = 2048;           // keys per radix-sort block (8 rounds of 256)
-constexpr int CDF_CHUNK = 8192;            // pmf floats staged in LDS per CDF round
+constexpr int CDF_CHUNK = 8192;            // pmf floats staged in LDS per CDF round (sequential CDF)
+constexpr int CDF_PAR_THREADS = 256;       // chunks of the parallel exact CDF
 constexpr int INTERIOR_LDS_FLOATS = 4096;  // largest gradient target accumulated in LDS (interior)
 constexpr int EDGE_LDS_FLOATS = 4096;      // largest position gradient accumulated in LDS (edges)
 constexpr int DEEP_F = 1 << 18;            // above this many triangles use the 64-deep traversal stack
@@ -159,6 +160,9 @@ struct EdgeArgs {
   const float* tr_inv0;   // 16 floats
   int lds_base, lds_n;
   int single_mesh, single_tr_off;
+  int* ekey;              // N: sampled edge of each sample, E = rejected
+  int* ebin;              // E+1: counts, then offsets, then ends; ebin[E] = accepted samples
+  int* order;             // N: accepted sample indices sorted by edge
 };
 
 constexpr int MAX_SEG = 8;
@@ -210,8 +214,11 @@ void launch_loss_final(const double* part, int n, float* out, double denom, hipS
 void launch_interior(int mode, bool deep, const InteriorArgs& a, int blocks, hipStream_t s);
 void launch_project(const ProjectArgs& a, hipStream_t s);
 void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s);
-void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s);
+void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s, bool parallel = true);
 void launch_edge_sample(int mode, bool deep, const EdgeArgs& a, int blocks, hipStream_t s);
+void launch_edge_bin(const EdgeArgs& a, int blocks, hipStream_t s);
+void launch_edge_scatter(const EdgeArgs& a, int blocks, hipStream_t s);
+void launch_scan_u32(uint32_t* p, int n, hipStream_t s);
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);
 void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_debug_hits(bool deep, const HitsArgs& a, int tiles, hipStream_t s);

@@ -541,6 +541,88 @@ __global__ __launch_bounds__(1024) void k_cdf(const float* pmf_raw, int E, float
   for (int i = threadIdx.x; i <= E; i += 1024) cdf[i] = (i == 0 ? 0.0f : cdf[i]) / total;
 }
 
+// The same sequential float running sum, computed in parallel and exactly.
+// Chunk j (of CDF_PAR_THREADS) starts from a speculated value s_j = float(double prefix) and
+// runs the float recurrence c <- p + c over its range.  If the true start T_j differs, the
+// true sequence is the speculated one shifted by d = T_j - s_j whenever every speculated value
+// stays in one binade (one grid spacing u), no addition was a rounding tie (TwoSum error
+// exactly u/2; ties-to-even depends on parity, which an odd shift flips), and the shifted
+// end stays in the binade: round-to-nearest commutes with adding a grid multiple.  One lane
+// walks the chunks applying shifts (recomputing a chunk where the conditions fail); then every
+// chunk is recomputed from its true start and written out normalised by the total.
+__device__ __forceinline__ int fexp(float x) { return (int)((__float_as_uint(x) >> 23) & 0xFFu); }
+
+__global__ __launch_bounds__(CDF_PAR_THREADS) void k_cdf_par(const float* __restrict__ pmf_raw, int E,
+                                                               float* __restrict__ pmf, float* __restrict__ cdf) {
+  __shared__ double dsum[CDF_PAR_THREADS];
+  __shared__ float spec_start[CDF_PAR_THREADS], spec_end[CDF_PAR_THREADS], true_start[CDF_PAR_THREADS + 1];
+  __shared__ int ok[CDF_PAR_THREADS];
+  const int j = threadIdx.x;
+  const int C = (E + CDF_PAR_THREADS - 1) / CDF_PAR_THREADS;
+  const int b = min(j * C, E), e = min(b + C, E);
+  // speculated starts: exclusive prefix of chunk sums in double
+  double s = 0.0;
+  for (int k = b; k < e; k++) s += (double)pmf_raw[k];
+  dsum[j] = s;
+  __syncthreads();
+  for (int o = 1; o < CDF_PAR_THREADS; o <<= 1) {
+    const double v = j >= o ? dsum[j - o] : 0.0;
+    __syncthreads();
+    dsum[j] += v;
+    __syncthreads();
+  }
+  const float s0 = j == 0 ? 0.0f : (float)dsum[j - 1];
+  float c = s0;
+  int good = 1;
+  const int ex = fexp(s0);
+  for (int k = b; k < e; k++) {
+    const float p = pmf_raw[k];
+    const float sum = p + c;
+    // TwoSum: err = (p + c) - sum exactly
+    const float bb = sum - c;
+    const float aa = sum - bb;
+    const float err = (p - bb) + (c - aa);
+    const float u = __uint_as_float((uint32_t)max(ex - 23, 1) << 23);  // grid spacing of the binade
+    if (fabsf(err) * 2.0f == u && err != 0.0f) good = 0;
+    c = sum;
+  }
+  // one binade for the whole chunk; tiny (grid below 2^-126) and non-finite values recompute
+  if (fexp(c) != ex || ex <= 24 || ex == 255) good = 0;
+  spec_start[j] = s0;
+  spec_end[j] = c;
+  ok[j] = good;
+  __syncthreads();
+  if (j == 0) {
+    float T = 0.0f;
+    for (int q = 0; q < CDF_PAR_THREADS; q++) {
+      true_start[q] = T;
+      const float ss = spec_start[q], se = spec_end[q];
+      bool done = false;
+      if (T == ss) { T = se; done = true; }
+      else if (ok[q] && fexp(T) == fexp(ss)) {
+        const float d = T - ss;       // exact: both on the binade's grid
+        const float te = se + d;
+        if (fexp(te) == fexp(ss)) { T = te; done = true; }
+      }
+      if (!done) {
+        const int qb = min(q * C, E), qe = min(qb + C, E);
+        for (int k = qb; k < qe; k++) T = pmf_raw[k] + T;
+      }
+    }
+    true_start[CDF_PAR_THREADS] = T;
+  }
+  __syncthreads();
+  const float total = true_start[CDF_PAR_THREADS];
+  c = true_start[j];
+  for (int k = b; k < e; k++) {
+    const float p = pmf_raw[k];
+    c = p + c;
+    cdf[k + 1] = c / total;
+    pmf[k] = p / total;
+  }
+  if (j == 0) cdf[0] = 0.0f / total;
+}
+
 // qmc::rndFloat (common_src/qmc.cpp:190-197)
 __device__ __forceinline__ float qmc_rnd(uint32_t pos, int dim, const uint32_t* table) {
   uint32_t r = 0, d = pos;
@@ -561,6 +643,67 @@ __device__ __forceinline__ int sample_edge(const float* cdf, int E, float u) {
   return clampi(first - 1, 0, E - 1);
 }
 
+// Edge samples are traced in edge order: a counting sort by the sampled edge (the edge is a
+// monotone function of rnd0 through the CDF) makes the rays of a wavefront coherent and lets
+// the gradient of every run of equal edges be reduced in the wave before one atomic.
+// Pass 1: sampled edge and the reference's rejection tests (off-screen pixel, |adjoint| <
+// 1e-3, src/drender.h:338-345); rejected samples go to bin E.
+__global__ __launch_bounds__(256) void k_edge_bin(EdgeArgs a) {
+  const DevCam& c = a.cams[a.view];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.N; i += stride) {
+    float rnd0, rnd1;
+    if (a.rnd) { const float2 r = a.rnd[i]; rnd0 = r.x; rnd1 = r.y; }
+    else { rnd0 = qmc_rnd((uint32_t)i, 0, a.qmc); rnd1 = qmc_rnd((uint32_t)i, 1, a.qmc); }
+    const int eid = sample_edge(a.cdf, a.E, rnd0);
+    const int2 ev = a.edges[eid];
+    const float4 P0 = a.proj[ev.x], P1 = a.proj[ev.y];
+    const float t = rnd1;
+    const float px = P0.x + t * (P1.x - P0.x), py = P0.y + t * (P1.y - P0.y);
+    int key = a.E;
+    // int() truncation; NaN and out-of-range values are off-screen (as x86 cvttss2si's INT_MIN)
+    if (px > -1.0f && py > -1.0f && px < (float)c.w && py < (float)c.h) {
+      const int xi = (int)px, yi = (int)py;
+      if (xi >= 0 && yi >= 0 && xi < c.w && yi < c.h) {
+        const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)yi * c.w + xi));
+        if (!((double)length(adj) < 1e-3)) key = eid;
+      }
+    }
+    a.ekey[i] = key;
+    atomicAdd(a.ebin + key, 1);
+  }
+}
+
+// Pass 2 (after an exclusive scan of the bins): accepted samples to their edge's range.
+__global__ __launch_bounds__(256) void k_edge_scatter(EdgeArgs a) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.N; i += stride) {
+    const int key = a.ekey[i];
+    if (key < a.E) a.order[atomicAdd(a.ebin + key, 1)] = (int)i;
+  }
+}
+
+// Sum `v` over each run of equal `key` in the wavefront (lanes hold consecutive sorted
+// samples); afterwards the last lane of a run holds the run's total.
+template <int K>
+__device__ __forceinline__ bool seg_reduce(int key, float (&v)[K]) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const int next = __shfl_down(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll((long long)(heads & upto));
+  for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const float o = __shfl_up(v[q], off, 64);
+      if (lane - off >= start) v[q] += o;
+    }
+  }
+  return lane == 63 || next != key;
+}
+
 template <int MODE, int DEPTH>
 __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
   __shared__ int stack[4 * DEPTH * 64];
@@ -579,22 +722,24 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
   __syncthreads();
   const float* Ti = a.tr_inv0;  // mesh 0's inverse transform for every mesh (drender.cpp:60)
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.N; i += stride) {
-    float rnd0, rnd1;
-    if (a.rnd) { const float2 r = a.rnd[i]; rnd0 = r.x; rnd1 = r.y; }
-    else { rnd0 = qmc_rnd((uint32_t)i, 0, a.qmc); rnd1 = qmc_rnd((uint32_t)i, 1, a.qmc); }
-    const int eid = sample_edge(a.cdf, a.E, rnd0);
+  const long long nacc = a.ebin[a.E];  // accepted samples: the rejected bin starts here
+  // whole wavefronts step together (the run reduction needs all 64 lanes)
+  for (long long wbase = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < nacc; wbase += stride) {
+    const long long slot = wbase + lane;
+    float gsum[6] = {0, 0, 0, 0, 0, 0};
+    int key = -1 - lane;   // lanes without a contribution form runs of their own
+    int vn0 = 0, vn1 = 0, goff = -1;
+    if (slot < nacc) {
+    const long long i = a.order[slot];
+    const int eid = a.ekey[i];
+    const float rnd1 = a.rnd ? a.rnd[i].y : qmc_rnd((uint32_t)i, 1, a.qmc);
     const int2 ev = a.edges[eid];
     const float pmf = a.pmf[eid];
     const float4 P0 = a.proj[ev.x], P1 = a.proj[ev.y];
     const float t = rnd1;
     const float px = P0.x + t * (P1.x - P0.x), py = P0.y + t * (P1.y - P0.y);
-    // int() truncation; NaN and out-of-range values are off-screen (as x86 cvttss2si's INT_MIN)
-    if (!(px > -1.0f) || !(py > -1.0f) || !(px < (float)c.w) || !(py < (float)c.h)) continue;
-    const int xi = (int)px, yi = (int)py;
-    if (xi < 0 || yi < 0 || xi >= c.w || yi >= c.h) continue;
+    const int xi = (int)px, yi = (int)py;   // accepted in k_edge_bin: on screen
     const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)yi * c.w + xi));
-    if ((double)length(adj) < 1e-3) continue;
     const float dvx = P1.x - P0.x, dvy = P1.y - P0.y;
     const float len = sqrtf(dvx * dvx + dvy * dvy);
     const float ndx = dvx / len, ndy = dvy / len;
@@ -604,7 +749,7 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     const float pdf = pmf / len;
     const float weight = 1.0f / (pdf * float(a.N));
     const float adjv = dot(sub(cin, cout), adj);
-    if (!(adjv * adjv > 0.0f)) continue;
+    if (adjv * adjv > 0.0f) {
     const float d0x = ((1 - t) * nx) * adjv * weight, d0y = ((1 - t) * ny) * adjv * weight;
     const float d1x = (t * nx) * adjv * weight, d1y = (t * ny) * adjv * weight;
     // edge_grad (src/drender.cpp:31-87): diagonal screen Jacobian, as the reference
@@ -614,20 +759,17 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     const float dv0_dx = g0x.x * d0x, dv0_dy = g0y.y * d0y, dv0_dz = (g0x.z * d0x + g0y.z * d0y);
     const float dv1_dx = g1x.x * d1x, dv1_dy = g1y.y * d1y, dv1_dz = (g1x.z * d1x + g1y.z * d1y);
     const int m = a.edge_mesh[eid];
-    const int goff = a.layout[4 * m + 0];
-    if (goff < 0) continue;
+    goff = a.layout[4 * m + 0];
+    if (goff >= 0) {
     const int vo = a.tr.s.mesh_vo[m];
-    const int vn0 = ev.x - vo, vn1 = ev.y - vo;
-    const float g0[3] = {Ti[0] * dv0_dx + Ti[4] * dv0_dy + Ti[8] * dv0_dz,
-                         Ti[1] * dv0_dx + Ti[5] * dv0_dy + Ti[9] * dv0_dz,
-                         Ti[2] * dv0_dx + Ti[6] * dv0_dy + Ti[10] * dv0_dz};
-    const float g1[3] = {Ti[0] * dv1_dx + Ti[4] * dv1_dy + Ti[8] * dv1_dz,
-                         Ti[1] * dv1_dx + Ti[5] * dv1_dy + Ti[9] * dv1_dz,
-                         Ti[2] * dv1_dx + Ti[6] * dv1_dy + Ti[10] * dv1_dz};
-    float* G = use_lds ? lpos : a.grad;
-    const int gb = use_lds ? goff - a.lds_base : goff;
-    atomicAdd(G + gb + 3 * vn0, g0[0]); atomicAdd(G + gb + 3 * vn0 + 1, g0[1]); atomicAdd(G + gb + 3 * vn0 + 2, g0[2]);
-    atomicAdd(G + gb + 3 * vn1, g1[0]); atomicAdd(G + gb + 3 * vn1 + 1, g1[1]); atomicAdd(G + gb + 3 * vn1 + 2, g1[2]);
+    vn0 = ev.x - vo; vn1 = ev.y - vo;
+    key = eid;
+    gsum[0] = Ti[0] * dv0_dx + Ti[4] * dv0_dy + Ti[8] * dv0_dz;
+    gsum[1] = Ti[1] * dv0_dx + Ti[5] * dv0_dy + Ti[9] * dv0_dz;
+    gsum[2] = Ti[2] * dv0_dx + Ti[6] * dv0_dy + Ti[10] * dv0_dz;
+    gsum[3] = Ti[0] * dv1_dx + Ti[4] * dv1_dy + Ti[8] * dv1_dz;
+    gsum[4] = Ti[1] * dv1_dx + Ti[5] * dv1_dy + Ti[9] * dv1_dz;
+    gsum[5] = Ti[2] * dv1_dx + Ti[6] * dv1_dy + Ti[10] * dv1_dz;
     const F3 o0 = ld3(a.tr.s.pos_orig + 3 * ev.x), o1 = ld3(a.tr.s.pos_orig + 3 * ev.y);
     const float tv[12] = {o0.x * dv0_dx + o1.x * dv1_dx, o0.y * dv0_dx + o1.y * dv1_dx,
                           o0.z * dv0_dx + o1.z * dv1_dx, 1 * dv0_dx + 1 * dv1_dx,
@@ -642,6 +784,17 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
       float* gt = a.grad + goff + a.layout[4 * m + 3];
 #pragma unroll
       for (int q = 0; q < 12; q++) atomicAdd(gt + q, tv[q]);
+    }
+    }  // differentiable mesh
+    }  // adj != 0
+    }  // live slot
+    // one atomic per run of equal edges for the position gradient of its two vertices
+    const bool tail = seg_reduce<6>(key, gsum);
+    if (tail && key >= 0) {
+      float* G = use_lds ? lpos : a.grad;
+      const int gb = use_lds ? goff - a.lds_base : goff;
+      atomicAdd(G + gb + 3 * vn0, gsum[0]); atomicAdd(G + gb + 3 * vn0 + 1, gsum[1]); atomicAdd(G + gb + 3 * vn0 + 2, gsum[2]);
+      atomicAdd(G + gb + 3 * vn1, gsum[3]); atomicAdd(G + gb + 3 * vn1 + 1, gsum[4]); atomicAdd(G + gb + 3 * vn1 + 2, gsum[5]);
     }
   }
   if (a.single_mesh) {
@@ -805,11 +958,21 @@ void launch_project(const ProjectArgs& a, hipStream_t s) {
 void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s) {
   hipLaunchKernelGGL(k_edge_len, dim3((E + 255) / 256), dim3(256), 0, s, edges, E, proj, pmf_raw);
 }
-void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s) {
-  hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
+void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s, bool parallel) {
+  if (parallel) hipLaunchKernelGGL(k_cdf_par, dim3(1), dim3(CDF_PAR_THREADS), 0, s, pmf_raw, E, pmf, cdf);
+  else hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
 }
 void launch_edge_sample(int mode, bool deep, const EdgeArgs& a, int blocks, hipStream_t s) {
   DRH_DISPATCH_MODE(mode, DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_edge_sample<M, D>), dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a)))
+}
+void launch_edge_bin(const EdgeArgs& a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_edge_bin, dim3(blocks), dim3(256), 0, s, a);
+}
+void launch_edge_scatter(const EdgeArgs& a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_edge_scatter, dim3(blocks), dim3(256), 0, s, a);
+}
+void launch_scan_u32(uint32_t* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, p, n);
 }
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s) {
   const int b = (a.n + 255) / 256;

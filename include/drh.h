@@ -161,6 +161,10 @@ drh_status drh_debug_hits(drh_handle h, const drh_camera* cam, uint32_t* prim, u
                           float* t, float* u, float* v);
 drh_status drh_debug_bvh(drh_handle h, uint32_t* morton_sorted, uint32_t* tri_order, float* scene_bounds);
 drh_status drh_debug_edges(drh_handle h, int* n_edges, int* v0, int* v1, float* pmf, float* cdf);
+/* The edge sampler's CDF over an arbitrary pmf (host arrays): pmf_out = pmf_raw / total,
+ * cdf_out[0..E] = sequential float running sum / total, by the parallel exact kernel
+ * (parallel != 0) or the single-lane chain. */
+drh_status drh_debug_cdf(drh_handle h, const float* pmf_raw, int E, float* pmf_out, float* cdf_out, int parallel);
 /* A plain device-to-device copy kernel (k_copy_f32) of n floats: the known-size reference
  * that hardware traffic counters (FETCH_SIZE / WRITE_SIZE) are calibrated against. */
 drh_status drh_debug_copy(drh_handle h, float* d_dst, const float* d_src, long long n);

@@ -232,3 +232,60 @@ def test_error_paths():
     with pytest.raises(drh.DrhError) as e:
         lam.commit([S.cube()])  # LAMBERT reads texture 0
     assert e.value.status == -5
+
+
+def _seq_cdf(p):
+    """The reference's running sum c <- p + c in float32 (build_edge_sampler,
+    src/drender.cpp:12-17), then both arrays divided by the total."""
+    c = np.add.accumulate(np.asarray(p, np.float32), dtype=np.float32)
+    total = c[-1]
+    return (p / total).astype(np.float32), np.concatenate([[np.float32(0) / total], c / total]).astype(np.float32)
+
+
+def test_cpu_reference_running_sum_is_sequential():
+    p = np.random.default_rng(0).uniform(0, 1, 1000).astype(np.float32)
+    c, out = np.float32(0), []
+    for x in p:
+        c = np.float32(x + c)
+        out.append(c)
+    assert np.array_equal(np.add.accumulate(p, dtype=np.float32), np.array(out, np.float32))
+
+
+@pytest.mark.parametrize("dist", ["uniform", "integer", "wide", "zeros", "lengths"])
+@pytest.mark.parametrize("E", [1, 2, 255, 256, 257, 1000, 30720, 300001])
+def test_parallel_cdf_is_bit_exact(dist, E):
+    rng = np.random.default_rng(E * 7 + len(dist))
+    if dist == "uniform":
+        p = rng.uniform(0, 1, E)
+    elif dist == "integer":     # many rounding ties
+        p = rng.integers(0, 64, E).astype(np.float64) * 0.5
+    elif dist == "wide":
+        p = np.exp(rng.uniform(-30, 10, E))
+    elif dist == "zeros":
+        p = rng.uniform(0, 1, E) * (rng.uniform(0, 1, E) < 0.3)
+    else:
+        p = np.abs(rng.normal(3.0, 2.0, E))
+    p = p.astype(np.float32)
+    if p.sum() == 0:
+        p[-1] = 1.0
+    ref_pmf, ref_cdf = _seq_cdf(p)
+    r = renderer(2, 1)
+    L = drh.load()
+    for par in (1, 0):
+        pmf = np.empty(E, np.float32)
+        cdf = np.empty(E + 1, np.float32)
+        r._check(L.drh_debug_cdf(r.h, drh._fp(p), E, drh._fp(pmf), drh._fp(cdf), par))
+        assert np.array_equal(pmf, ref_pmf), (par, dist, E)
+        assert np.array_equal(cdf, ref_cdf), (par, dist, E)
+
+
+def test_config2_device_cdf_matches_reference():
+    fx = load("cfg2_ico5_lambert")
+    meshes, _, cam = scene("cfg2_ico5_lambert")
+    r = renderer(4, 4)
+    r.commit(meshes)
+    adj = np.zeros((cam.height, cam.width, 3), np.float32)
+    r.d_render([cam], [adj], 1 << 10)
+    v0, v1, pmf, cdf = r.edges()
+    assert sha(cdf) == str(fx["edge_cdf_sha"]) and sha(pmf) == str(fx["edge_pmf_sha"])
+    assert sha(v0) == str(fx["edge_v0_sha"]) and sha(v1) == str(fx["edge_v1_sha"])

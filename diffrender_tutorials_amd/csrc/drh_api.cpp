@@ -235,7 +235,8 @@ struct drh_renderer {
   std::map<std::string, std::pair<double, int>> prof_acc;
   std::vector<hipEvent_t> event_pool;
 
-  bool deep() const { return F > DEEP_F; }
+  // traversal stack kind (see StackCfg): 0 = 16-bit entries, 1 = 32-bit, 2 = 32-bit and 64 deep
+  int deep() const { return F > DEEP_F ? 2 : (F > 65535 ? 1 : 0); }
 
   drh_status fail(drh_status s, const char* fmt, ...) {
     char buf[512];
@@ -293,7 +294,7 @@ struct drh_renderer {
     s.n_meshes = nm;
     t.bf_a = bf_a.as<float4>(); t.bf_e1 = bf_e1.as<float4>(); t.bf_e2 = bf_e2.as<float4>();
     t.use_bf = st.tracer == DRH_TRACER_BF;
-    t.depth = deep() ? 64 : 48;  // matches the kernel's LDS stack (DEPTH template argument)
+    t.depth = deep() == 2 ? 64 : 48;  // matches the kernel's LDS stack (DEPTH template argument)
     t.cam = {nullptr, nullptr};
     t.light = {light_sus ? sus_light.as<float4>() : nullptr, light_sus ? sus_light_cnt.as<int>() : nullptr};
     return t;

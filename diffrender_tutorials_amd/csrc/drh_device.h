@@ -146,9 +146,11 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
 // Closest hit with the brute-force tracer's answer: the same MT test, and on equal t the
 // triangle that comes first in (mesh, triangle) order wins (BF's strict `t < hit.t`).
 // `stack` is this lane's column of an LDS stack laid out [depth][64 lanes].
-template <bool ANY>
+// Only internal-node indices are pushed, so 16-bit entries serve F <= 65535 (SK = uint16_t)
+// and halve the stack's LDS footprint.
+template <bool ANY, class SK>
 __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar,
-                                         int* stack, int lane_stride, int depth) {
+                                         SK* stack, int lane_stride, int depth) {
   Hit h{NO_HIT, tFar, 0.0f, 0.0f};
   if (s.F == 0) return h;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
@@ -181,7 +183,7 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
     if (goL && goR) {
       const bool lfirst = tl <= tr;
       const int nearc = lfirst ? cl : cr, farc = lfirst ? cr : cl;
-      if (sp < depth) { stack[sp * lane_stride] = farc; sp++; }
+      if (sp < depth) { stack[sp * lane_stride] = (SK)farc; sp++; }
       node = nearc;
     } else if (goL) {
       node = cl;
@@ -190,7 +192,7 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
     } else {
       if (sp == 0) break;
       sp--;
-      node = stack[sp * lane_stride];
+      node = (int)stack[sp * lane_stride];
     }
   }
   return h;
@@ -258,10 +260,10 @@ __device__ __forceinline__ void test_suspects(const Tracer& T, const Suspects& S
   }
 }
 
-template <bool ANY>
-__device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* stack, int ls) {
+template <bool ANY, class SK>
+__device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, SK* stack, int ls) {
   if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
-  Hit h = trace_bvh<ANY>(T.s, o, d, tNear, tFar, stack, ls, T.depth);
+  Hit h = trace_bvh<ANY, SK>(T.s, o, d, tNear, tFar, stack, ls, T.depth);
   if (ANY && h.gtri != NO_HIT) return h;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
   test_suspects<ANY>(T, ANY ? T.light : T.cam, o, d, tNear, tFar, h);
@@ -306,9 +308,9 @@ __device__ __forceinline__ F2 blend2(const float* tc, uint32_t A, uint32_t B, ui
 
 // shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT.  `nrm` is the 3D scene's normals
 // for render and the projected scene's for edge samples (src/drender.h:353-354).
-template <int MODE>
+template <int MODE, class SK>
 __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
-                                    const float* nrm, int* stack, int ls) {
+                                    const float* nrm, SK* stack, int ls) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
   const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, stack, ls);

@@ -342,9 +342,11 @@ __global__ void k_reorder(ReorderArgs a) {
 // ---------------------------------------------------------------------------
 // render (src/drender.h:98-141) with the L2 loss fused (src/utils.cpp:15-27)
 // ---------------------------------------------------------------------------
-template <int MODE, int DEPTH>
+template <int MODE, int SC>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
-  __shared__ int stack[DEPTH * 64];
+  constexpr int DEPTH = StackCfg<SC>::depth;
+  using SK = typename StackCfg<SC>::T;
+  __shared__ SK stack[DEPTH * 64];
   Tracer T = a.tr;
   T.depth = DEPTH;
   const DevCam& c = a.cams[a.view];
@@ -400,9 +402,11 @@ __global__ __launch_bounds__(1024) void k_loss_final(const double* part, int n, 
 // interior_derivatives (src/drender.h:197-233): Hammersley subpixel offsets,
 // shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74) / <TEXTURE_COLOR> (:132-162, LAMBERT too)
 // ---------------------------------------------------------------------------
-template <int MODE, int DEPTH>
+template <int MODE, int SC>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
-  __shared__ int stack[DEPTH * 64];
+  constexpr int DEPTH = StackCfg<SC>::depth;
+  using SK = typename StackCfg<SC>::T;
+  __shared__ SK stack[DEPTH * 64];
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
   Tracer T = a.tr;
   T.depth = DEPTH;
@@ -552,17 +556,37 @@ __global__ __launch_bounds__(1024) void k_cdf(const float* pmf_raw, int E, float
 // chunk is recomputed from its true start and written out normalised by the total.
 __device__ __forceinline__ int fexp(float x) { return (int)((__float_as_uint(x) >> 23) & 0xFFu); }
 
+// Stage the next CDF_WIN elements of every chunk into tile[chunk][*]: consecutive lanes read
+// consecutive floats of a chunk row, so the block's loads coalesce and overlap.
+constexpr int CDF_WIN = 32;
+__device__ __forceinline__ void cdf_stage(const float* __restrict__ src, int E, int C, int off,
+                                          float (*tile)[CDF_WIN + 1]) {
+  for (int q = threadIdx.x; q < CDF_PAR_THREADS * CDF_WIN; q += CDF_PAR_THREADS) {
+    const int row = q / CDF_WIN, col = q % CDF_WIN;
+    const int k = row * C + off + col;
+    const int end = min(row * C + C, E);
+    tile[row][col] = k < end ? src[k] : 0.0f;
+  }
+}
+
 __global__ __launch_bounds__(CDF_PAR_THREADS) void k_cdf_par(const float* __restrict__ pmf_raw, int E,
                                                                float* __restrict__ pmf, float* __restrict__ cdf) {
+  __shared__ float tile[CDF_PAR_THREADS][CDF_WIN + 1];
   __shared__ double dsum[CDF_PAR_THREADS];
   __shared__ float spec_start[CDF_PAR_THREADS], spec_end[CDF_PAR_THREADS], true_start[CDF_PAR_THREADS + 1];
   __shared__ int ok[CDF_PAR_THREADS];
   const int j = threadIdx.x;
   const int C = (E + CDF_PAR_THREADS - 1) / CDF_PAR_THREADS;
-  const int b = min(j * C, E), e = min(b + C, E);
+  const int b = min(j * C, E), n = min(b + C, E) - b;
   // speculated starts: exclusive prefix of chunk sums in double
   double s = 0.0;
-  for (int k = b; k < e; k++) s += (double)pmf_raw[k];
+  for (int off = 0; off < C; off += CDF_WIN) {
+    __syncthreads();
+    cdf_stage(pmf_raw, E, C, off, tile);
+    __syncthreads();
+    const int m = min(CDF_WIN, n - off);
+    for (int q = 0; q < m; q++) s += (double)tile[j][q];
+  }
   dsum[j] = s;
   __syncthreads();
   for (int o = 1; o < CDF_PAR_THREADS; o <<= 1) {
@@ -575,16 +599,22 @@ __global__ __launch_bounds__(CDF_PAR_THREADS) void k_cdf_par(const float* __rest
   float c = s0;
   int good = 1;
   const int ex = fexp(s0);
-  for (int k = b; k < e; k++) {
-    const float p = pmf_raw[k];
-    const float sum = p + c;
-    // TwoSum: err = (p + c) - sum exactly
-    const float bb = sum - c;
-    const float aa = sum - bb;
-    const float err = (p - bb) + (c - aa);
-    const float u = __uint_as_float((uint32_t)max(ex - 23, 1) << 23);  // grid spacing of the binade
-    if (fabsf(err) * 2.0f == u && err != 0.0f) good = 0;
-    c = sum;
+  const float u = __uint_as_float((uint32_t)max(ex - 23, 1) << 23);  // grid spacing of the binade
+  for (int off = 0; off < C; off += CDF_WIN) {
+    __syncthreads();
+    cdf_stage(pmf_raw, E, C, off, tile);
+    __syncthreads();
+    const int m = min(CDF_WIN, n - off);
+    for (int q = 0; q < m; q++) {
+      const float p = tile[j][q];
+      const float sum = p + c;
+      // TwoSum: err = (p + c) - sum exactly
+      const float bb = sum - c;
+      const float aa = sum - bb;
+      const float err = (p - bb) + (c - aa);
+      if (fabsf(err) * 2.0f == u && err != 0.0f) good = 0;
+      c = sum;
+    }
   }
   // one binade for the whole chunk; tiny (grid below 2^-126) and non-finite values recompute
   if (fexp(c) != ex || ex <= 24 || ex == 255) good = 0;
@@ -614,11 +644,21 @@ __global__ __launch_bounds__(CDF_PAR_THREADS) void k_cdf_par(const float* __rest
   __syncthreads();
   const float total = true_start[CDF_PAR_THREADS];
   c = true_start[j];
-  for (int k = b; k < e; k++) {
-    const float p = pmf_raw[k];
-    c = p + c;
-    cdf[k + 1] = c / total;
-    pmf[k] = p / total;
+  for (int off = 0; off < C; off += CDF_WIN) {
+    __syncthreads();
+    cdf_stage(pmf_raw, E, C, off, tile);
+    __syncthreads();
+    const int m = min(CDF_WIN, n - off);
+    for (int q = 0; q < m; q++) { c = tile[j][q] + c; tile[j][q] = c; }
+    __syncthreads();
+    for (int q = threadIdx.x; q < CDF_PAR_THREADS * CDF_WIN; q += CDF_PAR_THREADS) {
+      const int row = q / CDF_WIN, col = q % CDF_WIN;
+      const int k = row * C + off + col;
+      if (k < min(row * C + C, E)) {
+        cdf[k + 1] = tile[row][col] / total;
+        pmf[k] = pmf_raw[k] / total;
+      }
+    }
   }
   if (j == 0) cdf[0] = 0.0f / total;
 }
@@ -670,7 +710,8 @@ __global__ __launch_bounds__(256) void k_edge_bin(EdgeArgs a) {
       }
     }
     a.ekey[i] = key;
-    atomicAdd(a.ebin + key, 1);
+    // the rejected bin is not counted: after the exclusive scan ebin[E] = accepted total
+    if (key < a.E) atomicAdd(a.ebin + key, 1);
   }
 }
 
@@ -704,16 +745,18 @@ __device__ __forceinline__ bool seg_reduce(int key, float (&v)[K]) {
   return lane == 63 || next != key;
 }
 
-template <int MODE, int DEPTH>
+template <int MODE, int SC>
 __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
-  __shared__ int stack[4 * DEPTH * 64];
+  constexpr int DEPTH = StackCfg<SC>::depth;
+  using SK = typename StackCfg<SC>::T;
+  __shared__ SK stack[4 * DEPTH * 64];
   extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
   __shared__ float ltr[4][12];
   Tracer T = a.tr;
   T.depth = DEPTH;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int* stk = stack + wave * DEPTH * 64 + lane;
+  SK* stk = stack + wave * DEPTH * 64 + lane;
   const bool use_lds = a.lds_n > 0;
   for (int i = threadIdx.x; i < a.lds_n; i += 256) lpos[i] = 0.0f;
   float tr_acc[12];
@@ -744,8 +787,16 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     const float len = sqrtf(dvx * dvx + dvy * dvy);
     const float ndx = dvx / len, ndy = dvy / len;
     const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
-    const F3 cin = shade<MODE>(T, a.light, c, px - 1e-3f * nx, py - 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
-    const F3 cout = shade<MODE>(T, a.light, c, px + 1e-3f * nx, py + 1e-3f * ny, a.tr.s.nrm2d, stk, 64);
+    // both sides of the edge through one (not unrolled) shade call: half the registers
+    F3 side[2];
+#pragma unroll 1
+    for (int sd = 0; sd < 2; sd++) {
+      const float qx = sd == 0 ? px - 1e-3f * nx : px + 1e-3f * nx;
+      const float qy = sd == 0 ? py - 1e-3f * ny : py + 1e-3f * ny;
+      const F3 col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, stk, 64);
+      if (sd == 0) side[0] = col; else side[1] = col;
+    }
+    const F3 cin = side[0], cout = side[1];
     const float pdf = pmf / len;
     const float weight = 1.0f / (pdf * float(a.N));
     const float adjv = dot(sub(cin, cout), adj);
@@ -857,9 +908,11 @@ __global__ void k_adam(AdamArgs a) {
 // ---------------------------------------------------------------------------
 // debug: closest hit at pixel centres
 // ---------------------------------------------------------------------------
-template <int DEPTH>
+template <int SC>
 __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
-  __shared__ int stack[DEPTH * 64];
+  constexpr int DEPTH = StackCfg<SC>::depth;
+  using SK = typename StackCfg<SC>::T;
+  __shared__ SK stack[DEPTH * 64];
   Tracer T = a.tr;
   T.depth = DEPTH;
   const DevCam& c = a.cams[0];
@@ -887,9 +940,11 @@ __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
     case 3: { constexpr int M = 3; BODY; } break;  \
     default: { constexpr int M = 4; BODY; } break; \
   }
-#define DRH_DISPATCH_DEPTH(DEEP, BODY)                 \
-  if (DEEP) { constexpr int D = 64; BODY; }             \
-  else { constexpr int D = 48; BODY; }
+// stack kinds: 0 = 48 x uint16 (F <= 65535), 1 = 48 x int (F <= 2^18), 2 = 64 x int
+#define DRH_DISPATCH_DEPTH(SKIND, BODY)                     \
+  if ((SKIND) == 2) { constexpr int D = 2; BODY; }          \
+  else if ((SKIND) == 1) { constexpr int D = 1; BODY; }     \
+  else { constexpr int D = 0; BODY; }
 
 void launch_fill_f32(float* p, long long n, float v, hipStream_t s) {
   if (n <= 0) return;
@@ -939,13 +994,13 @@ void launch_suspects(const SuspectArgs& a, hipStream_t s) {
 void launch_reorder(const ReorderArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
 }
-void launch_render(int mode, bool deep, const RenderArgs& a, int tiles, hipStream_t s) {
+void launch_render(int mode, int deep, const RenderArgs& a, int tiles, hipStream_t s) {
   DRH_DISPATCH_MODE(mode, DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_render<M, D>), dim3(tiles), dim3(64), 0, s, a)))
 }
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s) {
   hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1024), 0, s, part, n, out, denom);
 }
-void launch_interior(int mode, bool deep, const InteriorArgs& a, int blocks, hipStream_t s) {
+void launch_interior(int mode, int deep, const InteriorArgs& a, int blocks, hipStream_t s) {
   if (mode == 2) {
     DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_interior<2, D>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a))
   } else {
@@ -962,7 +1017,7 @@ void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t
   if (parallel) hipLaunchKernelGGL(k_cdf_par, dim3(1), dim3(CDF_PAR_THREADS), 0, s, pmf_raw, E, pmf, cdf);
   else hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
 }
-void launch_edge_sample(int mode, bool deep, const EdgeArgs& a, int blocks, hipStream_t s) {
+void launch_edge_sample(int mode, int deep, const EdgeArgs& a, int blocks, hipStream_t s) {
   DRH_DISPATCH_MODE(mode, DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_edge_sample<M, D>), dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a)))
 }
 void launch_edge_bin(const EdgeArgs& a, int blocks, hipStream_t s) {
@@ -982,7 +1037,7 @@ void launch_optsimple(const float* grad, float* m, float* G, int off, const OptA
 void launch_adam(const AdamArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_adam, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
-void launch_debug_hits(bool deep, const HitsArgs& a, int tiles, hipStream_t s) {
+void launch_debug_hits(int deep, const HitsArgs& a, int tiles, hipStream_t s) {
   DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_debug_hits<D>), dim3(tiles), dim3(64), 0, s, a))
 }
 

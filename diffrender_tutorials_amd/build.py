@@ -40,8 +40,10 @@ def build(verbose: bool = True) -> str:
     aobj = os.path.join(objdir, "drh_api.o")
     _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "drh_kernels.hip"), "-o", kobj])
     _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "drh_api.cpp"), "-o", aobj])
+    sobj = os.path.join(objdir, "drh_server.o")
+    _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "drh_server.cpp"), "-o", sobj])
     tmp = LIB + ".tmp"
-    _run([HIPCC, *DEVICE, "-shared", "-fPIC", kobj, aobj, "-o", tmp])
+    _run([HIPCC, *DEVICE, "-shared", "-fPIC", kobj, aobj, sobj, "-o", tmp, "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

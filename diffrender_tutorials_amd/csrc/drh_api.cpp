@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -169,12 +170,17 @@ void hammersley(int n, std::vector<float>& out) {
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
+// Bumped whenever any device buffer moves: a captured graph holds raw pointers, so it is only
+// replayed while the epoch it was captured in still holds.
+std::atomic<unsigned long long> g_realloc_epoch{0};
+
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
   ~DBuf() { if (p) (void)hipFree(p); }
   hipError_t reserve(size_t n) {
     if (n <= bytes && p) return hipSuccess;
+    g_realloc_epoch++;
     if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
     if (n == 0) return hipSuccess;
     hipError_t e = hipMalloc(&p, n);
@@ -228,7 +234,18 @@ struct drh_renderer {
   int opt_kind = -1;
   float pos_lr = 0.1f, col_lr = 0.1f;
   int opt_iter = 0;
-  DBuf opt_m, opt_v;
+  DBuf opt_m, opt_v, opt_t;
+  // captured optimisation step
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> graph_sig;
+  unsigned long long graph_epoch = 0;
+  void drop_graph() {
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    graph_sig.clear();
+  }
   // profiling
   bool prof = false;
   std::vector<ProfEvent> pending;
@@ -712,6 +729,7 @@ void drh_destroy(drh_handle h) {
   if (!h) return;
   (void)hipSetDevice(h->st.device);
   (void)hipStreamSynchronize(h->stream);
+  h->drop_graph();
   for (auto& p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto e : h->event_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(h->stream);
@@ -852,8 +870,11 @@ drh_status drh_opt_init(drh_handle h, int kind, float pos_lr, float col_lr) {
   if ((s = h->hip(h->opt_m.reserve(n), "alloc")) || (s = h->hip(h->opt_v.reserve(n), "alloc")) ||
       (s = h->hip(h->grad.reserve(n), "alloc")))
     return s;
+  if ((s = h->hip(h->opt_t.reserve(16), "alloc"))) return s;
   launch_fill_f32(h->opt_m.as<float>(), h->G, 0.0f, h->stream);
   launch_fill_f32(h->opt_v.as<float>(), h->G, 0.0f, h->stream);
+  launch_fill_u32(h->opt_t.as<uint32_t>(), 1, 1u, h->stream);  // first Adam step is t = 1
+  h->drop_graph();
   h->opt_kind = kind;
   h->pos_lr = pos_lr;
   h->col_lr = col_lr;
@@ -861,12 +882,10 @@ drh_status drh_opt_init(drh_handle h, int kind, float pos_lr, float col_lr) {
   return h->hip(hipGetLastError(), "opt_init");
 }
 
-drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const float* d_targets, int edge_samples,
-                        float* d_loss) {
-  if (!h) return DRH_E_INVALID;
-  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "opt_step: commit first");
-  if (h->opt_kind < 0) return h->fail(DRH_E_INVALID, "opt_step: call drh_opt_init first");
-  (void)hipSetDevice(h->st.device);
+// One optimisation step, every launch on the handle's stream (capturable: no allocation,
+// upload or synchronisation once the first step has sized every buffer).
+static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views, const float* d_targets,
+                                int edge_samples, float* d_loss) {
   drh_status s;
   // re-commit from the device parameters (the optimiser moved them)
   if (h->opt_iter > 0 && (s = h->build_bvh())) return s;
@@ -907,14 +926,73 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
       AdamArgs a{};
       a.n = n; a.off = off; a.nseg = nseg; a.grad = h->grad.as<float>(); a.m = h->opt_m.as<float>(); a.v = h->opt_v.as<float>();
       a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-8f;
-      a.bc1 = (float)(1.0 - std::pow(0.9, h->opt_iter + 1));
-      a.bc2 = (float)(1.0 - std::pow(0.999, h->opt_iter + 1));
+      a.step = h->opt_t.as<int>();
       for (int k = 0; k < nseg; k++) { a.seg_start[k] = seg_start[k]; a.seg_lr[k] = seg_lr[k]; a.seg_param[k] = seg_param[k]; }
       h->timed("k_adam", [&] { launch_adam(a, h->stream); });
     }
   }
-  h->opt_iter++;
+  if (h->opt_kind == DRH_OPT_ADAM) h->timed("k_step_inc", [&] { launch_step_inc(h->opt_t.as<int>(), h->stream); });
   return h->hip(hipGetLastError(), "opt_step");
+}
+
+drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const float* d_targets, int edge_samples,
+                        float* d_loss) {
+  if (!h) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "opt_step: commit first");
+  if (h->opt_kind < 0) return h->fail(DRH_E_INVALID, "opt_step: call drh_opt_init first");
+  if (!cams || views <= 0 || !d_targets || edge_samples < 0) return h->fail(DRH_E_INVALID, "opt_step: bad arguments");
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  // Adam steps are replayed from a hipGraph once the first (eager) step has sized every
+  // buffer; the graph is re-captured when the arguments change.  OptSimple stays eager (its
+  // step-decayed learning rates are host values).
+  std::vector<char> sig(sizeof(drh_camera) * views + 4 * sizeof(void*));
+  std::memcpy(sig.data(), cams, sizeof(drh_camera) * views);
+  const void* ptrs[4] = {d_targets, d_loss, (const void*)(intptr_t)edge_samples, (const void*)(intptr_t)views};
+  std::memcpy(sig.data() + sizeof(drh_camera) * views, ptrs, sizeof ptrs);
+  const bool graphable = h->use_graph && h->opt_kind == DRH_OPT_ADAM && h->opt_iter > 0 && !h->prof;
+  if (graphable) {
+    // the graph reads the camera block and the sample table: make sure they hold this call's
+    // (uploads only if another call changed them; a reallocation shows in the epoch)
+    long long total;
+    if ((s = h->set_cams(cams, views, &total)) || (s = h->ensure_rnd(edge_samples))) return s;
+  }
+  if (graphable && h->gexec && sig == h->graph_sig && h->graph_epoch == g_realloc_epoch.load()) {
+    if ((s = h->hip(hipGraphLaunch(h->gexec, h->stream), "graph launch"))) return s;
+    h->opt_iter++;
+    return DRH_OK;
+  }
+  if (graphable) {
+    h->drop_graph();
+    if ((s = h->hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return s;
+    const drh_status body = opt_step_body(h, cams, views, d_targets, edge_samples, d_loss);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (body == DRH_OK && ec == hipSuccess && g &&
+        hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0) == hipSuccess) {
+      h->graph = g;
+      h->graph_sig = sig;
+      h->graph_epoch = g_realloc_epoch.load();
+      if ((s = h->hip(hipGraphLaunch(h->gexec, h->stream), "graph launch"))) return s;
+      h->opt_iter++;
+      return DRH_OK;
+    }
+    // capture failed: forget it and run the step eagerly
+    if (g) (void)hipGraphDestroy(g);
+    h->gexec = nullptr;
+    (void)hipGetLastError();
+    h->use_graph = false;
+  }
+  if ((s = opt_step_body(h, cams, views, d_targets, edge_samples, d_loss))) return s;
+  h->opt_iter++;
+  return DRH_OK;
+}
+
+drh_status drh_opt_use_graph(drh_handle h, int on) {
+  if (!h) return DRH_E_INVALID;
+  h->use_graph = on != 0;
+  if (!on) h->drop_graph();
+  return DRH_OK;
 }
 
 drh_status drh_get_params(drh_handle h, float* pos, float* col) {

@@ -187,7 +187,8 @@ struct AdamArgs {
   const float* grad;
   float* m;
   float* v;
-  float b1, b2, eps, bc1, bc2;
+  float b1, b2, eps;
+  const int* step;        // 1-based Adam step, kept on the device so a captured graph replays correctly
   int seg_start[MAX_SEG];
   float seg_lr[MAX_SEG];
   float* seg_param[MAX_SEG];
@@ -228,6 +229,7 @@ void launch_edge_scatter(const EdgeArgs& a, int blocks, hipStream_t s);
 void launch_scan_u32(uint32_t* p, int n, hipStream_t s);
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);
 void launch_adam(const AdamArgs& a, hipStream_t s);
+void launch_step_inc(int* step, hipStream_t s);
 void launch_debug_hits(int deep, const HitsArgs& a, int tiles, hipStream_t s);
 
 }  // namespace drh

@@ -901,9 +901,16 @@ __global__ void k_adam(AdamArgs a) {
   const float v = a.b2 * a.v[a.off + i] + (1.0f - a.b2) * g * g;
   a.m[a.off + i] = m;
   a.v[a.off + i] = v;
-  const float mh = m / a.bc1, vh = v / a.bc2;
+  const int t = *a.step;
+  const float bc1 = (float)(1.0 - pow((double)a.b1, t)), bc2 = (float)(1.0 - pow((double)a.b2, t));
+  const float mh = m / bc1, vh = v / bc2;
   p[i - a.seg_start[seg]] -= a.seg_lr[seg] * mh / (sqrtf(vh) + a.eps);
 }
+
+__global__ void k_step_inc(int* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) step[0] += 1;
+}
+void launch_step_inc(int* step, hipStream_t s) { hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, s, step); }
 
 // ---------------------------------------------------------------------------
 // debug: closest hit at pixel centres

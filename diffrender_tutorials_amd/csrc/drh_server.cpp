@@ -1,0 +1,379 @@
+// drh_server.cpp -- continuous-batching server for mesh-fitting jobs (one MI355X).
+//
+// A job = a scene, its cameras, target images and an iteration count.  Jobs are queued by
+// drh_server_submit (inputs copied), admitted by a worker thread while a slot and the HBM
+// budget allow, and stepped round-robin: every active job keeps one fused fwd+bwd+Adam step
+// in flight on its own HIP stream (a hipGraph replay after its first step), so the device
+// overlaps the jobs' kernels.  A finished job's results are downloaded and its memory freed,
+// which admits the next queued job.  Per-iteration latency = HIP event time from a step's
+// first to last launch on its stream, contention included.
+//
+// The reference has no serving layer (its optimiser runs one scene at a time,
+// src/optimizer_simple.cpp:272-309); this is the "serving" pattern of the path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/drh.h"
+
+namespace {
+
+struct Job {
+  int id = 0;
+  drh_settings st{};
+  // owned copies of the scene arrays
+  int n_meshes = 0;
+  std::vector<int> vcount, fcount, tex_w, tex_h, diff;
+  std::vector<float> transforms, pos, col, tc, nrm, texels;
+  std::vector<uint32_t> idx;
+  bool has_col = false, has_tc = false, has_nrm = false;
+  std::vector<drh_camera> cams;
+  std::vector<float> targets;
+  int edge_samples = 0, iterations = 0;
+  float lr_pos = 1e-3f, lr_col = 1e-3f;
+  size_t bytes = 0;  // admission estimate
+  // run state
+  drh_handle h = nullptr;
+  void* d_targets = nullptr;
+  void* d_loss = nullptr;
+  int issued = 0, completed = 0;
+  std::deque<std::pair<hipEvent_t, hipEvent_t>> inflight;
+  // results
+  int status = 0;  // 0 queued / running, 1 done, <0 failed (drh_status)
+  std::string error;
+  std::vector<float> out_pos, out_col;
+  float final_loss = 0.0f;
+  std::vector<float> latencies_ms;
+  std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
+
+  int V() const { int v = 0; for (int c : vcount) v += c; return v; }
+  int F() const { int f = 0; for (int c : fcount) f += c; return f; }
+  drh_scene_desc desc() const {
+    return drh_scene_desc{n_meshes, vcount.data(), fcount.data(), transforms.data(), pos.data(),
+                          has_col ? col.data() : nullptr, has_tc ? tc.data() : nullptr, has_nrm ? nrm.data() : nullptr,
+                          idx.data(), tex_w.data(), tex_h.data(), texels.empty() ? nullptr : texels.data(), diff.data()};
+  }
+};
+
+// HBM a job holds while active (the buffers drh_renderer sizes for it), with 25% headroom.
+size_t estimate_bytes(const Job& j) {
+  const size_t V = j.V(), F = j.F();
+  size_t px = 0;
+  for (auto& c : j.cams) px += (size_t)c.width * (size_t)c.height;
+  const size_t N = (size_t)j.edge_samples;
+  const size_t E = 3 * F / 2 + 16;
+  size_t b = 0;
+  b += 12 * V * 7 + 8 * V + 16 * V;               // vertex arrays, projected vertices
+  b += 12 * F + 4 * F;                            // indices, triangle mesh ids
+  b += 16 * F * 6 + 4 * F * 8 + 16 * F * 4 + 64 * F + 16 * F * j.cams.size() + 16 * F;  // LBVH build + suspects
+  b += 24 * E + 8 * N + 8 * N;                    // edges, cdf; rng table, sample bins
+  b += 12 * px * 4;                               // image, adjoint, target (+ loss partials)
+  b += 4 * (size_t)(6 * V + 64) * 4;              // gradient + Adam moments
+  b += 4 * j.texels.size();
+  return b + b / 4 + (1u << 20);
+}
+
+}  // namespace
+
+struct drh_server_s {
+  int device = 0;
+  size_t budget = 0, used = 0;
+  int max_active = 4;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job*> pending;
+  std::vector<Job*> active;
+  std::map<int, std::unique_ptr<Job>> jobs;
+  int next_id = 1;
+  bool stop = false;
+  std::thread worker;
+  // stats
+  std::vector<float> lat_ms;
+  long long iters_done = 0;
+  int jobs_done = 0, jobs_failed = 0, peak_active = 0;
+  double active_sum = 0.0;
+  long long active_samples = 0;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  void run();
+  bool admit_one(Job* j);
+  void finish(Job* j, int status, const std::string& err);
+};
+
+void drh_server_s::finish(Job* j, int status, const std::string& err) {
+  if (j->h) {
+    if (status == 1) {
+      const int V = j->V();
+      j->out_pos.resize(3 * (size_t)V);
+      j->out_col.resize(3 * (size_t)V);
+      if (drh_get_params(j->h, j->out_pos.data(), j->out_col.data()) != DRH_OK) { status = DRH_E_HIP; }
+      if (drh_memcpy_dtoh(j->h, &j->final_loss, j->d_loss, 4) != DRH_OK) status = DRH_E_HIP;
+    }
+    for (auto& ev : j->inflight) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    j->inflight.clear();
+    (void)drh_free(j->h, j->d_targets);
+    (void)drh_free(j->h, j->d_loss);
+    drh_destroy(j->h);
+    j->h = nullptr;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  used -= std::min(used, j->bytes);
+  j->status = status;
+  j->error = err;
+  j->t_done = std::chrono::steady_clock::now();
+  if (status == 1) jobs_done++; else jobs_failed++;
+  cv_done.notify_all();
+}
+
+bool drh_server_s::admit_one(Job* j) {
+  drh_handle h = nullptr;
+  drh_status s = drh_create(&j->st, &h);
+  if (s != DRH_OK) { finish(j, s, "drh_create failed"); return false; }
+  j->h = h;
+  const drh_scene_desc d = j->desc();
+  const size_t tbytes = 4 * j->targets.size();
+  if ((s = drh_commit(h, &d)) != DRH_OK || (s = drh_opt_init(h, DRH_OPT_ADAM, j->lr_pos, j->lr_col)) != DRH_OK ||
+      (s = drh_malloc(h, tbytes, &j->d_targets)) != DRH_OK || (s = drh_malloc(h, 16, &j->d_loss)) != DRH_OK ||
+      (s = drh_memcpy_htod(h, j->d_targets, j->targets.data(), tbytes)) != DRH_OK) {
+    finish(j, s, drh_last_error(h));
+    return false;
+  }
+  j->t_admit = std::chrono::steady_clock::now();
+  return true;
+}
+
+void drh_server_s::run() {
+  (void)hipSetDevice(device);
+  while (true) {
+    // admission: queued jobs enter while a slot and the HBM budget allow
+    std::vector<Job*> to_admit;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      if (stop) break;
+      if (active.empty() && pending.empty()) {
+        cv_work.wait_for(lk, std::chrono::milliseconds(50));
+        continue;
+      }
+      while (!pending.empty() && (int)(active.size() + to_admit.size()) < max_active) {
+        Job* j = pending.front();
+        if (j->bytes > budget) {           // can never fit
+          pending.pop_front();
+          j->status = DRH_E_OOM;
+          j->error = "job exceeds the server's HBM budget";
+          jobs_failed++;
+          cv_done.notify_all();
+          continue;
+        }
+        if (used + j->bytes > budget) break;  // wait for memory to free up (FIFO)
+        used += j->bytes;
+        pending.pop_front();
+        to_admit.push_back(j);
+      }
+    }
+    for (Job* j : to_admit)
+      if (admit_one(j)) active.push_back(j);
+    peak_active = std::max(peak_active, (int)active.size());
+    active_sum += active.size();
+    active_samples++;
+    // stepping: keep up to two steps in flight per job; harvest completed ones
+    bool progressed = !to_admit.empty();
+    for (size_t k = 0; k < active.size();) {
+      Job* j = active[k];
+      // harvest
+      while (!j->inflight.empty() && hipEventQuery(j->inflight.front().second) == hipSuccess) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, j->inflight.front().first, j->inflight.front().second);
+        (void)hipEventDestroy(j->inflight.front().first);
+        (void)hipEventDestroy(j->inflight.front().second);
+        j->inflight.pop_front();
+        j->completed++;
+        j->latencies_ms.push_back(ms);
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          lat_ms.push_back(ms);
+          iters_done++;
+        }
+        progressed = true;
+      }
+      if (j->completed >= j->iterations) {
+        active.erase(active.begin() + k);
+        finish(j, 1, "");
+        progressed = true;
+        continue;
+      }
+      // issue
+      if (j->issued < j->iterations && j->inflight.size() < 2) {
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        hipStream_t st = (hipStream_t)drh_stream(j->h);
+        (void)hipEventRecord(a, st);
+        const drh_status s = drh_opt_step(j->h, j->cams.data(), (int)j->cams.size(), (const float*)j->d_targets,
+                                          j->edge_samples, (float*)j->d_loss);
+        (void)hipEventRecord(b, st);
+        if (s != DRH_OK) {
+          (void)hipEventDestroy(a);
+          (void)hipEventDestroy(b);
+          active.erase(active.begin() + k);
+          finish(j, s, drh_last_error(j->h));
+          continue;
+        }
+        j->inflight.push_back({a, b});
+        j->issued++;
+        progressed = true;
+      }
+      k++;
+    }
+    if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+  for (Job* j : active) finish(j, DRH_E_INVALID, "server stopped");
+  active.clear();
+}
+
+extern "C" {
+
+drh_status drh_server_create(int device, size_t hbm_budget_bytes, int max_active, drh_server* out) {
+  if (!out || max_active <= 0) return DRH_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return DRH_E_HIP;
+  auto* s = new drh_server_s();
+  s->device = device;
+  s->budget = hbm_budget_bytes;
+  s->max_active = max_active;
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return DRH_OK;
+}
+
+drh_status drh_server_submit(drh_server s, const drh_job_desc* d, int* job_id) {
+  if (!s || !d || !job_id || !d->cams || d->views <= 0 || !d->targets || d->iterations <= 0 || d->edge_samples < 0)
+    return DRH_E_INVALID;
+  const drh_scene_desc& sc = d->scene;
+  if (sc.n_meshes <= 0 || !sc.vcount || !sc.fcount || !sc.transforms || !sc.pos || !sc.idx) return DRH_E_INVALID;
+  auto j = std::make_unique<Job>();
+  j->st = d->settings;
+  j->n_meshes = sc.n_meshes;
+  j->vcount.assign(sc.vcount, sc.vcount + sc.n_meshes);
+  j->fcount.assign(sc.fcount, sc.fcount + sc.n_meshes);
+  int V = 0, F = 0, T = 0;
+  for (int i = 0; i < sc.n_meshes; i++) {
+    V += sc.vcount[i];
+    F += sc.fcount[i];
+    const int tw = sc.tex_w ? sc.tex_w[i] : 0, th = sc.tex_h ? sc.tex_h[i] : 0;
+    j->tex_w.push_back(tw);
+    j->tex_h.push_back(th);
+    T += 3 * tw * th;
+    j->diff.push_back(sc.differentiable ? sc.differentiable[i] : 1);
+  }
+  j->transforms.assign(sc.transforms, sc.transforms + 16 * sc.n_meshes);
+  j->pos.assign(sc.pos, sc.pos + 3 * (size_t)V);
+  if ((j->has_col = sc.col != nullptr)) j->col.assign(sc.col, sc.col + 3 * (size_t)V);
+  if ((j->has_tc = sc.tc != nullptr)) j->tc.assign(sc.tc, sc.tc + 2 * (size_t)V);
+  if ((j->has_nrm = sc.nrm != nullptr)) j->nrm.assign(sc.nrm, sc.nrm + 3 * (size_t)V);
+  j->idx.assign(sc.idx, sc.idx + 3 * (size_t)F);
+  if (T && sc.texels) j->texels.assign(sc.texels, sc.texels + T);
+  j->cams.assign(d->cams, d->cams + d->views);
+  size_t px = 0;
+  for (auto& c : j->cams) px += (size_t)c.width * (size_t)c.height;
+  j->targets.assign(d->targets, d->targets + 3 * px);
+  j->edge_samples = d->edge_samples;
+  j->iterations = d->iterations;
+  j->lr_pos = d->lr_pos;
+  j->lr_col = d->lr_col;
+  j->bytes = estimate_bytes(*j);
+  j->t_submit = std::chrono::steady_clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  j->id = s->next_id++;
+  *job_id = j->id;
+  s->pending.push_back(j.get());
+  s->jobs[j->id] = std::move(j);
+  s->cv_work.notify_all();
+  return DRH_OK;
+}
+
+drh_status drh_server_wait(drh_server s, int job_id, double timeout_s, drh_job_result* r) {
+  if (!s) return DRH_E_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return DRH_E_INVALID;
+  Job* j = it->second.get();
+  const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return j->status != 0; });
+  if (!ok) return DRH_E_INVALID;
+  if (r) {
+    r->status = j->status == 1 ? DRH_OK : (drh_status)j->status;
+    r->iterations = j->completed;
+    r->final_loss = j->final_loss;
+    r->queue_ms = std::chrono::duration<double, std::milli>(j->t_admit - j->t_submit).count();
+    r->run_ms = std::chrono::duration<double, std::milli>(j->t_done - j->t_admit).count();
+    if (r->pos && !j->out_pos.empty()) std::memcpy(r->pos, j->out_pos.data(), 4 * j->out_pos.size());
+    if (r->col && !j->out_col.empty()) std::memcpy(r->col, j->out_col.data(), 4 * j->out_col.size());
+    if (r->latencies_ms && r->latencies_cap > 0) {
+      const int n = std::min<int>(r->latencies_cap, (int)j->latencies_ms.size());
+      std::memcpy(r->latencies_ms, j->latencies_ms.data(), 4 * (size_t)n);
+    }
+  }
+  return j->status == 1 ? DRH_OK : (drh_status)j->status;
+}
+
+drh_status drh_server_stats(drh_server s, drh_server_stats_t* out) {
+  if (!s || !out) return DRH_E_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::vector<float> l = s->lat_ms;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double p) -> double {
+    if (l.empty()) return 0.0;
+    const size_t i = std::min(l.size() - 1, (size_t)(p * (double)(l.size() - 1) + 0.5));
+    return l[i];
+  };
+  out->iterations = s->iters_done;
+  out->jobs_done = s->jobs_done;
+  out->jobs_failed = s->jobs_failed;
+  out->jobs_queued = (int)s->pending.size();
+  out->p50_ms = pct(0.50);
+  out->p99_ms = pct(0.99);
+  out->peak_active = s->peak_active;
+  out->mean_active = s->active_samples ? s->active_sum / (double)s->active_samples : 0.0;
+  out->hbm_used = s->used;
+  out->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+  return DRH_OK;
+}
+
+void drh_server_destroy(drh_server s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    for (Job* j : s->pending) { j->status = DRH_E_INVALID; j->error = "server stopped"; }
+    s->pending.clear();
+    s->cv_work.notify_all();
+  }
+  s->worker.join();
+  delete s;
+}
+
+}  // extern "C"
+
+extern "C" size_t drh_server_job_bytes(const drh_job_desc* d) {
+  if (!d || d->scene.n_meshes <= 0 || !d->scene.vcount || !d->scene.fcount || !d->cams || d->views <= 0) return 0;
+  Job j;
+  j.n_meshes = d->scene.n_meshes;
+  j.vcount.assign(d->scene.vcount, d->scene.vcount + j.n_meshes);
+  j.fcount.assign(d->scene.fcount, d->scene.fcount + j.n_meshes);
+  j.cams.assign(d->cams, d->cams + d->views);
+  j.edge_samples = d->edge_samples;
+  size_t T = 0;
+  for (int i = 0; i < j.n_meshes; i++) T += 3 * (size_t)(d->scene.tex_w ? d->scene.tex_w[i] : 0) * (d->scene.tex_h ? d->scene.tex_h[i] : 0);
+  j.texels.resize(T);
+  return estimate_bytes(j);
+}

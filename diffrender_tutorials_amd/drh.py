@@ -33,7 +33,9 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params", "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
-           "drh_debug_copy", "drh_debug_cdf"]
+           "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
+           "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
+           "drh_server_job_bytes"]
 
 
 class DrhError(RuntimeError):
@@ -79,9 +81,17 @@ def load():
         L.drh_create.argtypes = [ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_void_p)]
         for n in EXPORTS:
             f = getattr(L, n)
-            if n not in ("drh_last_error", "drh_stream", "drh_destroy"):
+            if n not in ("drh_last_error", "drh_stream", "drh_destroy", "drh_server_destroy", "drh_server_job_bytes"):
                 f.restype = ctypes.c_int
         L.drh_destroy.restype = None
+        L.drh_server_destroy.restype = None
+        L.drh_server_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.drh_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_int)]
+        L.drh_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(JobResult)]
+        L.drh_server_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+        L.drh_server_destroy.argtypes = [ctypes.c_void_p]
+        L.drh_server_job_bytes.restype = ctypes.c_size_t
+        L.drh_server_job_bytes.argtypes = [ctypes.POINTER(JobDesc)]
         L.drh_malloc.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
         L.drh_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.drh_memcpy_htod.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
@@ -149,25 +159,10 @@ class DiffRender:
     def commit(self, meshes):
         """IDiffRender::commit (src/drender.h:91-96)."""
         L = load()
-        n = len(meshes)
-        vc = np.array([m.V for m in meshes], np.int32)
-        fc = np.array([m.F for m in meshes], np.int32)
-        tr = np.ascontiguousarray(np.concatenate([np.asarray(m.transform, np.float32).reshape(16) for m in meshes]))
-        pos = np.ascontiguousarray(np.concatenate([m.pos for m in meshes]).astype(np.float32))
-        has = lambda a: all(getattr(m, a) is not None for m in meshes)
-        cat = lambda a: np.ascontiguousarray(np.concatenate([getattr(m, a) for m in meshes]).astype(np.float32)) if has(a) else None
-        col, tc, nrm = cat("col"), cat("tc"), cat("nrm")
-        idx = np.ascontiguousarray(np.concatenate([m.idx for m in meshes]).astype(np.uint32))
-        tw = np.array([0 if m.tex is None else m.tex.shape[1] for m in meshes], np.int32)
-        th = np.array([0 if m.tex is None else m.tex.shape[0] for m in meshes], np.int32)
-        texs = [np.asarray(m.tex, np.float32).reshape(-1) for m in meshes if m.tex is not None]
-        tex = np.ascontiguousarray(np.concatenate(texs)) if texs else None
-        df = np.array([1 if m.differentiable else 0 for m in meshes], np.int32)
-        d = SceneDesc(n, _ip(vc), _ip(fc), _fp(tr), _fp(pos), _fp(col), _fp(tc), _fp(nrm),
-                      idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ip(tw), _ip(th), _fp(tex), _ip(df))
+        d, keep = scene_desc(meshes)
         self._check(L.drh_commit(self.h, ctypes.byref(d)))
-        self.V = int(vc.sum())
-        self.has_col = col is not None
+        self.V = int(sum(m.V for m in meshes))
+        self.has_col = keep[5] is not None
 
     def grad_size(self):
         return load().drh_grad_size(self.h)
@@ -289,6 +284,109 @@ class DeviceBuffer:
         self.ptr = None
 
     __del__ = free
+
+
+def scene_desc(meshes):
+    """drh_scene_desc over concatenated mesh arrays; returns (desc, arrays kept alive)."""
+    n = len(meshes)
+    vc = np.array([m.V for m in meshes], np.int32)
+    fc = np.array([m.F for m in meshes], np.int32)
+    tr = np.ascontiguousarray(np.concatenate([np.asarray(m.transform, np.float32).reshape(16) for m in meshes]))
+    pos = np.ascontiguousarray(np.concatenate([m.pos for m in meshes]).astype(np.float32))
+    has = lambda a: all(getattr(m, a) is not None for m in meshes)
+    cat = lambda a: np.ascontiguousarray(np.concatenate([getattr(m, a) for m in meshes]).astype(np.float32)) if has(a) else None
+    col, tc, nrm = cat("col"), cat("tc"), cat("nrm")
+    idx = np.ascontiguousarray(np.concatenate([m.idx for m in meshes]).astype(np.uint32))
+    tw = np.array([0 if m.tex is None else m.tex.shape[1] for m in meshes], np.int32)
+    th = np.array([0 if m.tex is None else m.tex.shape[0] for m in meshes], np.int32)
+    texs = [np.asarray(m.tex, np.float32).reshape(-1) for m in meshes if m.tex is not None]
+    tex = np.ascontiguousarray(np.concatenate(texs)) if texs else None
+    df = np.array([1 if m.differentiable else 0 for m in meshes], np.int32)
+    d = SceneDesc(n, _ip(vc), _ip(fc), _fp(tr), _fp(pos), _fp(col), _fp(tc), _fp(nrm),
+                  idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ip(tw), _ip(th), _fp(tex), _ip(df))
+    return d, (vc, fc, tr, pos, col, tc, nrm, idx, tw, th, tex, df)
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("settings", Settings), ("scene", SceneDesc), ("cams", ctypes.POINTER(CameraC)), ("views", ctypes.c_int),
+                ("targets", ctypes.POINTER(ctypes.c_float)), ("edge_samples", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("lr_pos", ctypes.c_float), ("lr_col", ctypes.c_float)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("final_loss", ctypes.c_float),
+                ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("pos", ctypes.POINTER(ctypes.c_float)),
+                ("col", ctypes.POINTER(ctypes.c_float)), ("latencies_ms", ctypes.POINTER(ctypes.c_float)),
+                ("latencies_cap", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_longlong), ("jobs_done", ctypes.c_int), ("jobs_failed", ctypes.c_int),
+                ("jobs_queued", ctypes.c_int), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("peak_active", ctypes.c_int), ("mean_active", ctypes.c_double), ("hbm_used", ctypes.c_size_t),
+                ("elapsed_s", ctypes.c_double)]
+
+
+def settings_struct(mode, spp, rng_mode=RNG_REF_PRNG14, tracer=TRACER_BVH, device=0, light=None):
+    s = Settings()
+    s.mode, s.spp, s.rng_mode, s.tracer, s.device = mode, spp, rng_mode, tracer, device
+    if light is not None and light.get("light_dir") is not None:
+        s.light_custom = 1
+        s.light_dir[:] = [float(x) for x in light["light_dir"]]
+        s.light_color[:] = [float(x) for x in light["light_color"]]
+        s.ambient[:] = [float(x) for x in light["ambient"]]
+    return s
+
+
+class Server:
+    """Continuous-batching server (drh_server_*): submit mesh-fitting jobs, wait for results."""
+
+    def __init__(self, device=0, hbm_budget=64 << 30, max_active=4):
+        L = load()
+        h = ctypes.c_void_p()
+        st = L.drh_server_create(device, int(hbm_budget), max_active, ctypes.byref(h))
+        if st != 0:
+            raise DrhError(st, "drh_server_create failed")
+        self.h = h
+        self.V = {}
+
+    def submit(self, meshes, cams, targets, iterations, edge_samples, mode=LAMBERT, spp=1, light=None,
+               lr_pos=1e-3, lr_col=1e-3, rng_mode=RNG_REF_PRNG14):
+        d, keep = scene_desc(meshes)
+        cs = cameras_array(cams)
+        tg = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float32).reshape(-1) for t in targets]))
+        j = JobDesc(settings_struct(mode, spp, rng_mode, light=light), d, cs, len(cams), _fp(tg), int(edge_samples),
+                    int(iterations), float(lr_pos), float(lr_col))
+        self.last_job_bytes = load().drh_server_job_bytes(ctypes.byref(j))
+        jid = ctypes.c_int()
+        st = load().drh_server_submit(self.h, ctypes.byref(j), ctypes.byref(jid))
+        if st != 0:
+            raise DrhError(st, "drh_server_submit failed")
+        self.V[jid.value] = (int(sum(m.V for m in meshes)), int(iterations))
+        return jid.value
+
+    def wait(self, job_id, timeout=600.0):
+        V, iters = self.V[job_id]
+        pos = np.empty(3 * V, np.float32)
+        col = np.empty(3 * V, np.float32)
+        lat = np.empty(iters, np.float32)
+        r = JobResult(0, 0, 0.0, 0.0, 0.0, _fp(pos), _fp(col), _fp(lat), iters)
+        st = load().drh_server_wait(self.h, job_id, float(timeout), ctypes.byref(r))
+        return {"status": st, "iterations": r.iterations, "final_loss": r.final_loss, "queue_ms": r.queue_ms,
+                "run_ms": r.run_ms, "pos": pos.reshape(-1, 3), "col": col.reshape(-1, 3),
+                "latencies_ms": lat[:r.iterations]}
+
+    def stats(self):
+        s = ServerStats()
+        load().drh_server_stats(self.h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.drh_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 def _split(flat, cams):

@@ -133,6 +133,9 @@ drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, in
 drh_status drh_opt_init(drh_handle h, int kind, float pos_lr, float col_lr);
 drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const float* d_targets,
                         int edge_samples, float* d_loss);
+/* DRH_OPT_ADAM steps after the first are captured once as a hipGraph and replayed (on by
+ * default); the graph is re-captured when the arguments or any device buffer change. */
+drh_status drh_opt_use_graph(drh_handle h, int on);
 /* Download the current (optimised) vertex positions (3V) and colours (3V, may be NULL). */
 drh_status drh_get_params(drh_handle h, float* pos, float* col);
 
@@ -143,6 +146,57 @@ drh_status drh_malloc(drh_handle h, size_t bytes, void** d_ptr);
 drh_status drh_free(drh_handle h, void* d_ptr);
 drh_status drh_memcpy_htod(drh_handle h, void* d_dst, const void* src, size_t bytes);
 drh_status drh_memcpy_dtoh(drh_handle h, void* dst, const void* d_src, size_t bytes);
+
+/* ---------------------------------------------------------------------------------------------
+ * Serving (no counterpart in the reference, whose optimiser fits one scene at a time,
+ * src/optimizer_simple.cpp:272-309): a continuous-batching server for mesh-fitting jobs.
+ * Jobs are queued by drh_server_submit (every input copied), admitted by a worker thread while
+ * fewer than max_active run and the HBM budget holds their estimated footprint, and stepped
+ * concurrently (one fused commit + render + loss + d_render + Adam step per iteration, each job
+ * on its own stream, replayed as a hipGraph).  A job larger than the whole budget is refused
+ * with DRH_E_OOM.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct drh_server_s* drh_server;
+
+typedef struct {
+  drh_settings settings;       /* mode, spp, light, rng (tracer / device are the server's) */
+  drh_scene_desc scene;        /* initial mesh */
+  const drh_camera* cams;
+  int views;
+  const float* targets;        /* views x H x W x 3 target images */
+  int edge_samples;            /* per view per iteration */
+  int iterations;
+  float lr_pos, lr_col;        /* Adam learning rates */
+} drh_job_desc;
+
+typedef struct {
+  drh_status status;
+  int iterations;              /* completed */
+  float final_loss;            /* loss of the last iteration */
+  double queue_ms, run_ms;     /* submit -> admit, admit -> done */
+  float* pos;                  /* out (may be NULL): 3V fitted positions */
+  float* col;                  /* out (may be NULL): 3V colours */
+  float* latencies_ms;         /* out (may be NULL): per-iteration latency */
+  int latencies_cap;
+} drh_job_result;
+
+typedef struct {
+  long long iterations;        /* completed over all jobs */
+  int jobs_done, jobs_failed, jobs_queued;
+  double p50_ms, p99_ms;       /* per-iteration latency */
+  int peak_active;
+  double mean_active;
+  size_t hbm_used;
+  double elapsed_s;
+} drh_server_stats_t;
+
+drh_status drh_server_create(int device, size_t hbm_budget_bytes, int max_active, drh_server* out);
+drh_status drh_server_submit(drh_server s, const drh_job_desc* job, int* job_id);
+drh_status drh_server_wait(drh_server s, int job_id, double timeout_s, drh_job_result* result);
+drh_status drh_server_stats(drh_server s, drh_server_stats_t* out);
+/* The HBM footprint the server charges a job against its budget (bytes). */
+size_t     drh_server_job_bytes(const drh_job_desc* job);
+void       drh_server_destroy(drh_server s);
 
 /* The handle's HIP stream (a hipStream_t) and device synchronisation of it. */
 void*      drh_stream(drh_handle h);

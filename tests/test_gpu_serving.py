@@ -1,0 +1,157 @@
+"""The device-resident optimisation step (graph replay) and the continuous-batching server:
+a job's result is the same alone and in a mix of concurrently served jobs."""
+from __future__ import annotations
+
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from diffrender_tutorials_amd import drh, scenes as S
+
+pytestmark = pytest.mark.gpu
+
+# Adam steps are sign-like, so ulp-level differences in the float-atomic gradient sums grow
+# over iterations; the bar is on the fitted positions after a short run.
+POS_TOL = 2e-3    # absolute, world units (the sphere has radius 1)
+LOSS_RTOL = 1e-3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _job(subdiv, res, seed, mode=drh.LAMBERT, views=1):
+    m = S.with_albedo(S.icosphere(subdiv), 0.8)
+    t = S.with_albedo(S.radial_displace(S.icosphere(subdiv), 0.1, seed=seed), 0.8)
+    cams = S.cameras_ring(views, res, res) if views > 1 else [S.camera_look((0.4, 0.6, 2.9), res, res)]
+    return m, t, cams
+
+
+def _targets(t, cams, spp, lam):
+    r = drh.DiffRender(mode=drh.LAMBERT, spp=spp, light=lam)
+    r.commit([t])
+    imgs = r.render(cams)
+    r.close()
+    return imgs
+
+
+def _fit_alone(m, cams, targets, iters, N, spp, lam, graph=True):
+    r = drh.DiffRender(mode=drh.LAMBERT, spp=spp, light=lam)
+    r.commit([m])
+    r.opt_init(drh.OPT_ADAM, 1e-3, 1e-3)
+    drh.load().drh_opt_use_graph(r.h, int(graph))
+    dt = r.device_array(np.concatenate([t.reshape(-1) for t in targets]))
+    dl = r.device_array(np.zeros(1, np.float32))
+    cs = drh.cameras_array(cams)
+    losses = []
+    for _ in range(iters):
+        r.opt_step_dev(cs, len(cams), dt.ptr, N, dl.ptr)
+        losses.append(float(dl.download(1)[0]))
+    pos, _ = r.get_params()
+    r.close()
+    return pos, np.array(losses)
+
+
+def test_graph_replay_matches_eager_steps():
+    lam = S.lambert_config2()
+    m, t, cams = _job(3, 96, 11)
+    tg = _targets(t, cams, 4, lam)
+    say("eager")
+    pe, le = _fit_alone(m, cams, tg, 12, 1 << 14, 4, lam, graph=False)
+    say("graph")
+    pg, lg = _fit_alone(m, cams, tg, 12, 1 << 14, 4, lam, graph=True)
+    assert np.allclose(lg, le, rtol=LOSS_RTOL), (lg, le)
+    assert np.abs(pg - pe).max() < POS_TOL
+    assert lg[-1] < lg[0]   # it fits
+
+
+def test_graph_survives_interleaved_calls():
+    """Another render with other cameras between replays must not leak into the graph."""
+    lam = S.lambert_config2()
+    m, t, cams = _job(3, 96, 12)
+    tg = _targets(t, cams, 4, lam)
+    pe, le = _fit_alone(m, cams, tg, 8, 1 << 14, 4, lam, graph=False)
+    r = drh.DiffRender(mode=drh.LAMBERT, spp=4, light=lam)
+    r.commit([m])
+    r.opt_init(drh.OPT_ADAM, 1e-3, 1e-3)
+    dt = r.device_array(tg[0].reshape(-1))
+    dl = r.device_array(np.zeros(1, np.float32))
+    cs = drh.cameras_array(cams)
+    other = S.cameras_ring(2, 64, 64)
+    losses = []
+    for k in range(8):
+        r.opt_step_dev(cs, 1, dt.ptr, 1 << 14, dl.ptr)
+        losses.append(float(dl.download(1)[0]))
+        if k == 3:
+            r.render(other)   # different camera block and image buffers
+    pos, _ = r.get_params()
+    assert np.allclose(losses, le, rtol=LOSS_RTOL)
+    assert np.abs(pos - pe).max() < POS_TOL
+
+
+def test_server_result_alone_equals_result_in_mix():
+    lam = S.lambert_config2()
+    iters, N, spp = 10, 1 << 14, 4
+    jobs = [_job(3, 96, 21), _job(4, 128, 22), _job(2, 64, 23), _job(3, 128, 24, views=2)]
+    tgts = [_targets(t, c, spp, lam) for _, t, c in jobs]
+    m0, _, c0 = jobs[0]
+    say("alone (handle)")
+    p_alone, l_alone = _fit_alone(m0, c0, tgts[0], iters, N, spp, lam)
+    srv = drh.Server(hbm_budget=8 << 30, max_active=4)
+    say("alone (server)")
+    j = srv.submit([m0], c0, tgts[0], iters, N, mode=drh.LAMBERT, spp=spp, light=lam)
+    r_alone = srv.wait(j)
+    assert r_alone["status"] == 0 and r_alone["iterations"] == iters
+    say("mix")
+    ids = [srv.submit([m], c, tg, iters, N, mode=drh.LAMBERT, spp=spp, light=lam)
+           for (m, _, c), tg in zip(jobs, tgts)]
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    say("stats", st)
+    assert all(r["status"] == 0 and r["iterations"] == iters for r in res)
+    assert st["peak_active"] >= 2
+    for r in (r_alone, res[0]):
+        assert np.abs(r["pos"] - p_alone).max() < POS_TOL
+        assert np.isclose(r["final_loss"], l_alone[-1], rtol=LOSS_RTOL)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_server_admission_by_hbm_budget():
+    lam = S.lambert_config2()
+    m, t, cams = _job(2, 64, 31)
+    tg = _targets(t, cams, 1, lam)
+    # a budget below one job's footprint refuses it
+    srv = drh.Server(hbm_budget=1 << 16, max_active=4)
+    j = srv.submit([m], cams, tg, 2, 1 << 10, mode=drh.LAMBERT, spp=1, light=lam)
+    need = srv.last_job_bytes
+    r = srv.wait(j, timeout=60)
+    assert need > (1 << 16) and r["status"] == -4
+    srv.close()
+    # a budget that holds one job at a time still completes every job, one by one
+    srv = drh.Server(hbm_budget=int(1.5 * need), max_active=4)
+    ids = [srv.submit([m], cams, tg, 3, 1 << 10, mode=drh.LAMBERT, spp=1, light=lam) for _ in range(3)]
+    res = [srv.wait(i, timeout=120) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert all(x["status"] == 0 for x in res)
+    assert st["peak_active"] == 1 and st["jobs_done"] == 3
+
+
+def test_server_admits_jobs_arriving_while_others_run():
+    lam = S.lambert_config2()
+    srv = drh.Server(hbm_budget=8 << 30, max_active=3)
+    jobs = [_job(3, 96, 40 + k) for k in range(5)]
+    tgts = [_targets(t, c, 1, lam) for _, t, c in jobs]
+    ids = []
+    for (m, _, c), tg in zip(jobs, tgts):
+        ids.append(srv.submit([m], c, tg, 15, 1 << 12, mode=drh.LAMBERT, spp=1, light=lam))
+        time.sleep(0.02)
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert all(r["status"] == 0 and r["iterations"] == 15 for r in res)
+    assert st["jobs_done"] == 5 and st["peak_active"] <= 3

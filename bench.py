@@ -4,9 +4,11 @@ directional light and constant albedo 0.8, 1024x1024, 4 spp, 2^20 edge samples, 
 
 A step = re-commit from the device-resident vertices (LBVH rebuild), render + L2 loss +
 adjoint, interior adjoint, edge sampling, Adam update of the vertex positions -- all on the
-handle's stream, inputs resident in HBM.  Prints one JSON line.
+handle's stream, inputs resident in HBM.  Beside it: the plain one-job-at-a-time loop
+through the host interface, and the served form (several config-2 jobs on the
+continuous-batching server) with its p50/p99 per-iteration latency.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-serve]
 """
 from __future__ import annotations
 
@@ -47,6 +49,63 @@ def measured_traffic(kernel):
         return None
 
 
+def plain_leg(drh, meshes, cam, target, N, spp, lam, iters):
+    """The plain one-job-at-a-time GPU path: the reference's optimiser loop
+    (src/optimizer_simple.cpp:272-309) driving the drop-in through its host interface --
+    per iteration commit(scene), d_render_and_compare with host target and host gradient
+    (src/drender.h:169-191), then Adam on the host and the scene update.  Returns iters/s."""
+    r = drh.DiffRender(mode=drh.LAMBERT, spp=spp, light=lam)
+    m = meshes[0].copy()
+    V = m.V
+    r.commit([m])
+    P = r.grad_size()
+    mom, vel = np.zeros(P, np.float32), np.zeros(P, np.float32)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+
+    def it(t):
+        r.commit([m])
+        _, g, _ = r.d_render_and_compare([cam], [target], N, images=False)
+        mom[:] = b1 * mom + (1 - b1) * g
+        vel[:] = b2 * vel + (1 - b2) * g * g
+        step = lr * (mom / (1 - b1 ** t)) / (np.sqrt(vel / (1 - b2 ** t)) + eps)
+        m.pos = (m.pos.reshape(-1) - step[:3 * V]).reshape(-1, 3).astype(np.float32)
+
+    it(1)
+    t0 = time.perf_counter()
+    for t in range(2, iters + 2):
+        it(t)
+    dt = time.perf_counter() - t0
+    r.close()
+    return iters / dt
+
+
+def served_leg(drh, S, meshes, cam, N, spp, lam, jobs, iters):
+    """The served form: `jobs` config-2 mesh-fitting jobs (distinct targets) submitted to the
+    continuous-batching server at once; each keeps its scene, LBVH, Adam moments and target
+    on the device and steps by graph replay, the jobs' steps overlapping on the GPU.  Wall
+    time from the first submit to the last result (admission and downloads included)."""
+    targets = []
+    for k in range(jobs):
+        t = S.with_albedo(S.radial_displace(S.icosphere(5, 1.0), 0.1, seed=100 + k), 0.8)
+        rt = drh.DiffRender(mode=drh.LAMBERT, spp=spp, light=lam)
+        rt.commit([t])
+        targets.append(rt.render([cam]))
+        rt.close()
+    srv = drh.Server(hbm_budget=64 << 30, max_active=jobs)
+    t0 = time.perf_counter()
+    ids = [srv.submit(meshes, [cam], targets[k], iters, N, mode=drh.LAMBERT, spp=spp, light=lam)
+           for k in range(jobs)]
+    res = [srv.wait(i) for i in ids]
+    dt = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    assert all(x["status"] == 0 for x in res), [x["status"] for x in res]
+    return {"iters_per_s": round(jobs * iters / dt, 3), "jobs": jobs, "iterations_per_job": iters,
+            "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4),
+            "peak_active": st["peak_active"], "mean_active": round(st["mean_active"], 3),
+            "wall_s": round(dt, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +113,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-jobs", type=int, default=4, help="concurrent config-2 jobs in the served leg")
+    ap.add_argument("--serve-iters", type=int, default=50, help="Adam iterations per served job")
+    ap.add_argument("--plain-iters", type=int, default=20, help="iterations of the plain host-driven loop")
+    ap.add_argument("--no-serve", action="store_true", help="skip the plain and served legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (k_copy_f32), the known-size reference "
                          "for FETCH_SIZE / WRITE_SIZE counters")
@@ -159,6 +222,14 @@ def main():
                      "algorithmic_bytes": bytes_dom},
         "cpu_baseline": None,
     }
+
+    if not args.no_serve:
+        plain = plain_leg(drh, meshes, cam, target, N, spp, lam, args.plain_iters)
+        served = served_leg(drh, S, meshes, cam, N, spp, lam, args.serve_jobs, args.serve_iters)
+        result["plain_one_job_iters_per_s"] = round(plain, 3)
+        result["served"] = served
+        result["served_over_plain"] = round(served["iters_per_s"] / plain, 3)
+        result["p50_ms"], result["p99_ms"] = served["p50_ms"], served["p99_ms"]
 
     if not args.no_cpu:
         import oracle as O

@@ -252,8 +252,6 @@ struct drh_renderer {
   std::map<std::string, std::pair<double, int>> prof_acc;
   std::vector<hipEvent_t> event_pool;
 
-  // traversal stack kind (see StackCfg): 0 = 16-bit entries, 1 = 32-bit, 2 = 32-bit and 64 deep
-  int deep() const { return F > DEEP_F ? 2 : (F > 65535 ? 1 : 0); }
 
   drh_status fail(drh_status s, const char* fmt, ...) {
     char buf[512];
@@ -311,7 +309,6 @@ struct drh_renderer {
     s.n_meshes = nm;
     t.bf_a = bf_a.as<float4>(); t.bf_e1 = bf_e1.as<float4>(); t.bf_e2 = bf_e2.as<float4>();
     t.use_bf = st.tracer == DRH_TRACER_BF;
-    t.depth = deep() == 2 ? 64 : 48;  // matches the kernel's LDS stack (DEPTH template argument)
     t.cam = {nullptr, nullptr};
     t.light = {light_sus ? sus_light.as<float4>() : nullptr, light_sus ? sus_light_cnt.as<int>() : nullptr};
     return t;
@@ -550,7 +547,7 @@ drh_status drh_renderer::build_bvh() {
                  node_parent.as<int>(), leaf_parent.as<int>(), counter.as<uint32_t>(), leaf_lo.as<float4>(),
                  leaf_hi.as<float4>(), node_lo.as<float4>(), node_hi.as<float4>()};
     timed("k_refit", [&] { launch_refit(ra, stream); });
-    PackArgs pk{F, child.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
+    PackArgs pk{F, bounds.as<uint32_t>(), child.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
                 node_hi.as<float4>(), nodes.as<float4>()};
     timed("k_pack_nodes", [&] { launch_pack(pk, stream); });
   }
@@ -602,7 +599,7 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   int part_off = 0;
   for (int v = 0; v < views; v++) {
     RenderArgs ra{tracer_view(v), light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off};
-    timed("k_render", [&] { launch_render(st.mode, deep(), ra, tiles[v], stream); });
+    timed("k_render", [&] { launch_render(st.mode, ra, tiles[v], stream); });
     part_off += tiles[v];
   }
   if (d_target && d_loss) {
@@ -648,7 +645,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         ia.lds_base = 0; ia.lds_n = G;
         blocks = std::min(tiles, 1024);
       }
-      timed("k_interior", [&] { launch_interior(st.mode, deep(), ia, blocks, stream); });
+      timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
     }
     if (E > 0 && N > 0) {
       ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>()};
@@ -677,7 +674,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       timed("k_edge_bin", [&] { launch_edge_bin(ea, blocks, stream); });
       timed("k_edge_scan", [&] { launch_scan_u32(ebin.as<uint32_t>(), E + 1, stream); });
       timed("k_edge_scatter", [&] { launch_edge_scatter(ea, blocks, stream); });
-      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, deep(), ea, std::min(blocks, 1024), stream); });
+      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, ea, std::min(blocks, 1024), stream); });
     }
   }
   return hip(hipGetLastError(), "d_render kernels");
@@ -1045,7 +1042,7 @@ drh_status drh_debug_hits(drh_handle h, const drh_camera* cam, uint32_t* prim, u
   HitsArgs a{h->tracer_view(0), h->cams.as<DevCam>(), b.as<uint32_t>(), b.as<uint32_t>() + n, (float*)(b.as<uint32_t>() + 2 * n),
              (float*)(b.as<uint32_t>() + 3 * n), (float*)(b.as<uint32_t>() + 4 * n)};
   const int tiles = ((h->cam_cache[0].w + 7) / 8) * ((h->cam_cache[0].h + 7) / 8);
-  launch_debug_hits(h->deep(), a, tiles, h->stream);
+  launch_debug_hits(a, tiles, h->stream);
   if ((s = h->hip(hipGetLastError(), "debug_hits"))) return s;
   std::vector<uint32_t> hb(5 * (size_t)n);
   if ((s = h->hip(hipMemcpyAsync(hb.data(), b.p, 20 * (size_t)n, hipMemcpyDeviceToHost, h->stream), "download"))) return s;

@@ -127,63 +127,87 @@ __device__ __forceinline__ bool mt_test(float4 A4, float4 E14, float4 E24, F3 o,
   return v > 0.0f && u > 0.0f && (u + v < 1.0f) && t > tNear;
 }
 
-// Slab test against a padded child box; NaN terms (0 * inf) are dropped by fminf/fmaxf,
-// which keeps the test conservative.
+// Read-only scene data through the constant address space: a wave-uniform index then
+// compiles to a scalar load into SGPRs, shared by the 64 rays of the wavefront.
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) v4f cv4f;
+typedef const __attribute__((address_space(4))) int cint;
+__device__ __forceinline__ float4 uload(const float4* p, int i) {
+  const v4f v = ((cv4f*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int uload(const int* p) { return *(cint*)p; }
+
+// 1/d for the slab test; a zero (or denormal) component becomes +-1e30 so the fused form
+// below stays finite: that axis' slab is then all-pass or all-fail, as for an infinite 1/d.
+__device__ __forceinline__ float slab_rcp(float d) { return fabsf(d) < 1e-30f ? copysignf(1e30f, d) : 1.0f / d; }
+
+// Slab test of a padded child box, fused form t = fma(l, 1/d, -o/d).  Its rounding differs
+// from (l - o) * (1/d) by about 2^-24 (|o| + |l - o|) in world units, which the boxes'
+// padding (1e-5 relative plus an absolute term of the scene's scale) absorbs; the triangle
+// test itself stays exact, so the padding only decides how many boxes are opened.
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
-                                     F3 o, F3 id, float tmax, float& tn) {
-  const float t0x = (lx - o.x) * id.x, t1x = (hx - o.x) * id.x;
-  const float t0y = (ly - o.y) * id.y, t1y = (hy - o.y) * id.y;
-  const float t0z = (lz - o.z) * id.z, t1z = (hz - o.z) * id.z;
+                                     F3 id, F3 nod, float tmax, float& tn) {
+  const float t0x = fmaf(lx, id.x, nod.x), t1x = fmaf(hx, id.x, nod.x);
+  const float t0y = fmaf(ly, id.y, nod.y), t1y = fmaf(hy, id.y, nod.y);
+  const float t0z = fmaf(lz, id.z, nod.z), t1z = fmaf(hz, id.z, nod.z);
   const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
   const float tmx = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
   tn = tmin;
   return tmin <= tmx;
 }
 
-// LDS traversal stack depth: the Karras tree over (30-bit Morton key, triangle index) is at
-// most 30 + ceil(log2 F) levels deep, so 48 entries cover F <= 2^18 and 64 any F < 2^32.
-
-// Closest hit with the brute-force tracer's answer: the same MT test, and on equal t the
-// triangle that comes first in (mesh, triangle) order wins (BF's strict `t < hit.t`).
-// `stack` is this lane's column of an LDS stack laid out [depth][64 lanes].
-// Only internal-node indices are pushed, so 16-bit entries serve F <= 65535 (SK = uint16_t)
-// and halve the stack's LDS footprint.
-template <bool ANY, class SK>
-__device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar,
-                                         SK* stack, int lane_stride, int depth) {
+// LBVH walk, wavefront-coherent: the active lanes of a wave traverse together.  The node
+// index is wave-uniform, so a node's two child boxes and links arrive by one scalar load
+// for all 64 rays; each lane tests its own ray (per-lane closest-hit culling), ballots pick
+// the children any lane needs (nearer child first by majority), and the traversal stack is
+// wave-uniform too: 64 ints of LDS per wave (`wst`), written by one lane, read as a
+// broadcast; 64 entries cover the Karras tree's depth (<= 30 + ceil(log2 F) levels) for
+// F < 2^34.
+// A lane tests every leaf its own slab test opens and possibly more (leaves opened for
+// other lanes): the closest hit with the (t, triangle id) tie rule is the same over any
+// superset of the triangles a lane's own walk would test, so the result is the brute-force
+// tracer's answer for every lane.  Any-hit lanes drop out at their first hit.
+template <bool ANY>
+__device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar, int* wst) {
   Hit h{NO_HIT, tFar, 0.0f, 0.0f};
   if (s.F == 0) return h;
-  if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
-  const F3 id = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-  auto leaf = [&](int slot) -> bool {
+  bool live = !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z);
+  const F3 id = {slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z)};
+  const F3 nod = {-(o.x * id.x), -(o.y * id.y), -(o.z * id.z)};
+  auto leaf = [&](int slot, bool mine) {
+    const float4 A = uload(s.tri_a, slot), E1 = uload(s.tri_e1, slot), E2 = uload(s.tri_e2, slot);
     float t, u, v;
-    const float4 A = s.tri_a[slot];
-    if (mt_test(A, s.tri_e1[slot], s.tri_e2[slot], o, d, tNear, t, u, v)) {
+    if (mine && mt_test(A, E1, E2, o, d, tNear, t, u, v)) {
       const uint32_t g = __float_as_uint(A.w);
-      if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; return true; } }
+      if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; live = false; } }
       else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) { h.gtri = g; h.t = t; h.u = u; h.v = v; }
     }
-    return false;
   };
-  if (s.F == 1) { leaf(0); return h; }
+  if (s.F == 1) { leaf(0, live); return h; }
+  const bool leader = (int)(threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(1)) - 1;
   int sp = 0;
   int node = 0;
-  // a tree over F leaves has F-1 internal nodes, each visited at most once per ray
+  // a tree over F leaves has F-1 internal nodes, each visited at most once per wave
   for (int visits = 0; visits < s.F; visits++) {
-    const float4* n = s.nodes + 4 * node;
-    const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
+    const float4 n0 = uload(s.nodes, 4 * node), n1 = uload(s.nodes, 4 * node + 1);
+    const float4 n2 = uload(s.nodes, 4 * node + 2), n3 = uload(s.nodes, 4 * node + 3);
     const int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
     const float lim = h.t * 1.00002f + 1e-6f;
     float tl, tr;
-    const bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, id, lim, tl);
-    const bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, id, lim, tr);
+    const bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, id, nod, lim, tl) & live;
+    const bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, id, nod, lim, tr) & live;
+    const uint64_t bl = __ballot(hl), br = __ballot(hr);
     bool goL = false, goR = false;
-    if (hl) { if (cl < 0) { if (leaf(~cl) && ANY) return h; } else goL = true; }
-    if (hr) { if (cr < 0) { if (leaf(~cr) && ANY) return h; } else goR = true; }
+    if (bl) { if (cl < 0) leaf(~cl, hl); else goL = true; }
+    if (br) { if (cr < 0) leaf(~cr, hr); else goR = true; }
+    if (ANY && (bl | br) && !__ballot(live)) break;
     if (goL && goR) {
-      const bool lfirst = tl <= tr;
+      const uint64_t both = bl & br;
+      const uint64_t lf = __ballot(hl && hr && tl <= tr);
+      const bool lfirst = both ? 2 * __popcll(lf) >= __popcll(both) : __popcll(bl) >= __popcll(br);
       const int nearc = lfirst ? cl : cr, farc = lfirst ? cr : cl;
-      if (sp < depth) { stack[sp * lane_stride] = (SK)farc; sp++; }
+      if (sp < 64) { if (leader) wst[sp] = farc; sp++; }
       node = nearc;
     } else if (goL) {
       node = cl;
@@ -192,7 +216,7 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
     } else {
       if (sp == 0) break;
       sp--;
-      node = (int)stack[sp * lane_stride];
+      node = __builtin_amdgcn_readfirstlane(wst[sp]);
     }
   }
   return h;
@@ -206,7 +230,7 @@ __device__ __forceinline__ Hit trace_bf(const DevScene& s, const float4* tri_a_o
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
   for (int g = 0; g < s.F; g++) {
     float t, u, v;
-    if (mt_test(tri_a_orig[g], tri_e1_orig[g], tri_e2_orig[g], o, d, tNear, t, u, v) && t < h.t) {
+    if (mt_test(uload(tri_a_orig, g), uload(tri_e1_orig, g), uload(tri_e2_orig, g), o, d, tNear, t, u, v) && t < h.t) {
       h.gtri = (uint32_t)g; h.t = t; h.u = u; h.v = v;
       if (ANY && t < tFar) return h;
     }
@@ -237,7 +261,6 @@ struct Tracer {
   const float4* bf_e1;
   const float4* bf_e2;
   int use_bf;
-  int depth;        // entries of the caller's LDS stack column
   Suspects cam;     // for closest-hit (camera) rays of the current view
   Suspects light;   // for any-hit (shadow) rays
 };
@@ -246,24 +269,24 @@ template <bool ANY>
 __device__ __forceinline__ void test_suspects(const Tracer& T, const Suspects& S, F3 o, F3 d, float tNear, float tFar,
                                               Hit& h) {
   if (!S.count) return;
-  const int n = *S.count;
+  const int n = uload(S.count);
   for (int k = 0; k < n; k++) {
-    const float4 e = S.list[k];
+    const float4 e = uload(S.list, k);
     const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
     if (!(dn < 1.5e-6f)) continue;
     const uint32_t g = __float_as_uint(e.w);
     float t, u, v;
-    if (mt_test(T.bf_a[g], T.bf_e1[g], T.bf_e2[g], o, d, tNear, t, u, v)) {
+    if (mt_test(uload(T.bf_a, (int)g), uload(T.bf_e1, (int)g), uload(T.bf_e2, (int)g), o, d, tNear, t, u, v)) {
       if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; return; } }
       else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) { h.gtri = g; h.t = t; h.u = u; h.v = v; }
     }
   }
 }
 
-template <bool ANY, class SK>
-__device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, SK* stack, int ls) {
+template <bool ANY>
+__device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* wst) {
   if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
-  Hit h = trace_bvh<ANY, SK>(T.s, o, d, tNear, tFar, stack, ls, T.depth);
+  Hit h = trace_bvh<ANY>(T.s, o, d, tNear, tFar, wst);
   if (ANY && h.gtri != NO_HIT) return h;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
   test_suspects<ANY>(T, ANY ? T.light : T.cam, o, d, tNear, tFar, h);
@@ -308,12 +331,12 @@ __device__ __forceinline__ F2 blend2(const float* tc, uint32_t A, uint32_t B, ui
 
 // shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT.  `nrm` is the 3D scene's normals
 // for render and the projected scene's for edge samples (src/drender.h:353-354).
-template <int MODE, class SK>
+template <int MODE>
 __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
-                                    const float* nrm, SK* stack, int ls) {
+                                    const float* nrm, int* wst) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
-  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, stack, ls);
+  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
   if (h.gtri == NO_HIT) return {0, 0, 0};
   if (MODE == 1) return {1, 1, 1};
   const DevScene& s = T.s;
@@ -325,11 +348,17 @@ __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCa
   if (MODE == 3) return diffuse;
   const F3 n = blend3(ld3(nrm + 3 * A), ld3(nrm + 3 * B), ld3(nrm + 3 * C), h.u, h.v);
   const float BIAS = 1e-5f;
-  const F3 surf = add(o, mul(h.t - BIAS, d));
   const F3 negL = mul(-1.0f, L.L);
-  const Hit sh = trace<true>(T, surf, negL, 0.0f, 1e9f, stack, ls);
-  const float shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
-  const F3 lit = mul(mul(shadow, L.Lc), max_std(0.0f, dot(n, negL)));
+  const float ndl = max_std(0.0f, dot(n, negL));   // +0 or positive
+  // lit = (shadow * Lc) * ndl: with ndl == +0 and a finite light colour the shadow ray
+  // cannot change the result (bit for bit), so it is only cast where ndl > 0
+  float shadow = 1.0f;
+  if (ndl != 0.0f || !(fabsf(L.Lc.x) <= 3.4e38f && fabsf(L.Lc.y) <= 3.4e38f && fabsf(L.Lc.z) <= 3.4e38f)) {
+    const F3 surf = add(o, mul(h.t - BIAS, d));
+    const Hit sh = trace<true>(T, surf, negL, 0.0f, 1e9f, wst);
+    shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
+  }
+  const F3 lit = mul(mul(shadow, L.Lc), ndl);
   return mulv(add(lit, L.amb), diffuse);
 }
 

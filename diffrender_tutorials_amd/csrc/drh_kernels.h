@@ -5,7 +5,6 @@
 #include <stdint.h>
 
 #include <algorithm>
-#include <type_traits>
 
 #include "drh_device.h"
 
@@ -16,13 +15,6 @@ constexpr int CDF_CHUNK = 8192;            // pmf floats staged in LDS per CDF r
 constexpr int CDF_PAR_THREADS = 256;       // chunks of the parallel exact CDF
 constexpr int INTERIOR_LDS_FLOATS = 4096;  // largest gradient target accumulated in LDS (interior)
 constexpr int EDGE_LDS_FLOATS = 4096;      // largest position gradient accumulated in LDS (edges)
-constexpr int DEEP_F = 1 << 18;            // above this many triangles use the 64-deep traversal stack
-
-// LDS traversal stack per kind: depth and entry type (16-bit node indices when F <= 65535)
-template <int SC> struct StackCfg {
-  static constexpr int depth = SC == 2 ? 64 : 48;
-  using T = typename std::conditional<SC == 0, uint16_t, int>::type;
-};
 
 struct PrepareArgs {
   int V, n_meshes, has_nrm;
@@ -83,6 +75,7 @@ struct RefitArgs {
 
 struct PackArgs {
   int F;
+  const uint32_t* bounds;  // scene centroid bounds (ordered keys), for the absolute padding
   const int* child;
   const float4* leaf_lo;
   const float4* leaf_hi;
@@ -217,19 +210,19 @@ void launch_refit(const RefitArgs& a, hipStream_t s);
 void launch_pack(const PackArgs& a, hipStream_t s);
 void launch_reorder(const ReorderArgs& a, hipStream_t s);
 void launch_suspects(const SuspectArgs& a, hipStream_t s);
-void launch_render(int mode, int deep, const RenderArgs& a, int tiles, hipStream_t s);
+void launch_render(int mode, const RenderArgs& a, int tiles, hipStream_t s);
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s);
-void launch_interior(int mode, int deep, const InteriorArgs& a, int blocks, hipStream_t s);
+void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s);
 void launch_project(const ProjectArgs& a, hipStream_t s);
 void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s);
 void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s, bool parallel = true);
-void launch_edge_sample(int mode, int deep, const EdgeArgs& a, int blocks, hipStream_t s);
+void launch_edge_sample(int mode, const EdgeArgs& a, int blocks, hipStream_t s);
 void launch_edge_bin(const EdgeArgs& a, int blocks, hipStream_t s);
 void launch_edge_scatter(const EdgeArgs& a, int blocks, hipStream_t s);
 void launch_scan_u32(uint32_t* p, int n, hipStream_t s);
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);
 void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_step_inc(int* step, hipStream_t s);
-void launch_debug_hits(int deep, const HitsArgs& a, int tiles, hipStream_t s);
+void launch_debug_hits(const HitsArgs& a, int tiles, hipStream_t s);
 
 }  // namespace drh

@@ -281,10 +281,12 @@ __global__ void k_refit(RefitArgs a) {
 
 // Box padding: the MT test can accept a point a few ulps outside its triangle, and the slab
 // test rounds; widen every box so traversal never culls a triangle BF would accept.
-__device__ __forceinline__ void pad_box(float4& lo, float4& hi) {
-  const float px = 1e-5f * (fabsf(lo.x) + fabsf(hi.x) + (hi.x - lo.x)) + 1e-30f;
-  const float py = 1e-5f * (fabsf(lo.y) + fabsf(hi.y) + (hi.y - lo.y)) + 1e-30f;
-  const float pz = 1e-5f * (fabsf(lo.z) + fabsf(hi.z) + (hi.z - lo.z)) + 1e-30f;
+// Child boxes are padded by 1e-5 of their own magnitude plus 1e-6 of the scene's (the largest
+// |coordinate| of the triangle centroids): the slab test's rounding scales with |origin|.
+__device__ __forceinline__ void pad_box(float4& lo, float4& hi, float pa) {
+  const float px = 1e-5f * (fabsf(lo.x) + fabsf(hi.x) + (hi.x - lo.x)) + pa;
+  const float py = 1e-5f * (fabsf(lo.y) + fabsf(hi.y) + (hi.y - lo.y)) + pa;
+  const float pz = 1e-5f * (fabsf(lo.z) + fabsf(hi.z) + (hi.z - lo.z)) + pa;
   lo.x -= px; lo.y -= py; lo.z -= pz;
   hi.x += px; hi.y += py; hi.z += pz;
 }
@@ -296,8 +298,14 @@ __global__ void k_pack_nodes(PackArgs a) {
   float4 h0 = c0 < 0 ? a.leaf_hi[~c0] : a.node_hi[c0];
   float4 l1 = c1 < 0 ? a.leaf_lo[~c1] : a.node_lo[c1];
   float4 h1 = c1 < 0 ? a.leaf_hi[~c1] : a.node_hi[c1];
-  pad_box(l0, h0);
-  pad_box(l1, h1);
+  float S = 0.0f;
+  for (int k = 0; k < 6; k++) {
+    const float b = key2f(a.bounds[k]);
+    if (fabsf(b) <= 3.4e38f) S = fmaxf(S, fabsf(b));
+  }
+  const float pa = 1e-6f * S + 1e-30f;
+  pad_box(l0, h0, pa);
+  pad_box(l1, h1, pa);
   float4* n = a.nodes + 4 * i;
   n[0] = make_float4(l0.x, l0.y, l0.z, h0.x);
   n[1] = make_float4(h0.y, h0.z, l1.x, l1.y);
@@ -342,13 +350,10 @@ __global__ void k_reorder(ReorderArgs a) {
 // ---------------------------------------------------------------------------
 // render (src/drender.h:98-141) with the L2 loss fused (src/utils.cpp:15-27)
 // ---------------------------------------------------------------------------
-template <int MODE, int SC>
+template <int MODE>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
-  constexpr int DEPTH = StackCfg<SC>::depth;
-  using SK = typename StackCfg<SC>::T;
-  __shared__ SK stack[DEPTH * 64];
-  Tracer T = a.tr;
-  T.depth = DEPTH;
+  __shared__ int wst[64];   // the wave's traversal stack
+  const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
   const int tiles_x = (c.w + 7) >> 3;
@@ -365,7 +370,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
       for (int dx = 0; dx < sq; dx++) {
         const float xo = (dx + 0.5f) / float(sq);
         const float yo = (dy + 0.5f) / float(sq);
-        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, stack + lane, 64);
+        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst);
         acc = add(acc, divs(col, ns));
       }
     const long long p = c.img_off + 3ll * ((long long)y * c.w + x);
@@ -402,14 +407,11 @@ __global__ __launch_bounds__(1024) void k_loss_final(const double* part, int n, 
 // interior_derivatives (src/drender.h:197-233): Hammersley subpixel offsets,
 // shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74) / <TEXTURE_COLOR> (:132-162, LAMBERT too)
 // ---------------------------------------------------------------------------
-template <int MODE, int SC>
+template <int MODE>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
-  constexpr int DEPTH = StackCfg<SC>::depth;
-  using SK = typename StackCfg<SC>::T;
-  __shared__ SK stack[DEPTH * 64];
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
-  Tracer T = a.tr;
-  T.depth = DEPTH;
+  __shared__ int wst[64];
+  const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
   const bool use_lds = a.lds_n > 0;
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
       const F3 val = divs(adj, (float)a.nsamp);
       F3 o, d;
       eye_ray(c, x + xo, y + yo, o, d);
-      const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, stack + lane, 64);
+      const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
       if (h.gtri == NO_HIT) continue;
       const int m = a.tr.s.tri_mesh[h.gtri];
       const int goff = a.layout[4 * m + 0];
@@ -745,18 +747,15 @@ __device__ __forceinline__ bool seg_reduce(int key, float (&v)[K]) {
   return lane == 63 || next != key;
 }
 
-template <int MODE, int SC>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
-  constexpr int DEPTH = StackCfg<SC>::depth;
-  using SK = typename StackCfg<SC>::T;
-  __shared__ SK stack[4 * DEPTH * 64];
   extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
   __shared__ float ltr[4][12];
-  Tracer T = a.tr;
-  T.depth = DEPTH;
+  __shared__ int wstack[4][64];
+  const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  SK* stk = stack + wave * DEPTH * 64 + lane;
+  int* wst = wstack[wave];
   const bool use_lds = a.lds_n > 0;
   for (int i = threadIdx.x; i < a.lds_n; i += 256) lpos[i] = 0.0f;
   float tr_acc[12];
@@ -793,7 +792,7 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     for (int sd = 0; sd < 2; sd++) {
       const float qx = sd == 0 ? px - 1e-3f * nx : px + 1e-3f * nx;
       const float qy = sd == 0 ? py - 1e-3f * ny : py + 1e-3f * ny;
-      const F3 col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, stk, 64);
+      const F3 col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
       if (sd == 0) side[0] = col; else side[1] = col;
     }
     const F3 cin = side[0], cout = side[1];
@@ -915,13 +914,9 @@ void launch_step_inc(int* step, hipStream_t s) { hipLaunchKernelGGL(k_step_inc, 
 // ---------------------------------------------------------------------------
 // debug: closest hit at pixel centres
 // ---------------------------------------------------------------------------
-template <int SC>
 __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
-  constexpr int DEPTH = StackCfg<SC>::depth;
-  using SK = typename StackCfg<SC>::T;
-  __shared__ SK stack[DEPTH * 64];
-  Tracer T = a.tr;
-  T.depth = DEPTH;
+  __shared__ int wst[64];
+  const Tracer& T = a.tr;
   const DevCam& c = a.cams[0];
   const int lane = threadIdx.x;
   const int tiles_x = (c.w + 7) >> 3;
@@ -930,7 +925,7 @@ __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
   if (x >= c.w || y >= c.h) return;
   F3 o, d;
   eye_ray(c, x + 0.5f, y + 0.5f, o, d);
-  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, stack + lane, 64);
+  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
   const int p = y * c.w + x;
   uint32_t prim = NO_HIT, geom = NO_HIT;
   if (h.gtri != NO_HIT) { geom = (uint32_t)a.tr.s.tri_mesh[h.gtri]; prim = h.gtri - (uint32_t)a.tr.s.mesh_to[geom]; }
@@ -947,12 +942,6 @@ __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
     case 3: { constexpr int M = 3; BODY; } break;  \
     default: { constexpr int M = 4; BODY; } break; \
   }
-// stack kinds: 0 = 48 x uint16 (F <= 65535), 1 = 48 x int (F <= 2^18), 2 = 64 x int
-#define DRH_DISPATCH_DEPTH(SKIND, BODY)                     \
-  if ((SKIND) == 2) { constexpr int D = 2; BODY; }          \
-  else if ((SKIND) == 1) { constexpr int D = 1; BODY; }     \
-  else { constexpr int D = 0; BODY; }
-
 void launch_fill_f32(float* p, long long n, float v, hipStream_t s) {
   if (n <= 0) return;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
@@ -1001,17 +990,17 @@ void launch_suspects(const SuspectArgs& a, hipStream_t s) {
 void launch_reorder(const ReorderArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
 }
-void launch_render(int mode, int deep, const RenderArgs& a, int tiles, hipStream_t s) {
-  DRH_DISPATCH_MODE(mode, DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_render<M, D>), dim3(tiles), dim3(64), 0, s, a)))
+void launch_render(int mode, const RenderArgs& a, int tiles, hipStream_t s) {
+  DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_render<M>), dim3(tiles), dim3(64), 0, s, a))
 }
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s) {
   hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1024), 0, s, part, n, out, denom);
 }
-void launch_interior(int mode, int deep, const InteriorArgs& a, int blocks, hipStream_t s) {
+void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s) {
   if (mode == 2) {
-    DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_interior<2, D>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a))
+    hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a);
   } else {
-    DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_interior<3, D>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a))
+    hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a);
   }
 }
 void launch_project(const ProjectArgs& a, hipStream_t s) {
@@ -1024,8 +1013,8 @@ void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t
   if (parallel) hipLaunchKernelGGL(k_cdf_par, dim3(1), dim3(CDF_PAR_THREADS), 0, s, pmf_raw, E, pmf, cdf);
   else hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
 }
-void launch_edge_sample(int mode, int deep, const EdgeArgs& a, int blocks, hipStream_t s) {
-  DRH_DISPATCH_MODE(mode, DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_edge_sample<M, D>), dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a)))
+void launch_edge_sample(int mode, const EdgeArgs& a, int blocks, hipStream_t s) {
+  DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_sample<M>), dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a))
 }
 void launch_edge_bin(const EdgeArgs& a, int blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_edge_bin, dim3(blocks), dim3(256), 0, s, a);
@@ -1044,8 +1033,8 @@ void launch_optsimple(const float* grad, float* m, float* G, int off, const OptA
 void launch_adam(const AdamArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_adam, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
-void launch_debug_hits(int deep, const HitsArgs& a, int tiles, hipStream_t s) {
-  DRH_DISPATCH_DEPTH(deep, hipLaunchKernelGGL((k_debug_hits<D>), dim3(tiles), dim3(64), 0, s, a))
+void launch_debug_hits(const HitsArgs& a, int tiles, hipStream_t s) {
+  hipLaunchKernelGGL(k_debug_hits, dim3(tiles), dim3(64), 0, s, a);
 }
 
 }  // namespace drh

@@ -184,17 +184,17 @@ class DiffRender:
         self._check(load().drh_d_render(self.h, cs, len(cams), _fp(adj), int(edge_samples), _fp(grad), int(acc)))
         return grad
 
-    def d_render_and_compare(self, cams, targets, edge_samples):
+    def d_render_and_compare(self, cams, targets, edge_samples, images=True):
         """IDiffRender::d_render_and_compare (src/drender.h:169-191): returns
-        (loss, grad, images)."""
+        (loss, grad, images); images is None when not requested (outImages = nullptr)."""
         cs = cameras_array(cams)
         tgt = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1) for a in targets]))
         grad = np.zeros(self.grad_size(), np.float32)
-        imgs = np.empty_like(tgt)
+        imgs = np.empty_like(tgt) if images else None
         loss = ctypes.c_float()
         self._check(load().drh_d_render_and_compare(self.h, cs, len(cams), _fp(tgt), int(edge_samples), _fp(grad),
-                                                    _fp(imgs), ctypes.byref(loss)))
-        return float(loss.value), grad, _split(imgs, cams)
+                                                    _fp(imgs) if images else None, ctypes.byref(loss)))
+        return float(loss.value), grad, (_split(imgs, cams) if images else None)
 
     # -- device-resident optimisation --------------------------------------
     def opt_init(self, kind=OPT_SIMPLE, pos_lr=0.1, col_lr=0.1):

@@ -188,6 +188,13 @@ def main():
         if cnt:
             prof[k] = {"ms_per_launch": tot / cnt, "launches_per_step": cnt / args.steps}
     r.profile(False)
+    # the same step without per-kernel events: graph replay, as the served form runs it
+    r.sync()
+    g0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    r.sync()
+    graph_ms = 1e3 * (time.perf_counter() - g0) / args.steps
     dom = max(("k_render", "k_interior", "k_edge_sample"), key=lambda k: prof.get(k, {}).get("ms_per_launch", 0))
     tiles = ((W + 7) // 8) * ((H + 7) // 8)
     E = len(r.edges()[0])
@@ -216,6 +223,7 @@ def main():
         "primary_rays_per_s": round(rays_per_step * args.steps / dt, 1),
         "edge_samples_per_s": round(N * args.steps / dt, 1),
         "loss_first_last": [loss0, loss1],
+        "ms_per_step_graph_replay": round(graph_ms, 4),
         "kernels": {k: round(v["ms_per_launch"], 5) for k, v in prof.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dom),

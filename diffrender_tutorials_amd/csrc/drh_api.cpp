@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -155,6 +156,58 @@ void niederreiter_table(uint32_t table[11 * 31]) {
   }
 }
 
+// qmc::rndFloat (common_src/qmc.cpp:190-197)
+float qmc_rnd_host(uint32_t pos, int dim, const uint32_t* table) {
+  uint32_t r = 0, d = pos;
+  for (int bit = 0; bit < 31; bit++, d >>= 1)
+    if (d & 1) r ^= table[bit + dim * 31];
+  return (float)(r + 1) * (1.0f / 2147483648.0f);
+}
+
+// The (rnd0, rnd1) pairs of sample indices 0..N-1 in either mode, ordered by rnd0 (ties by
+// index).  The sampled edge is a non-decreasing function of rnd0 (upper_bound in the CDF), so
+// in this order consecutive slots sample consecutive edges: the edge kernel's wavefronts trace
+// nearby rays and reduce each run of equal edges before one atomic, with no per-step sort.
+// A sample's contribution depends on its (rnd0, rnd1) only, so the order changes nothing but
+// the summation order of the gradient.  Tables are cached per (mode, N) for the process.
+std::shared_ptr<const std::vector<float>> sorted_sample_table(int mode, long long N) {
+  static std::mutex mu;
+  static std::map<std::pair<int, long long>, std::shared_ptr<const std::vector<float>>> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({mode, N});
+    if (it != cache.end()) return it->second;
+  }
+  std::vector<float> raw;
+  if (mode == DRH_RNG_REF_PRNG14) {
+    prng14_table(N, raw);
+  } else {
+    uint32_t tab[11 * 31];
+    niederreiter_table(tab);
+    raw.resize(2 * (size_t)N);
+    for (long long i = 0; i < N; i++) {
+      raw[2 * i] = qmc_rnd_host((uint32_t)i, 0, tab);
+      raw[2 * i + 1] = qmc_rnd_host((uint32_t)i, 1, tab);
+    }
+  }
+  // rnd0 is in [0, 1]: its bit pattern orders like its value
+  std::vector<uint64_t> key((size_t)N);
+  for (long long i = 0; i < N; i++) {
+    uint32_t b;
+    std::memcpy(&b, &raw[2 * i], 4);
+    key[i] = ((uint64_t)b << 32) | (uint64_t)i;
+  }
+  std::sort(key.begin(), key.end());
+  auto out = std::make_shared<std::vector<float>>(2 * (size_t)N);
+  for (long long k = 0; k < N; k++) {
+    const uint64_t i = key[k] & 0xFFFFFFFFull;
+    (*out)[2 * k] = raw[2 * i];
+    (*out)[2 * k + 1] = raw[2 * i + 1];
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  return cache.emplace(std::make_pair(mode, N), out).first->second;
+}
+
 // planeHammersley (common_src/qmc.cpp:227-243)
 void hammersley(int n, std::vector<float>& out) {
   out.resize(2 * (size_t)std::max(n, 1));
@@ -217,15 +270,15 @@ struct drh_renderer {
       dtex_off, dmats, dlayout, dtr_inv0;
   // LBVH
   DBuf bf_a, bf_e1, bf_e2, box_lo, box_hi, cen, bounds, keys0, vals0, keys1, vals1, hist, child, node_parent,
-      leaf_parent, counter, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
+      leaf_parent, counter, nrange, frontier, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
   // clamped-determinant suspects (per view for camera rays, per light for shadow rays)
   DBuf sus_view, sus_view_cnt, sus_light, sus_light_cnt;
   bool light_sus = false;
+  bool tc_zero = false;     // every texture coordinate of the committed scene is 0
   // edges
-  DBuf edges, edge_mesh, pmf_raw, pmf, cdf, proj;
-  DBuf ekey, ebin, eorder;   // edge-sample binning (sort by sampled edge)
+  DBuf edges, edge_mesh, pmf_raw, pmf, cdf, cdfw, proj;
   // sampling
-  DBuf rnd, qmc, hamm;
+  DBuf rnd, hamm;   // sorted edge-sample table, Hammersley offsets
   long long rnd_n = -1;
   // per call
   DBuf cams, img, adj, tgt, loss_part, loss, grad;
@@ -463,8 +516,13 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
   if ((s = up(pos_orig, d->pos, v3))) return s;
   if (d->col) { if ((s = up(col, d->col, v3))) return s; }
   else { launch_fill_f32(col.as<float>(), 3ll * V, 0.0f, stream); }
-  if (d->tc) { if ((s = up(tc, d->tc, 8 * (size_t)V))) return s; }
-  else { launch_fill_f32(tc.as<float>(), 2ll * V, 0.0f, stream); }
+  tc_zero = true;
+  if (d->tc) {
+    if ((s = up(tc, d->tc, 8 * (size_t)V))) return s;
+    for (size_t k = 0; k < 2 * (size_t)V && tc_zero; k++) tc_zero = d->tc[k] == 0.0f;
+  } else {
+    launch_fill_f32(tc.as<float>(), 2ll * V, 0.0f, stream);
+  }
   if (d->nrm) { if ((s = up(nrm_in, d->nrm, v3))) return s; }
   else {
     std::vector<float> def(3 * (size_t)V, 0.0f);
@@ -483,7 +541,7 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     E = (int)hedges.size();
     if ((s = R(edges, 8 * (size_t)E, "alloc")) || (s = R(edge_mesh, 4 * (size_t)E, "alloc")) ||
         (s = R(pmf_raw, 4 * (size_t)E, "alloc")) || (s = R(pmf, 4 * (size_t)E, "alloc")) ||
-        (s = R(cdf, 4 * (size_t)(E + 1), "alloc")))
+        (s = R(cdf, 4 * (size_t)(E + 1), "alloc")) || (s = R(cdfw, cdf_work_bytes(E), "alloc")))
       return s;
     if ((s = up(edges, hedges.data(), 8 * (size_t)E)) || (s = up(edge_mesh, hemesh.data(), 4 * (size_t)E))) return s;
     idx_local_cache = idx_local;
@@ -507,7 +565,8 @@ drh_status drh_renderer::build_bvh() {
   if ((s = R(bf_a, f16)) || (s = R(bf_e1, f16)) || (s = R(bf_e2, f16)) || (s = R(box_lo, f16)) || (s = R(box_hi, f16)) ||
       (s = R(cen, f16)) || (s = R(bounds, 32)) || (s = R(keys0, f4)) || (s = R(vals0, f4)) || (s = R(keys1, f4)) ||
       (s = R(vals1, f4)) || (s = R(hist, 4 * 256 * (size_t)nb)) || (s = R(child, 8 * (size_t)F)) ||
-      (s = R(node_parent, f4)) || (s = R(leaf_parent, f4)) || (s = R(counter, f4)) || (s = R(leaf_lo, f16)) ||
+      (s = R(node_parent, f4)) || (s = R(leaf_parent, f4)) || (s = R(counter, f4 + 4)) ||
+      (s = R(nrange, 2 * f4)) || (s = R(frontier, f4)) || (s = R(leaf_lo, f16)) ||
       (s = R(leaf_hi, f16)) || (s = R(node_lo, f16)) || (s = R(node_hi, f16)) || (s = R(nodes, 4 * f16)) ||
       (s = R(tri_a, f16)) || (s = R(tri_e1, f16)) || (s = R(tri_e2, f16)))
     return s;
@@ -540,12 +599,13 @@ drh_status drh_renderer::build_bvh() {
   }
   // after 4 passes the sorted keys / triangle order are back in keys0 / vals0
   if (F > 1) {
-    KarrasArgs ka{F, keys0.as<uint32_t>(), child.as<int>(), node_parent.as<int>(), leaf_parent.as<int>()};
+    KarrasArgs ka{F, keys0.as<uint32_t>(), nrange.as<int2>(), child.as<int>(), node_parent.as<int>(), leaf_parent.as<int>()};
     timed("k_karras", [&] { launch_karras(ka, stream); });
-    timed("k_fill_u32", [&] { launch_fill_u32(counter.as<uint32_t>(), F, 0u, stream); });
+    timed("k_fill_u32", [&] { launch_fill_u32(counter.as<uint32_t>(), F + 1, 0u, stream); });
     RefitArgs ra{F, vals0.as<uint32_t>(), box_lo.as<float4>(), box_hi.as<float4>(), child.as<int>(),
-                 node_parent.as<int>(), leaf_parent.as<int>(), counter.as<uint32_t>(), leaf_lo.as<float4>(),
-                 leaf_hi.as<float4>(), node_lo.as<float4>(), node_hi.as<float4>()};
+                 node_parent.as<int>(), leaf_parent.as<int>(), nrange.as<int2>(), counter.as<uint32_t>(),
+                 frontier.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
+                 node_hi.as<float4>()};
     timed("k_refit", [&] { launch_refit(ra, stream); });
     PackArgs pk{F, bounds.as<uint32_t>(), child.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
                 node_hi.as<float4>(), nodes.as<float4>()};
@@ -611,12 +671,14 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
 }
 
 drh_status drh_renderer::ensure_rnd(long long N) {
-  if (st.rng_mode != DRH_RNG_REF_PRNG14 || N == rnd_n) return DRH_OK;
-  std::vector<float> t;
-  prng14_table(N, t);
+  if (N == rnd_n) return DRH_OK;
+  if (N > 0x7FFFFFFFll) return fail(DRH_E_INVALID, "edge_samples too large");
   drh_status s;
   if ((s = hip(rnd.reserve(std::max<size_t>(8 * (size_t)N, 16)), "alloc"))) return s;
-  if (N > 0 && (s = hip(hipMemcpy(rnd.p, t.data(), 8 * (size_t)N, hipMemcpyHostToDevice), "rng upload"))) return s;
+  if (N > 0) {
+    const auto t = sorted_sample_table(st.rng_mode, N);
+    if ((s = hip(hipMemcpy(rnd.p, t->data(), 8 * (size_t)N, hipMemcpyHostToDevice), "rng upload"))) return s;
+  }
   rnd_n = N;
   return DRH_OK;
 }
@@ -640,6 +702,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       int blocks = tiles;
       if ((st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) && nm == 1 && diff[0] && tex_w[0] * tex_h[0] == 1) {
         ia.uniform = 1; ia.uniform_off = layout[2];
+        ia.coverage = tc_zero ? 1 : 0;
         blocks = std::min(tiles, 4096);
       } else if (G <= INTERIOR_LDS_FLOATS) {
         ia.lds_base = 0; ia.lds_n = G;
@@ -651,13 +714,13 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>()};
       timed("k_project", [&] { launch_project(pa, stream); });
       timed("k_edge_len", [&] { launch_edge_len(edges.as<int2>(), E, proj.as<float4>(), pmf_raw.as<float>(), stream); });
-      timed("k_cdf_par", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), stream, true); });
+      timed("k_cdf_par", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), cdfw.p, stream, true); });
       EdgeArgs ea{};
       ea.tr = tracer_view(v); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
       ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
       ea.proj = proj.as<float4>();
-      ea.rnd = st.rng_mode == DRH_RNG_REF_PRNG14 ? rnd.as<float2>() : nullptr;
-      ea.qmc = qmc.as<uint32_t>(); ea.adj = d_adj; ea.grad = d_grad; ea.layout = dlayout.as<int>();
+      ea.rnd = rnd.as<float2>();
+      ea.adj = d_adj; ea.grad = d_grad; ea.layout = dlayout.as<int>();
       ea.tr_inv0 = dtr_inv0.as<float>();
       ea.lds_base = 0; ea.lds_n = 0;
       if (single >= 0 && 3 * (vo[single + 1] - vo[single]) <= EDGE_LDS_FLOATS) {
@@ -665,16 +728,8 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       }
       ea.single_mesh = single >= 0 ? 1 : 0;
       ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
-      if ((s = hip(ekey.reserve(4 * (size_t)N), "alloc")) || (s = hip(eorder.reserve(4 * (size_t)N), "alloc")) ||
-          (s = hip(ebin.reserve(4 * (size_t)(E + 1)), "alloc")))
-        return s;
-      ea.ekey = ekey.as<int>(); ea.ebin = ebin.as<int>(); ea.order = eorder.as<int>();
-      const int blocks = (int)std::max<long long>(1, std::min<long long>((N + 255) / 256, 2048));
-      timed("k_fill_u32", [&] { launch_fill_u32(ebin.as<uint32_t>(), E + 1, 0u, stream); });
-      timed("k_edge_bin", [&] { launch_edge_bin(ea, blocks, stream); });
-      timed("k_edge_scan", [&] { launch_scan_u32(ebin.as<uint32_t>(), E + 1, stream); });
-      timed("k_edge_scatter", [&] { launch_edge_scatter(ea, blocks, stream); });
-      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, ea, std::min(blocks, 1024), stream); });
+      const int blocks = (int)std::max<long long>(1, std::min<long long>((N + 255) / 256, 1024));
+      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, ea, blocks, stream); });
     }
   }
   return hip(hipGetLastError(), "d_render kernels");
@@ -712,11 +767,8 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   }
   std::vector<float> hm;
   hammersley(settings->spp, hm);
-  uint32_t tab[11 * 31];
-  niederreiter_table(tab);
-  if (r->hamm.reserve(4 * hm.size()) != hipSuccess || r->qmc.reserve(sizeof tab) != hipSuccess) return DRH_E_OOM;
-  if (hipMemcpy(r->hamm.p, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(r->qmc.p, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess || r->loss.reserve(16) != hipSuccess)
+  if (r->hamm.reserve(4 * hm.size()) != hipSuccess) return DRH_E_OOM;
+  if (hipMemcpy(r->hamm.p, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice) != hipSuccess || r->loss.reserve(16) != hipSuccess)
     return DRH_E_HIP;
   *out = r.release();
   return DRH_OK;
@@ -1139,13 +1191,15 @@ extern "C" drh_status drh_debug_cdf(drh_handle h, const float* pmf_raw, int E, f
                                     int parallel) {
   if (!h || E <= 0 || !pmf_raw || !pmf_out || !cdf_out) return DRH_E_INVALID;
   (void)hipSetDevice(h->st.device);
-  DBuf a, p, c;
+  DBuf a, p, c, w;
   drh_status s;
   if ((s = h->hip(a.reserve(4 * (size_t)E), "alloc")) || (s = h->hip(p.reserve(4 * (size_t)E), "alloc")) ||
-      (s = h->hip(c.reserve(4 * (size_t)(E + 1)), "alloc")))
+      (s = h->hip(c.reserve(4 * (size_t)(E + 1)), "alloc")) || (s = h->hip(w.reserve(cdf_work_bytes(E)), "alloc")))
     return s;
   if ((s = h->hip(hipMemcpyAsync(a.p, pmf_raw, 4 * (size_t)E, hipMemcpyHostToDevice, h->stream), "upload"))) return s;
-  h->timed(parallel ? "k_cdf_par" : "k_cdf", [&] { launch_cdf(a.as<float>(), E, p.as<float>(), c.as<float>(), h->stream, parallel != 0); });
+  h->timed(parallel ? "k_cdf_par" : "k_cdf", [&] {
+    launch_cdf(a.as<float>(), E, p.as<float>(), c.as<float>(), w.p, h->stream, parallel != 0);
+  });
   if ((s = h->hip(hipGetLastError(), "debug_cdf"))) return s;
   if ((s = h->hip(hipMemcpyAsync(pmf_out, p.p, 4 * (size_t)E, hipMemcpyDeviceToHost, h->stream), "download")) ||
       (s = h->hip(hipMemcpyAsync(cdf_out, c.p, 4 * (size_t)(E + 1), hipMemcpyDeviceToHost, h->stream), "download")))

@@ -293,6 +293,18 @@ __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, f
   return h;
 }
 
+// Does a camera ray hit anything?  Any-hit traversal with the view's suspect list: the same
+// yes / no as the closest-hit query (both accept the first triangle whose MT test passes with
+// 0 < t < tFar; the closest-hit query only keeps looking for a nearer one).
+__device__ __forceinline__ bool covered(const Tracer& T, F3 o, F3 d, int* wst) {
+  if (T.use_bf) return trace_bf<true>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, 0.0f, 1e9f).gtri != NO_HIT;
+  Hit h = trace_bvh<true>(T.s, o, d, 0.0f, 1e9f, wst);
+  if (h.gtri != NO_HIT) return true;
+  if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return false;
+  test_suspects<true>(T, T.cam, o, d, 0.0f, 1e9f, h);
+  return h.gtri != NO_HIT;
+}
+
 // ---------------------------------------------------------------------------
 // Shading (src/shade_simple.cpp, src/shade_lambert.cpp, src/shade_common.cpp)
 // ---------------------------------------------------------------------------

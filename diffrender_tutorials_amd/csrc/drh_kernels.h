@@ -11,8 +11,12 @@
 namespace drh {
 
 constexpr int RADIX_TILE = 2048;           // keys per radix-sort block (8 rounds of 256)
+constexpr int REFIT_CHUNK = 1024;          // sorted leaves refit per workgroup in LDS (one per thread)
 constexpr int CDF_CHUNK = 8192;            // pmf floats staged in LDS per CDF round (sequential CDF)
-constexpr int CDF_PAR_THREADS = 256;       // chunks of the parallel exact CDF
+constexpr int CDF_CH = 128;                // edges per chunk of the parallel exact CDF
+constexpr int CDF_CPB = 64;                // CDF chunks per workgroup (one per lane)
+constexpr int CDF_WALK_STAGE = 8192;       // floats of LDS for the chunks replayed serially
+constexpr int CDF_WALK_CHUNKS = 8192;      // chunk records the serial walk reads from LDS
 constexpr int INTERIOR_LDS_FLOATS = 4096;  // largest gradient target accumulated in LDS (interior)
 constexpr int EDGE_LDS_FLOATS = 4096;      // largest position gradient accumulated in LDS (edges)
 
@@ -53,6 +57,7 @@ struct MortonArgs {
 struct KarrasArgs {
   int F;
   const uint32_t* keys;   // sorted
+  int2* range;            // per internal node: first and last sorted leaf it covers
   int* child;             // 2 per internal node
   int* node_parent;
   int* leaf_parent;
@@ -66,7 +71,9 @@ struct RefitArgs {
   const int* child;
   const int* node_parent;
   const int* leaf_parent;
-  uint32_t* counter;
+  const int2* range;
+  uint32_t* counter;      // F + 1, zeroed; counter[F] counts the frontier
+  int* frontier;          // F: nodes (or ~leaves) whose parent spans refit chunks
   float4* leaf_lo;
   float4* leaf_hi;
   float4* node_lo;
@@ -130,6 +137,7 @@ struct InteriorArgs {
   const int* layout;      // per mesh: grad offset (-1: not differentiable), colour, texture, transform offsets
   int lds_base, lds_n;    // gradient range accumulated in LDS (lds_n = 0: global atomics)
   int uniform, uniform_off;  // one 1x1 texture: wave-reduce into grad[uniform_off..+3)
+  int coverage;           // uniform and tc == 0 everywhere: only whether a sample hits matters
 };
 
 struct ProjectArgs {
@@ -152,17 +160,13 @@ struct EdgeArgs {
   const float* pmf;
   const float* cdf;
   const float4* proj;
-  const float2* rnd;      // REF_PRNG14 table, or NULL for QMC
-  const uint32_t* qmc;
+  const float2* rnd;      // N (rnd0, rnd1) pairs ordered by rnd0
   const float* adj;
   float* grad;
   const int* layout;
   const float* tr_inv0;   // 16 floats
   int lds_base, lds_n;
   int single_mesh, single_tr_off;
-  int* ekey;              // N: sampled edge of each sample, E = rejected
-  int* ebin;              // E+1: counts, then offsets, then ends; ebin[E] = accepted samples
-  int* order;             // N: accepted sample indices sorted by edge
 };
 
 constexpr int MAX_SEG = 8;
@@ -215,11 +219,10 @@ void launch_loss_final(const double* part, int n, float* out, double denom, hipS
 void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s);
 void launch_project(const ProjectArgs& a, hipStream_t s);
 void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s);
-void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s, bool parallel = true);
+// exact CDF; `work` holds cdf_work_bytes(E) of device memory (parallel form only)
+size_t cdf_work_bytes(int E);
+void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work, hipStream_t s, bool parallel = true);
 void launch_edge_sample(int mode, const EdgeArgs& a, int blocks, hipStream_t s);
-void launch_edge_bin(const EdgeArgs& a, int blocks, hipStream_t s);
-void launch_edge_scatter(const EdgeArgs& a, int blocks, hipStream_t s);
-void launch_scan_u32(uint32_t* p, int n, hipStream_t s);
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);
 void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_step_inc(int* step, hipStream_t s);

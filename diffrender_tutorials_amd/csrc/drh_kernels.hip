@@ -249,33 +249,97 @@ __global__ void k_karras(KarrasArgs a) {
   const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
   a.child[2 * i] = left;
   a.child[2 * i + 1] = right;
+  a.range[i] = make_int2(lo, hi);
   if (left < 0) a.leaf_parent[~left] = i; else a.node_parent[left] = i;
   if (right < 0) a.leaf_parent[~right] = i; else a.node_parent[right] = i;
   if (i == 0) a.node_parent[0] = -1;
 }
 
-__global__ void k_refit(RefitArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.F) return;
-  const int g = (int)a.order[s];
-  a.leaf_lo[s] = a.box_lo[g];
-  a.leaf_hi[s] = a.box_hi[g];
-  int p = a.leaf_parent[s];
-  for (int level = 0; p >= 0 && level < 64; level++) {  // tree depth <= 62
-    __threadfence();
-    if (atomicAdd(&a.counter[p], 1u) == 0u) return;  // the other child is still on its way
-    __threadfence();
-    const int c0 = a.child[2 * p], c1 = a.child[2 * p + 1];
-    // the sibling's box was written by another thread: read it past the non-coherent caches
-    const volatile float* lo0 = (const volatile float*)(c0 < 0 ? a.leaf_lo + ~c0 : a.node_lo + c0);
-    const volatile float* hi0 = (const volatile float*)(c0 < 0 ? a.leaf_hi + ~c0 : a.node_hi + c0);
-    const volatile float* lo1 = (const volatile float*)(c1 < 0 ? a.leaf_lo + ~c1 : a.node_lo + c1);
-    const volatile float* hi1 = (const volatile float*)(c1 < 0 ? a.leaf_hi + ~c1 : a.node_hi + c1);
-    const float4 L = make_float4(fminf(lo0[0], lo1[0]), fminf(lo0[1], lo1[1]), fminf(lo0[2], lo1[2]), 0.0f);
-    const float4 H = make_float4(fmaxf(hi0[0], hi1[0]), fmaxf(hi0[1], hi1[1]), fmaxf(hi0[2], hi1[2]), 0.0f);
+// Bottom-up refit in two passes without device-scope fences.  k_refit_local: one workgroup per
+// chunk of REFIT_CHUNK sorted leaves climbs from its leaves with LDS arrival counters and LDS
+// boxes while the parent's leaf range stays inside the chunk (the second arrival at a node
+// merges both child boxes and goes on); a node whose parent spans chunks is handed to the top
+// pass through `frontier`.  k_refit_top: one workgroup finishes the few spanning nodes the
+// same way with global counters; its boxes move between its own waves through L2 (volatile
+// loads), so workgroup-scope fences order them.
+__device__ __forceinline__ void box_union(float4& L, float4& H, float4 l, float4 h) {
+  L = make_float4(fminf(L.x, l.x), fminf(L.y, l.y), fminf(L.z, l.z), 0.0f);
+  H = make_float4(fmaxf(H.x, h.x), fmaxf(H.y, h.y), fmaxf(H.z, h.z), 0.0f);
+}
+__global__ __launch_bounds__(REFIT_CHUNK) void k_refit_local(RefitArgs a) {
+  __shared__ int cnt[REFIT_CHUNK];
+  __shared__ float lbox[REFIT_CHUNK][6];   // leaf boxes of the chunk's slots
+  __shared__ float nbox[REFIT_CHUNK][6];   // boxes of the internal nodes numbered in the chunk
+  __shared__ int2 lrange[REFIT_CHUNK], lchild[REFIT_CHUNK];
+  __shared__ int lpar[REFIT_CHUNK];
+  const int base = blockIdx.x * REFIT_CHUNK, end = min(base + REFIT_CHUNK, a.F);
+  const int k = threadIdx.x, sl = base + k;
+  // stage the chunk's leaves and the internal nodes numbered base..end-1 (coalesced)
+  cnt[k] = 0;
+  float4 L = {0, 0, 0, 0}, H = {0, 0, 0, 0};
+  int p = -1;
+  if (sl < end) {
+    const int g = (int)a.order[sl];
+    L = a.box_lo[g];
+    H = a.box_hi[g];
+    a.leaf_lo[sl] = L;
+    a.leaf_hi[sl] = H;
+    lbox[k][0] = L.x; lbox[k][1] = L.y; lbox[k][2] = L.z; lbox[k][3] = H.x; lbox[k][4] = H.y; lbox[k][5] = H.z;
+    p = a.leaf_parent[sl];
+  }
+  if (sl < end && sl < a.F - 1) {
+    lrange[k] = a.range[sl];
+    lchild[k] = make_int2(a.child[2 * sl], a.child[2 * sl + 1]);
+    lpar[k] = a.node_parent[sl];
+  }
+  __syncthreads();
+  if (sl >= end) return;
+  int q = ~sl;
+  // an internal node's number lies in its own leaf range, so a parent numbered outside the
+  // chunk spans chunks; one numbered inside spans iff its staged range leaves the chunk
+  for (int level = 0; p >= 0 && level < 64; level++) {
+    const int pk = p - base;
+    if (p < base || p >= end || lrange[pk].x < base || lrange[pk].y >= end) {
+      a.frontier[atomicAdd(a.counter + a.F, 1u)] = q;
+      return;
+    }
+    __threadfence_block();
+    if (atomicAdd(&cnt[pk], 1) == 0) return;  // the other child is still on its way
+    __threadfence_block();
+    const int2 c = lchild[pk];
+    const int sib = c.x == q ? c.y : c.x;
+    const float* sb = sib < 0 ? lbox[~sib - base] : nbox[sib - base];
+    box_union(L, H, make_float4(sb[0], sb[1], sb[2], 0.0f), make_float4(sb[3], sb[4], sb[5], 0.0f));
+    nbox[pk][0] = L.x; nbox[pk][1] = L.y; nbox[pk][2] = L.z; nbox[pk][3] = H.x; nbox[pk][4] = H.y; nbox[pk][5] = H.z;
     a.node_lo[p] = L;
     a.node_hi[p] = H;
-    p = a.node_parent[p];
+    q = p;
+    p = lpar[pk];
+  }
+}
+__global__ __launch_bounds__(1024) void k_refit_top(RefitArgs a) {
+  const int n = (int)a.counter[a.F];
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int q = a.frontier[k];
+    float4 L = q < 0 ? a.leaf_lo[~q] : a.node_lo[q];
+    float4 H = q < 0 ? a.leaf_hi[~q] : a.node_hi[q];
+    int p = q < 0 ? a.leaf_parent[~q] : a.node_parent[q];
+    for (int level = 0; p >= 0 && level < 64; level++) {
+      const int c0 = a.child[2 * p], c1 = a.child[2 * p + 1];   // in flight during the atomic
+      const int pp = a.node_parent[p];
+      __threadfence_block();
+      if (atomicAdd(&a.counter[p], 1u) == 0u) break;
+      __threadfence_block();
+      const int sib = c0 == q ? c1 : c0;
+      // a spanning sibling was written by another wave of this workgroup: read it from L2
+      const volatile float* sl = (const volatile float*)(sib < 0 ? a.leaf_lo + ~sib : a.node_lo + sib);
+      const volatile float* sh = (const volatile float*)(sib < 0 ? a.leaf_hi + ~sib : a.node_hi + sib);
+      box_union(L, H, make_float4(sl[0], sl[1], sl[2], 0.0f), make_float4(sh[0], sh[1], sh[2], 0.0f));
+      a.node_lo[p] = L;
+      a.node_hi[p] = H;
+      q = p;
+      p = pp;
+    }
   }
 }
 
@@ -430,6 +494,12 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
       const F3 val = divs(adj, (float)a.nsamp);
       F3 o, d;
       eye_ray(c, x + xo, y + yo, o, d);
+      if (a.coverage) {
+        // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
+        // hit, so the sample adds exactly `val` if the ray hits anything
+        if (covered(T, o, d, wst)) uni = add(uni, val);
+        continue;
+      }
       const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
       if (h.gtri == NO_HIT) continue;
       const int m = a.tr.s.tri_mesh[h.gtri];
@@ -548,129 +618,224 @@ __global__ __launch_bounds__(1024) void k_cdf(const float* pmf_raw, int E, float
 }
 
 // The same sequential float running sum, computed in parallel and exactly.
-// Chunk j (of CDF_PAR_THREADS) starts from a speculated value s_j = float(double prefix) and
-// runs the float recurrence c <- p + c over its range.  If the true start T_j differs, the
-// true sequence is the speculated one shifted by d = T_j - s_j whenever every speculated value
+// Chunk j (CDF_CH edges) starts from a speculated value s_j = float(double prefix) and runs
+// the float recurrence c <- p + c over its range.  If the true start T_j differs, the true
+// sequence is the speculated one shifted by d = T_j - s_j whenever every speculated value
 // stays in one binade (one grid spacing u), no addition was a rounding tie (TwoSum error
 // exactly u/2; ties-to-even depends on parity, which an odd shift flips), and the shifted
 // end stays in the binade: round-to-nearest commutes with adding a grid multiple.  One lane
-// walks the chunks applying shifts (recomputing a chunk where the conditions fail); then every
-// chunk is recomputed from its true start and written out normalised by the total.
+// walks the chunks applying shifts (replaying a chunk where the conditions fail, from LDS);
+// then every chunk is recomputed from its true start and written out normalised by the total.
+//   k_cdf_sums   double sum per chunk            k_cdf_scan   speculated starts
+//   k_cdf_spec   speculated ends, shift test     k_cdf_walk   true starts (serial)
+//   k_cdf_final  cdf / pmf normalised
 __device__ __forceinline__ int fexp(float x) { return (int)((__float_as_uint(x) >> 23) & 0xFFu); }
 
-// Stage the next CDF_WIN elements of every chunk into tile[chunk][*]: consecutive lanes read
-// consecutive floats of a chunk row, so the block's loads coalesce and overlap.
-constexpr int CDF_WIN = 32;
-__device__ __forceinline__ void cdf_stage(const float* __restrict__ src, int E, int C, int off,
-                                          float (*tile)[CDF_WIN + 1]) {
-  for (int q = threadIdx.x; q < CDF_PAR_THREADS * CDF_WIN; q += CDF_PAR_THREADS) {
-    const int row = q / CDF_WIN, col = q % CDF_WIN;
-    const int k = row * C + off + col;
-    const int end = min(row * C + C, E);
-    tile[row][col] = k < end ? src[k] : 0.0f;
+struct CdfWork {
+  double* csum;   // NCH
+  float* s0;      // NCH speculated starts
+  float* se;      // NCH speculated ends
+  int* ok;        // NCH shift-invariant
+  float* ts;      // NCH + 1 true starts, ts[NCH] = total
+};
+__host__ __device__ inline int cdf_chunks(int E) { return (E + CDF_CH - 1) / CDF_CH; }
+size_t cdf_work_bytes(int E) {
+  const size_t n = (size_t)cdf_chunks(E);
+  return 8 * n + 4 * n * 3 + 4 * (n + 1) + 64;
+}
+static CdfWork cdf_work(void* w, int E) {
+  const size_t n = (size_t)cdf_chunks(E);
+  char* p = (char*)w;
+  CdfWork c;
+  c.csum = (double*)p; p += 8 * n;
+  c.s0 = (float*)p; p += 4 * n;
+  c.se = (float*)p; p += 4 * n;
+  c.ok = (int*)p; p += 4 * n;
+  c.ts = (float*)p;
+  return c;
+}
+
+// CDF_CPB chunks of src into t[chunk][*] by the workgroup's CDF_STAGE_T threads, coalesced,
+// every load of a thread issued before its LDS stores; lane L of wave 0 then walks row L
+// (stride CDF_CH + 1 floats: conflict-free)
+constexpr int CDF_STAGE_T = 256;
+__device__ __forceinline__ void cdf_stage(const float* __restrict__ src, int E, int chunk0,
+                                          float (*t)[CDF_CH + 1]) {
+  constexpr int PER = CDF_CPB * CDF_CH / CDF_STAGE_T;
+  const int base = chunk0 * CDF_CH;
+  float v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    const int k = base + (int)threadIdx.x + i * CDF_STAGE_T;
+    v[i] = k < E ? src[k] : 0.0f;
+  }
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    const int q = (int)threadIdx.x + i * CDF_STAGE_T;
+    t[q / CDF_CH][q % CDF_CH] = v[i];
   }
 }
 
-__global__ __launch_bounds__(CDF_PAR_THREADS) void k_cdf_par(const float* __restrict__ pmf_raw, int E,
-                                                               float* __restrict__ pmf, float* __restrict__ cdf) {
-  __shared__ float tile[CDF_PAR_THREADS][CDF_WIN + 1];
-  __shared__ double dsum[CDF_PAR_THREADS];
-  __shared__ float spec_start[CDF_PAR_THREADS], spec_end[CDF_PAR_THREADS], true_start[CDF_PAR_THREADS + 1];
-  __shared__ int ok[CDF_PAR_THREADS];
-  const int j = threadIdx.x;
-  const int C = (E + CDF_PAR_THREADS - 1) / CDF_PAR_THREADS;
-  const int b = min(j * C, E), n = min(b + C, E) - b;
-  // speculated starts: exclusive prefix of chunk sums in double
-  double s = 0.0;
-  for (int off = 0; off < C; off += CDF_WIN) {
-    __syncthreads();
-    cdf_stage(pmf_raw, E, C, off, tile);
-    __syncthreads();
-    const int m = min(CDF_WIN, n - off);
-    for (int q = 0; q < m; q++) s += (double)tile[j][q];
-  }
-  dsum[j] = s;
+__global__ __launch_bounds__(CDF_STAGE_T) void k_cdf_sums(const float* __restrict__ raw, int E, CdfWork w) {
+  __shared__ float t[CDF_CPB][CDF_CH + 1];
+  cdf_stage(raw, E, blockIdx.x * CDF_CPB, t);
   __syncthreads();
-  for (int o = 1; o < CDF_PAR_THREADS; o <<= 1) {
-    const double v = j >= o ? dsum[j - o] : 0.0;
+  const int j = blockIdx.x * CDF_CPB + threadIdx.x;
+  if (threadIdx.x >= CDF_CPB || j >= cdf_chunks(E)) return;
+  const int n = min(CDF_CH, E - j * CDF_CH);
+  double s = 0.0;
+  for (int q = 0; q < n; q++) s += (double)t[threadIdx.x][q];
+  w.csum[j] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_cdf_scan(int E, CdfWork w) {
+  __shared__ double part[1024];
+  const int n = cdf_chunks(E);
+  const int per = (n + 1023) / 1024;
+  const int b = min((int)threadIdx.x * per, n), e = min(b + per, n);
+  double s = 0.0;
+  for (int j = b; j < e; j++) s += w.csum[j];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const double v = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0.0;
     __syncthreads();
-    dsum[j] += v;
+    part[threadIdx.x] += v;
     __syncthreads();
   }
-  const float s0 = j == 0 ? 0.0f : (float)dsum[j - 1];
-  float c = s0;
-  int good = 1;
+  double run = threadIdx.x == 0 ? 0.0 : part[threadIdx.x - 1];
+  for (int j = b; j < e; j++) {
+    w.s0[j] = j == 0 ? 0.0f : (float)run;
+    run += w.csum[j];
+  }
+}
+
+__global__ __launch_bounds__(CDF_STAGE_T) void k_cdf_spec(const float* __restrict__ raw, int E, CdfWork w) {
+  __shared__ float t[CDF_CPB][CDF_CH + 1];
+  cdf_stage(raw, E, blockIdx.x * CDF_CPB, t);
+  __syncthreads();
+  const int j = blockIdx.x * CDF_CPB + threadIdx.x;
+  if (threadIdx.x >= CDF_CPB || j >= cdf_chunks(E)) return;
+  const int n = min(CDF_CH, E - j * CDF_CH);
+  const float s0 = w.s0[j];
   const int ex = fexp(s0);
   const float u = __uint_as_float((uint32_t)max(ex - 23, 1) << 23);  // grid spacing of the binade
-  for (int off = 0; off < C; off += CDF_WIN) {
-    __syncthreads();
-    cdf_stage(pmf_raw, E, C, off, tile);
-    __syncthreads();
-    const int m = min(CDF_WIN, n - off);
-    for (int q = 0; q < m; q++) {
-      const float p = tile[j][q];
-      const float sum = p + c;
-      // TwoSum: err = (p + c) - sum exactly
-      const float bb = sum - c;
-      const float aa = sum - bb;
-      const float err = (p - bb) + (c - aa);
-      if (fabsf(err) * 2.0f == u && err != 0.0f) good = 0;
-      c = sum;
-    }
+  float c = s0;
+  int good = 1;
+  for (int q = 0; q < n; q++) {
+    const float p = t[threadIdx.x][q];
+    const float sum = p + c;
+    // TwoSum: err = (p + c) - sum exactly
+    const float bb = sum - c;
+    const float aa = sum - bb;
+    const float err = (p - bb) + (c - aa);
+    if (fabsf(err) * 2.0f == u && err != 0.0f) good = 0;
+    c = sum;
   }
-  // one binade for the whole chunk; tiny (grid below 2^-126) and non-finite values recompute
+  // one binade for the whole chunk; tiny (grid below 2^-126) and non-finite values replay
   if (fexp(c) != ex || ex <= 24 || ex == 255) good = 0;
-  spec_start[j] = s0;
-  spec_end[j] = c;
-  ok[j] = good;
-  __syncthreads();
-  if (j == 0) {
-    float T = 0.0f;
-    for (int q = 0; q < CDF_PAR_THREADS; q++) {
-      true_start[q] = T;
-      const float ss = spec_start[q], se = spec_end[q];
-      bool done = false;
-      if (T == ss) { T = se; done = true; }
-      else if (ok[q] && fexp(T) == fexp(ss)) {
-        const float d = T - ss;       // exact: both on the binade's grid
-        const float te = se + d;
-        if (fexp(te) == fexp(ss)) { T = te; done = true; }
-      }
-      if (!done) {
-        const int qb = min(q * C, E), qe = min(qb + C, E);
-        for (int k = qb; k < qe; k++) T = pmf_raw[k] + T;
-      }
-    }
-    true_start[CDF_PAR_THREADS] = T;
-  }
-  __syncthreads();
-  const float total = true_start[CDF_PAR_THREADS];
-  c = true_start[j];
-  for (int off = 0; off < C; off += CDF_WIN) {
-    __syncthreads();
-    cdf_stage(pmf_raw, E, C, off, tile);
-    __syncthreads();
-    const int m = min(CDF_WIN, n - off);
-    for (int q = 0; q < m; q++) { c = tile[j][q] + c; tile[j][q] = c; }
-    __syncthreads();
-    for (int q = threadIdx.x; q < CDF_PAR_THREADS * CDF_WIN; q += CDF_PAR_THREADS) {
-      const int row = q / CDF_WIN, col = q % CDF_WIN;
-      const int k = row * C + off + col;
-      if (k < min(row * C + C, E)) {
-        cdf[k + 1] = tile[row][col] / total;
-        pmf[k] = pmf_raw[k] / total;
-      }
-    }
-  }
-  if (j == 0) cdf[0] = 0.0f / total;
+  w.se[j] = c;
+  w.ok[j] = good;
 }
 
-// qmc::rndFloat (common_src/qmc.cpp:190-197)
-__device__ __forceinline__ float qmc_rnd(uint32_t pos, int dim, const uint32_t* table) {
-  uint32_t r = 0, d = pos;
-  for (int bit = 0; bit < 31; bit++, d >>= 1)
-    if (d & 1) r ^= table[bit + dim * 31];
-  return (float)(r + 1) * (1.0f / 2147483648.0f);
+// True chunk starts.  The chunks that are not shift-invariant are staged in LDS first (in
+// chunk order, by the whole workgroup) so the serial replay reads LDS, 16 values ahead.
+__global__ __launch_bounds__(1024) void k_cdf_walk(const float* __restrict__ raw, int E, CdfWork w) {
+  __shared__ int soff[1024];
+  __shared__ float stage[CDF_WALK_STAGE];
+  __shared__ float ls0[CDF_WALK_CHUNKS], lse[CDF_WALK_CHUNKS];
+  __shared__ int lok[CDF_WALK_CHUNKS];
+  const int n = cdf_chunks(E);
+  const bool lds_spec = n <= CDF_WALK_CHUNKS;
+  if (lds_spec)
+    for (int j = threadIdx.x; j < n; j += 1024) { ls0[j] = w.s0[j]; lse[j] = w.se[j]; lok[j] = w.ok[j]; }
+  const float* S0 = lds_spec ? ls0 : w.s0;
+  const float* SE = lds_spec ? lse : w.se;
+  const int* OK = lds_spec ? lok : w.ok;
+  const int per = (n + 1023) / 1024;
+  const int b = min((int)threadIdx.x * per, n), e = min(b + per, n);
+  int need = 0;
+  for (int j = b; j < e; j++) if (!w.ok[j]) need += min(CDF_CH, E - j * CDF_CH);
+  soff[threadIdx.x] = need;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = (int)threadIdx.x >= o ? soff[threadIdx.x - o] : 0;
+    __syncthreads();
+    soff[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int at = soff[threadIdx.x] - need;
+  for (int j = b; j < e; j++) {
+    if (w.ok[j]) continue;
+    const int len = min(CDF_CH, E - j * CDF_CH);
+    if (at + len <= CDF_WALK_STAGE)
+      for (int q = 0; q < len; q++) stage[at + q] = raw[j * CDF_CH + q];
+    at += len;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  float T = 0.0f;
+  int staged = 0;   // LDS offset of the next not-invariant chunk
+  float ssn = S0[0], sen = SE[0];
+  int okn = OK[0];
+  for (int j = 0; j < n; j++) {
+    w.ts[j] = T;
+    const float ss = ssn, se = sen;
+    const int okq = okn;
+    if (j + 1 < n) { ssn = S0[j + 1]; sen = SE[j + 1]; okn = OK[j + 1]; }  // ahead of the chain
+    const int len = min(CDF_CH, E - j * CDF_CH);
+    bool done = false;
+    if (T == ss) { T = se; done = true; }
+    else if (okq && fexp(T) == fexp(ss)) {
+      const float d = T - ss;       // exact: both on the binade's grid
+      const float te = se + d;
+      if (fexp(te) == fexp(ss)) { T = te; done = true; }
+    }
+    if (!done) {
+      if (!okq && staged + len <= CDF_WALK_STAGE) {
+        const float* v = stage + staged;
+        int k = 0;
+        for (; k + 16 <= len; k += 16) {
+          float x[16];
+#pragma unroll
+          for (int i = 0; i < 16; i++) x[i] = v[k + i];
+#pragma unroll
+          for (int i = 0; i < 16; i++) T = x[i] + T;
+        }
+        for (; k < len; k++) T = v[k] + T;
+      } else {
+        const float* v = raw + (size_t)j * CDF_CH;
+        for (int k = 0; k < len; k++) T = v[k] + T;
+      }
+    }
+    if (!okq) staged += len;
+  }
+  w.ts[n] = T;
+}
+
+__global__ __launch_bounds__(CDF_STAGE_T) void k_cdf_final(const float* __restrict__ raw, int E, CdfWork w,
+                                                       float* __restrict__ pmf, float* __restrict__ cdf) {
+  __shared__ float t[CDF_CPB][CDF_CH + 1];
+  const int chunk0 = blockIdx.x * CDF_CPB;
+  cdf_stage(raw, E, chunk0, t);
+  __syncthreads();
+  const int j = chunk0 + threadIdx.x;
+  const float total = w.ts[cdf_chunks(E)];
+  if (threadIdx.x < CDF_CPB && j < cdf_chunks(E)) {
+    const int n = min(CDF_CH, E - j * CDF_CH);
+    float c = w.ts[j];
+    for (int q = 0; q < n; q++) { c = t[threadIdx.x][q] + c; t[threadIdx.x][q] = c; }
+  }
+  __syncthreads();
+  const int base = chunk0 * CDF_CH;
+  for (int q = threadIdx.x; q < CDF_CPB * CDF_CH; q += CDF_STAGE_T) {
+    const int k = base + q;
+    if (k < E) {
+      cdf[k + 1] = t[q / CDF_CH][q % CDF_CH] / total;
+      pmf[k] = raw[k] / total;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cdf[0] = 0.0f / total;
 }
 
 // sample(): clamp(upper_bound(cdf, u) - 1, 0, E - 1) (src/drender.cpp:25-29)
@@ -683,47 +848,6 @@ __device__ __forceinline__ int sample_edge(const float* cdf, int E, float u) {
     else count = step;
   }
   return clampi(first - 1, 0, E - 1);
-}
-
-// Edge samples are traced in edge order: a counting sort by the sampled edge (the edge is a
-// monotone function of rnd0 through the CDF) makes the rays of a wavefront coherent and lets
-// the gradient of every run of equal edges be reduced in the wave before one atomic.
-// Pass 1: sampled edge and the reference's rejection tests (off-screen pixel, |adjoint| <
-// 1e-3, src/drender.h:338-345); rejected samples go to bin E.
-__global__ __launch_bounds__(256) void k_edge_bin(EdgeArgs a) {
-  const DevCam& c = a.cams[a.view];
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.N; i += stride) {
-    float rnd0, rnd1;
-    if (a.rnd) { const float2 r = a.rnd[i]; rnd0 = r.x; rnd1 = r.y; }
-    else { rnd0 = qmc_rnd((uint32_t)i, 0, a.qmc); rnd1 = qmc_rnd((uint32_t)i, 1, a.qmc); }
-    const int eid = sample_edge(a.cdf, a.E, rnd0);
-    const int2 ev = a.edges[eid];
-    const float4 P0 = a.proj[ev.x], P1 = a.proj[ev.y];
-    const float t = rnd1;
-    const float px = P0.x + t * (P1.x - P0.x), py = P0.y + t * (P1.y - P0.y);
-    int key = a.E;
-    // int() truncation; NaN and out-of-range values are off-screen (as x86 cvttss2si's INT_MIN)
-    if (px > -1.0f && py > -1.0f && px < (float)c.w && py < (float)c.h) {
-      const int xi = (int)px, yi = (int)py;
-      if (xi >= 0 && yi >= 0 && xi < c.w && yi < c.h) {
-        const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)yi * c.w + xi));
-        if (!((double)length(adj) < 1e-3)) key = eid;
-      }
-    }
-    a.ekey[i] = key;
-    // the rejected bin is not counted: after the exclusive scan ebin[E] = accepted total
-    if (key < a.E) atomicAdd(a.ebin + key, 1);
-  }
-}
-
-// Pass 2 (after an exclusive scan of the bins): accepted samples to their edge's range.
-__global__ __launch_bounds__(256) void k_edge_scatter(EdgeArgs a) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.N; i += stride) {
-    const int key = a.ekey[i];
-    if (key < a.E) a.order[atomicAdd(a.ebin + key, 1)] = (int)i;
-  }
 }
 
 // Sum `v` over each run of equal `key` in the wavefront (lanes hold consecutive sorted
@@ -764,23 +888,38 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
   __syncthreads();
   const float* Ti = a.tr_inv0;  // mesh 0's inverse transform for every mesh (drender.cpp:60)
   const long long stride = (long long)gridDim.x * blockDim.x;
-  const long long nacc = a.ebin[a.E];  // accepted samples: the rejected bin starts here
-  // whole wavefronts step together (the run reduction needs all 64 lanes)
-  for (long long wbase = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < nacc; wbase += stride) {
+  // Slots hold the samples in rnd0 order, so a wavefront's 64 slots sample a few consecutive
+  // edges.  Whole wavefronts step together (the run reduction needs all 64 lanes).
+  for (long long wbase = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < a.N; wbase += stride) {
     const long long slot = wbase + lane;
     float gsum[6] = {0, 0, 0, 0, 0, 0};
     int key = -1 - lane;   // lanes without a contribution form runs of their own
     int vn0 = 0, vn1 = 0, goff = -1;
-    if (slot < nacc) {
-    const long long i = a.order[slot];
-    const int eid = a.ekey[i];
-    const float rnd1 = a.rnd ? a.rnd[i].y : qmc_rnd((uint32_t)i, 1, a.qmc);
-    const int2 ev = a.edges[eid];
+    bool live = false;
+    int eid = 0, xi = 0, yi = 0;
+    float t = 0.0f, px = 0.0f, py = 0.0f;
+    float4 P0 = {0, 0, 0, 0}, P1 = {0, 0, 0, 0};
+    int2 ev = {0, 0};
+    if (slot < a.N) {
+      const float2 r = a.rnd[slot];
+      eid = sample_edge(a.cdf, a.E, r.x);
+      ev = a.edges[eid];
+      P0 = a.proj[ev.x]; P1 = a.proj[ev.y];
+      t = r.y;
+      px = P0.x + t * (P1.x - P0.x); py = P0.y + t * (P1.y - P0.y);
+      // the reference's rejection tests (src/drender.h:338-345): off-screen pixel (int()
+      // truncation; NaN and out-of-range values are off-screen as x86 cvttss2si's INT_MIN)
+      // and |adjoint| < 1e-3
+      if (px > -1.0f && py > -1.0f && px < (float)c.w && py < (float)c.h) {
+        xi = (int)px; yi = (int)py;
+        if (xi >= 0 && yi >= 0 && xi < c.w && yi < c.h) {
+          const F3 aj = ld3(a.adj + c.img_off + 3ll * ((long long)yi * c.w + xi));
+          live = !((double)length(aj) < 1e-3);
+        }
+      }
+    }
+    if (live) {
     const float pmf = a.pmf[eid];
-    const float4 P0 = a.proj[ev.x], P1 = a.proj[ev.y];
-    const float t = rnd1;
-    const float px = P0.x + t * (P1.x - P0.x), py = P0.y + t * (P1.y - P0.y);
-    const int xi = (int)px, yi = (int)py;   // accepted in k_edge_bin: on screen
     const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)yi * c.w + xi));
     const float dvx = P1.x - P0.x, dvy = P1.y - P0.y;
     const float len = sqrtf(dvx * dvx + dvy * dvy);
@@ -837,7 +976,7 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     }
     }  // differentiable mesh
     }  // adj != 0
-    }  // live slot
+    }  // accepted sample
     // one atomic per run of equal edges for the position gradient of its two vertices
     const bool tail = seg_reduce<6>(key, gsum);
     if (tail && key >= 0) {
@@ -979,7 +1118,8 @@ void launch_karras(const KarrasArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_karras, dim3((a.F + 254) / 256), dim3(256), 0, s, a);
 }
 void launch_refit(const RefitArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_refit, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_refit_local, dim3((a.F + REFIT_CHUNK - 1) / REFIT_CHUNK), dim3(REFIT_CHUNK), 0, s, a);
+  hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, a);
 }
 void launch_pack(const PackArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_pack_nodes, dim3((a.F + 254) / 256), dim3(256), 0, s, a);
@@ -1009,21 +1149,22 @@ void launch_project(const ProjectArgs& a, hipStream_t s) {
 void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s) {
   hipLaunchKernelGGL(k_edge_len, dim3((E + 255) / 256), dim3(256), 0, s, edges, E, proj, pmf_raw);
 }
-void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, hipStream_t s, bool parallel) {
-  if (parallel) hipLaunchKernelGGL(k_cdf_par, dim3(1), dim3(CDF_PAR_THREADS), 0, s, pmf_raw, E, pmf, cdf);
-  else hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
+void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work, hipStream_t s, bool parallel) {
+  if (E <= 0) return;
+  if (!parallel) {
+    hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
+    return;
+  }
+  const CdfWork w = cdf_work(work, E);
+  const int blocks = (cdf_chunks(E) + CDF_CPB - 1) / CDF_CPB;
+  hipLaunchKernelGGL(k_cdf_sums, dim3(blocks), dim3(CDF_STAGE_T), 0, s, pmf_raw, E, w);
+  hipLaunchKernelGGL(k_cdf_scan, dim3(1), dim3(1024), 0, s, E, w);
+  hipLaunchKernelGGL(k_cdf_spec, dim3(blocks), dim3(CDF_STAGE_T), 0, s, pmf_raw, E, w);
+  hipLaunchKernelGGL(k_cdf_walk, dim3(1), dim3(1024), 0, s, pmf_raw, E, w);
+  hipLaunchKernelGGL(k_cdf_final, dim3(blocks), dim3(CDF_STAGE_T), 0, s, pmf_raw, E, w, pmf, cdf);
 }
 void launch_edge_sample(int mode, const EdgeArgs& a, int blocks, hipStream_t s) {
   DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_sample<M>), dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a))
-}
-void launch_edge_bin(const EdgeArgs& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_edge_bin, dim3(blocks), dim3(256), 0, s, a);
-}
-void launch_edge_scatter(const EdgeArgs& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_edge_scatter, dim3(blocks), dim3(256), 0, s, a);
-}
-void launch_scan_u32(uint32_t* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, p, n);
 }
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s) {
   const int b = (a.n + 255) / 256;

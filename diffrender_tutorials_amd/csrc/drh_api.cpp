@@ -651,7 +651,7 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   if ((s0 = build_view_suspects(views))) return s0;
   int part = 0;
   std::vector<int> tiles(views);
-  for (int v = 0; v < views; v++) { tiles[v] = ((cam_cache[v].w + 7) / 8) * ((cam_cache[v].h + 7) / 8); part += tiles[v]; }
+  for (int v = 0; v < views; v++) { tiles[v] = num_tiles(sq, cam_cache[v].w, cam_cache[v].h); part += tiles[v]; }
   if (d_target) {
     drh_status s;
     if ((s = hip(loss_part.reserve(8 * (size_t)std::max(part, 1)), "alloc"))) return s;
@@ -693,10 +693,10 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   for (int i = 0; i < nm; i++) if (diff[i]) { single = (single == -1) ? i : -2; }
   for (int v = 0; v < views; v++) {
     const DevCam& c = cam_cache[v];
-    const int tiles = ((c.w + 7) / 8) * ((c.h + 7) / 8);
+    const int tiles = num_tiles(sq, c.w, c.h);   // interior samples: nsamp = sq*sq Hammersley points
     if (st.mode != DRH_SILHOUETTE && G > 0) {
       InteriorArgs ia{};
-      ia.tr = tracer_view(v); ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.hamm = hamm.as<float>();
+      ia.tr = tracer_view(v); ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.sq = sq; ia.hamm = hamm.as<float>();
       ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
       ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
       int blocks = tiles;

@@ -142,10 +142,12 @@ __device__ __forceinline__ int uload(const int* p) { return *(cint*)p; }
 // below stays finite: that axis' slab is then all-pass or all-fail, as for an infinite 1/d.
 __device__ __forceinline__ float slab_rcp(float d) { return fabsf(d) < 1e-30f ? copysignf(1e30f, d) : 1.0f / d; }
 
-// Slab test of a padded child box, fused form t = fma(l, 1/d, -o/d).  Its rounding differs
-// from (l - o) * (1/d) by about 2^-24 (|o| + |l - o|) in world units, which the boxes'
+// Slab test of a padded child box, fused form t = fma(plane, 1/d, -o/d).  Its rounding differs
+// from (plane - o) * (1/d) by about 2^-24 (|o| + |plane - o|) in world units, which the boxes'
 // padding (1e-5 relative plus an absolute term of the scene's scale) absorbs; the triangle
-// test itself stays exact, so the padding only decides how many boxes are opened.
+// test itself stays exact, so the padding only decides how many boxes are opened.  (Picking
+// the entry / exit planes by a wave-uniform direction octant saves the min / max pairs on the
+// vector unit but costs more on the scalar unit, which issues at the same rate.)
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
                                      F3 id, F3 nod, float tmax, float& tn) {
   const float t0x = fmaf(lx, id.x, nod.x), t1x = fmaf(hx, id.x, nod.x);
@@ -159,49 +161,57 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
 
 // LBVH walk, wavefront-coherent: the active lanes of a wave traverse together.  The node
 // index is wave-uniform, so a node's two child boxes and links arrive by one scalar load
-// for all 64 rays; each lane tests its own ray (per-lane closest-hit culling), ballots pick
-// the children any lane needs (nearer child first by majority), and the traversal stack is
-// wave-uniform too: 64 ints of LDS per wave (`wst`), written by one lane, read as a
-// broadcast; 64 entries cover the Karras tree's depth (<= 30 + ceil(log2 F) levels) for
-// F < 2^34.
+// for all 64 rays; each lane tests its own ray (per-lane closest-hit culling through `lim`),
+// ballots pick the children any lane needs (nearer child first by majority), and the
+// traversal stack is wave-uniform too: 64 ints of LDS per wave (`wst`), written by one lane,
+// read as a broadcast; 64 entries cover the Karras tree's depth (<= 30 + ceil(log2 F)
+// levels) for F < 2^34.  A lane whose ray is NaN, or an any-hit lane that has its hit, gets
+// lim = -1 and no box passes for it any more.
 // A lane tests every leaf its own slab test opens and possibly more (leaves opened for
 // other lanes): the closest hit with the (t, triangle id) tie rule is the same over any
 // superset of the triangles a lane's own walk would test, so the result is the brute-force
-// tracer's answer for every lane.  Any-hit lanes drop out at their first hit.
+// tracer's answer for every lane.
 template <bool ANY>
 __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar, int* wst) {
+  if (s.F == 0) return Hit{NO_HIT, tFar, 0.0f, 0.0f};
+  const bool retired = o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z;
   Hit h{NO_HIT, tFar, 0.0f, 0.0f};
-  if (s.F == 0) return h;
-  bool live = !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z);
   const F3 id = {slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z)};
   const F3 nod = {-(o.x * id.x), -(o.y * id.y), -(o.z * id.z)};
+  float lim = retired ? -1.0f : tFar * 1.00002f + 1e-6f;
+  bool live = !retired;   // any-hit: still looking
   auto leaf = [&](int slot, bool mine) {
     const float4 A = uload(s.tri_a, slot), E1 = uload(s.tri_e1, slot), E2 = uload(s.tri_e2, slot);
     float t, u, v;
     if (mine && mt_test(A, E1, E2, o, d, tNear, t, u, v)) {
       const uint32_t g = __float_as_uint(A.w);
-      if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; live = false; } }
-      else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) { h.gtri = g; h.t = t; h.u = u; h.v = v; }
+      if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; lim = -1.0f; live = false; } }
+      else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) {
+        h.gtri = g; h.t = t; h.u = u; h.v = v;
+        lim = t * 1.00002f + 1e-6f;
+      }
     }
   };
-  if (s.F == 1) { leaf(0, live); return h; }
+  if (s.F == 1) { leaf(0, !retired); return h; }
   const bool leader = (int)(threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(1)) - 1;
   int sp = 0;
   int node = 0;
   // a tree over F leaves has F-1 internal nodes, each visited at most once per wave
   for (int visits = 0; visits < s.F; visits++) {
+    node = __builtin_amdgcn_readfirstlane(node);   // wave-uniform by construction; keep it scalar
     const float4 n0 = uload(s.nodes, 4 * node), n1 = uload(s.nodes, 4 * node + 1);
     const float4 n2 = uload(s.nodes, 4 * node + 2), n3 = uload(s.nodes, 4 * node + 3);
     const int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
-    const float lim = h.t * 1.00002f + 1e-6f;
     float tl, tr;
-    const bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, id, nod, lim, tl) & live;
-    const bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, id, nod, lim, tr) & live;
+    // any-hit lanes that are done skip the box tests (exec-masked); closest-hit lanes are
+    // culled by lim alone
+    const bool hl = (!ANY || live) && slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, id, nod, lim, tl);
+    const bool hr = (!ANY || live) && slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, id, nod, lim, tr);
     const uint64_t bl = __ballot(hl), br = __ballot(hr);
     bool goL = false, goR = false;
     if (bl) { if (cl < 0) leaf(~cl, hl); else goL = true; }
     if (br) { if (cr < 0) leaf(~cr, hr); else goR = true; }
-    if (ANY && (bl | br) && !__ballot(live)) break;
+    if (ANY && (bl | br) && !__ballot(lim >= 0.0f)) break;
     if (goL && goR) {
       const uint64_t both = bl & br;
       const uint64_t lf = __ballot(hl && hr && tl <= tr);

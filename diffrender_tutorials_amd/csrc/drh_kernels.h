@@ -20,6 +20,12 @@ constexpr int CDF_WALK_CHUNKS = 8192;      // chunk records the serial walk read
 constexpr int INTERIOR_LDS_FLOATS = 4096;  // largest gradient target accumulated in LDS (interior)
 constexpr int EDGE_LDS_FLOATS = 4096;      // largest position gradient accumulated in LDS (edges)
 
+// pixel tiles per view, one wavefront each (matches tile_map in the kernels)
+__host__ __device__ inline int num_tiles(int sq, int w, int h) {
+  const int ts = (sq == 1 || sq == 2 || sq == 4 || sq == 8) ? 8 / sq : 8;
+  return ((w + ts - 1) / ts) * ((h + ts - 1) / ts);
+}
+
 struct PrepareArgs {
   int V, n_meshes, has_nrm;
   const int* mesh_vo;
@@ -130,7 +136,7 @@ struct InteriorArgs {
   Tracer tr;
   const DevCam* cams;
   int view;
-  int nsamp;
+  int nsamp, sq;          // nsamp = sq*sq
   const float* hamm;      // 2*nsamp Hammersley offsets
   const float* adj;
   float* grad;

@@ -414,22 +414,48 @@ __global__ void k_reorder(ReorderArgs a) {
 // ---------------------------------------------------------------------------
 // render (src/drender.h:98-141) with the L2 loss fused (src/utils.cpp:15-27)
 // ---------------------------------------------------------------------------
+// Work of one wavefront.  With sq*sq in {1, 4, 16, 64} samples per pixel a wave takes a
+// (8/sq) x (8/sq) pixel tile with one sample per lane, so its 64 rays cover a small footprint
+// and walk the LBVH as one coherent packet; otherwise an 8x8 tile, one pixel per lane, whose
+// samples are traced one after another.
+struct TileMap { int x, y, S, s; };   // pixel, lanes per pixel, this lane's sample (0 if S == 1)
+__device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
+  const bool inter = sq == 1 || sq == 2 || sq == 4 || sq == 8;
+  const int S = inter ? sq * sq : 1;
+  const int ts = inter ? 8 / sq : 8;          // tile side in pixels
+  const int tiles_x = (w + ts - 1) / ts;
+  const int p = lane / S;                     // pixel within the tile
+  return TileMap{(tile % tiles_x) * ts + p % ts, (tile / tiles_x) * ts + p / ts, S, lane % S};
+}
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   __shared__ int wst[64];   // the wave's traversal stack
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
-  const int tiles_x = (c.w + 7) >> 3;
-  const int tile = blockIdx.x;
-  const int x = (tile % tiles_x) * 8 + (lane & 7);
-  const int y = (tile / tiles_x) * 8 + (lane >> 3);
+  const int sq = a.sq;
+  const float ns = (float)(sq * sq);
+  const TileMap tm = tile_map(sq, c.w, blockIdx.x, lane);
+  const int x = tm.x, y = tm.y;
   const bool inside = x < c.w && y < c.h;
-  double l2 = 0.0;
-  if (inside) {
-    const int sq = a.sq;
-    const float ns = (float)(sq * sq);
-    F3 acc = {0, 0, 0};
+  F3 acc = {0, 0, 0};
+  if (tm.S > 1) {
+    // one subsample per lane; the pixel's S lanes are adjacent, sample k = dy*sq + dx in lane
+    // base + k, so the gather below adds col/spp in the reference's dy-major order
+    F3 col = {0, 0, 0};
+    if (inside) {
+      const int dx = tm.s % sq, dy = tm.s / sq;
+      const float xo = (dx + 0.5f) / float(sq);
+      const float yo = (dy + 0.5f) / float(sq);
+      col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst);
+    }
+    const int base = lane - tm.s;
+    for (int k = 0; k < tm.S; k++) {
+      const F3 ck = {__shfl(col.x, base + k, 64), __shfl(col.y, base + k, 64), __shfl(col.z, base + k, 64)};
+      acc = add(acc, divs(ck, ns));
+    }
+  } else if (inside) {
     for (int dy = 0; dy < sq; dy++)
       for (int dx = 0; dx < sq; dx++) {
         const float xo = (dx + 0.5f) / float(sq);
@@ -437,6 +463,9 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
         const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst);
         acc = add(acc, divs(col, ns));
       }
+  }
+  double l2 = 0.0;
+  if (inside && tm.s == 0) {
     const long long p = c.img_off + 3ll * ((long long)y * c.w + x);
     if (a.img) { a.img[p] = acc.x; a.img[p + 1] = acc.y; a.img[p + 2] = acc.z; }
     if (a.target) {
@@ -449,7 +478,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   }
   if (a.target) {
     l2 = wave_sum_d(l2);
-    if (lane == 0) a.loss_part[a.part_off + tile] = l2;
+    if (lane == 0) a.loss_part[a.part_off + blockIdx.x] = l2;
   }
 }
 
@@ -481,15 +510,17 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   const bool use_lds = a.lds_n > 0;
   for (int i = lane; i < a.lds_n; i += 64) lgrad[i] = 0.0f;
   __syncthreads();
-  const int tiles_x = (c.w + 7) >> 3;
-  const int ntiles = tiles_x * ((c.h + 7) >> 3);
+  const int sq = a.sq;
+  const int ntiles = num_tiles(sq, c.w, c.h);
   F3 uni = {0, 0, 0};  // uniform-target accumulator (single 1x1 texture)
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int x = (tile % tiles_x) * 8 + (lane & 7);
-    const int y = (tile / tiles_x) * 8 + (lane >> 3);
+    const TileMap tm = tile_map(sq, c.w, tile, lane);
+    const int x = tm.x, y = tm.y;
     if (x >= c.w || y >= c.h) continue;
     const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)y * c.w + x));
-    for (int si = 0; si < a.nsamp; si++) {
+    // one Hammersley sample per lane in the interleaved layout, all of them otherwise
+    const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
+    for (int si = s_begin; si < s_end; si++) {
       const float xo = a.hamm[2 * si], yo = a.hamm[2 * si + 1];
       const F3 val = divs(adj, (float)a.nsamp);
       F3 o, d;
@@ -853,7 +884,7 @@ __device__ __forceinline__ int sample_edge(const float* cdf, int E, float u) {
 // Sum `v` over each run of equal `key` in the wavefront (lanes hold consecutive sorted
 // samples); afterwards the last lane of a run holds the run's total.
 template <int K>
-__device__ __forceinline__ bool seg_reduce(int key, float (&v)[K]) {
+__device__ __forceinline__ bool seg_reduce(int key, float* v) {
   const int lane = threadIdx.x & 63;
   const int prev = __shfl_up(key, 1, 64);
   const int next = __shfl_down(key, 1, 64);
@@ -872,7 +903,7 @@ __device__ __forceinline__ bool seg_reduce(int key, float (&v)[K]) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_edge_sample(EdgeArgs a) {
   extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
   __shared__ float ltr[4][12];
   __shared__ int wstack[4][64];
@@ -882,9 +913,7 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
   int* wst = wstack[wave];
   const bool use_lds = a.lds_n > 0;
   for (int i = threadIdx.x; i < a.lds_n; i += 256) lpos[i] = 0.0f;
-  float tr_acc[12];
-#pragma unroll
-  for (int q = 0; q < 12; q++) tr_acc[q] = 0.0f;
+  if (threadIdx.x < 48) ltr[threadIdx.x / 12][threadIdx.x % 12] = 0.0f;
   __syncthreads();
   const float* Ti = a.tr_inv0;  // mesh 0's inverse transform for every mesh (drender.cpp:60)
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -892,7 +921,9 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
   // edges.  Whole wavefronts step together (the run reduction needs all 64 lanes).
   for (long long wbase = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < a.N; wbase += stride) {
     const long long slot = wbase + lane;
-    float gsum[6] = {0, 0, 0, 0, 0, 0};
+    // position gradient of v0, v1 (after mesh 0's inverse transform), then for a single
+    // mesh the raw screen-chain terms of v0, v1 that its transform gradient is built from
+    float gsum[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int key = -1 - lane;   // lanes without a contribution form runs of their own
     int vn0 = 0, vn1 = 0, goff = -1;
     bool live = false;
@@ -959,17 +990,18 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     gsum[3] = Ti[0] * dv1_dx + Ti[4] * dv1_dy + Ti[8] * dv1_dz;
     gsum[4] = Ti[1] * dv1_dx + Ti[5] * dv1_dy + Ti[9] * dv1_dz;
     gsum[5] = Ti[2] * dv1_dx + Ti[6] * dv1_dy + Ti[10] * dv1_dz;
-    const F3 o0 = ld3(a.tr.s.pos_orig + 3 * ev.x), o1 = ld3(a.tr.s.pos_orig + 3 * ev.y);
-    const float tv[12] = {o0.x * dv0_dx + o1.x * dv1_dx, o0.y * dv0_dx + o1.y * dv1_dx,
-                          o0.z * dv0_dx + o1.z * dv1_dx, 1 * dv0_dx + 1 * dv1_dx,
-                          o0.x * dv0_dy + o1.x * dv1_dy, o0.y * dv0_dy + o1.y * dv1_dy,
-                          o0.z * dv0_dy + o1.z * dv1_dy, 1 * dv0_dy + 1 * dv1_dy,
-                          o0.x * dv0_dz + o1.x * dv1_dz, o0.y * dv0_dz + o1.y * dv1_dz,
-                          o0.z * dv0_dz + o1.z * dv1_dz, 1 * dv0_dz + 1 * dv1_dz};
     if (a.single_mesh) {
-#pragma unroll
-      for (int q = 0; q < 12; q++) tr_acc[q] += tv[q];
+      gsum[6] = dv0_dx; gsum[7] = dv0_dy; gsum[8] = dv0_dz;
+      gsum[9] = dv1_dx; gsum[10] = dv1_dy; gsum[11] = dv1_dz;
     } else {
+      // d_tr (src/drender.cpp:73-86): orig_vertices times the screen-chain terms
+      const F3 o0 = ld3(a.tr.s.pos_orig + 3 * ev.x), o1 = ld3(a.tr.s.pos_orig + 3 * ev.y);
+      const float tv[12] = {o0.x * dv0_dx + o1.x * dv1_dx, o0.y * dv0_dx + o1.y * dv1_dx,
+                            o0.z * dv0_dx + o1.z * dv1_dx, 1 * dv0_dx + 1 * dv1_dx,
+                            o0.x * dv0_dy + o1.x * dv1_dy, o0.y * dv0_dy + o1.y * dv1_dy,
+                            o0.z * dv0_dy + o1.z * dv1_dy, 1 * dv0_dy + 1 * dv1_dy,
+                            o0.x * dv0_dz + o1.x * dv1_dz, o0.y * dv0_dz + o1.y * dv1_dz,
+                            o0.z * dv0_dz + o1.z * dv1_dz, 1 * dv0_dz + 1 * dv1_dz};
       float* gt = a.grad + goff + a.layout[4 * m + 3];
 #pragma unroll
       for (int q = 0; q < 12; q++) atomicAdd(gt + q, tv[q]);
@@ -978,19 +1010,28 @@ __global__ __launch_bounds__(256) void k_edge_sample(EdgeArgs a) {
     }  // adj != 0
     }  // accepted sample
     // one atomic per run of equal edges for the position gradient of its two vertices
-    const bool tail = seg_reduce<6>(key, gsum);
+    bool tail;
+    if (a.single_mesh) tail = seg_reduce<12>(key, gsum);
+    else tail = seg_reduce<6>(key, gsum);
     if (tail && key >= 0) {
       float* G = use_lds ? lpos : a.grad;
       const int gb = use_lds ? goff - a.lds_base : goff;
       atomicAdd(G + gb + 3 * vn0, gsum[0]); atomicAdd(G + gb + 3 * vn0 + 1, gsum[1]); atomicAdd(G + gb + 3 * vn0 + 2, gsum[2]);
       atomicAdd(G + gb + 3 * vn1, gsum[3]); atomicAdd(G + gb + 3 * vn1 + 1, gsum[4]); atomicAdd(G + gb + 3 * vn1 + 2, gsum[5]);
-    }
-  }
-  if (a.single_mesh) {
+      if (a.single_mesh) {
+        // the run's transform gradient: orig_vertices of its two (shared) vertices times the
+        // run's summed screen-chain terms (src/drender.cpp:73-86), into the wave's LDS row
+        const F3 o0 = ld3(a.tr.s.pos_orig + 3 * ev.x), o1 = ld3(a.tr.s.pos_orig + 3 * ev.y);
 #pragma unroll
-    for (int q = 0; q < 12; q++) tr_acc[q] = wave_sum(tr_acc[q]);
-    if (lane == 0)
-      for (int q = 0; q < 12; q++) ltr[wave][q] = tr_acc[q];
+        for (int r = 0; r < 3; r++) {
+          const float d0 = gsum[6 + r], d1 = gsum[9 + r];
+          atomicAdd(&ltr[wave][4 * r + 0], o0.x * d0 + o1.x * d1);
+          atomicAdd(&ltr[wave][4 * r + 1], o0.y * d0 + o1.y * d1);
+          atomicAdd(&ltr[wave][4 * r + 2], o0.z * d0 + o1.z * d1);
+          atomicAdd(&ltr[wave][4 * r + 3], 1 * d0 + 1 * d1);
+        }
+      }
+    }
   }
   __syncthreads();
   if (a.single_mesh && threadIdx.x < 12) {

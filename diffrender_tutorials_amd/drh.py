@@ -17,7 +17,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdrh.so")
+# DRH_LIB: load another build of the library (e.g. to compare builds side by side)
+LIB_PATH = os.environ.get("DRH_LIB") or os.path.join(HERE, "libdrh.so")
 
 SILHOUETTE, VERTEX_COLOR, TEXTURE_COLOR, LAMBERT = 1, 2, 3, 4
 RNG_REF_PRNG14, RNG_QMC = 0, 1

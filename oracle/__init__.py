@@ -23,6 +23,9 @@ Parity status of each function (how it is pinned):
   * CPU BVH tracer: pinned against the BF tracer (same hits).
   * orc_optsimple_step: pinned by the reference OptSimple run (golden vectors).
   * orc_adam_step (standard Adam): pinned against torch.optim.Adam golden vectors.
+  * orc_silhouette_functional / orc_fd_silhouette (FP64 box-filtered silhouette and its
+    central differences): pinned by the closed-form area of a cube's front face and by
+    analytic derivatives of that area (tests/test_oracle.py).
 """
 from __future__ import annotations
 
@@ -62,6 +65,7 @@ def lib():
         _lib.orc_scene_create.restype = ctypes.c_void_p
         _lib.orc_loss.restype = ctypes.c_float
         _lib.orc_qmc_rnd.restype = ctypes.c_float
+        _lib.orc_silhouette_functional.restype = ctypes.c_double
     return _lib
 
 
@@ -154,6 +158,23 @@ class Scene:
             grad = np.zeros(self.grad_size(mode), np.float32)
         lib().orc_d_render(self.h, mode, spp, lam_struct(lam), cs, len(cams), _p(adj), edge_samples, rng_mode, _p(grad))
         return grad
+
+    def silhouette_functional(self, cam, adjoint):
+        """FP64 box-filtered silhouette functional sum_p (sum of adjoint channels)_p * covered
+        area of pixel p (closed convex mesh; front-facing triangles)."""
+        c = cam_struct(cam)
+        adj = np.ascontiguousarray(np.asarray(adjoint, np.float32).reshape(-1))
+        return lib().orc_silhouette_functional(self.h, ctypes.byref(c), _p(adj))
+
+    def fd_silhouette(self, cam, adjoint, coords, h=1e-4):
+        """FP64 central differences of silhouette_functional for coordinates 3*vertex+axis."""
+        c = cam_struct(cam)
+        adj = np.ascontiguousarray(np.asarray(adjoint, np.float32).reshape(-1))
+        co = np.ascontiguousarray(np.asarray(coords, np.int32))
+        out = np.empty(len(co), np.float64)
+        lib().orc_fd_silhouette(self.h, ctypes.byref(c), _p(adj), len(co), _p(co, ctypes.c_int), ctypes.c_double(h),
+                                _p(out, ctypes.c_double))
+        return out
 
     def edge_sampler(self, cam):
         c = cam_struct(cam)

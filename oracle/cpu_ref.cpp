@@ -1072,3 +1072,136 @@ void orc_adam_step(int n, float* p, const float* g, float* m, float* v, int step
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FP64 finite-difference pin of the edge-sampled position gradient (the check of
+// src/fin_diff.cpp / Tester::test_fin_diff, src/tests.cpp:491-676, made exact): in SILHOUETTE
+// mode the image, box-filtered over each pixel, is the area of the pixel the mesh covers.
+// For a closed convex mesh seen from outside the front-facing triangles tile the silhouette
+// without overlap, so L(x) = sum_p a_p * covered_area_p(x), a_p = sum of the pixel's adjoint
+// channels, is a sum over front-facing triangles of sum_p a_p * area(triangle ∩ pixel p),
+// each area computed exactly (Sutherland-Hodgman clip to the pixel square, shoelace) in
+// double.  Edge sampling estimates dL/dx (its colour jump is 1 across the silhouette and 0
+// elsewhere), so central differences of L in double pin it.  The projection is the
+// reference's VertexShader (src/functions.h:5-13) in double with mProj*mWorldView.
+// ---------------------------------------------------------------------------
+namespace {
+struct P2 { double x, y; };
+double clip_area(const P2* tri, double x0, double y0) {
+  P2 a[8], b[8];
+  int n = 3;
+  for (int i = 0; i < 3; i++) a[i] = tri[i];
+  const double lo[2] = {x0, y0}, hi[2] = {x0 + 1.0, y0 + 1.0};
+  for (int e = 0; e < 4 && n > 0; e++) {
+    const int ax = e & 1;          // 0: x, 1: y
+    const bool upper = e >= 2;
+    const double c = upper ? hi[ax] : lo[ax];
+    auto inside = [&](const P2& p) { const double v = ax ? p.y : p.x; return upper ? v <= c : v >= c; };
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+      const P2 &P = a[i], &Q = a[(i + 1) % n];
+      const bool ip = inside(P), iq = inside(Q);
+      if (ip) b[m++] = P;
+      if (ip != iq) {
+        const double pv = ax ? P.y : P.x, qv = ax ? Q.y : Q.x;
+        const double t = (c - pv) / (qv - pv);
+        b[m++] = P2{P.x + t * (Q.x - P.x), P.y + t * (Q.y - P.y)};
+      }
+    }
+    n = m;
+    for (int i = 0; i < n; i++) a[i] = b[i];
+  }
+  double ar = 0.0;
+  for (int i = 0; i < n; i++) { const P2 &P = a[i], &Q = a[(i + 1) % n]; ar += P.x * Q.y - Q.x * P.y; }
+  return std::fabs(0.5 * ar);
+}
+struct FdCam { double m[16]; double W, H; int w, h; };
+P2 project_d(const FdCam& c, const double* v) {
+  const double* M = c.m;  // column-major mWVP
+  const double W = v[0] * M[3] + v[1] * M[7] + v[2] * M[11] + M[15];
+  const double xN = (v[0] * M[0] + v[1] * M[4] + v[2] * M[8] + M[12]) / W;
+  const double yN = -(v[0] * M[1] + v[1] * M[5] + v[2] * M[9] + M[13]) / W;
+  return P2{(xN * 0.5 + 0.5) * c.W, (yN * 0.5 + 0.5) * c.H};
+}
+double tri_functional(const FdCam& c, const double* A, const double* B, const double* C, const double* a) {
+  const P2 t[3] = {project_d(c, A), project_d(c, B), project_d(c, C)};
+  const double xmin = std::min(t[0].x, std::min(t[1].x, t[2].x)), xmax = std::max(t[0].x, std::max(t[1].x, t[2].x));
+  const double ymin = std::min(t[0].y, std::min(t[1].y, t[2].y)), ymax = std::max(t[0].y, std::max(t[1].y, t[2].y));
+  const int x0 = std::max(0, (int)std::floor(xmin)), x1 = std::min(c.w - 1, (int)std::floor(xmax));
+  const int y0 = std::max(0, (int)std::floor(ymin)), y1 = std::min(c.h - 1, (int)std::floor(ymax));
+  double L = 0.0;
+  for (int y = y0; y <= y1; y++)
+    for (int x = x0; x <= x1; x++) {
+      const double ap = a[(size_t)y * c.w + x];
+      if (ap != 0.0) L += ap * clip_area(t, x, y);
+    }
+  return L;
+}
+}  // namespace
+
+extern "C" {
+double orc_silhouette_functional(void* sp, const orc_camera* cam, const float* adjoint) {
+  Scene& s = *(Scene*)sp;
+  FdCam c;
+  M44 wvp = mm(load44(cam->proj), load44(cam->world_view));
+  for (int i = 0; i < 16; i++) c.m[i] = wvp.m[i];
+  c.W = cam->width; c.H = cam->height; c.w = (int)cam->width; c.h = (int)cam->height;
+  std::vector<double> a((size_t)c.w * c.h);
+  for (size_t p = 0; p < a.size(); p++) a[p] = (double)adjoint[3 * p] + adjoint[3 * p + 1] + adjoint[3 * p + 2];
+  const M44 wi = inverse4x4(load44(cam->world_view));
+  const double eye[3] = {wi.m[12], wi.m[13], wi.m[14]};
+  double L = 0.0;
+  for (int g = 0; g < s.F; g++) {
+    double v[3][3];
+    for (int k = 0; k < 3; k++) { const f3 q = s.pos[s.idx[3 * g + k]]; v[k][0] = q.x; v[k][1] = q.y; v[k][2] = q.z; }
+    const double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double vd = n[0] * (v[0][0] - eye[0]) + n[1] * (v[0][1] - eye[1]) + n[2] * (v[0][2] - eye[2]);
+    if (vd < 0.0) L += tri_functional(c, v[0], v[1], v[2], a.data());
+  }
+  return L;
+}
+
+// Central differences (L(x + h) - L(x - h)) / 2h of the functional above for `n` coordinates
+// (3 * global vertex + axis); only the triangles around the vertex change, and a triangle
+// keeps the facing it has at x (h is far below any facing change of a non-degenerate one).
+void orc_fd_silhouette(void* sp, const orc_camera* cam, const float* adjoint, int n, const int* coords, double h,
+                       double* out) {
+  Scene& s = *(Scene*)sp;
+  FdCam c;
+  M44 wvp = mm(load44(cam->proj), load44(cam->world_view));
+  for (int i = 0; i < 16; i++) c.m[i] = wvp.m[i];
+  c.W = cam->width; c.H = cam->height; c.w = (int)cam->width; c.h = (int)cam->height;
+  std::vector<double> a((size_t)c.w * c.h);
+  for (size_t p = 0; p < a.size(); p++) a[p] = (double)adjoint[3 * p] + adjoint[3 * p + 1] + adjoint[3 * p + 2];
+  const M44 wi = inverse4x4(load44(cam->world_view));
+  const double eye[3] = {wi.m[12], wi.m[13], wi.m[14]};
+  std::vector<std::vector<int>> adj_tri(s.V);
+  for (int g = 0; g < s.F; g++)
+    for (int k = 0; k < 3; k++) adj_tri[s.idx[3 * g + k]].push_back(g);
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < n; i++) {
+    const int vtx = coords[i] / 3, ax = coords[i] % 3;
+    double acc[2] = {0.0, 0.0};
+    for (int g : adj_tri[vtx]) {
+      double v[3][3];
+      for (int k = 0; k < 3; k++) { const f3 q = s.pos[s.idx[3 * g + k]]; v[k][0] = q.x; v[k][1] = q.y; v[k][2] = q.z; }
+      const double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+      const double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+      const double nn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const double vd = nn[0] * (v[0][0] - eye[0]) + nn[1] * (v[0][1] - eye[1]) + nn[2] * (v[0][2] - eye[2]);
+      if (!(vd < 0.0)) continue;
+      for (int sgn = 0; sgn < 2; sgn++) {
+        double w[3][3];
+        std::memcpy(w, v, sizeof w);
+        for (int k = 0; k < 3; k++)
+          if ((int)s.idx[3 * g + k] == vtx) w[k][ax] += sgn == 0 ? h : -h;
+        acc[sgn] += tri_functional(c, w[0], w[1], w[2], a.data());
+      }
+    }
+    out[i] = (acc[0] - acc[1]) / (2.0 * h);
+  }
+}
+}  // extern "C"
+

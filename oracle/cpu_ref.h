@@ -87,6 +87,12 @@ void  orc_optsimple_step(int n, float* grad, float* m, float* G, int nblocks, co
                          const float* block_lr);
 
 /* Standard Adam (Kingma & Ba) on a flat vector; step is 1-based. */
+/* FP64 box-filtered silhouette functional L = sum_p (sum of adjoint channels)_p * covered area
+ * of pixel p (front-facing triangles of a closed convex mesh), and its central differences for
+ * coordinates 3*vertex+axis of the committed scene (pins the edge-sampled position gradient). */
+double orc_silhouette_functional(void* s, const orc_camera* cam, const float* adjoint);
+void  orc_fd_silhouette(void* s, const orc_camera* cam, const float* adjoint, int n, const int* coords, double h,
+                        double* out);
 void  orc_adam_step(int n, float* param, const float* grad, float* m, float* v, int step, float lr,
                     float beta1, float beta2, float eps);
 

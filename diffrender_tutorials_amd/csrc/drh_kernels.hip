@@ -1,14 +1,15 @@
 // drh_kernels.hip -- HIP kernels of the render / d_render path for gfx950 (MI355X).
 //
 //   commit   k_prepare, k_tri_setup, k_morton, k_radix_{hist,scan,scatter},
-//            k_karras, k_refit, k_pack_nodes, k_reorder      (replaces IRayTracer::Init)
+//            k_karras, k_refit_{local,top}, k_pack_nodes, k_reorder   (replaces IRayTracer::Init)
 //   render   k_render<MODE> (+ fused L2 loss / adjoint), k_loss_final
-//   interior k_interior<MODE>                                 (interior_derivatives)
-//   edges    k_project, k_edge_len, k_cdf, k_edge_sample<MODE> (edge_derivatives)
-//   optimise k_optsimple_{moments,apply}, k_adam
+//   interior k_interior<MODE>                                          (interior_derivatives)
+//   edges    k_project, k_edge_len, k_cdf_{sums,scan,spec,walk,final}, k_edge_sample<MODE>
+//                                                                      (edge_derivatives)
+//   optimise k_optsimple_{moments,apply}, k_adam, k_step_inc
 //
-// Wave64 everywhere: one 8x8 pixel tile per wavefront, LDS-resident traversal stacks
-// laid out [depth][lane] so pushes and pops of a wave hit 64 different banks.
+// Wave64 everywhere.  Rays are traced a wavefront at a time (trace_bvh in drh_device.h): a
+// pixel tile or a run of edge samples per wave, one wave-uniform LBVH walk per packet.
 #include <hip/hip_runtime.h>
 
 #include "drh_device.h"

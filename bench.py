@@ -217,7 +217,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "config2: icosphere-5 (10242 V / 20480 F), Lambert, 1024x1024, spp 4, 2^20 edge samples,"
-                               " fused commit(LBVH)+render+loss+d_render+Adam per step",
+                               " fused commit(LBVH)+render+loss+d_render+Adam per step on the vertex positions"
+                               " (texture frozen, so the interior pass -- texture adjoint only in Lambert -- is left"
+                               " out; the plain leg computes the full gradient)",
                    "triangles": F, "vertices": V, "resolution": [W, H], "spp": spp, "edge_samples": N,
                    "parallelism": "single GPU"},
         "primary_rays_per_s": round(rays_per_step * args.steps / dt, 1),

@@ -285,7 +285,8 @@ struct drh_renderer {
   std::vector<DevCam> cam_cache;
   // optimiser
   int opt_kind = -1;
-  float pos_lr = 0.1f, col_lr = 0.1f;
+  float pos_lr = 0.1f, col_lr = 0.1f, tex_lr = 0.0f;
+  bool skip_tex_interior = false;   // set by the fused step: the interior pass feeds nothing
   int opt_iter = 0;
   DBuf opt_m, opt_v, opt_t;
   // captured optimisation step
@@ -694,7 +695,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   for (int v = 0; v < views; v++) {
     const DevCam& c = cam_cache[v];
     const int tiles = num_tiles(sq, c.w, c.h);   // interior samples: nsamp = sq*sq Hammersley points
-    if (st.mode != DRH_SILHOUETTE && G > 0) {
+    if (st.mode != DRH_SILHOUETTE && G > 0 && !skip_tex_interior) {
       InteriorArgs ia{};
       ia.tr = tracer_view(v); ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.sq = sq; ia.hamm = hamm.as<float>();
       ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
@@ -938,8 +939,12 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
   drh_status s;
   // re-commit from the device parameters (the optimiser moved them)
   if (h->opt_iter > 0 && (s = h->build_bvh())) return s;
-  if ((s = drh_d_render_and_compare_dev(h, cams, views, d_targets, edge_samples, h->grad.as<float>(), nullptr, d_loss)))
-    return s;
+  // TEXTURE / LAMBERT interior adjoints reach only the texture block (shade_grad,
+  // src/shade_lambert.cpp:37-42): with the texture frozen they feed nothing
+  h->skip_tex_interior = (h->st.mode == DRH_TEXTURE_COLOR || h->st.mode == DRH_LAMBERT) && h->tex_lr == 0.0f;
+  s = drh_d_render_and_compare_dev(h, cams, views, d_targets, edge_samples, h->grad.as<float>(), nullptr, d_loss);
+  h->skip_tex_interior = false;
+  if (s) return s;
   // StepDecay (src/optimizer_simple.cpp:82-90): decayPeriod 100, decay_mult 0.75 for GD_Adam,
   // applied after iterations 100, 200, ...
   float plr = h->pos_lr, clr = h->col_lr;
@@ -963,7 +968,12 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
       seg_start[nseg] = colo; seg_lr[nseg] = clr; seg_param[nseg] = h->col.as<float>() + 3ll * h->vo[i]; nseg++;
     }
     const int texo = h->layout[4 * i + 2];
-    if (texo >= 0) { seg_start[nseg] = texo; seg_lr[nseg] = 0.0f; seg_param[nseg] = nullptr; nseg++; }
+    if (texo >= 0) {
+      seg_start[nseg] = texo;
+      seg_lr[nseg] = h->tex_lr;
+      seg_param[nseg] = h->tex_lr != 0.0f ? h->texels.as<float>() + h->tex_off[i] : nullptr;
+      nseg++;
+    }
     seg_start[nseg] = h->layout[4 * i + 3]; seg_lr[nseg] = 0.0f; seg_param[nseg] = nullptr; nseg++;
     (void)nv;
     if (h->opt_kind == DRH_OPT_SIMPLE) {
@@ -1052,6 +1062,25 @@ drh_status drh_get_params(drh_handle h, float* pos, float* col) {
   if (pos && (s = h->hip(hipMemcpyAsync(pos, h->pos_orig.p, 12 * (size_t)h->V, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
   if (col && (s = h->hip(hipMemcpyAsync(col, h->col.p, 12 * (size_t)h->V, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
   return h->hip(hipStreamSynchronize(h->stream), "get_params");
+}
+
+drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr) {
+  if (!h) return DRH_E_INVALID;
+  if (!(tex_lr >= 0.0f) || tex_lr > 1e30f) return h->fail(DRH_E_INVALID, "opt_set_texture_lr: bad rate");
+  if (tex_lr != h->tex_lr) h->drop_graph();   // the captured step bakes the segment table in
+  h->tex_lr = tex_lr;
+  return DRH_OK;
+}
+
+drh_status drh_get_texels(drh_handle h, float* out) {
+  if (!h || !out) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "get_texels: commit first");
+  (void)hipSetDevice(h->st.device);
+  size_t n = 0;
+  for (int i = 0; i < h->nm; i++) n += 3 * (size_t)h->tex_w[i] * h->tex_h[i];
+  drh_status s;
+  if (n && (s = h->hip(hipMemcpyAsync(out, h->texels.p, 4 * n, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
+  return h->hip(hipStreamSynchronize(h->stream), "get_texels");
 }
 
 void* drh_stream(drh_handle h) { return h ? (void*)h->stream : nullptr; }

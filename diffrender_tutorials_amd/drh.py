@@ -31,7 +31,8 @@ STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_
 # every symbol include/drh.h declares
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
-           "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params", "drh_stream",
+           "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params",
+           "drh_opt_set_texture_lr", "drh_get_texels", "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
@@ -204,6 +205,14 @@ class DiffRender:
     def opt_step_dev(self, cams_c, views, d_targets_ptr, edge_samples, d_loss_ptr=None):
         self._check(load().drh_opt_step(self.h, cams_c, views, ctypes.c_void_p(d_targets_ptr), int(edge_samples),
                                         ctypes.c_void_p(d_loss_ptr) if d_loss_ptr else None))
+
+    def opt_set_texture_lr(self, lr):
+        self._check(load().drh_opt_set_texture_lr(self.h, ctypes.c_float(lr)))
+
+    def get_texels(self, n):
+        out = np.empty(n, np.float32)
+        self._check(load().drh_get_texels(self.h, _fp(out)))
+        return out
 
     def get_params(self):
         pos = np.empty(3 * self.V, np.float32)

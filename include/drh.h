@@ -138,6 +138,14 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
 drh_status drh_opt_use_graph(drh_handle h, int on);
 /* Download the current (optimised) vertex positions (3V) and colours (3V, may be NULL). */
 drh_status drh_get_params(drh_handle h, float* pos, float* col);
+/* Learning rate of the texture block (OptimizerParameters::textures_lr with _OPT_TEX,
+ * src/optimizer.h:26-45, src/optimizer_simple.cpp:168-183); 0 (the default) keeps textures
+ * fixed.  With 0 in TEXTURE_COLOR / LAMBERT mode the fused step leaves out the interior pass,
+ * whose only output there is the texture gradient: the step's results are unchanged. */
+drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr);
+/* Download the committed scene's textures (all meshes, in commit order; the floats of
+ * drh_scene_desc.texels), as the optimiser has left them. */
+drh_status drh_get_texels(drh_handle h, float* texels);
 
 /* Device memory plumbing (no counterpart in the reference): for callers that hold no HIP
  * allocator of their own, e.g. Python tests and the benchmark.  Allocations are on the

@@ -192,3 +192,62 @@ def test_server_admits_jobs_arriving_while_others_run():
     srv.close()
     assert all(r["status"] == 0 and r["iterations"] == 15 for r in res)
     assert st["jobs_done"] == 5 and st["peak_active"] <= 3
+
+
+def test_fused_lambert_step_without_interior_pass_matches_full_gradient_step():
+    """TEXTURE / LAMBERT interior adjoints only reach the texture block; with the texture
+    frozen the fused step leaves that pass out.  One fused Adam step must equal the same step
+    done from the full d_render_and_compare gradient (interior included) on the host."""
+    lam = S.lambert_config2()
+    m, t, cams = _job(3, 96, 51)
+    tg = _targets(t, cams, 4, lam)
+    N = 1 << 14
+    r = drh.DiffRender(mode=drh.LAMBERT, spp=4, light=lam)
+    r.commit([m])
+    r.opt_init(drh.OPT_ADAM, 1e-3, 1e-3)
+    dt = r.device_array(tg[0].reshape(-1))
+    dl = r.device_array(np.zeros(1, np.float32))
+    r.opt_step_dev(drh.cameras_array(cams), 1, dt.ptr, N, dl.ptr)
+    pos, _ = r.get_params()
+    fused_loss = float(dl.download(1)[0])
+    r.close()
+    h = drh.DiffRender(mode=drh.LAMBERT, spp=4, light=lam)
+    h.commit([m])
+    loss, g, _ = h.d_render_and_compare(cams, tg, N, images=False)
+    h.close()
+    tex_block = g[3 * m.V: 3 * m.V + 3]
+    assert np.abs(tex_block).max() > 0        # the full gradient has a texture part ...
+    gp = g[: 3 * m.V].reshape(-1, 3)
+    mh = 0.1 * gp / (1 - 0.9)                 # ... the first Adam step uses only positions
+    vh = 0.001 * gp * gp / (1 - 0.999)
+    want = m.pos - 1e-3 * mh / (np.sqrt(vh) + 1e-8)
+    assert np.isclose(fused_loss, loss, rtol=1e-6)
+    assert np.abs(pos - want).max() < 1e-6
+
+
+def test_texture_learning_rate_fits_albedo():
+    """With a texture learning rate the optimiser fits the albedo (OPT_TEX): a sphere rendered
+    with albedo 0.5 as the target, starting from 0.8 with the positions frozen."""
+    lam = S.lambert_config2()
+    m = S.with_albedo(S.icosphere(3), 0.8)
+    t = S.with_albedo(S.icosphere(3), 0.5)
+    cams = [S.camera_look((0.4, 0.6, 2.9), 96, 96)]
+    tg = _targets(t, cams, 4, lam)
+    r = drh.DiffRender(mode=drh.LAMBERT, spp=4, light=lam)
+    r.commit([m])
+    r.opt_init(drh.OPT_ADAM, 0.0, 0.0)
+    r.opt_set_texture_lr(2e-2)
+    dt = r.device_array(tg[0].reshape(-1))
+    dl = r.device_array(np.zeros(1, np.float32))
+    cs = drh.cameras_array(cams)
+    losses = []
+    for _ in range(20):
+        r.opt_step_dev(cs, 1, dt.ptr, 1 << 12, dl.ptr)
+        losses.append(float(dl.download(1)[0]))
+    tex = r.get_texels(3)
+    pos, _ = r.get_params()
+    r.close()
+    say("albedo", tex, "loss", losses[0], losses[-1])
+    assert np.array_equal(pos, m.pos)
+    assert losses[-1] < 0.2 * losses[0]
+    assert np.all(np.abs(tex - 0.5) < np.abs(0.8 - 0.5) * 0.5)

@@ -79,6 +79,100 @@ def plain_leg(drh, meshes, cam, target, N, spp, lam, iters):
     return iters / dt
 
 
+def plain_fit(drh, meshes, cams, targets, mode, spp, N, lam, iters):
+    """The plain one-job-at-a-time path for one job: create + commit, then per iteration
+    commit, d_render_and_compare with host targets / host gradient, host Adam (lr 1e-3) on
+    positions (and colours in vertex-colour mode) -- the reference's optimiser loop over the
+    drop-in.  Returns the parameters after `iters` iterations."""
+    r = drh.DiffRender(mode=mode, spp=spp, light=lam)
+    m = meshes[0].copy()
+    V = m.V
+    r.commit([m])
+    P = r.grad_size()
+    mom, vel = np.zeros(P, np.float32), np.zeros(P, np.float32)
+    b1, b2, eps, lr = 0.9, 0.999, 1e-8, 1e-3
+    nparam = 6 * V if mode == drh.VERTEX_COLOR else 3 * V
+    for t in range(1, iters + 1):
+        r.commit([m])
+        _, g, _ = r.d_render_and_compare(cams, targets, N, images=False)
+        mom[:] = b1 * mom + (1 - b1) * g
+        vel[:] = b2 * vel + (1 - b2) * g * g
+        step = lr * (mom / (1 - b1 ** t)) / (np.sqrt(vel / (1 - b2 ** t)) + eps)
+        m.pos = (m.pos.reshape(-1) - step[:3 * V]).reshape(-1, 3).astype(np.float32)
+        if nparam > 3 * V:
+            m.col = (m.col.reshape(-1) - step[3 * V:6 * V]).reshape(-1, 3).astype(np.float32)
+    r.close()
+    return m
+
+
+def mix_legs(drh, S, lam, njobs, iters, max_active):
+    """Config 3's uneven batch (64 jobs by default): the plain path fits the jobs one after
+    another; the server takes them all at once (continuous batching, `max_active` slots).
+    Wall time of each includes per-job setup.  Returns a dict for the bench line."""
+    jobs = S.config3_jobs(njobs)
+    targets = []
+    for j in jobs:
+        rt = drh.DiffRender(mode=j["mode"], spp=j["spp"], light=lam)
+        rt.commit(j["target"])
+        targets.append(rt.render(j["cams"]))
+        rt.close()
+    t0 = time.perf_counter()
+    for j, tg in zip(jobs, targets):
+        plain_fit(drh, j["meshes"], j["cams"], tg, j["mode"], j["spp"], j["edge_samples"], lam, iters)
+    plain_s = time.perf_counter() - t0
+    srv = drh.Server(hbm_budget=64 << 30, max_active=max_active)
+    t0 = time.perf_counter()
+    ids = [srv.submit(j["meshes"], j["cams"], tg, iters, j["edge_samples"], mode=j["mode"], spp=j["spp"], light=lam)
+           for j, tg in zip(jobs, targets)]
+    res = [srv.wait(i) for i in ids]
+    served_s = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    assert all(x["status"] == 0 for x in res), [x["status"] for x in res]
+    total = njobs * iters
+    return {"workload": f"config3: {njobs} uneven jobs (F log-uniform 12..80k, 128-512 px, VCOL/Lambert, N = 4 px),"
+                        f" {iters} Adam iterations each",
+            "plain_iters_per_s": round(total / plain_s, 2), "served_iters_per_s": round(total / served_s, 2),
+            "served_over_plain": round(plain_s / served_s, 3), "p50_ms": round(st["p50_ms"], 4),
+            "p99_ms": round(st["p99_ms"], 4), "peak_active": st["peak_active"],
+            "mean_active": round(st["mean_active"], 3), "triangles": [int(j["meshes"][0].F) for j in jobs]}
+
+
+def poisson_leg(drh, S, lam, njobs, rate, max_active, iter_scale):
+    """Config 5's serving stream: `njobs` mesh-fitting jobs submitted at Poisson arrival times
+    (rate jobs/s, seed 7) to one server under a 288 GB HBM budget; reports per-iteration
+    p50 / p99 latency, jobs/s and iters/s over the whole stream.  `iter_scale` shrinks the
+    50-500 iteration counts to keep the run short (the same scale for every job)."""
+    jobs = S.config5_jobs(njobs)
+    targets = []
+    for j in jobs:
+        rt = drh.DiffRender(mode=j["mode"], spp=j["spp"], light=lam)
+        rt.commit(j["target"])
+        targets.append(rt.render(j["cams"]))
+        rt.close()
+    gaps = np.random.default_rng(7).exponential(1.0 / rate, size=njobs)
+    srv = drh.Server(hbm_budget=288 << 30, max_active=max_active)
+    t0 = time.perf_counter()
+    ids, iters_total, due = [], 0, 0.0
+    for j, tg, g in zip(jobs, targets, gaps):
+        due += g
+        while time.perf_counter() - t0 < due:
+            time.sleep(0.0005)
+        it = max(1, int(round(j["iterations"] * iter_scale)))
+        iters_total += it
+        ids.append(srv.submit(j["meshes"], j["cams"], tg, it, j["edge_samples"], mode=j["mode"], spp=j["spp"], light=lam))
+    res = [srv.wait(i) for i in ids]
+    dt = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    assert all(x["status"] == 0 for x in res), [x["status"] for x in res]
+    return {"workload": f"config5: {njobs} Lambert jobs at Poisson rate {rate}/s (F log-uniform 12..200k, 128-1024 px, "
+                        f"1-8 views, 50-500 iterations x {iter_scale}), 288 GB budget, {max_active} slots",
+            "jobs_per_s": round(njobs / dt, 3), "iters_per_s": round(iters_total / dt, 2),
+            "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4), "peak_active": st["peak_active"],
+            "mean_active": round(st["mean_active"], 3), "wall_s": round(dt, 3)}
+
+
 def served_leg(drh, S, meshes, cam, N, spp, lam, jobs, iters):
     """The served form: `jobs` config-2 mesh-fitting jobs (distinct targets) submitted to the
     continuous-batching server at once; each keeps its scene, LBVH, Adam moments and target
@@ -117,6 +211,12 @@ def main():
     ap.add_argument("--serve-iters", type=int, default=50, help="Adam iterations per served job")
     ap.add_argument("--plain-iters", type=int, default=20, help="iterations of the plain host-driven loop")
     ap.add_argument("--no-serve", action="store_true", help="skip the plain and served legs")
+    ap.add_argument("--mix-jobs", type=int, default=64, help="jobs in the config-3 uneven mix (0: skip)")
+    ap.add_argument("--mix-iters", type=int, default=30, help="Adam iterations per mix job")
+    ap.add_argument("--mix-active", type=int, default=16, help="server slots for the mix")
+    ap.add_argument("--stream-jobs", type=int, default=24, help="jobs in the config-5 Poisson stream (0: skip)")
+    ap.add_argument("--stream-rate", type=float, default=20.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--stream-iter-scale", type=float, default=0.1, help="scale of the 50-500 iteration counts")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (k_copy_f32), the known-size reference "
                          "for FETCH_SIZE / WRITE_SIZE counters")
@@ -240,6 +340,10 @@ def main():
         result["served"] = served
         result["served_over_plain"] = round(served["iters_per_s"] / plain, 3)
         result["p50_ms"], result["p99_ms"] = served["p50_ms"], served["p99_ms"]
+        if args.mix_jobs > 0:
+            result["mix"] = mix_legs(drh, S, lam, args.mix_jobs, args.mix_iters, args.mix_active)
+        if args.stream_jobs > 0:
+            result["stream"] = poisson_leg(drh, S, lam, args.stream_jobs, args.stream_rate, 8, args.stream_iter_scale)
 
     if not args.no_cpu:
         import oracle as O

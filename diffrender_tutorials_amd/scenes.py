@@ -287,3 +287,57 @@ def lambert_config2():
     """Config-2 light: one directional light toward (1,1,1)/sqrt3, white, ambient 0.1."""
     d = -_normalize(np.array([1.0, 1.0, 1.0], dtype=np.float32))
     return dict(light_dir=d, light_color=np.ones(3, np.float32), ambient=np.full(3, 0.1, np.float32))
+
+
+def config3_jobs(n: int = 64, seed: int = 5):
+    """Config 3's uneven batch: per job a triangle count F = round(exp(U(ln 12, ln 80000)))
+    (seed 5) met by the closest icosphere (20 * 4^k triangles) or the 12-triangle cube,
+    resolution uniform in {128, 256, 512}, vertex-colour or Lambert shading at random, edge
+    samples = 4 x pixels; the target is the mesh with radial noise 0.05.  Returns dicts with
+    meshes, target, cams, mode (2 VCOL / 4 LAMBERT), spp, edge_samples."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(n):
+        F = int(round(math.exp(rng.uniform(math.log(12), math.log(80000)))))
+        res = int(rng.choice([128, 256, 512]))
+        mode = 2 if rng.uniform() < 0.5 else 4
+        if F < 16:
+            base = cube(0.5, seed=100 + k)
+        else:
+            lvl = int(np.clip(round(math.log(F / 20.0) / math.log(4.0)), 0, 6))
+            base = icosphere(lvl, 1.0)
+        target = radial_displace(base, 0.05, seed=200 + k, smooth=False)
+        if mode == 2:
+            m, t = with_random_colors(base, 300 + k), with_random_colors(target, 400 + k)
+        else:
+            m, t = with_albedo(base, 0.8), with_albedo(target, 0.8)
+        az = rng.uniform(0, 2 * math.pi)
+        eye = (3.0 * math.sin(az), 0.8, 3.0 * math.cos(az))
+        cam = camera_look(eye, res, res)
+        jobs.append(dict(meshes=[m], target=[t], cams=[cam], mode=mode, spp=1, edge_samples=4 * res * res))
+    return jobs
+
+
+def config5_jobs(n: int = 32, seed: int = 7):
+    """Config 5's serving stream: per job a triangle count log-uniform in [12, 200k] (closest
+    icosphere or the cube), resolution uniform in {128, 256, 512, 1024}, 1-8 views on a ring,
+    50-500 Adam iterations, Lambert shading, radial-noise-0.05 targets, N = 4 x pixels of one
+    view.  `n` jobs with seed 7; arrival times are drawn by the caller."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(n):
+        F = int(round(math.exp(rng.uniform(math.log(12), math.log(200000)))))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        views = int(rng.integers(1, 9))
+        iters = int(rng.integers(50, 501))
+        if F < 16:
+            base = cube(0.5, seed=500 + k)
+        else:
+            lvl = int(np.clip(round(math.log(F / 20.0) / math.log(4.0)), 0, 6))   # <= 81920 triangles
+            base = icosphere(lvl, 1.0)
+        target = radial_displace(base, 0.05, seed=600 + k, smooth=False)
+        cams = cameras_ring(views, res, res)
+        jobs.append(dict(meshes=[with_albedo(base, 0.8)], target=[with_albedo(target, 0.8)], cams=cams, mode=4,
+                         spp=1, edge_samples=4 * res * res, iterations=iters))
+    return jobs
+

@@ -251,6 +251,12 @@ struct drh_renderer {
   drh_settings st{};
   int sq = 1, nsamp = 1;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for the edge sampler's preparation of view 0, which
+  // depends on the geometry only and so runs beside the render; fork / join by events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool prep0_forked = false;
+  int sharing = 1;          // handles sharing the device (drh_set_device_sharing)
   std::string err;
   Light light{};
   bool committed = false;
@@ -400,6 +406,26 @@ struct drh_renderer {
   drh_status run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss);
   drh_status run_d_render(int views, const float* d_adj, long long N, float* d_grad, bool accumulate);
   drh_status ensure_rnd(long long N);
+  // projection, edge lengths and the exact CDF of view v on stream q
+  void edge_prep(int v, hipStream_t q) {
+    ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>()};
+    timed("k_project", [&] { launch_project(pa, q); });
+    timed("k_edge_len", [&] { launch_edge_len(edges.as<int2>(), E, proj.as<float4>(), pmf_raw.as<float>(), q); });
+    timed("k_cdf_par", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), cdfw.p, q, true); });
+  }
+  // fork view 0's edge preparation onto the side stream (not while profiling: per-kernel
+  // events time the main stream)
+  drh_status fork_prep0(long long N) {
+    prep0_forked = false;
+    if (prof || sharing > 1 || E <= 0 || N <= 0) return DRH_OK;
+    drh_status s;
+    if ((s = hip(hipEventRecord(ev_fork, stream), "fork")) || (s = hip(hipStreamWaitEvent(side, ev_fork, 0), "fork")))
+      return s;
+    edge_prep(0, side);
+    if ((s = hip(hipEventRecord(ev_join, side), "fork"))) return s;
+    prep0_forked = true;
+    return hip(hipGetLastError(), "edge prep");
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -712,10 +738,11 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
     }
     if (E > 0 && N > 0) {
-      ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>()};
-      timed("k_project", [&] { launch_project(pa, stream); });
-      timed("k_edge_len", [&] { launch_edge_len(edges.as<int2>(), E, proj.as<float4>(), pmf_raw.as<float>(), stream); });
-      timed("k_cdf_par", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), cdfw.p, stream, true); });
+      if (v == 0 && prep0_forked) {
+        if ((s = hip(hipStreamWaitEvent(stream, ev_join, 0), "join"))) return s;   // prepared beside the render
+      } else {
+        edge_prep(v, stream);
+      }
       EdgeArgs ea{};
       ea.tr = tracer_view(v); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
       ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
@@ -752,6 +779,10 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   r->st = *settings;
   if (hipSetDevice(settings->device) != hipSuccess) return DRH_E_HIP;
   if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return DRH_E_HIP;
+  if (hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess)
+    return DRH_E_HIP;
   r->sq = (int)std::sqrt((float)settings->spp);
   r->nsamp = r->sq * r->sq;
   // light (src/shade_lambert.cpp:10-12 unless a custom one is given)
@@ -782,6 +813,10 @@ void drh_destroy(drh_handle h) {
   h->drop_graph();
   for (auto& p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto e : h->event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamSynchronize(h->side);
+  (void)hipEventDestroy(h->ev_fork);
+  (void)hipEventDestroy(h->ev_join);
+  (void)hipStreamDestroy(h->side);
   (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -879,8 +914,11 @@ drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, in
     img = h->img.as<float>();
   }
   float* loss = d_loss ? d_loss : h->loss.as<float>();
+  if ((s = h->ensure_rnd(edge_samples)) || (s = h->fork_prep0(edge_samples))) return s;
   if ((s = h->run_render(views, img, d_targets, h->adj.as<float>(), loss))) return s;
-  return h->run_d_render(views, h->adj.as<float>(), edge_samples, d_grad, false);
+  s = h->run_d_render(views, h->adj.as<float>(), edge_samples, d_grad, false);
+  h->prep0_forked = false;
+  return s;
 }
 
 drh_status drh_d_render_and_compare(drh_handle h, const drh_camera* cams, int views, const float* targets,
@@ -1069,6 +1107,13 @@ drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr) {
   if (!(tex_lr >= 0.0f) || tex_lr > 1e30f) return h->fail(DRH_E_INVALID, "opt_set_texture_lr: bad rate");
   if (tex_lr != h->tex_lr) h->drop_graph();   // the captured step bakes the segment table in
   h->tex_lr = tex_lr;
+  return DRH_OK;
+}
+
+drh_status drh_set_device_sharing(drh_handle h, int handles) {
+  if (!h || handles < 1) return DRH_E_INVALID;
+  if ((handles > 1) != (h->sharing > 1)) h->drop_graph();   // the captured step bakes the fork in
+  h->sharing = handles;
   return DRH_OK;
 }
 

@@ -140,6 +140,7 @@ bool drh_server_s::admit_one(Job* j) {
   drh_status s = drh_create(&j->st, &h);
   if (s != DRH_OK) { finish(j, s, "drh_create failed"); return false; }
   j->h = h;
+  (void)drh_set_device_sharing(h, max_active);
   const drh_scene_desc d = j->desc();
   const size_t tbytes = 4 * j->targets.size();
   if ((s = drh_commit(h, &d)) != DRH_OK || (s = drh_opt_init(h, DRH_OPT_ADAM, j->lr_pos, j->lr_col)) != DRH_OK ||

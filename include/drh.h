@@ -143,6 +143,12 @@ drh_status drh_get_params(drh_handle h, float* pos, float* col);
  * fixed.  With 0 in TEXTURE_COLOR / LAMBERT mode the fused step leaves out the interior pass,
  * whose only output there is the texture gradient: the step's results are unchanged. */
 drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr);
+/* How many handles share the device (1 by default).  A handle alone runs view 0's edge
+ * preparation on a second stream beside the render; when several share the device (the
+ * server sets its slot count) that overlap comes from the other handles instead and the
+ * second stream is left idle (it would contend for hardware queues).  No counterpart in the
+ * reference. */
+drh_status drh_set_device_sharing(drh_handle h, int handles);
 /* Download the committed scene's textures (all meshes, in commit order; the floats of
  * drh_scene_desc.texels), as the optimiser has left them. */
 drh_status drh_get_texels(drh_handle h, float* texels);

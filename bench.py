@@ -173,6 +173,43 @@ def poisson_leg(drh, S, lam, njobs, rate, max_active, iter_scale):
             "mean_active": round(st["mean_active"], 3), "wall_s": round(dt, 3)}
 
 
+def multiview_leg(drh, S, steps, N):
+    """Config 4's multi-view adjoint: a displaced sphere of 199,808 triangles (radial noise
+    0.02, seed 6), per-vertex colours, 16 cameras on a ring (radius 3, 20 deg), 512x512 each,
+    spp 4; a fused step fits positions and colours (vertex-colour shading, so the interior
+    pass produces the colour gradient).  Times the step with the reference's Hammersley
+    interior samples (recomputed), and with aligned samples recomputed vs stored."""
+    base = S.uv_sphere(224, 448)
+    m = S.with_random_colors(S.radial_displace(base, 0.02, seed=6, smooth=False), 6)
+    t = S.with_random_colors(S.radial_displace(base, 0.02, seed=16, smooth=False), 16)
+    cams = S.cameras_ring(16, 512, 512)
+    rt = drh.DiffRender(mode=drh.VERTEX_COLOR, spp=4)
+    rt.commit([t])
+    targets = rt.render(cams)
+    rt.close()
+    out = {"workload": f"config4: {m.F} triangles / {m.V} vertices, 16 views x 512x512, spp 4, vertex colours, "
+                       f"{N} edge samples per view, Adam on positions + colours",
+           "hit_buffer_bytes": 16 * 16 * 512 * 512 * 4}
+    for name, aligned, store in (("hammersley_recompute", 0, 0), ("aligned_recompute", 1, 0), ("aligned_store", 1, 1)):
+        r = drh.DiffRender(mode=drh.VERTEX_COLOR, spp=4)
+        r.commit([m])
+        r.set_interior_samples(aligned, store)
+        r.opt_init(drh.OPT_ADAM, 1e-3, 1e-3)
+        dt = r.device_array(np.concatenate([x.reshape(-1) for x in targets]))
+        dl = r.device_array(np.zeros(1, np.float32))
+        cs = drh.cameras_array(cams)
+        for _ in range(2):
+            r.opt_step_dev(cs, 16, dt.ptr, N, dl.ptr)
+        r.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.opt_step_dev(cs, 16, dt.ptr, N, dl.ptr)
+        r.sync()
+        out[name + "_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
+        r.close()
+    return out
+
+
 def served_leg(drh, S, meshes, cam, N, spp, lam, jobs, iters):
     """The served form: `jobs` config-2 mesh-fitting jobs (distinct targets) submitted to the
     continuous-batching server at once; each keeps its scene, LBVH, Adam moments and target
@@ -214,6 +251,7 @@ def main():
     ap.add_argument("--mix-jobs", type=int, default=64, help="jobs in the config-3 uneven mix (0: skip)")
     ap.add_argument("--mix-iters", type=int, default=30, help="Adam iterations per mix job")
     ap.add_argument("--mix-active", type=int, default=16, help="server slots for the mix")
+    ap.add_argument("--multiview-steps", type=int, default=5, help="timed config-4 steps per interior mode (0: skip)")
     ap.add_argument("--stream-jobs", type=int, default=24, help="jobs in the config-5 Poisson stream (0: skip)")
     ap.add_argument("--stream-rate", type=float, default=20.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--stream-iter-scale", type=float, default=0.1, help="scale of the 50-500 iteration counts")
@@ -342,6 +380,8 @@ def main():
         result["p50_ms"], result["p99_ms"] = served["p50_ms"], served["p99_ms"]
         if args.mix_jobs > 0:
             result["mix"] = mix_legs(drh, S, lam, args.mix_jobs, args.mix_iters, args.mix_active)
+        if args.multiview_steps > 0:
+            result["multiview"] = multiview_leg(drh, S, args.multiview_steps, 1 << 18)
         if args.stream_jobs > 0:
             result["stream"] = poisson_leg(drh, S, lam, args.stream_jobs, args.stream_rate, 8, args.stream_iter_scale)
 

@@ -257,6 +257,11 @@ struct drh_renderer {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool prep0_forked = false;
   int sharing = 1;          // handles sharing the device (drh_set_device_sharing)
+  // interior samples: Hammersley (reference) or the render's grid (aligned); store: the render
+  // keeps each sample's closest hit and the interior pass reads it back
+  int interior_aligned = 0, interior_store = 0;
+  DBuf hitbuf;
+  bool hits_stored = false;
   std::string err;
   Light light{};
   bool committed = false;
@@ -403,7 +408,8 @@ struct drh_renderer {
   drh_status build_bvh();
   drh_status upload_scene(const drh_scene_desc* d);
   drh_status set_cams(const drh_camera* cams, int views, long long* total_floats);
-  drh_status run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss);
+  drh_status run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss,
+                        bool store_hits = false);
   drh_status run_d_render(int views, const float* d_adj, long long N, float* d_grad, bool accumulate);
   drh_status ensure_rnd(long long N);
   // projection, edge lengths and the exact CDF of view v on stream q
@@ -673,9 +679,16 @@ drh_status drh_renderer::set_cams(const drh_camera* c, int views, long long* tot
   return DRH_OK;
 }
 
-drh_status drh_renderer::run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss) {
+drh_status drh_renderer::run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss,
+                                    bool store_hits) {
   drh_status s0;
   if ((s0 = build_view_suspects(views))) return s0;
+  hits_stored = false;
+  if (store_hits) {
+    long long px = 0;
+    for (int v = 0; v < views; v++) px += (long long)cam_cache[v].w * cam_cache[v].h;
+    if ((s0 = hip(hitbuf.reserve(16 * (size_t)px * nsamp), "alloc"))) return s0;
+  }
   int part = 0;
   std::vector<int> tiles(views);
   for (int v = 0; v < views; v++) { tiles[v] = num_tiles(sq, cam_cache[v].w, cam_cache[v].h); part += tiles[v]; }
@@ -685,7 +698,8 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   }
   int part_off = 0;
   for (int v = 0; v < views; v++) {
-    RenderArgs ra{tracer_view(v), light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off};
+    RenderArgs ra{tracer_view(v), light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off,
+                  store_hits ? hitbuf.as<float4>() : nullptr};
     timed("k_render", [&] { launch_render(st.mode, ra, tiles[v], stream); });
     part_off += tiles[v];
   }
@@ -694,6 +708,7 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
     for (int v = 0; v < views; v++) denom += (double)cam_cache[0].w * cam_cache[0].h;
     timed("k_loss_final", [&] { launch_loss_final(loss_part.as<double>(), part, d_loss, denom, stream); });
   }
+  hits_stored = store_hits;
   return hip(hipGetLastError(), "render kernels");
 }
 
@@ -726,6 +741,8 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ia.tr = tracer_view(v); ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.sq = sq; ia.hamm = hamm.as<float>();
       ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
       ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
+      ia.aligned = interior_aligned;
+      ia.hits = (interior_store && hits_stored) ? hitbuf.as<float4>() : nullptr;
       int blocks = tiles;
       if ((st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) && nm == 1 && diff[0] && tex_w[0] * tex_h[0] == 1) {
         ia.uniform = 1; ia.uniform_off = layout[2];
@@ -915,9 +932,11 @@ drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, in
   }
   float* loss = d_loss ? d_loss : h->loss.as<float>();
   if ((s = h->ensure_rnd(edge_samples)) || (s = h->fork_prep0(edge_samples))) return s;
-  if ((s = h->run_render(views, img, d_targets, h->adj.as<float>(), loss))) return s;
+  const bool store = h->interior_store && h->interior_aligned && !h->skip_tex_interior && h->st.mode != DRH_SILHOUETTE;
+  if ((s = h->run_render(views, img, d_targets, h->adj.as<float>(), loss, store))) return s;
   s = h->run_d_render(views, h->adj.as<float>(), edge_samples, d_grad, false);
   h->prep0_forked = false;
+  h->hits_stored = false;
   return s;
 }
 
@@ -1107,6 +1126,15 @@ drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr) {
   if (!(tex_lr >= 0.0f) || tex_lr > 1e30f) return h->fail(DRH_E_INVALID, "opt_set_texture_lr: bad rate");
   if (tex_lr != h->tex_lr) h->drop_graph();   // the captured step bakes the segment table in
   h->tex_lr = tex_lr;
+  return DRH_OK;
+}
+
+drh_status drh_set_interior_samples(drh_handle h, int aligned, int store) {
+  if (!h || (aligned != 0 && aligned != 1) || (store != 0 && store != 1)) return DRH_E_INVALID;
+  if (store && !aligned) return h->fail(DRH_E_INVALID, "set_interior_samples: storing hits needs aligned samples");
+  if (aligned != h->interior_aligned || store != h->interior_store) h->drop_graph();
+  h->interior_aligned = aligned;
+  h->interior_store = store;
   return DRH_OK;
 }
 

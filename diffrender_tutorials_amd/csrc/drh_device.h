@@ -355,10 +355,11 @@ __device__ __forceinline__ F2 blend2(const float* tc, uint32_t A, uint32_t B, ui
 // for render and the projected scene's for edge samples (src/drender.h:353-354).
 template <int MODE>
 __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
-                                    const float* nrm, int* wst) {
+                                    const float* nrm, int* wst, Hit* hit_out = nullptr) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
   const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
+  if (hit_out) *hit_out = h;
   if (h.gtri == NO_HIT) return {0, 0, 0};
   if (MODE == 1) return {1, 1, 1};
   const DevScene& s = T.s;

@@ -130,6 +130,8 @@ struct RenderArgs {
   float* adj;
   double* loss_part;
   int part_off;
+  float4* hits;           // non-NULL: store each sample's closest hit (id bits, u, v, t) for the
+                          // interior pass, at ((pixel) * sq*sq + sample) of the views' pixels
 };
 
 struct InteriorArgs {
@@ -144,6 +146,8 @@ struct InteriorArgs {
   int lds_base, lds_n;    // gradient range accumulated in LDS (lds_n = 0: global atomics)
   int uniform, uniform_off;  // one 1x1 texture: wave-reduce into grad[uniform_off..+3)
   int coverage;           // uniform and tc == 0 everywhere: only whether a sample hits matters
+  int aligned;            // 1: samples at the render's grid offsets instead of Hammersley points
+  const float4* hits;     // non-NULL (aligned only): the render's stored hits instead of tracing
 };
 
 struct ProjectArgs {

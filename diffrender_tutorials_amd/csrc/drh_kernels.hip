@@ -449,7 +449,10 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
       const int dx = tm.s % sq, dy = tm.s / sq;
       const float xo = (dx + 0.5f) / float(sq);
       const float yo = (dy + 0.5f) / float(sq);
-      col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst);
+      Hit h;
+      col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h);
+      if (a.hits)
+        a.hits[(c.img_off / 3 + (long long)y * c.w + x) * tm.S + tm.s] = make_float4(__uint_as_float(h.gtri), h.u, h.v, h.t);
     }
     const int base = lane - tm.s;
     for (int k = 0; k < tm.S; k++) {
@@ -461,7 +464,11 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
       for (int dx = 0; dx < sq; dx++) {
         const float xo = (dx + 0.5f) / float(sq);
         const float yo = (dy + 0.5f) / float(sq);
-        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst);
+        Hit h;
+        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h);
+        if (a.hits)
+          a.hits[(c.img_off / 3 + (long long)y * c.w + x) * (sq * sq) + dy * sq + dx] =
+              make_float4(__uint_as_float(h.gtri), h.u, h.v, h.t);
         acc = add(acc, divs(col, ns));
       }
   }
@@ -522,17 +529,26 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
     // one Hammersley sample per lane in the interleaved layout, all of them otherwise
     const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
     for (int si = s_begin; si < s_end; si++) {
-      const float xo = a.hamm[2 * si], yo = a.hamm[2 * si + 1];
       const F3 val = divs(adj, (float)a.nsamp);
-      F3 o, d;
-      eye_ray(c, x + xo, y + yo, o, d);
-      if (a.coverage) {
-        // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
-        // hit, so the sample adds exactly `val` if the ray hits anything
-        if (covered(T, o, d, wst)) uni = add(uni, val);
-        continue;
+      Hit h;
+      if (a.hits) {
+        // the render's stored closest hit of this (grid) sample
+        const float4 r = a.hits[(c.img_off / 3 + (long long)y * c.w + x) * a.nsamp + si];
+        h = Hit{__float_as_uint(r.x), r.w, r.y, r.z};
+        if (a.coverage) { if (h.gtri != NO_HIT) uni = add(uni, val); continue; }
+      } else {
+        const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
+        const float yo = a.aligned ? ((si / sq) + 0.5f) / float(sq) : a.hamm[2 * si + 1];
+        F3 o, d;
+        eye_ray(c, x + xo, y + yo, o, d);
+        if (a.coverage) {
+          // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
+          // hit, so the sample adds exactly `val` if the ray hits anything
+          if (covered(T, o, d, wst)) uni = add(uni, val);
+          continue;
+        }
+        h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
       }
-      const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
       if (h.gtri == NO_HIT) continue;
       const int m = a.tr.s.tri_mesh[h.gtri];
       const int goff = a.layout[4 * m + 0];

@@ -32,7 +32,8 @@ STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params",
-           "drh_opt_set_texture_lr", "drh_get_texels", "drh_set_device_sharing", "drh_stream",
+           "drh_opt_set_texture_lr", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples",
+           "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
@@ -205,6 +206,9 @@ class DiffRender:
     def opt_step_dev(self, cams_c, views, d_targets_ptr, edge_samples, d_loss_ptr=None):
         self._check(load().drh_opt_step(self.h, cams_c, views, ctypes.c_void_p(d_targets_ptr), int(edge_samples),
                                         ctypes.c_void_p(d_loss_ptr) if d_loss_ptr else None))
+
+    def set_interior_samples(self, aligned, store=False):
+        self._check(load().drh_set_interior_samples(self.h, int(aligned), int(store)))
 
     def opt_set_texture_lr(self, lr):
         self._check(load().drh_opt_set_texture_lr(self.h, ctypes.c_float(lr)))

@@ -341,3 +341,29 @@ def config5_jobs(n: int = 32, seed: int = 7):
                          spp=1, edge_samples=4 * res * res, iterations=iters))
     return jobs
 
+
+def uv_sphere(n_lat: int, n_lon: int, radius: float = 1.0) -> Mesh:
+    """Latitude-longitude sphere: two poles, n_lat - 1 rings of n_lon vertices, outward
+    winding; F = 2 * n_lon * (n_lat - 1) (n_lat 224, n_lon 448: 199,808 triangles, 99,906
+    vertices -- config 4's ~200k-triangle sphere)."""
+    th = np.linspace(0.0, math.pi, n_lat + 1)[1:-1]
+    ph = np.linspace(0.0, 2 * math.pi, n_lon, endpoint=False)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    ring = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
+    pos = np.concatenate([[[0.0, 1.0, 0.0]], ring, [[0.0, -1.0, 0.0]]]) * radius
+    nr = n_lat - 1
+    S_ = pos.shape[0] - 1
+    ring_id = lambda r, k: 1 + r * n_lon + (k % n_lon)
+    tris = []
+    for k in range(n_lon):
+        tris.append((0, ring_id(0, k + 1), ring_id(0, k)))
+        tris.append((S_, ring_id(nr - 1, k), ring_id(nr - 1, k + 1)))
+    r = np.arange(nr - 1)[:, None]
+    k = np.arange(n_lon)[None, :]
+    a, b = 1 + r * n_lon + k, 1 + r * n_lon + (k + 1) % n_lon
+    c, d = a + n_lon, b + n_lon
+    quads = np.stack([np.stack([a, b, c], -1), np.stack([b, d, c], -1)], -2).reshape(-1, 3)
+    idx = np.concatenate([np.array(tris), quads]).astype(np.uint32)
+    pos = pos.astype(np.float32)
+    return Mesh(pos=pos, idx=idx, nrm=vertex_normals(pos, idx))
+

@@ -143,6 +143,14 @@ drh_status drh_get_params(drh_handle h, float* pos, float* col);
  * fixed.  With 0 in TEXTURE_COLOR / LAMBERT mode the fused step leaves out the interior pass,
  * whose only output there is the texture gradient: the step's results are unchanged. */
 drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr);
+/* Interior samples (interior_derivatives, src/drender.h:197-233).  aligned = 0 (default): the
+ * reference's Hammersley offsets, so every interior sample is a fresh ray (recompute).
+ * aligned = 1: the render's grid offsets (d + 0.5)/sqrt(spp) instead -- another unbiased
+ * estimate of the same pixel integral, not the reference's samples.  store = 1 (aligned only):
+ * d_render_and_compare and the fused step keep each render sample's closest hit (16 B per
+ * sample in HBM) and the interior pass reads it back instead of tracing; d_render alone
+ * recomputes.  No counterpart in the reference. */
+drh_status drh_set_interior_samples(drh_handle h, int aligned, int store);
 /* How many handles share the device (1 by default).  A handle alone runs view 0's edge
  * preparation on a second stream beside the render; when several share the device (the
  * server sets its slot count) that overlap comes from the other handles instead and the

@@ -751,7 +751,12 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       } else if (G <= INTERIOR_LDS_FLOATS) {
         ia.lds_base = 0; ia.lds_n = G;
         blocks = std::min(tiles, 1024);
+      } else {
+        ia.hash = INTERIOR_HASH;
+        ia.region = (sq == 1 || sq == 2 || sq == 4 || sq == 8) ? INTERIOR_REGION : 1;
+        blocks = num_regions(sq, c.w, c.h, ia.region);
       }
+      if (ia.region == 0) ia.region = 1;
       timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
     }
     if (E > 0 && N > 0) {

@@ -21,10 +21,22 @@ constexpr int INTERIOR_LDS_FLOATS = 4096;  // largest gradient target accumulate
 constexpr int EDGE_LDS_FLOATS = 4096;      // largest position gradient accumulated in LDS (edges)
 
 // pixel tiles per view, one wavefront each (matches tile_map in the kernels)
+__host__ __device__ inline int tile_side(int sq) { return (sq == 1 || sq == 2 || sq == 4 || sq == 8) ? 8 / sq : 8; }
 __host__ __device__ inline int num_tiles(int sq, int w, int h) {
-  const int ts = (sq == 1 || sq == 2 || sq == 4 || sq == 8) ? 8 / sq : 8;
+  const int ts = tile_side(sq);
   return ((w + ts - 1) / ts) * ((h + ts - 1) / ts);
 }
+// regions of rt x rt tiles (one wavefront each in the interior pass)
+__host__ __device__ inline int num_regions(int sq, int w, int h, int rt) {
+  const int ts = tile_side(sq) * rt;
+  return ((w + ts - 1) / ts) * ((h + ts - 1) / ts);
+}
+// Interior gradient targets (float3 each: a vertex colour or a texel) are summed in a
+// per-wavefront LDS hash table of this many entries before the global atomics; a region of
+// INTERIOR_REGION x INTERIOR_REGION interleaved tiles (256 samples) shares one table.
+constexpr int INTERIOR_HASH = 256;
+constexpr int INTERIOR_HASH_PROBES = 16;
+constexpr int INTERIOR_REGION = 2;
 
 struct PrepareArgs {
   int V, n_meshes, has_nrm;
@@ -148,6 +160,8 @@ struct InteriorArgs {
   int coverage;           // uniform and tc == 0 everywhere: only whether a sample hits matters
   int aligned;            // 1: samples at the render's grid offsets instead of Hammersley points
   const float4* hits;     // non-NULL (aligned only): the render's stored hits instead of tracing
+  int hash;               // entries of the per-wave LDS hash table (0: atomics straight to grad)
+  int region;             // tiles per side of a wave's region (1: one tile)
 };
 
 struct ProjectArgs {

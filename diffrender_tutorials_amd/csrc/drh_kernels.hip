@@ -420,13 +420,16 @@ __global__ void k_reorder(ReorderArgs a) {
 // and walk the LBVH as one coherent packet; otherwise an 8x8 tile, one pixel per lane, whose
 // samples are traced one after another.
 struct TileMap { int x, y, S, s; };   // pixel, lanes per pixel, this lane's sample (0 if S == 1)
-__device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
+__device__ __forceinline__ TileMap tile_map_xy(int sq, int tx, int ty, int lane) {
   const bool inter = sq == 1 || sq == 2 || sq == 4 || sq == 8;
   const int S = inter ? sq * sq : 1;
-  const int ts = inter ? 8 / sq : 8;          // tile side in pixels
-  const int tiles_x = (w + ts - 1) / ts;
+  const int ts = tile_side(sq);               // tile side in pixels
   const int p = lane / S;                     // pixel within the tile
-  return TileMap{(tile % tiles_x) * ts + p % ts, (tile / tiles_x) * ts + p / ts, S, lane % S};
+  return TileMap{tx * ts + p % ts, ty * ts + p / ts, S, lane % S};
+}
+__device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
+  const int tiles_x = (w + tile_side(sq) - 1) / tile_side(sq);
+  return tile_map_xy(sq, tile % tiles_x, tile / tiles_x, lane);
 }
 
 template <int MODE>
@@ -504,6 +507,46 @@ __global__ __launch_bounds__(1024) void k_loss_final(const double* part, int n, 
   if (threadIdx.x == 0) out[0] = float(float(s[0]) / float(denom));
 }
 
+// Sum `v` over each run of equal `key` in the wavefront (all 64 lanes active; lanes that
+// hold no contribution carry a key of their own); afterwards the last lane of a run holds
+// the run's total.  The doubling steps stop once no run is longer than the step.
+template <int K, typename Key>
+__device__ __forceinline__ bool seg_reduce(Key key, float* v) {
+  const int lane = threadIdx.x & 63;
+  const Key prev = __shfl_up(key, 1, 64);
+  const Key next = __shfl_down(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll((long long)(heads & upto));
+  for (int off = 1; off < 64; off <<= 1) {
+    if (!__ballot(lane - off >= start)) break;
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      const float o = __shfl_up(v[q], off, 64);
+      if (lane - off >= start) v[q] += o;
+    }
+  }
+  return lane == 63 || next != key;
+}
+
+// Add a float3 to gradient target `key` (its offset in grad) through the wave's LDS hash
+// table: linear probing, slots claimed by compare-and-swap; a target that finds no slot
+// within INTERIOR_HASH_PROBES goes straight to grad.
+__device__ __forceinline__ void hash_add(int* hkey, float* hval, int nh, float* grad, int key, float x, float y, float z) {
+  unsigned h = ((unsigned)key * 0x9E3779B1u) >> 16;
+  for (int p = 0; p < INTERIOR_HASH_PROBES; p++) {
+    h &= (unsigned)(nh - 1);
+    const int k = atomicCAS(&hkey[h], -1, key);
+    if (k == -1 || k == key) {
+      atomicAdd(&hval[3 * h], x); atomicAdd(&hval[3 * h + 1], y); atomicAdd(&hval[3 * h + 2], z);
+      return;
+    }
+    h++;
+  }
+  atomicAdd(grad + key, x); atomicAdd(grad + key + 1, y); atomicAdd(grad + key + 2, z);
+}
+
 // ---------------------------------------------------------------------------
 // interior_derivatives (src/drender.h:197-233): Hammersley subpixel offsets,
 // shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74) / <TEXTURE_COLOR> (:132-162, LAMBERT too)
@@ -516,85 +559,136 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
   const bool use_lds = a.lds_n > 0;
+  const int nh = a.hash;                       // hash table: nh keys, then 3*nh sums
+  int* hkey = reinterpret_cast<int*>(lgrad);
+  float* hval = lgrad + nh;
   for (int i = lane; i < a.lds_n; i += 64) lgrad[i] = 0.0f;
+  for (int i = lane; i < nh; i += 64) { hkey[i] = -1; hval[3 * i] = 0.0f; hval[3 * i + 1] = 0.0f; hval[3 * i + 2] = 0.0f; }
   __syncthreads();
   const int sq = a.sq;
-  const int ntiles = num_tiles(sq, c.w, c.h);
+  const int rt = a.region;
+  const int ts = tile_side(sq);
+  const int tiles_x = (c.w + ts - 1) / ts, tiles_y = (c.h + ts - 1) / ts;
+  const int units_x = (tiles_x + rt - 1) / rt;
+  const int nunits = units_x * ((tiles_y + rt - 1) / rt);
   F3 uni = {0, 0, 0};  // uniform-target accumulator (single 1x1 texture)
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const TileMap tm = tile_map(sq, c.w, tile, lane);
-    const int x = tm.x, y = tm.y;
-    if (x >= c.w || y >= c.h) continue;
-    const F3 adj = ld3(a.adj + c.img_off + 3ll * ((long long)y * c.w + x));
-    // one Hammersley sample per lane in the interleaved layout, all of them otherwise
-    const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
-    for (int si = s_begin; si < s_end; si++) {
-      const F3 val = divs(adj, (float)a.nsamp);
-      Hit h;
-      if (a.hits) {
-        // the render's stored closest hit of this (grid) sample
-        const float4 r = a.hits[(c.img_off / 3 + (long long)y * c.w + x) * a.nsamp + si];
-        h = Hit{__float_as_uint(r.x), r.w, r.y, r.z};
-        if (a.coverage) { if (h.gtri != NO_HIT) uni = add(uni, val); continue; }
-      } else {
-        const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
-        const float yo = a.aligned ? ((si / sq) + 0.5f) / float(sq) : a.hamm[2 * si + 1];
-        F3 o, d;
-        eye_ray(c, x + xo, y + yo, o, d);
-        if (a.coverage) {
-          // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
-          // hit, so the sample adds exactly `val` if the ray hits anything
-          if (covered(T, o, d, wst)) uni = add(uni, val);
-          continue;
+  // A wave's samples cover one compact pixel tile, so neighbouring lanes mostly hit the same
+  // triangle (vertex colours) or texel quad (texture): the per-sample terms are summed over
+  // runs of equal target in the wave and each run issues one set of atomics.
+  const bool runs = !a.coverage && (MODE == 2 || !a.uniform);
+  for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
+    for (int j = 0; j < rt * rt; j++) {
+      const int tx = (unit % units_x) * rt + j % rt, ty = (unit / units_x) * rt + j / rt;
+      if (tx >= tiles_x || ty >= tiles_y) continue;   // whole tile outside the image
+      const TileMap tm = tile_map_xy(sq, tx, ty, lane);
+      const int x = tm.x, y = tm.y;
+      const bool inside = x < c.w && y < c.h;
+      const F3 adj = inside ? ld3(a.adj + c.img_off + 3ll * ((long long)y * c.w + x)) : F3{0, 0, 0};
+      // one Hammersley sample per lane in the interleaved layout, all of them otherwise (the
+      // trip count is the same in every lane)
+      const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
+      for (int si = s_begin; si < s_end; si++) {
+        const F3 val = divs(adj, (float)a.nsamp);
+        long long key = -1 - lane;   // lanes without a contribution form runs of their own
+        float g[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int o0 = 0, o1 = 0, o2 = 0, o3 = 0;   // offsets (in G) the four float3 terms go to
+        do {
+          if (!inside) break;
+          Hit h;
+          if (a.hits) {
+            // the render's stored closest hit of this (grid) sample
+            const float4 r = a.hits[(c.img_off / 3 + (long long)y * c.w + x) * a.nsamp + si];
+            h = Hit{__float_as_uint(r.x), r.w, r.y, r.z};
+            if (a.coverage) { if (h.gtri != NO_HIT) uni = add(uni, val); break; }
+          } else {
+            const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
+            const float yo = a.aligned ? ((si / sq) + 0.5f) / float(sq) : a.hamm[2 * si + 1];
+            F3 o, d;
+            eye_ray(c, x + xo, y + yo, o, d);
+            if (a.coverage) {
+              // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
+              // hit, so the sample adds exactly `val` if the ray hits anything
+              if (covered(T, o, d, wst)) uni = add(uni, val);
+              break;
+            }
+            h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
+          }
+          if (h.gtri == NO_HIT) break;
+          const int m = a.tr.s.tri_mesh[h.gtri];
+          const int goff = a.layout[4 * m + 0];
+          if (goff < 0) break;
+          const int vo = a.tr.s.mesh_vo[m];
+          if (MODE == 2) {
+            const float w = 1.0f - h.u - h.v;
+            const F3 cA = mul(w, val), cB = mul(h.v, val), cC = mul(h.u, val);
+            g[0] = cA.x; g[1] = cA.y; g[2] = cA.z;
+            g[3] = cB.x; g[4] = cB.y; g[5] = cB.z;
+            g[6] = cC.x; g[7] = cC.y; g[8] = cC.z;
+            const int base = goff + a.layout[4 * m + 1];
+            const uint32_t* I = a.tr.s.idx + 3 * h.gtri;
+            const int gb = use_lds ? base - a.lds_base : base;
+            o0 = gb + 3 * ((int)I[0] - vo); o1 = gb + 3 * ((int)I[1] - vo); o2 = gb + 3 * ((int)I[2] - vo);
+            key = h.gtri;
+          } else {
+            const int w = a.tr.s.tex_w[m], hh = a.tr.s.tex_h[m];
+            if (w == 0 || hh == 0) break;
+            const uint32_t* I = a.tr.s.idx + 3 * h.gtri;
+            const F2 tcv = blend2(a.tr.s.tc, I[0], I[1], I[2], h.u, h.v);
+            // sample_bilinear_clamp_3f_grad (shade_common.cpp:23-45): tc not clamped
+            const float px = tcv.x * (float)w, py = tcv.y * (float)hh;
+            const int ix = (int)px, iy = (int)py;
+            const int x0 = clampi(ix, 0, w - 1), y0 = clampi(iy, 0, hh - 1);
+            const int x1 = clampi(ix + 1, 0, w - 1), y1 = clampi(iy + 1, 0, hh - 1);
+            const float dx = px - (float)x0, dy = py - (float)y0;
+            const float w00 = (1 - dx) * (1 - dy), w01 = (1 - dx) * dy, w10 = dx * (1 - dy), w11 = dx * dy;
+            if (a.uniform) {
+              uni.x += w00 * val.x + w01 * val.x + w10 * val.x + w11 * val.x;
+              uni.y += w00 * val.y + w01 * val.y + w10 * val.y + w11 * val.y;
+              uni.z += w00 * val.z + w01 * val.z + w10 * val.z + w11 * val.z;
+              break;
+            }
+            const int base = goff + a.layout[4 * m + 2];
+            const int gb = use_lds ? base - a.lds_base : base;
+            o0 = gb + 3 * (y0 * w + x0); o1 = gb + 3 * (y1 * w + x0);
+            o2 = gb + 3 * (y0 * w + x1); o3 = gb + 3 * (y1 * w + x1);
+            g[0] = w00 * val.x; g[1] = w00 * val.y; g[2] = w00 * val.z;
+            g[3] = w01 * val.x; g[4] = w01 * val.y; g[5] = w01 * val.z;
+            g[6] = w10 * val.x; g[7] = w10 * val.y; g[8] = w10 * val.z;
+            g[9] = w11 * val.x; g[10] = w11 * val.y; g[11] = w11 * val.z;
+            // the quad's corner (o0) and whether its far corners clamp onto it fix all four
+            key = 4ll * o0 + (x1 != x0 ? 2 : 0) + (y1 != y0 ? 1 : 0);
+          }
+        } while (false);
+        if (runs) {
+          const bool tail = seg_reduce<MODE == 2 ? 9 : 12>(key, g);
+          if (tail && key >= 0) {
+            if (nh > 0) {
+              hash_add(hkey, hval, nh, a.grad, o0, g[0], g[1], g[2]);
+              hash_add(hkey, hval, nh, a.grad, o1, g[3], g[4], g[5]);
+              hash_add(hkey, hval, nh, a.grad, o2, g[6], g[7], g[8]);
+              if (MODE != 2) hash_add(hkey, hval, nh, a.grad, o3, g[9], g[10], g[11]);
+            } else {
+              float* G = use_lds ? lgrad : a.grad;
+              atomicAdd(G + o0, g[0]); atomicAdd(G + o0 + 1, g[1]); atomicAdd(G + o0 + 2, g[2]);
+              atomicAdd(G + o1, g[3]); atomicAdd(G + o1 + 1, g[4]); atomicAdd(G + o1 + 2, g[5]);
+              atomicAdd(G + o2, g[6]); atomicAdd(G + o2 + 1, g[7]); atomicAdd(G + o2 + 2, g[8]);
+              if (MODE != 2) { atomicAdd(G + o3, g[9]); atomicAdd(G + o3 + 1, g[10]); atomicAdd(G + o3 + 2, g[11]); }
+            }
+          }
         }
-        h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
       }
-      if (h.gtri == NO_HIT) continue;
-      const int m = a.tr.s.tri_mesh[h.gtri];
-      const int goff = a.layout[4 * m + 0];
-      if (goff < 0) continue;
-      const int vo = a.tr.s.mesh_vo[m];
-      if (MODE == 2) {
-        const float w = 1.0f - h.u - h.v;
-        const F3 cA = mul(w, val), cB = mul(h.v, val), cC = mul(h.u, val);
-        const int base = goff + a.layout[4 * m + 1];
-        const uint32_t* I = a.tr.s.idx + 3 * h.gtri;
-        const int A = (int)I[0] - vo, B = (int)I[1] - vo, C = (int)I[2] - vo;
-        float* G = use_lds ? lgrad : a.grad;
-        const int gb = use_lds ? base - a.lds_base : base;
-        atomicAdd(G + gb + 3 * A, cA.x); atomicAdd(G + gb + 3 * A + 1, cA.y); atomicAdd(G + gb + 3 * A + 2, cA.z);
-        atomicAdd(G + gb + 3 * B, cB.x); atomicAdd(G + gb + 3 * B + 1, cB.y); atomicAdd(G + gb + 3 * B + 2, cB.z);
-        atomicAdd(G + gb + 3 * C, cC.x); atomicAdd(G + gb + 3 * C + 1, cC.y); atomicAdd(G + gb + 3 * C + 2, cC.z);
-      } else {
-        const int w = a.tr.s.tex_w[m], hh = a.tr.s.tex_h[m];
-        if (w == 0 || hh == 0) continue;
-        const uint32_t* I = a.tr.s.idx + 3 * h.gtri;
-        const F2 tcv = blend2(a.tr.s.tc, I[0], I[1], I[2], h.u, h.v);
-        // sample_bilinear_clamp_3f_grad (shade_common.cpp:23-45): tc not clamped
-        const float px = tcv.x * (float)w, py = tcv.y * (float)hh;
-        const int ix = (int)px, iy = (int)py;
-        const int x0 = clampi(ix, 0, w - 1), y0 = clampi(iy, 0, hh - 1);
-        const int x1 = clampi(ix + 1, 0, w - 1), y1 = clampi(iy + 1, 0, hh - 1);
-        const float dx = px - (float)x0, dy = py - (float)y0;
-        const float w00 = (1 - dx) * (1 - dy), w01 = (1 - dx) * dy, w10 = dx * (1 - dy), w11 = dx * dy;
-        if (a.uniform) {
-          uni.x += w00 * val.x + w01 * val.x + w10 * val.x + w11 * val.x;
-          uni.y += w00 * val.y + w01 * val.y + w10 * val.y + w11 * val.y;
-          uni.z += w00 * val.z + w01 * val.z + w10 * val.z + w11 * val.z;
-          continue;
+    }
+    if (nh > 0) {
+      // the region's sums: one set of atomics per target, then an empty table
+      __syncthreads();
+      for (int i = lane; i < nh; i += 64) {
+        const int k = hkey[i];
+        if (k >= 0) {
+          atomicAdd(a.grad + k, hval[3 * i]); atomicAdd(a.grad + k + 1, hval[3 * i + 1]); atomicAdd(a.grad + k + 2, hval[3 * i + 2]);
+          hkey[i] = -1; hval[3 * i] = 0.0f; hval[3 * i + 1] = 0.0f; hval[3 * i + 2] = 0.0f;
         }
-        const int base = goff + a.layout[4 * m + 2];
-        float* G = use_lds ? lgrad : a.grad;
-        const int gb = use_lds ? base - a.lds_base : base;
-        float* d00 = G + gb + 3 * (y0 * w + x0);
-        float* d01 = G + gb + 3 * (y1 * w + x0);
-        float* d10 = G + gb + 3 * (y0 * w + x1);
-        float* d11 = G + gb + 3 * (y1 * w + x1);
-        atomicAdd(d00, w00 * val.x); atomicAdd(d00 + 1, w00 * val.y); atomicAdd(d00 + 2, w00 * val.z);
-        atomicAdd(d01, w01 * val.x); atomicAdd(d01 + 1, w01 * val.y); atomicAdd(d01 + 2, w01 * val.z);
-        atomicAdd(d10, w10 * val.x); atomicAdd(d10 + 1, w10 * val.y); atomicAdd(d10 + 2, w10 * val.z);
-        atomicAdd(d11, w11 * val.x); atomicAdd(d11 + 1, w11 * val.y); atomicAdd(d11 + 2, w11 * val.z);
       }
+      __syncthreads();
     }
   }
   if (a.uniform) {
@@ -898,27 +992,6 @@ __device__ __forceinline__ int sample_edge(const float* cdf, int E, float u) {
   return clampi(first - 1, 0, E - 1);
 }
 
-// Sum `v` over each run of equal `key` in the wavefront (lanes hold consecutive sorted
-// samples); afterwards the last lane of a run holds the run's total.
-template <int K>
-__device__ __forceinline__ bool seg_reduce(int key, float* v) {
-  const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(key, 1, 64);
-  const int next = __shfl_down(key, 1, 64);
-  const bool head = lane == 0 || prev != key;
-  const uint64_t heads = __ballot(head);
-  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const int start = 63 - __clzll((long long)(heads & upto));
-  for (int off = 1; off < 64; off <<= 1) {
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-      const float o = __shfl_up(v[q], off, 64);
-      if (lane - off >= start) v[q] += o;
-    }
-  }
-  return lane == 63 || next != key;
-}
-
 template <int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_edge_sample(EdgeArgs a) {
   extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
@@ -1195,10 +1268,11 @@ void launch_loss_final(const double* part, int n, float* out, double denom, hipS
   hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1024), 0, s, part, n, out, denom);
 }
 void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s) {
+  const size_t lds = sizeof(float) * (size_t)std::max(a.lds_n, 4 * a.hash);
   if (mode == 2) {
-    hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a);
+    hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), lds, s, a);
   } else {
-    hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), a.lds_n * sizeof(float), s, a);
+    hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), lds, s, a);
   }
 }
 void launch_project(const ProjectArgs& a, hipStream_t s) {

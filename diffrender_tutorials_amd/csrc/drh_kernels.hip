@@ -765,11 +765,12 @@ __global__ __launch_bounds__(1024) void k_cdf(const float* pmf_raw, int E, float
 // sequence is the speculated one shifted by d = T_j - s_j whenever every speculated value
 // stays in one binade (one grid spacing u), no addition was a rounding tie (TwoSum error
 // exactly u/2; ties-to-even depends on parity, which an odd shift flips), and the shifted
-// end stays in the binade: round-to-nearest commutes with adding a grid multiple.  One lane
-// walks the chunks applying shifts (replaying a chunk where the conditions fail, from LDS);
-// then every chunk is recomputed from its true start and written out normalised by the total.
+// end stays in the binade: round-to-nearest commutes with adding a grid multiple.  One
+// wavefront walks the chunks applying shifts, 64 chunks per round (replaying a chunk where
+// the conditions fail, from LDS); then every chunk is recomputed from its true start and
+// written out normalised by the total.
 //   k_cdf_sums   double sum per chunk            k_cdf_scan   speculated starts
-//   k_cdf_spec   speculated ends, shift test     k_cdf_walk   true starts (serial)
+//   k_cdf_spec   speculated ends, shift test     k_cdf_walk   true starts (64 per round)
 //   k_cdf_final  cdf / pmf normalised
 __device__ __forceinline__ int fexp(float x) { return (int)((__float_as_uint(x) >> 23) & 0xFFu); }
 
@@ -880,79 +881,128 @@ __global__ __launch_bounds__(CDF_STAGE_T) void k_cdf_spec(const float* __restric
   w.ok[j] = good;
 }
 
-// True chunk starts.  The chunks that are not shift-invariant are staged in LDS first (in
-// chunk order, by the whole workgroup) so the serial replay reads LDS, 16 values ahead.
+// Serial float sum T = x + T over the len (<= CDF_CH = 128) values lanes hold in v0 (first
+// 64) and v1 (next 64), in order; T and the result are the same in every lane.
+static_assert(CDF_CH == 128, "chunk_sum reads two values per lane");
+__device__ __forceinline__ float chunk_sum(float T, float v0, float v1, int len) {
+  const int a0 = __float_as_int(v0), a1 = __float_as_int(v1);
+  if (len == CDF_CH) {
+#pragma unroll
+    for (int k = 0; k < 64; k++) T = __int_as_float(__builtin_amdgcn_readlane(a0, k)) + T;
+#pragma unroll
+    for (int k = 0; k < 64; k++) T = __int_as_float(__builtin_amdgcn_readlane(a1, k)) + T;
+  } else {
+    for (int k = 0; k < len; k++) T = __int_as_float(__builtin_amdgcn_readlane(k < 64 ? a0 : a1, k & 63)) + T;
+  }
+  return T;
+}
+
+// Inclusive prefix sum over the 64 lanes of a fully active wavefront (mod 2^32): row shifts
+// within each 16-lane row, then row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+// True chunk starts.  The chunks that are not shift-invariant are staged in LDS first (one
+// wavefront per chunk).  Wave 0 then resolves up to 64 chunks per round.  With
+// D_k = bits(T_k) - bits(s0_k) (true and speculated start of chunk k as integers), a chunk
+// that takes a fast path -- T_k == s0_k, or a shift-invariant chunk whose true start lies in
+// s0_k's binade and whose shifted end stays there -- ends at bits(se_k) + D_k, so
+//   D_{k+1} = D_k + bits(se_k) - bits(s0_{k+1}),
+// and a wave prefix sum of those differences gives every start up to the round's first chunk
+// that fails both tests; that chunk is summed serially from its true start and the next
+// round begins after it.  The starts are bit-identical to the chunk-by-chunk walk.
 __global__ __launch_bounds__(1024) void k_cdf_walk(const float* __restrict__ raw, int E, CdfWork w) {
-  __shared__ int soff[1024];
+  constexpr int MAXST = CDF_WALK_STAGE / CDF_CH;   // staged chunks
+  __shared__ int scnt[1024];
+  __shared__ int slot_chunk[MAXST];
   __shared__ float stage[CDF_WALK_STAGE];
   __shared__ float ls0[CDF_WALK_CHUNKS], lse[CDF_WALK_CHUNKS];
   __shared__ int lok[CDF_WALK_CHUNKS];
   const int n = cdf_chunks(E);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool lds_spec = n <= CDF_WALK_CHUNKS;
   if (lds_spec)
-    for (int j = threadIdx.x; j < n; j += 1024) { ls0[j] = w.s0[j]; lse[j] = w.se[j]; lok[j] = w.ok[j]; }
+    for (int j = tid; j < n; j += 1024) { ls0[j] = w.s0[j]; lse[j] = w.se[j]; lok[j] = w.ok[j]; }
   const float* S0 = lds_spec ? ls0 : w.s0;
   const float* SE = lds_spec ? lse : w.se;
   const int* OK = lds_spec ? lok : w.ok;
+  __syncthreads();
+  // rank of every chunk that is not shift-invariant, in chunk order
   const int per = (n + 1023) / 1024;
-  const int b = min((int)threadIdx.x * per, n), e = min(b + per, n);
-  int need = 0;
-  for (int j = b; j < e; j++) if (!w.ok[j]) need += min(CDF_CH, E - j * CDF_CH);
-  soff[threadIdx.x] = need;
+  const int b = min(tid * per, n), e = min(b + per, n);
+  int cnt = 0;
+  for (int j = b; j < e; j++) cnt += OK[j] ? 0 : 1;
+  scnt[tid] = cnt;
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {
-    const int v = (int)threadIdx.x >= o ? soff[threadIdx.x - o] : 0;
+    const int v = tid >= o ? scnt[tid - o] : 0;
     __syncthreads();
-    soff[threadIdx.x] += v;
+    scnt[tid] += v;
     __syncthreads();
   }
-  int at = soff[threadIdx.x] - need;
-  for (int j = b; j < e; j++) {
-    if (w.ok[j]) continue;
+  int r = scnt[tid] - cnt;
+  for (int j = b; j < e; j++)
+    if (!OK[j]) { if (r < MAXST) slot_chunk[r] = j; r++; }
+  const int nst = min(scnt[1023], MAXST);
+  __syncthreads();
+  for (int q = wave; q < nst; q += 16) {
+    const int j = slot_chunk[q];
     const int len = min(CDF_CH, E - j * CDF_CH);
-    if (at + len <= CDF_WALK_STAGE)
-      for (int q = 0; q < len; q++) stage[at + q] = raw[j * CDF_CH + q];
-    at += len;
+    for (int k = lane; k < len; k += 64) stage[q * CDF_CH + k] = raw[(size_t)j * CDF_CH + k];
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (wave != 0) return;
   float T = 0.0f;
-  int staged = 0;   // LDS offset of the next not-invariant chunk
-  float ssn = S0[0], sen = SE[0];
-  int okn = OK[0];
-  for (int j = 0; j < n; j++) {
-    w.ts[j] = T;
-    const float ss = ssn, se = sen;
-    const int okq = okn;
-    if (j + 1 < n) { ssn = S0[j + 1]; sen = SE[j + 1]; okn = OK[j + 1]; }  // ahead of the chain
-    const int len = min(CDF_CH, E - j * CDF_CH);
-    bool done = false;
-    if (T == ss) { T = se; done = true; }
-    else if (okq && fexp(T) == fexp(ss)) {
-      const float d = T - ss;       // exact: both on the binade's grid
-      const float te = se + d;
-      if (fexp(te) == fexp(ss)) { T = te; done = true; }
+  int rank = 0;   // chunks before j0 that are not shift-invariant (= LDS slot of the next one)
+  for (int j0 = 0; j0 < n;) {
+    const int j = j0 + lane;
+    const bool in = j < n;
+    const uint32_t uss = in ? __float_as_uint(S0[j]) : 0u, use = in ? __float_as_uint(SE[j]) : 0u;
+    const bool okj = in && OK[j] != 0;
+    // exclusive prefix over lanes of bits(se_k) - bits(s0_{k+1}), mod 2^32.  Up to the first
+    // slow chunk every start and speculated start is a float with the sign bit clear, so the
+    // true D_k lies in (-2^31, 2^31) and the wrapped sum read as signed is exact.
+    const uint32_t dl = use - (uint32_t)__shfl_down((int)uss, 1, 64);
+    const uint32_t pre = wave_incl_scan(dl) - dl;
+    const uint32_t tbits = __float_as_uint(T);
+    const long long D = (long long)(int32_t)(tbits - (uint32_t)__builtin_amdgcn_readfirstlane((int)uss) + pre);
+    const int bx = (int)((uss >> 23) & 0xFF);
+    const bool nan = bx == 255 && (uss & 0x7FFFFFu) != 0;
+    const long long tb = (long long)uss + D, teb = (long long)use + D;   // true start / shifted end
+    const bool sane = (tbits >> 31) == 0 && (uss >> 31) == 0 && (use >> 31) == 0;
+    const bool fast = in && sane && ((D == 0 && !nan) || (okj && (tb >> 23) == bx && (teb >> 23) == bx));
+    const uint64_t inm = __ballot(in), slow = inm & ~__ballot(fast), notok = inm & ~__ballot(okj);
+    const int L = __popcll(inm);
+    const int k = slow ? __ffsll((long long)slow) - 1 : L;
+    const float start = lane == 0 ? T : __uint_as_float((uint32_t)tb);
+    if (in && lane <= k) w.ts[j] = start;
+    if (k == L) {
+      T = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)teb, L - 1));
+      rank += __popcll(notok);
+      j0 += L;
+      continue;
     }
-    if (!done) {
-      if (!okq && staged + len <= CDF_WALK_STAGE) {
-        const float* v = stage + staged;
-        int k = 0;
-        for (; k + 16 <= len; k += 16) {
-          float x[16];
-#pragma unroll
-          for (int i = 0; i < 16; i++) x[i] = v[k + i];
-#pragma unroll
-          for (int i = 0; i < 16; i++) T = x[i] + T;
-        }
-        for (; k < len; k++) T = v[k] + T;
-      } else {
-        const float* v = raw + (size_t)j * CDF_CH;
-        for (int k = 0; k < len; k++) T = v[k] + T;
-      }
-    }
-    if (!okq) staged += len;
+    // chunk j0 + k: summed from its true start
+    const int jr = j0 + k;
+    T = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(start), k));
+    const uint64_t before = (1ull << k) - 1ull;      // k <= 63
+    const int slot = rank + __popcll(notok & before);
+    const bool staged = ((notok >> k) & 1ull) && slot < MAXST;
+    rank = slot + (int)((notok >> k) & 1ull);
+    const int len = min(CDF_CH, E - jr * CDF_CH);
+    const float* src = staged ? stage + slot * CDF_CH : raw + (size_t)jr * CDF_CH;
+    const float v0 = lane < len ? src[lane] : 0.0f, v1 = 64 + lane < len ? src[64 + lane] : 0.0f;
+    T = chunk_sum(T, v0, v1, len);
+    j0 = jr + 1;
   }
-  w.ts[n] = T;
+  if (lane == 0) w.ts[n] = T;
 }
 
 __global__ __launch_bounds__(CDF_STAGE_T) void k_cdf_final(const float* __restrict__ raw, int E, CdfWork w,

@@ -251,11 +251,13 @@ def test_cpu_reference_running_sum_is_sequential():
     assert np.array_equal(np.add.accumulate(p, dtype=np.float32), np.array(out, np.float32))
 
 
-@pytest.mark.parametrize("dist", ["uniform", "integer", "wide", "zeros", "lengths"])
-@pytest.mark.parametrize("E", [1, 2, 255, 256, 257, 1000, 30720, 300001])
+@pytest.mark.parametrize("dist", ["uniform", "integer", "wide", "zeros", "lengths", "ties"])
+@pytest.mark.parametrize("E", [1, 2, 255, 256, 257, 1000, 30720, 300001, 1100000])
 def test_parallel_cdf_is_bit_exact(dist, E):
     rng = np.random.default_rng(E * 7 + len(dist))
-    if dist == "uniform":
+    if dist == "ties":          # past 2^24 every addition of 65 is a rounding tie
+        p = np.full(E, 65.0)
+    elif dist == "uniform":
         p = rng.uniform(0, 1, E)
     elif dist == "integer":     # many rounding ties
         p = rng.integers(0, 64, E).astype(np.float64) * 0.5

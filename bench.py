@@ -34,8 +34,10 @@ def algorithmic_bytes(kernel, F, V, E, W, H, spp_sq, N, P, tiles):
         return scene + 12 * px + 12 * px + 12 * px + 8 * tiles   # image write, target read, adjoint write
     if kernel == "k_interior":
         return scene + 12 * px + 4 * P                            # adjoint read, gradient
-    if kernel == "k_edge_sample":
-        return scene + N * (12 + 24 + 8 + 8) + 12 * E + 4 * P     # adjoint, endpoints, edge ids, rng; edge/cdf; gradient
+    if kernel == "k_edge_trace":   # rng, endpoints, adjoint, side colours written; edges + cdf
+        return scene + N * (8 + 32 + 12 + 32) + 12 * E
+    if kernel == "k_edge_grad":    # side colours, rng, endpoints, adjoint, positions; pmf; gradient
+        return N * (32 + 8 + 32 + 12 + 24) + 4 * E + 4 * P
     return None
 
 
@@ -316,7 +318,7 @@ def main():
     dt = t1 - t0
     ms = 1e3 * dt / args.steps
 
-    kernels = ["k_render", "k_interior", "k_edge_sample", "k_cdf_par", "k_edge_bin", "k_edge_scan", "k_edge_scatter",
+    kernels = ["k_render", "k_interior", "k_edge_trace", "k_edge_grad", "k_cdf_par", "k_edge_bin",
                "k_suspects", "k_radix_sort", "k_refit", "k_karras",
                "k_tri_setup", "k_morton", "k_reorder", "k_pack_nodes", "k_prepare", "k_project", "k_edge_len",
                "k_loss_final", "k_fill_f32", "k_fill_u32", "k_adam"]
@@ -333,7 +335,7 @@ def main():
         step()
     r.sync()
     graph_ms = 1e3 * (time.perf_counter() - g0) / args.steps
-    dom = max(("k_render", "k_interior", "k_edge_sample"), key=lambda k: prof.get(k, {}).get("ms_per_launch", 0))
+    dom = max(("k_render", "k_interior", "k_edge_trace", "k_edge_grad"), key=lambda k: prof.get(k, {}).get("ms_per_launch", 0))
     tiles = ((W + 7) // 8) * ((H + 7) // 8)
     E = len(r.edges()[0])
     sq = int(np.sqrt(spp))

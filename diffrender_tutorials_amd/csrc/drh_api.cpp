@@ -260,6 +260,9 @@ struct drh_renderer {
   // interior samples: Hammersley (reference) or the render's grid (aligned); store: the render
   // keeps each sample's closest hit and the interior pass reads it back
   int interior_aligned = 0, interior_store = 0;
+  int edge_order = DRH_EDGE_ORDER_AUTO;   // sample order of the edge pass
+  DBuf bin_count, bin_rec, bin_order;     // pixel-bin order of the kept edge samples
+  DBuf ecol;                              // edge samples' side colours (2 float4 each)
   DBuf hitbuf;
   bool hits_stored = false;
   std::string err;
@@ -733,6 +736,22 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   // single differentiable mesh: position / transform gradient offsets
   int single = -1;
   for (int i = 0; i < nm; i++) if (diff[i]) { single = (single == -1) ? i : -2; }
+  // Edge samples in pixel-bin order when there are few samples per edge (rnd0 order then
+  // scatters a wavefront's rays over distant edges); buffers sized before any launch.
+  const bool binned = E > 0 && N > 0 &&
+                      (edge_order == DRH_EDGE_ORDER_BINNED ||
+                       (edge_order == DRH_EDGE_ORDER_AUTO && N < (long long)EDGE_BIN_AUTO_PER_EDGE * E));
+  if (E > 0 && N > 0 && (s = hip(ecol.reserve(32 * (size_t)N), "alloc"))) return s;
+  if (binned) {
+    int max_bins = 1;
+    for (int v = 0; v < views; v++) {
+      const DevCam& c = cam_cache[v];
+      max_bins = std::max(max_bins, ((c.w + EDGE_BIN - 1) / EDGE_BIN) * ((c.h + EDGE_BIN - 1) / EDGE_BIN));
+    }
+    if ((s = hip(bin_count.reserve(4 * ((size_t)max_bins + 1)), "alloc")) ||
+        (s = hip(bin_rec.reserve(16 * (size_t)N), "alloc")) || (s = hip(bin_order.reserve(8 * (size_t)N), "alloc")))
+      return s;
+  }
   for (int v = 0; v < views; v++) {
     const DevCam& c = cam_cache[v];
     const int tiles = num_tiles(sq, c.w, c.h);   // interior samples: nsamp = sq*sq Hammersley points
@@ -776,10 +795,23 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       if (single >= 0 && 3 * (vo[single + 1] - vo[single]) <= EDGE_LDS_FLOATS) {
         ea.lds_base = layout[4 * single]; ea.lds_n = 3 * (vo[single + 1] - vo[single]);
       }
+      if (binned) {
+        EdgeBinArgs ba{};
+        ba.cams = cams.as<DevCam>(); ba.view = v; ba.N = N; ba.E = E;
+        ba.edges = edges.as<int2>(); ba.cdf = cdf.as<float>(); ba.proj = proj.as<float4>();
+        ba.rnd = rnd.as<float2>(); ba.adj = d_adj;
+        ba.bins_x = (c.w + EDGE_BIN - 1) / EDGE_BIN;
+        ba.nbins = ba.bins_x * ((c.h + EDGE_BIN - 1) / EDGE_BIN);
+        ba.bin_count = bin_count.as<int>(); ba.rec = bin_rec.as<int4>(); ba.order = bin_order.as<int2>();
+        timed("k_fill_u32", [&] { launch_fill_u32(bin_count.as<uint32_t>(), ba.nbins + 1, 0u, stream); });
+        timed("k_edge_bin", [&] { launch_edge_bin(ba, stream); });
+        ea.order = ba.order; ea.nkept = ba.bin_count + ba.nbins;
+      }
       ea.single_mesh = single >= 0 ? 1 : 0;
       ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
-      const int blocks = (int)std::max<long long>(1, std::min<long long>((N + 255) / 256, 1024));
-      timed("k_edge_sample", [&] { launch_edge_sample(st.mode, ea, blocks, stream); });
+      ea.ecol = ecol.as<float4>();
+      timed("k_edge_trace", [&] { launch_edge_trace(st.mode, ea, stream); });
+      timed("k_edge_grad", [&] { launch_edge_grad(ea, stream); });
     }
   }
   return hip(hipGetLastError(), "d_render kernels");
@@ -1140,6 +1172,13 @@ drh_status drh_set_interior_samples(drh_handle h, int aligned, int store) {
   if (aligned != h->interior_aligned || store != h->interior_store) h->drop_graph();
   h->interior_aligned = aligned;
   h->interior_store = store;
+  return DRH_OK;
+}
+
+drh_status drh_set_edge_order(drh_handle h, int order) {
+  if (!h || order < DRH_EDGE_ORDER_AUTO || order > DRH_EDGE_ORDER_BINNED) return DRH_E_INVALID;
+  if (order != h->edge_order) h->drop_graph();
+  h->edge_order = order;
   return DRH_OK;
 }
 

@@ -185,12 +185,36 @@ struct EdgeArgs {
   const float* cdf;
   const float4* proj;
   const float2* rnd;      // N (rnd0, rnd1) pairs ordered by rnd0
+  const int2* order;      // non-NULL: the kept samples as (slot, edge) in pixel-bin order ...
+  const int* nkept;       // ... and their count (device); NULL: every slot in rnd0 order
+  float4* ecol;           // 2 per item: side colours (w: edge id, -1 rejected)
   const float* adj;
   float* grad;
   const int* layout;
   const float* tr_inv0;   // 16 floats
   int lds_base, lds_n;
   int single_mesh, single_tr_off;
+};
+
+// Edge samples binned by the EDGE_BIN x EDGE_BIN pixel tile of their sample point.  Only the
+// samples the reference keeps (on screen, adjoint not ~0) are binned, so the sampling pass
+// traces rays that share a small screen area whatever the number of samples per edge.
+constexpr int EDGE_BIN = 8;
+constexpr int EDGE_BIN_AUTO_PER_EDGE = 1;   // automatic binning below this many samples per edge
+struct EdgeBinArgs {
+  const DevCam* cams;
+  int view;
+  long long N;
+  int E;
+  const int2* edges;
+  const float* cdf;
+  const float4* proj;
+  const float2* rnd;
+  const float* adj;
+  int bins_x, nbins;
+  int* bin_count;         // nbins + 1, zero on entry; then bin starts and [nbins] = kept samples
+  int4* rec;              // per slot: edge, bin (negative: rejected), rank within the bin
+  int2* order;            // kept samples (slot, edge) in bin order
 };
 
 constexpr int MAX_SEG = 8;
@@ -246,7 +270,9 @@ void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_ra
 // exact CDF; `work` holds cdf_work_bytes(E) of device memory (parallel form only)
 size_t cdf_work_bytes(int E);
 void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work, hipStream_t s, bool parallel = true);
-void launch_edge_sample(int mode, const EdgeArgs& a, int blocks, hipStream_t s);
+void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s);   // side colours into ecol
+void launch_edge_grad(const EdgeArgs& a, hipStream_t s);              // gradients from ecol
+void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s);   // bin_count must be zero
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);
 void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_step_inc(int* step, hipStream_t s);

@@ -4,12 +4,13 @@
 //            k_karras, k_refit_{local,top}, k_pack_nodes, k_reorder   (replaces IRayTracer::Init)
 //   render   k_render<MODE> (+ fused L2 loss / adjoint), k_loss_final
 //   interior k_interior<MODE>                                          (interior_derivatives)
-//   edges    k_project, k_edge_len, k_cdf_{sums,scan,spec,walk,final}, k_edge_sample<MODE>
-//                                                                      (edge_derivatives)
+//   edges    k_project, k_edge_len, k_cdf_{sums,scan,spec,walk,final},
+//            k_edge_bin, k_bin_scan, k_edge_scatter (pixel-bin order),
+//            k_edge_trace<MODE> (side colours), k_edge_grad               (edge_derivatives)
 //   optimise k_optsimple_{moments,apply}, k_adam, k_step_inc
 //
 // Wave64 everywhere.  Rays are traced a wavefront at a time (trace_bvh in drh_device.h): a
-// pixel tile or a run of edge samples per wave, one wave-uniform LBVH walk per packet.
+// pixel tile or 32 edge samples (both sides) per wave, one wave-uniform LBVH walk per packet.
 #include <hip/hip_runtime.h>
 
 #include "drh_device.h"
@@ -1042,25 +1043,152 @@ __device__ __forceinline__ int sample_edge(const float* cdf, int E, float u) {
   return clampi(first - 1, 0, E - 1);
 }
 
+// Sample point of sample (rnd0 -> edge, rnd1 = t) and the reference's rejection tests
+// (src/drender.h:338-345): off-screen pixel (int() truncation; NaN and out-of-range values
+// are off-screen as x86 cvttss2si's INT_MIN) and |adjoint| < 1e-3.
+struct EdgePoint { int2 ev; float4 P0, P1; float t, px, py; int xi, yi; };
+__device__ __forceinline__ void edge_point(const int2* edges, const float4* proj, int eid, float t, EdgePoint& p) {
+  p.ev = edges[eid];
+  p.P0 = proj[p.ev.x]; p.P1 = proj[p.ev.y];
+  p.t = t;
+  p.px = p.P0.x + t * (p.P1.x - p.P0.x); p.py = p.P0.y + t * (p.P1.y - p.P0.y);
+  p.xi = 0; p.yi = 0;
+}
+__device__ __forceinline__ bool edge_keep(const DevCam& c, const float* adj, EdgePoint& p) {
+  if (p.px > -1.0f && p.py > -1.0f && p.px < (float)c.w && p.py < (float)c.h) {
+    p.xi = (int)p.px; p.yi = (int)p.py;
+    if (p.xi >= 0 && p.yi >= 0 && p.xi < c.w && p.yi < c.h) {
+      const F3 aj = ld3(adj + c.img_off + 3ll * ((long long)p.yi * c.w + p.xi));
+      return !((double)length(aj) < 1e-3);
+    }
+  }
+  return false;
+}
+
+// Bin every kept sample: its edge, bin and rank within the bin.  Slots in rnd0 order sample
+// a few consecutive edges per wavefront, so lanes form runs of equal bin that take their
+// ranks with one counter increment per run (rnd0 order kept within a run).
+__global__ __launch_bounds__(256) void k_edge_bin(EdgeBinArgs a) {
+  const DevCam& c = a.cams[a.view];
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long wbase = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < a.N; wbase += stride) {
+    const long long slot = wbase + lane;
+    int bin = -1 - lane, eid = 0;   // rejected and past-the-end lanes: runs of their own
+    if (slot < a.N) {
+      const float2 r = a.rnd[slot];
+      eid = sample_edge(a.cdf, a.E, r.x);
+      EdgePoint p;
+      edge_point(a.edges, a.proj, eid, r.y, p);
+      if (edge_keep(c, a.adj, p)) bin = (p.yi / EDGE_BIN) * a.bins_x + p.xi / EDGE_BIN;
+    }
+    const int prev = __shfl_up(bin, 1, 64);
+    const bool head = lane == 0 || prev != bin;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __clzll((long long)(heads & upto));
+    const uint64_t later = lane == 63 ? 0ull : (heads >> (lane + 1));
+    const int end = later ? lane + __ffsll((long long)later) : 64;   // one past the run
+    int base = 0;
+    if (head && bin >= 0) base = atomicAdd(&a.bin_count[bin], end - lane);
+    base = __shfl(base, start, 64);
+    if (slot < a.N) a.rec[slot] = make_int4(eid, bin, base + lane - start, 0);
+  }
+}
+
+// Bin counts -> exclusive starts in place; cnt[n] = total.
+__global__ __launch_bounds__(1024) void k_bin_scan(int* cnt, int n) {
+  __shared__ int part[1024];
+  const int per = (n + 1023) / 1024;
+  const int b = min((int)threadIdx.x * per, n), e = min(b + per, n);
+  int s = 0;
+  for (int j = b; j < e; j++) s += cnt[j];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - s;
+  for (int j = b; j < e; j++) { const int x = cnt[j]; cnt[j] = run; run += x; }
+  if (threadIdx.x == 1023) cnt[n] = part[1023];
+}
+
+__global__ void k_edge_scatter(EdgeBinArgs a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const int4 r = a.rec[i];
+  if (r.y >= 0) a.order[a.bin_count[r.y] + r.z] = make_int2((int)i, r.x);
+}
+
+// Edge pass, part 1: the colour on either side of every kept sample (shade at the sample
+// point offset by -/+1e-3 px along the edge normal, src/drender.h:347-360).  A wavefront takes
+// 32 samples with both sides in adjacent lanes, one ray (and shadow ray) per lane, so its
+// packet covers 32 nearby points and the kernel stays small enough for 8 waves per SIMD.
+// ecol[2i + side] = (colour, edge id bits; -1 for a rejected sample).
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_edge_sample(EdgeArgs a) {
-  extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
-  __shared__ float ltr[4][12];
+__global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
   __shared__ int wstack[4][64];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
+  const int lane = threadIdx.x & 63;
+  int* wst = wstack[threadIdx.x >> 6];
+  const long long items = a.order ? (long long)__builtin_amdgcn_readfirstlane(*a.nkept) : a.N;
+  const long long stride = (long long)gridDim.x * (blockDim.x / 2);
+  for (long long wbase = ((long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) / 2; wbase < items; wbase += stride) {
+    const long long item = wbase + (lane >> 1);
+    const int sd = lane & 1;
+    if (item >= items) continue;
+    EdgePoint p;
+    int eid;
+    bool live;
+    if (a.order) {
+      const int2 o = a.order[item];   // binned samples passed the rejection tests
+      eid = o.y;
+      edge_point(a.edges, a.proj, eid, a.rnd[o.x].y, p);
+      live = true;
+    } else {
+      const float2 r = a.rnd[item];
+      eid = sample_edge(a.cdf, a.E, r.x);
+      edge_point(a.edges, a.proj, eid, r.y, p);
+      live = edge_keep(c, a.adj, p);
+    }
+    float4 out = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(live ? eid : -1));
+    if (live) {
+      const float dvx = p.P1.x - p.P0.x, dvy = p.P1.y - p.P0.y;
+      const float len = sqrtf(dvx * dvx + dvy * dvy);
+      const float ndx = dvx / len, ndy = dvy / len;
+      const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
+      const float qx = sd == 0 ? p.px - 1e-3f * nx : p.px + 1e-3f * nx;
+      const float qy = sd == 0 ? p.py - 1e-3f * ny : p.py + 1e-3f * ny;
+      const F3 col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
+      out.x = col.x; out.y = col.y; out.z = col.z;
+    }
+    a.ecol[2 * item + sd] = out;
+  }
+}
+
+// Edge pass, part 2: each kept sample's position / transform gradient from its two side
+// colours (src/drender.h:361-425, edge_grad src/drender.cpp:31-87).
+__global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
+  extern __shared__ float lpos[];  // a.lds_n floats (dynamic)
+  __shared__ float ltr[4][12];
+  const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int* wst = wstack[wave];
   const bool use_lds = a.lds_n > 0;
   for (int i = threadIdx.x; i < a.lds_n; i += 256) lpos[i] = 0.0f;
   if (threadIdx.x < 48) ltr[threadIdx.x / 12][threadIdx.x % 12] = 0.0f;
   __syncthreads();
   const float* Ti = a.tr_inv0;  // mesh 0's inverse transform for every mesh (drender.cpp:60)
   const long long stride = (long long)gridDim.x * blockDim.x;
-  // Slots hold the samples in rnd0 order, so a wavefront's 64 slots sample a few consecutive
-  // edges.  Whole wavefronts step together (the run reduction needs all 64 lanes).
-  for (long long wbase = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < a.N; wbase += stride) {
-    const long long slot = wbase + lane;
+  // Items are the slots in rnd0 order (a wavefront's 64 slots sample a few consecutive edges)
+  // or the kept samples in pixel-bin order.  Whole wavefronts step together (the run
+  // reduction needs all 64 lanes).
+  const long long items = a.order ? (long long)__builtin_amdgcn_readfirstlane(*a.nkept) : a.N;
+  for (long long wbase = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < items; wbase += stride) {
+    const long long item = wbase + lane;
     // position gradient of v0, v1 (after mesh 0's inverse transform), then for a single
     // mesh the raw screen-chain terms of v0, v1 that its transform gradient is built from
     float gsum[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1068,25 +1196,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     int vn0 = 0, vn1 = 0, goff = -1;
     bool live = false;
     int eid = 0, xi = 0, yi = 0;
-    float t = 0.0f, px = 0.0f, py = 0.0f;
+    float t = 0.0f;
     float4 P0 = {0, 0, 0, 0}, P1 = {0, 0, 0, 0};
     int2 ev = {0, 0};
-    if (slot < a.N) {
-      const float2 r = a.rnd[slot];
-      eid = sample_edge(a.cdf, a.E, r.x);
-      ev = a.edges[eid];
-      P0 = a.proj[ev.x]; P1 = a.proj[ev.y];
-      t = r.y;
-      px = P0.x + t * (P1.x - P0.x); py = P0.y + t * (P1.y - P0.y);
-      // the reference's rejection tests (src/drender.h:338-345): off-screen pixel (int()
-      // truncation; NaN and out-of-range values are off-screen as x86 cvttss2si's INT_MIN)
-      // and |adjoint| < 1e-3
-      if (px > -1.0f && py > -1.0f && px < (float)c.w && py < (float)c.h) {
-        xi = (int)px; yi = (int)py;
-        if (xi >= 0 && yi >= 0 && xi < c.w && yi < c.h) {
-          const F3 aj = ld3(a.adj + c.img_off + 3ll * ((long long)yi * c.w + xi));
-          live = !((double)length(aj) < 1e-3);
-        }
+    F3 cin = {0, 0, 0}, cout = {0, 0, 0};
+    if (item < items) {
+      const float4 c0 = a.ecol[2 * item], c1 = a.ecol[2 * item + 1];
+      eid = __float_as_int(c0.w);
+      if (eid >= 0) {
+        // the sample point exactly as the trace pass computed it
+        EdgePoint p;
+        edge_point(a.edges, a.proj, eid, a.rnd[a.order ? a.order[item].x : item].y, p);
+        p.xi = (int)p.px; p.yi = (int)p.py;
+        live = true;
+        ev = p.ev; P0 = p.P0; P1 = p.P1; t = p.t; xi = p.xi; yi = p.yi;
+        cin = F3{c0.x, c0.y, c0.z}; cout = F3{c1.x, c1.y, c1.z};
       }
     }
     if (live) {
@@ -1096,16 +1220,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const float len = sqrtf(dvx * dvx + dvy * dvy);
     const float ndx = dvx / len, ndy = dvy / len;
     const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
-    // both sides of the edge through one (not unrolled) shade call: half the registers
-    F3 side[2];
-#pragma unroll 1
-    for (int sd = 0; sd < 2; sd++) {
-      const float qx = sd == 0 ? px - 1e-3f * nx : px + 1e-3f * nx;
-      const float qy = sd == 0 ? py - 1e-3f * ny : py + 1e-3f * ny;
-      const F3 col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
-      if (sd == 0) side[0] = col; else side[1] = col;
-    }
-    const F3 cin = side[0], cout = side[1];
     const float pdf = pmf / len;
     const float weight = 1.0f / (pdf * float(a.N));
     const float adjv = dot(sub(cin, cout), adj);
@@ -1345,8 +1459,20 @@ void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work,
   hipLaunchKernelGGL(k_cdf_walk, dim3(1), dim3(1024), 0, s, pmf_raw, E, w);
   hipLaunchKernelGGL(k_cdf_final, dim3(blocks), dim3(CDF_STAGE_T), 0, s, pmf_raw, E, w, pmf, cdf);
 }
-void launch_edge_sample(int mode, const EdgeArgs& a, int blocks, hipStream_t s) {
-  DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_sample<M>), dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a))
+void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s) {
+  if (a.N <= 0) return;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((a.N + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_edge_bin, dim3(blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, a.bin_count, a.nbins);
+  hipLaunchKernelGGL(k_edge_scatter, dim3((unsigned)((a.N + 255) / 256)), dim3(256), 0, s, a);
+}
+void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s) {
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((2 * a.N + 255) / 256, 8192));
+  DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M>), dim3(blocks), dim3(256), 0, s, a))
+}
+void launch_edge_grad(const EdgeArgs& a, hipStream_t s) {
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((a.N + 255) / 256, 2048));
+  hipLaunchKernelGGL(k_edge_grad, dim3(blocks), dim3(256), a.lds_n * sizeof(float), s, a);
 }
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s) {
   const int b = (a.n + 255) / 256;

@@ -23,6 +23,7 @@ LIB_PATH = os.environ.get("DRH_LIB") or os.path.join(HERE, "libdrh.so")
 SILHOUETTE, VERTEX_COLOR, TEXTURE_COLOR, LAMBERT = 1, 2, 3, 4
 RNG_REF_PRNG14, RNG_QMC = 0, 1
 TRACER_BVH, TRACER_BF = 0, 1
+EDGE_ORDER_AUTO, EDGE_ORDER_SAMPLE, EDGE_ORDER_BINNED = 0, 1, 2
 OPT_SIMPLE, OPT_ADAM = 0, 1
 
 STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_E_HIP", -4: "DRH_E_OOM",
@@ -38,7 +39,7 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
-           "drh_server_job_bytes"]
+           "drh_server_job_bytes", "drh_set_edge_order"]
 
 
 class DrhError(RuntimeError):
@@ -209,6 +210,10 @@ class DiffRender:
 
     def set_interior_samples(self, aligned, store=False):
         self._check(load().drh_set_interior_samples(self.h, int(aligned), int(store)))
+
+    def set_edge_order(self, order):
+        """EDGE_ORDER_AUTO / _SAMPLE (rnd0 slot order) / _BINNED (kept samples by 8x8 pixel bin)."""
+        self._check(load().drh_set_edge_order(self.h, int(order)))
 
     def opt_set_texture_lr(self, lr):
         self._check(load().drh_opt_set_texture_lr(self.h, ctypes.c_float(lr)))

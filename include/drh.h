@@ -151,6 +151,18 @@ drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr);
  * sample in HBM) and the interior pass reads it back instead of tracing; d_render alone
  * recomputes.  No counterpart in the reference. */
 drh_status drh_set_interior_samples(drh_handle h, int aligned, int store);
+/* Order in which the edge pass (edge_derivatives, src/drender.h:235-425) visits its samples.
+ * The samples, their edges and their contributions are the reference's in every order; only
+ * the order of the gradient sums (float rounding) differs.
+ *   DRH_EDGE_ORDER_SAMPLE  slots in rnd0 order: a wavefront samples a few consecutive edges.
+ *   DRH_EDGE_ORDER_BINNED  the kept samples (on screen, adjoint not ~0) binned by the 8x8
+ *                          pixel tile of their sample point first: a wavefront's rays share a
+ *                          small screen area whatever the number of samples per edge.
+ *   DRH_EDGE_ORDER_AUTO    (default) binned when there are fewer samples than edges, else
+ *                          sample order.
+ * No counterpart in the reference. */
+enum { DRH_EDGE_ORDER_AUTO = 0, DRH_EDGE_ORDER_SAMPLE = 1, DRH_EDGE_ORDER_BINNED = 2 };
+drh_status drh_set_edge_order(drh_handle h, int order);
 /* How many handles share the device (1 by default).  A handle alone runs view 0's edge
  * preparation on a second stream beside the render; when several share the device (the
  * server sets its slot count) that overlap comes from the other handles instead and the

@@ -305,14 +305,17 @@ struct drh_renderer {
   DBuf opt_m, opt_v, opt_t;
   // captured optimisation step
   bool use_graph = true;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<char> graph_sig;
-  unsigned long long graph_epoch = 0;
+  // two captured forms: [1] with the full LBVH build, [0] with the refit only
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<char> graph_sig[2];
+  unsigned long long graph_epoch[2] = {0, 0};
   void drop_graph() {
-    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
-    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
-    graph_sig.clear();
+    for (int k = 0; k < 2; k++) {
+      if (gexec[k]) { (void)hipGraphExecDestroy(gexec[k]); gexec[k] = nullptr; }
+      if (graph[k]) { (void)hipGraphDestroy(graph[k]); graph[k] = nullptr; }
+      graph_sig[k].clear();
+    }
   }
   // profiling
   bool prof = false;
@@ -408,7 +411,11 @@ struct drh_renderer {
     return hip(hipGetLastError(), "suspects");
   }
 
-  drh_status build_bvh();
+  // full: Morton sort + Karras topology + refit; otherwise (same triangle count, a topology
+  // from an earlier full build) only the boxes are refitted over the moved geometry
+  drh_status build_bvh(bool full = true);
+  bool topo_valid = false;   // child / parent / leaf-order buffers hold a topology for F
+  int rebuild_every = 8;     // fused steps: a full build every this many steps, refits between
   drh_status upload_scene(const drh_scene_desc* d);
   drh_status set_cams(const drh_camera* cams, int views, long long* total_floats);
   drh_status run_render(int views, float* d_img, const float* d_target, float* d_adj, float* d_loss,
@@ -589,7 +596,7 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
   return DRH_OK;
 }
 
-drh_status drh_renderer::build_bvh() {
+drh_status drh_renderer::build_bvh(bool full) {
   PrepareArgs pa{V, nm, has_nrm ? 1 : 0, mesh_vo.as<int>(), dmats.as<float>(), pos_orig.as<float>(),
                  nrm_in.as<float>(), pos.as<float>(), nrm.as<float>(), nrm2d.as<float>()};
   if (V > 0) timed("k_prepare", [&] { launch_prepare(pa, stream); });
@@ -623,6 +630,8 @@ drh_status drh_renderer::build_bvh() {
     timed("k_suspects", [&] { launch_suspects(la, stream); });
     light_sus = true;
   }
+  if (!topo_valid) full = true;
+  if (full) {
   MortonArgs ma{F, cen.as<float4>(), bounds.as<uint32_t>(), keys0.as<uint32_t>(), vals0.as<uint32_t>()};
   timed("k_morton", [&] { launch_morton(ma, stream); });
   for (int pass = 0; pass < 4; pass++) {
@@ -637,6 +646,10 @@ drh_status drh_renderer::build_bvh() {
   if (F > 1) {
     KarrasArgs ka{F, keys0.as<uint32_t>(), nrange.as<int2>(), child.as<int>(), node_parent.as<int>(), leaf_parent.as<int>()};
     timed("k_karras", [&] { launch_karras(ka, stream); });
+  }
+  }  // full
+  topo_valid = true;
+  if (F > 1) {
     timed("k_fill_u32", [&] { launch_fill_u32(counter.as<uint32_t>(), F + 1, 0u, stream); });
     RefitArgs ra{F, vals0.as<uint32_t>(), box_lo.as<float4>(), box_hi.as<float4>(), child.as<int>(),
                  node_parent.as<int>(), leaf_parent.as<int>(), nrange.as<int2>(), counter.as<uint32_t>(),
@@ -881,6 +894,7 @@ drh_status drh_commit(drh_handle h, const drh_scene_desc* scene) {
   if (!h) return DRH_E_INVALID;
   (void)hipSetDevice(h->st.device);
   h->committed = false;
+  h->topo_valid = false;
   drh_status s = h->upload_scene(scene);
   if (s) return s;
   if ((s = h->build_bvh())) return s;
@@ -1032,7 +1046,7 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
                                 int edge_samples, float* d_loss) {
   drh_status s;
   // re-commit from the device parameters (the optimiser moved them)
-  if (h->opt_iter > 0 && (s = h->build_bvh())) return s;
+  if (h->opt_iter > 0 && (s = h->build_bvh(h->opt_iter % h->rebuild_every == 0))) return s;   // a fresh commit's tree is current
   // TEXTURE / LAMBERT interior adjoints reach only the texture block (shade_grad,
   // src/shade_lambert.cpp:37-42): with the texture frozen they feed nothing
   h->skip_tex_interior = (h->st.mode == DRH_TEXTURE_COLOR || h->st.mode == DRH_LAMBERT) && h->tex_lr == 0.0f;
@@ -1110,29 +1124,32 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
     long long total;
     if ((s = h->set_cams(cams, views, &total)) || (s = h->ensure_rnd(edge_samples))) return s;
   }
-  if (graphable && h->gexec && sig == h->graph_sig && h->graph_epoch == g_realloc_epoch.load()) {
-    if ((s = h->hip(hipGraphLaunch(h->gexec, h->stream), "graph launch"))) return s;
+  // this step's form: full LBVH build or refit only (opt_step_body makes the same choice)
+  const int form = (h->opt_iter % h->rebuild_every == 0 || !h->topo_valid) ? 1 : 0;
+  if (graphable && h->gexec[form] && sig == h->graph_sig[form] && h->graph_epoch[form] == g_realloc_epoch.load()) {
+    if ((s = h->hip(hipGraphLaunch(h->gexec[form], h->stream), "graph launch"))) return s;
     h->opt_iter++;
     return DRH_OK;
   }
   if (graphable) {
-    h->drop_graph();
+    // arguments or buffers changed: both captured forms are stale
+    if (sig != h->graph_sig[form] || h->graph_epoch[form] != g_realloc_epoch.load()) h->drop_graph();
     if ((s = h->hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return s;
     const drh_status body = opt_step_body(h, cams, views, d_targets, edge_samples, d_loss);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(h->stream, &g);
     if (body == DRH_OK && ec == hipSuccess && g &&
-        hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0) == hipSuccess) {
-      h->graph = g;
-      h->graph_sig = sig;
-      h->graph_epoch = g_realloc_epoch.load();
-      if ((s = h->hip(hipGraphLaunch(h->gexec, h->stream), "graph launch"))) return s;
+        hipGraphInstantiate(&h->gexec[form], g, nullptr, nullptr, 0) == hipSuccess) {
+      h->graph[form] = g;
+      h->graph_sig[form] = sig;
+      h->graph_epoch[form] = g_realloc_epoch.load();
+      if ((s = h->hip(hipGraphLaunch(h->gexec[form], h->stream), "graph launch"))) return s;
       h->opt_iter++;
       return DRH_OK;
     }
     // capture failed: forget it and run the step eagerly
     if (g) (void)hipGraphDestroy(g);
-    h->gexec = nullptr;
+    h->gexec[form] = nullptr;
     (void)hipGetLastError();
     h->use_graph = false;
   }
@@ -1172,6 +1189,12 @@ drh_status drh_set_interior_samples(drh_handle h, int aligned, int store) {
   if (aligned != h->interior_aligned || store != h->interior_store) h->drop_graph();
   h->interior_aligned = aligned;
   h->interior_store = store;
+  return DRH_OK;
+}
+
+drh_status drh_set_bvh_rebuild(drh_handle h, int every) {
+  if (!h || every < 1) return DRH_E_INVALID;
+  h->rebuild_every = every;
   return DRH_OK;
 }
 

@@ -39,7 +39,7 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
-           "drh_server_job_bytes", "drh_set_edge_order"]
+           "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild"]
 
 
 class DrhError(RuntimeError):
@@ -214,6 +214,10 @@ class DiffRender:
     def set_edge_order(self, order):
         """EDGE_ORDER_AUTO / _SAMPLE (rnd0 slot order) / _BINNED (kept samples by 8x8 pixel bin)."""
         self._check(load().drh_set_edge_order(self.h, int(order)))
+
+    def set_bvh_rebuild(self, every):
+        """Full LBVH build every `every` fused steps (1: each step), box refits between."""
+        self._check(load().drh_set_bvh_rebuild(self.h, int(every)))
 
     def opt_set_texture_lr(self, lr):
         self._check(load().drh_opt_set_texture_lr(self.h, ctypes.c_float(lr)))

@@ -34,10 +34,10 @@ def algorithmic_bytes(kernel, F, V, E, W, H, spp_sq, N, P, tiles):
         return scene + 12 * px + 12 * px + 12 * px + 8 * tiles   # image write, target read, adjoint write
     if kernel == "k_interior":
         return scene + 12 * px + 4 * P                            # adjoint read, gradient
-    if kernel == "k_edge_trace":   # rng, endpoints, adjoint, side colours written; edges + cdf
-        return scene + N * (8 + 32 + 12 + 32) + 12 * E
-    if kernel == "k_edge_grad":    # side colours, rng, endpoints, adjoint, positions; pmf; gradient
-        return N * (32 + 8 + 32 + 12 + 24) + 4 * E + 4 * P
+    if kernel == "k_edge_trace":   # rng, endpoints, adjoint, colour differences written; edges + cdf
+        return scene + N * (8 + 32 + 12 + 16) + 12 * E
+    if kernel == "k_edge_grad":    # colour differences, rng, endpoints, adjoint, positions; pmf; gradient
+        return N * (16 + 8 + 32 + 12 + 24) + 4 * E + 4 * P
     return None
 
 

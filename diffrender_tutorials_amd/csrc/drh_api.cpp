@@ -262,7 +262,7 @@ struct drh_renderer {
   int interior_aligned = 0, interior_store = 0;
   int edge_order = DRH_EDGE_ORDER_AUTO;   // sample order of the edge pass
   DBuf bin_count, bin_rec, bin_order;     // pixel-bin order of the kept edge samples
-  DBuf ecol;                              // edge samples' side colours (2 float4 each)
+  DBuf ecol;                              // edge samples' side colour differences (float4 each)
   DBuf hitbuf;
   bool hits_stored = false;
   std::string err;
@@ -754,7 +754,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   const bool binned = E > 0 && N > 0 &&
                       (edge_order == DRH_EDGE_ORDER_BINNED ||
                        (edge_order == DRH_EDGE_ORDER_AUTO && N < (long long)EDGE_BIN_AUTO_PER_EDGE * E));
-  if (E > 0 && N > 0 && (s = hip(ecol.reserve(32 * (size_t)N), "alloc"))) return s;
+  if (E > 0 && N > 0 && (s = hip(ecol.reserve(16 * (size_t)N), "alloc"))) return s;
   if (binned) {
     int max_bins = 1;
     for (int v = 0; v < views; v++) {
@@ -784,8 +784,11 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         ia.lds_base = 0; ia.lds_n = G;
         blocks = std::min(tiles, 1024);
       } else {
+        // regions of several tiles only when the samples come from the stored hits: a wave
+        // that traces walks its region's tiles one after another, and one tile per wave
+        // keeps as many packets in flight as the render
         ia.hash = INTERIOR_HASH;
-        ia.region = (sq == 1 || sq == 2 || sq == 4 || sq == 8) ? INTERIOR_REGION : 1;
+        ia.region = (ia.hits && (sq == 1 || sq == 2 || sq == 4 || sq == 8)) ? INTERIOR_REGION : 1;
         blocks = num_regions(sq, c.w, c.h, ia.region);
       }
       if (ia.region == 0) ia.region = 1;

@@ -187,7 +187,7 @@ struct EdgeArgs {
   const float2* rnd;      // N (rnd0, rnd1) pairs ordered by rnd0
   const int2* order;      // non-NULL: the kept samples as (slot, edge) in pixel-bin order ...
   const int* nkept;       // ... and their count (device); NULL: every slot in rnd0 order
-  float4* ecol;           // 2 per item: side colours (w: edge id, -1 rejected)
+  float4* ecol;           // per item: inside - outside colour (w: edge id, -1 rejected)
   const float* adj;
   float* grad;
   const int* layout;
@@ -270,7 +270,7 @@ void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_ra
 // exact CDF; `work` holds cdf_work_bytes(E) of device memory (parallel form only)
 size_t cdf_work_bytes(int E);
 void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work, hipStream_t s, bool parallel = true);
-void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s);   // side colours into ecol
+void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s);   // side colour differences into ecol
 void launch_edge_grad(const EdgeArgs& a, hipStream_t s);              // gradients from ecol
 void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s);   // bin_count must be zero
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);

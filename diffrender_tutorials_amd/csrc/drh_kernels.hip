@@ -498,7 +498,16 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
 __global__ __launch_bounds__(1024) void k_loss_final(const double* part, int n, float* out, double denom) {
   __shared__ double s[1024];
   double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) v += part[i];
+  int i = threadIdx.x;
+  // eight loads in flight per thread, added in the same order as one at a time
+  for (; i + 7 * 1024 < n; i += 8 * 1024) {
+    double x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = part[i + k * 1024];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v += x[k];
+  }
+  for (; i < n; i += 1024) v += part[i];
   s[threadIdx.x] = v;
   __syncthreads();
   for (int o = 512; o > 0; o >>= 1) {
@@ -1127,7 +1136,7 @@ __global__ void k_edge_scatter(EdgeBinArgs a) {
 // point offset by -/+1e-3 px along the edge normal, src/drender.h:347-360).  A wavefront takes
 // 32 samples with both sides in adjacent lanes, one ray (and shadow ray) per lane, so its
 // packet covers 32 nearby points and the kernel stays small enough for 8 waves per SIMD.
-// ecol[2i + side] = (colour, edge id bits; -1 for a rejected sample).
+// ecol[i] = (colour inside - colour outside, edge id bits; -1 for a rejected sample).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
   __shared__ int wstack[4][64];
@@ -1155,7 +1164,7 @@ __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
       edge_point(a.edges, a.proj, eid, r.y, p);
       live = edge_keep(c, a.adj, p);
     }
-    float4 out = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(live ? eid : -1));
+    F3 col = {0.0f, 0.0f, 0.0f};
     if (live) {
       const float dvx = p.P1.x - p.P0.x, dvy = p.P1.y - p.P0.y;
       const float len = sqrtf(dvx * dvx + dvy * dvy);
@@ -1163,10 +1172,11 @@ __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
       const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
       const float qx = sd == 0 ? p.px - 1e-3f * nx : p.px + 1e-3f * nx;
       const float qy = sd == 0 ? p.py - 1e-3f * ny : p.py + 1e-3f * ny;
-      const F3 col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
-      out.x = col.x; out.y = col.y; out.z = col.z;
+      col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
     }
-    a.ecol[2 * item + sd] = out;
+    // both lanes of a sample are here together (same item, same `live`)
+    const float ox = __shfl_xor(col.x, 1, 64), oy = __shfl_xor(col.y, 1, 64), oz = __shfl_xor(col.z, 1, 64);
+    if (sd == 0) a.ecol[item] = make_float4(col.x - ox, col.y - oy, col.z - oz, __int_as_float(live ? eid : -1));
   }
 }
 
@@ -1199,9 +1209,9 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
     float t = 0.0f;
     float4 P0 = {0, 0, 0, 0}, P1 = {0, 0, 0, 0};
     int2 ev = {0, 0};
-    F3 cin = {0, 0, 0}, cout = {0, 0, 0};
+    F3 dcol = {0, 0, 0};
     if (item < items) {
-      const float4 c0 = a.ecol[2 * item], c1 = a.ecol[2 * item + 1];
+      const float4 c0 = a.ecol[item];
       eid = __float_as_int(c0.w);
       if (eid >= 0) {
         // the sample point exactly as the trace pass computed it
@@ -1210,7 +1220,7 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
         p.xi = (int)p.px; p.yi = (int)p.py;
         live = true;
         ev = p.ev; P0 = p.P0; P1 = p.P1; t = p.t; xi = p.xi; yi = p.yi;
-        cin = F3{c0.x, c0.y, c0.z}; cout = F3{c1.x, c1.y, c1.z};
+        dcol = F3{c0.x, c0.y, c0.z};
       }
     }
     if (live) {
@@ -1222,7 +1232,7 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
     const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
     const float pdf = pmf / len;
     const float weight = 1.0f / (pdf * float(a.N));
-    const float adjv = dot(sub(cin, cout), adj);
+    const float adjv = dot(dcol, adj);   // dot(cin - cout, adj)
     if (adjv * adjv > 0.0f) {
     const float d0x = ((1 - t) * nx) * adjv * weight, d0y = ((1 - t) * ny) * adjv * weight;
     const float d1x = (t * nx) * adjv * weight, d1y = (t * ny) * adjv * weight;

@@ -76,7 +76,7 @@ size_t estimate_bytes(const Job& j) {
   b += 12 * V * 7 + 8 * V + 16 * V;               // vertex arrays, projected vertices
   b += 12 * F + 4 * F;                            // indices, triangle mesh ids
   b += 16 * F * 6 + 4 * F * 8 + 16 * F * 4 + 64 * F + 16 * F * j.cams.size() + 16 * F;  // LBVH build + suspects
-  b += 24 * E + 8 * N + 32 * N;                   // edges, cdf; rng table, side colours
+  b += 24 * E + 8 * N + 16 * N;                   // edges, cdf; rng table, side colour differences
   if (N < E) b += 24 * N + px / 16 + 64;          // pixel-bin order of the edge samples
   b += 12 * px * 4;                               // image, adjoint, target (+ loss partials)
   b += 4 * (size_t)(6 * V + 64) * 4;              // gradient + Adam moments

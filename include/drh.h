@@ -165,10 +165,11 @@ enum { DRH_EDGE_ORDER_AUTO = 0, DRH_EDGE_ORDER_SAMPLE = 1, DRH_EDGE_ORDER_BINNED
 drh_status drh_set_edge_order(drh_handle h, int order);
 /* LBVH upkeep in drh_opt_step.  The reference re-commits the scene every iteration (a new
  * accelerator over the moved vertices).  every = 1 does that: Morton sort, Karras topology
- * and refit on each step.  every = k > 1 (default 8) runs the full build on steps 0, k, 2k, ...
- * and on the others refits the last topology's boxes over the moved geometry.  Traversal
- * is exact over any valid tree, so hits, images, gradients and the optimisation are the same
- * for every k; only the traversal cost changes.  No counterpart in the reference. */
+ * and refit on each step.  every = k > 1 (default 8): step i (counted from drh_opt_init;
+ * step 0 uses the commit's tree) runs the full build when i is a multiple of k and otherwise
+ * refits the last topology's boxes over the moved geometry.  Traversal is exact over any
+ * valid tree, so hits and images are bit-identical for every k and gradients differ only by
+ * float summation order, as between any two runs.  No counterpart in the reference. */
 drh_status drh_set_bvh_rebuild(drh_handle h, int every);
 /* How many handles share the device (1 by default).  A handle alone runs view 0's edge
  * preparation on a second stream beside the render; when several share the device (the

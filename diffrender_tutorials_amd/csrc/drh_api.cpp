@@ -223,17 +223,27 @@ void hammersley(int n, std::vector<float>& out) {
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
-// Bumped whenever any device buffer moves: a captured graph holds raw pointers, so it is only
-// replayed while the epoch it was captured in still holds.
+// A buffer that moves bumps its owner's reallocation epoch: a handle's captured graph holds
+// raw pointers into that handle's buffers, so it is replayed only while the epoch it was
+// captured in still holds.  Buffers built as members of a handle count into the handle's
+// epoch (its EpochOwner base is constructed before the members and names itself in
+// t_epoch_owner); other buffers count into a process-wide one.
 std::atomic<unsigned long long> g_realloc_epoch{0};
+thread_local std::atomic<unsigned long long>* t_epoch_owner = nullptr;
+
+struct EpochOwner {
+  std::atomic<unsigned long long> realloc_epoch{0};
+  EpochOwner() { t_epoch_owner = &realloc_epoch; }
+};
 
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  std::atomic<unsigned long long>* epoch = t_epoch_owner ? t_epoch_owner : &g_realloc_epoch;
   ~DBuf() { if (p) (void)hipFree(p); }
   hipError_t reserve(size_t n) {
     if (n <= bytes && p) return hipSuccess;
-    g_realloc_epoch++;
+    (*epoch)++;
     if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
     if (n == 0) return hipSuccess;
     hipError_t e = hipMalloc(&p, n);
@@ -247,7 +257,7 @@ struct ProfEvent { std::string name; hipEvent_t a, b; };
 
 }  // namespace
 
-struct drh_renderer {
+struct drh_renderer : EpochOwner {
   drh_settings st{};
   int sq = 1, nsamp = 1;
   hipStream_t stream = nullptr;
@@ -310,13 +320,14 @@ struct drh_renderer {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<char> graph_sig[2];
   unsigned long long graph_epoch[2] = {0, 0};
-  void drop_graph() {
-    for (int k = 0; k < 2; k++) {
-      if (gexec[k]) { (void)hipGraphExecDestroy(gexec[k]); gexec[k] = nullptr; }
-      if (graph[k]) { (void)hipGraphDestroy(graph[k]); graph[k] = nullptr; }
-      graph_sig[k].clear();
-    }
+  // a graph may still run from earlier launches on the stream: wait before destroying it
+  void drop_form(int k) {
+    if (gexec[k] || graph[k]) (void)hipStreamSynchronize(stream);
+    if (gexec[k]) { (void)hipGraphExecDestroy(gexec[k]); gexec[k] = nullptr; }
+    if (graph[k]) { (void)hipGraphDestroy(graph[k]); graph[k] = nullptr; }
+    graph_sig[k].clear();
   }
+  void drop_graph() { drop_form(0); drop_form(1); }
   // profiling
   bool prof = false;
   std::vector<ProfEvent> pending;
@@ -846,6 +857,7 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
       (settings->tracer != DRH_TRACER_BVH && settings->tracer != DRH_TRACER_BF))
     return DRH_E_INVALID;
   std::unique_ptr<drh_renderer> r(new drh_renderer());
+  t_epoch_owner = nullptr;   // later buffers of this thread belong to no handle
   r->st = *settings;
   if (hipSetDevice(settings->device) != hipSuccess) return DRH_E_HIP;
   if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return DRH_E_HIP;
@@ -1129,14 +1141,16 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
   }
   // this step's form: full LBVH build or refit only (opt_step_body makes the same choice)
   const int form = (h->opt_iter % h->rebuild_every == 0 || !h->topo_valid) ? 1 : 0;
-  if (graphable && h->gexec[form] && sig == h->graph_sig[form] && h->graph_epoch[form] == g_realloc_epoch.load()) {
+  const unsigned long long epoch = h->realloc_epoch.load();
+  auto valid = [&](int k) { return h->gexec[k] && sig == h->graph_sig[k] && h->graph_epoch[k] == epoch; };
+  if (graphable && valid(form)) {
     if ((s = h->hip(hipGraphLaunch(h->gexec[form], h->stream), "graph launch"))) return s;
     h->opt_iter++;
     return DRH_OK;
   }
   if (graphable) {
-    // arguments or buffers changed: both captured forms are stale
-    if (sig != h->graph_sig[form] || h->graph_epoch[form] != g_realloc_epoch.load()) h->drop_graph();
+    // this form is missing or stale; the other one is kept while it still holds
+    if (valid(1 - form)) h->drop_form(form); else h->drop_graph();
     if ((s = h->hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return s;
     const drh_status body = opt_step_body(h, cams, views, d_targets, edge_samples, d_loss);
     hipGraph_t g = nullptr;
@@ -1145,7 +1159,7 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
         hipGraphInstantiate(&h->gexec[form], g, nullptr, nullptr, 0) == hipSuccess) {
       h->graph[form] = g;
       h->graph_sig[form] = sig;
-      h->graph_epoch[form] = g_realloc_epoch.load();
+      h->graph_epoch[form] = h->realloc_epoch.load();
       if ((s = h->hip(hipGraphLaunch(h->gexec[form], h->stream), "graph launch"))) return s;
       h->opt_iter++;
       return DRH_OK;

@@ -163,8 +163,9 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
 // index is wave-uniform, so a node's two child boxes and links arrive by one scalar load
 // for all 64 rays; each lane tests its own ray (per-lane closest-hit culling through `lim`),
 // ballots pick the children any lane needs (nearer child first by majority), and the
-// traversal stack is wave-uniform too: 64 ints of LDS per wave (`wst`), written by one lane,
-// read as a broadcast; 64 entries cover the Karras tree's depth (<= 30 + ceil(log2 F)
+// traversal stack is wave-uniform too: 64 ints of LDS per wave (`wst`), written by every
+// active lane with the same value (no exec-mask juggling for a single writer), read as a
+// broadcast; 64 entries cover the Karras tree's depth (<= 30 + ceil(log2 F)
 // levels) for F < 2^34.  A lane whose ray is NaN, or an any-hit lane that has its hit, gets
 // lim = -1 and no box passes for it any more.
 // A lane tests every leaf its own slab test opens and possibly more (leaves opened for
@@ -193,7 +194,6 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
     }
   };
   if (s.F == 1) { leaf(0, !retired); return h; }
-  const bool leader = (int)(threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(1)) - 1;
   int sp = 0;
   int node = 0;
   // a tree over F leaves has F-1 internal nodes, each visited at most once per wave
@@ -217,7 +217,7 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
       const uint64_t lf = __ballot(hl && hr && tl <= tr);
       const bool lfirst = both ? 2 * __popcll(lf) >= __popcll(both) : __popcll(bl) >= __popcll(br);
       const int nearc = lfirst ? cl : cr, farc = lfirst ? cr : cl;
-      if (sp < 64) { if (leader) wst[sp] = farc; sp++; }
+      if (sp < 64) { wst[sp] = farc; sp++; }   // every active lane writes the same value
       node = nearc;
     } else if (goL) {
       node = cl;

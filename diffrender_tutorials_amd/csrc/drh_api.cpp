@@ -403,8 +403,10 @@ struct drh_renderer : EpochOwner {
   }
   // Suspect triangles of every view's camera origin (CastSingleRay's origin,
   // WorldViewInv * (0,0,0,1), raytrace.h:67-76).
+  int views_built = -1;   // views whose suspect lists match the current geometry and cameras
   drh_status build_view_suspects(int views) {
     if (st.tracer == DRH_TRACER_BF || F == 0) return DRH_OK;
+    if (views_built == views) return DRH_OK;
     drh_status s;
     if ((s = hip(sus_view.reserve(16 * (size_t)F * views), "alloc")) ||
         (s = hip(sus_view_cnt.reserve(4 * (size_t)views), "alloc")))
@@ -419,6 +421,7 @@ struct drh_renderer : EpochOwner {
                     sus_view.as<float4>() + (size_t)v * F, sus_view_cnt.as<int>() + v};
       timed("k_suspects", [&] { launch_suspects(a, stream); });
     }
+    views_built = views;
     return hip(hipGetLastError(), "suspects");
   }
 
@@ -608,10 +611,13 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
 }
 
 drh_status drh_renderer::build_bvh(bool full) {
+  views_built = -1;   // the views' suspect lists belong to the old geometry
   PrepareArgs pa{V, nm, has_nrm ? 1 : 0, mesh_vo.as<int>(), dmats.as<float>(), pos_orig.as<float>(),
-                 nrm_in.as<float>(), pos.as<float>(), nrm.as<float>(), nrm2d.as<float>()};
-  if (V > 0) timed("k_prepare", [&] { launch_prepare(pa, stream); });
-  if (F == 0) return hip(hipGetLastError(), "commit kernels");
+                 nrm_in.as<float>(), pos.as<float>(), nrm.as<float>(), nrm2d.as<float>(), nullptr, nullptr};
+  if (F == 0) {
+    if (V > 0) timed("k_prepare", [&] { launch_prepare(pa, stream); });
+    return hip(hipGetLastError(), "commit kernels");
+  }
   auto R = [&](DBuf& b, size_t bytes) -> drh_status { return hip(b.reserve(std::max<size_t>(bytes, 16)), "alloc"); };
   const size_t f16 = 16 * (size_t)F, f4 = 4 * (size_t)F;
   const int nb = (F + RADIX_TILE - 1) / RADIX_TILE;
@@ -624,22 +630,21 @@ drh_status drh_renderer::build_bvh(bool full) {
       (s = R(leaf_hi, f16)) || (s = R(node_lo, f16)) || (s = R(node_hi, f16)) || (s = R(nodes, 4 * f16)) ||
       (s = R(tri_a, f16)) || (s = R(tri_e1, f16)) || (s = R(tri_e2, f16)))
     return s;
-  timed("k_fill_u32", [&] {
-    launch_fill_u32(bounds.as<uint32_t>(), 3, 0xFFFFFFFFu, stream);
-    launch_fill_u32(bounds.as<uint32_t>() + 3, 3, 0u, stream);
-  });
+  light_sus = st.mode == DRH_LAMBERT && st.tracer != DRH_TRACER_BF;
+  if (light_sus && ((s = R(sus_light, f16)) || (s = R(sus_light_cnt, 16)))) return s;
+  // k_prepare also empties the centroid bounds and the light's suspect count, and
+  // k_tri_setup clears the refit counters
+  pa.bounds = bounds.as<uint32_t>();
+  pa.zero = light_sus ? sus_light_cnt.as<uint32_t>() : nullptr;
+  timed("k_prepare", [&] { launch_prepare(pa, stream); });
   TriSetupArgs ta{F, tri_mesh.as<int>(), dmats.as<float>(), idx.as<uint32_t>(), pos.as<float>(), bf_a.as<float4>(),
                   bf_e1.as<float4>(), bf_e2.as<float4>(), box_lo.as<float4>(), box_hi.as<float4>(), cen.as<float4>(),
-                  bounds.as<uint32_t>()};
+                  bounds.as<uint32_t>(), F > 1 ? counter.as<uint32_t>() : nullptr};
   timed("k_tri_setup", [&] { launch_tri_setup(ta, stream); });
-  light_sus = false;
-  if (st.mode == DRH_LAMBERT && st.tracer != DRH_TRACER_BF) {
-    if ((s = R(sus_light, f16)) || (s = R(sus_light_cnt, 16))) return s;
-    timed("k_fill_u32", [&] { launch_fill_u32(sus_light_cnt.as<uint32_t>(), 1, 0u, stream); });
+  if (light_sus) {
     SuspectArgs la{F, 1, light.L.x, light.L.y, light.L.z, bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
                    sus_light.as<float4>(), sus_light_cnt.as<int>()};
     timed("k_suspects", [&] { launch_suspects(la, stream); });
-    light_sus = true;
   }
   if (!topo_valid) full = true;
   if (full) {
@@ -661,7 +666,6 @@ drh_status drh_renderer::build_bvh(bool full) {
   }  // full
   topo_valid = true;
   if (F > 1) {
-    timed("k_fill_u32", [&] { launch_fill_u32(counter.as<uint32_t>(), F + 1, 0u, stream); });
     RefitArgs ra{F, vals0.as<uint32_t>(), box_lo.as<float4>(), box_hi.as<float4>(), child.as<int>(),
                  node_parent.as<int>(), leaf_parent.as<int>(), nrange.as<int2>(), counter.as<uint32_t>(),
                  frontier.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
@@ -702,6 +706,7 @@ drh_status drh_renderer::set_cams(const drh_camera* c, int views, long long* tot
     if ((s = hip(hipMemcpyAsync(cams.p, dc.data(), sizeof(DevCam) * views, hipMemcpyHostToDevice, stream), "camera upload"))) return s;
     if ((s = hip(hipStreamSynchronize(stream), "camera upload"))) return s;
     cam_cache = dc;
+    views_built = -1;
   }
   return DRH_OK;
 }

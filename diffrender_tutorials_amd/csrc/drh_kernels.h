@@ -47,6 +47,8 @@ struct PrepareArgs {
   float* pos;
   float* nrm;
   float* nrm2d;
+  uint32_t* bounds;       // optional: set to the empty key range (6 values) by thread 0
+  uint32_t* zero;         // optional: one counter cleared by thread 0
 };
 
 struct TriSetupArgs {
@@ -62,6 +64,7 @@ struct TriSetupArgs {
   float4* box_hi;
   float4* cen;
   uint32_t* bounds;       // 6 ordered float keys: min xyz, max xyz
+  uint32_t* counter;      // optional: F + 1 refit arrival counters cleared here
 };
 
 struct MortonArgs {

@@ -54,6 +54,14 @@ __global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
 // normals are I*(T*(n,0), 1) (transform() then an identity prepare, src/drender.h:260-268).
 __global__ void k_prepare(PrepareArgs a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    // state for the kernels that follow (one launch fewer each)
+    if (a.bounds) {
+      a.bounds[0] = a.bounds[1] = a.bounds[2] = 0xFFFFFFFFu;
+      a.bounds[3] = a.bounds[4] = a.bounds[5] = 0u;
+    }
+    if (a.zero) *a.zero = 0u;
+  }
   if (k >= a.V) return;
   int m = 0;
   while (m + 1 < a.n_meshes && a.mesh_vo[m + 1] <= k) m++;
@@ -80,6 +88,7 @@ __global__ void k_tri_setup(TriSetupArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ uint32_t smin[3], smax[3];
   if (threadIdx.x < 3) { smin[threadIdx.x] = 0xFFFFFFFFu; smax[threadIdx.x] = 0u; }
+  if (a.counter && g <= a.F) a.counter[g] = 0u;
   __syncthreads();
   if (g < a.F) {
     const int m = a.tri_mesh[g];
@@ -1407,7 +1416,8 @@ void launch_prepare(const PrepareArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_prepare, dim3((a.V + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_tri_setup(const TriSetupArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_tri_setup, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
+  // F + 1 threads at least: the last counter is cleared too
+  hipLaunchKernelGGL(k_tri_setup, dim3((a.F + 1 + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_morton(const MortonArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_morton, dim3((a.F + 255) / 256), dim3(256), 0, s, a);

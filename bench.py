@@ -23,6 +23,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The served legs run up to 16 jobs, each on its own stream: give the HIP runtime as many
+# hardware queues (read once, at its initialisation; the default is 4) so their steps run
+# side by side.  The single-stream legs are unaffected.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 HBM_PEAK_GBS = 8000.0
 
 

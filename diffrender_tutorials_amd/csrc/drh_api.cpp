@@ -223,41 +223,83 @@ void hammersley(int n, std::vector<float>& out) {
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
-// A buffer that moves bumps its owner's reallocation epoch: a handle's captured graph holds
-// raw pointers into that handle's buffers, so it is replayed only while the epoch it was
-// captured in still holds.  Buffers built as members of a handle count into the handle's
-// epoch (its EpochOwner base is constructed before the members and names itself in
-// t_epoch_owner); other buffers count into a process-wide one.
+// Buffers built as members of a handle belong to it (its BufOwner base is constructed before
+// the members and names itself in t_buf_owner):
+//  - a buffer that moves bumps its owner's reallocation epoch: a captured graph holds raw
+//    pointers into the handle's buffers, so it is replayed only while the epoch it was
+//    captured in still holds;
+//  - once the handle's stream exists, its buffers come from the device's stream-ordered pool
+//    (hipMallocAsync / hipFreeAsync on that stream): freeing them does not wait for the whole
+//    device, which matters when a server tears jobs down while others run.
+// Other buffers use hipMalloc / hipFree and count into a process-wide epoch.
 std::atomic<unsigned long long> g_realloc_epoch{0};
-thread_local std::atomic<unsigned long long>* t_epoch_owner = nullptr;
 
-struct EpochOwner {
+struct BufOwner;
+thread_local BufOwner* t_buf_owner = nullptr;
+
+struct BufOwner {
   std::atomic<unsigned long long> realloc_epoch{0};
-  EpochOwner() { t_epoch_owner = &realloc_epoch; }
+  hipStream_t pool_stream = nullptr;
+  BufOwner() { t_buf_owner = this; }
 };
 
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  std::atomic<unsigned long long>* epoch = t_epoch_owner ? t_epoch_owner : &g_realloc_epoch;
-  ~DBuf() { if (p) (void)hipFree(p); }
+  BufOwner* owner = t_buf_owner;
+  bool pooled = false;
+  ~DBuf() { release(); }
+  void release() {
+    if (!p) return;
+    if (pooled) (void)hipFreeAsync(p, owner->pool_stream);   // after the stream's work
+    else (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
   hipError_t reserve(size_t n) {
     if (n <= bytes && p) return hipSuccess;
-    (*epoch)++;
-    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (owner) owner->realloc_epoch++; else g_realloc_epoch++;
+    release();
     if (n == 0) return hipSuccess;
-    hipError_t e = hipMalloc(&p, n);
-    if (e == hipSuccess) bytes = n;
+    hipStream_t st = owner ? owner->pool_stream : nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (st) (void)hipStreamIsCapturing(st, &cs);
+    hipError_t e;
+    if (st && cs == hipStreamCaptureStatusNone) {
+      // ordered on the handle's stream: its copies and kernels follow on that stream, and the
+      // second stream starts from an event recorded there later
+      e = hipMallocAsync(&p, n, st);
+      pooled = true;
+    } else {
+      e = hipMalloc(&p, n);
+      pooled = false;
+    }
+    if (e == hipSuccess) bytes = n; else p = nullptr;
     return e;
   }
   template <class T> T* as() const { return (T*)p; }
 };
 
+// Pooled memory stays with the process once freed (reused by the next allocation) instead
+// of going back to the driver at every synchronisation.
+void keep_pool_memory(int device) {
+  static std::mutex m;
+  static std::vector<int> done;
+  std::lock_guard<std::mutex> lk(m);
+  if (std::find(done.begin(), done.end(), device) != done.end()) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    uint64_t thr = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  }
+  done.push_back(device);
+}
+
 struct ProfEvent { std::string name; hipEvent_t a, b; };
 
 }  // namespace
 
-struct drh_renderer : EpochOwner {
+struct drh_renderer : BufOwner {
   drh_settings st{};
   int sq = 1, nsamp = 1;
   hipStream_t stream = nullptr;
@@ -751,7 +793,9 @@ drh_status drh_renderer::ensure_rnd(long long N) {
   if ((s = hip(rnd.reserve(std::max<size_t>(8 * (size_t)N, 16)), "alloc"))) return s;
   if (N > 0) {
     const auto t = sorted_sample_table(st.rng_mode, N);
-    if ((s = hip(hipMemcpy(rnd.p, t->data(), 8 * (size_t)N, hipMemcpyHostToDevice), "rng upload"))) return s;
+    if ((s = hip(hipMemcpyAsync(rnd.p, t->data(), 8 * (size_t)N, hipMemcpyHostToDevice, stream), "rng upload")) ||
+        (s = hip(hipStreamSynchronize(stream), "rng upload")))
+      return s;
   }
   rnd_n = N;
   return DRH_OK;
@@ -862,10 +906,12 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
       (settings->tracer != DRH_TRACER_BVH && settings->tracer != DRH_TRACER_BF))
     return DRH_E_INVALID;
   std::unique_ptr<drh_renderer> r(new drh_renderer());
-  t_epoch_owner = nullptr;   // later buffers of this thread belong to no handle
+  t_buf_owner = nullptr;   // later buffers of this thread belong to no handle
   r->st = *settings;
   if (hipSetDevice(settings->device) != hipSuccess) return DRH_E_HIP;
   if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return DRH_E_HIP;
+  keep_pool_memory(settings->device);
+  r->pool_stream = r->stream;
   if (hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess)
@@ -887,7 +933,8 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   std::vector<float> hm;
   hammersley(settings->spp, hm);
   if (r->hamm.reserve(4 * hm.size()) != hipSuccess) return DRH_E_OOM;
-  if (hipMemcpy(r->hamm.p, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice) != hipSuccess || r->loss.reserve(16) != hipSuccess)
+  if (hipMemcpyAsync(r->hamm.p, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice, r->stream) != hipSuccess ||
+      hipStreamSynchronize(r->stream) != hipSuccess || r->loss.reserve(16) != hipSuccess)
     return DRH_E_HIP;
   *out = r.release();
   return DRH_OK;
@@ -903,9 +950,10 @@ void drh_destroy(drh_handle h) {
   (void)hipStreamSynchronize(h->side);
   (void)hipEventDestroy(h->ev_fork);
   (void)hipEventDestroy(h->ev_join);
-  (void)hipStreamDestroy(h->side);
-  (void)hipStreamDestroy(h->stream);
-  delete h;
+  hipStream_t side = h->side, main = h->stream;
+  delete h;   // pooled buffers are freed on the main stream, which still exists
+  (void)hipStreamDestroy(side);
+  (void)hipStreamDestroy(main);
 }
 
 const char* drh_last_error(drh_handle h) { return h ? h->err.c_str() : "null handle"; }
@@ -1342,7 +1390,9 @@ drh_status drh_malloc(drh_handle h, size_t bytes, void** d_ptr) {
   if (!h || !d_ptr) return DRH_E_INVALID;
   (void)hipSetDevice(h->st.device);
   *d_ptr = nullptr;
-  return h->hip(hipMalloc(d_ptr, std::max<size_t>(bytes, 4)), "drh_malloc");
+  drh_status s = h->hip(hipMallocAsync(d_ptr, std::max<size_t>(bytes, 4), h->stream), "drh_malloc");
+  if (s) return s;
+  return h->hip(hipStreamSynchronize(h->stream), "drh_malloc");
 }
 
 drh_status drh_free(drh_handle h, void* d_ptr) {
@@ -1350,7 +1400,7 @@ drh_status drh_free(drh_handle h, void* d_ptr) {
   (void)hipSetDevice(h->st.device);
   drh_status s = h->hip(hipStreamSynchronize(h->stream), "drh_free");
   if (s) return s;
-  return d_ptr ? h->hip(hipFree(d_ptr), "drh_free") : DRH_OK;
+  return d_ptr ? h->hip(hipFreeAsync(d_ptr, h->stream), "drh_free") : DRH_OK;
 }
 
 drh_status drh_memcpy_htod(drh_handle h, void* d_dst, const void* src, size_t bytes) {

@@ -1,12 +1,14 @@
 // drh_server.cpp -- continuous-batching server for mesh-fitting jobs (one MI355X).
 //
 // A job = a scene, its cameras, target images and an iteration count.  Jobs are queued by
-// drh_server_submit (inputs copied), admitted by a worker thread while a slot and the HBM
-// budget allow, and stepped round-robin: every active job keeps one fused fwd+bwd+Adam step
-// in flight on its own HIP stream (a hipGraph replay after its first step), so the device
-// overlaps the jobs' kernels.  A finished job's results are downloaded and its memory freed,
-// which admits the next queued job.  Per-iteration latency = HIP event time from a step's
-// first to last launch on its stream, contention included.
+// drh_server_submit (inputs copied).  Threads serve them: admitters set queued jobs up while
+// a slot and the HBM budget allow (handle, commit with its LBVH build, uploads); the reaper
+// tears finished ones down (downloads, frees), which admits the next.
+// The stepper steps the admitted jobs round-robin: every job keeps up to four fused
+// fwd+bwd+Adam steps in flight on its own HIP stream (hipGraph replays after its first
+// step), so the device overlaps the jobs' kernels while set-up and teardown of others go on
+// beside them.  Per-iteration latency = HIP event time from a step's first to last launch on
+// its stream, contention included.
 //
 // The reference has no serving layer (its optimiser runs one scene at a time,
 // src/optimizer_simple.cpp:272-309); this is the "serving" pattern of the path.
@@ -65,6 +67,12 @@ struct Job {
   }
 };
 
+// Steps a job keeps queued on its stream (graph replays back to back; more than two keeps
+// the device fed while the stepping thread serves the other jobs).
+constexpr size_t MAX_INFLIGHT = 4;
+// Threads setting jobs up (each job's set-up is serial host work: handle, uploads, LBVH build).
+constexpr int ADMITTERS = 1;
+
 // HBM a job holds while active (the buffers drh_renderer sizes for it), with 25% headroom.
 size_t estimate_bytes(const Job& j) {
   const size_t V = j.V(), F = j.F();
@@ -91,24 +99,42 @@ struct drh_server_s {
   size_t budget = 0, used = 0;
   int max_active = 4;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
-  std::deque<Job*> pending;
-  std::vector<Job*> active;
+  std::condition_variable cv_work;   // stepper: admitted jobs ready, or stop
+  std::condition_variable cv_help;   // admitter: a queued job may be set up now, or stop
+  std::condition_variable cv_reap;   // reaper: jobs to finish, or stop
+  std::condition_variable cv_done;   // drh_server_wait
+  std::deque<Job*> pending;          // submitted, not yet admitted (FIFO)
+  std::deque<Job*> ready;            // set up by the admitter, not yet stepped
+  struct Done { Job* j; int status; std::string err; };
+  std::deque<Done> to_finish;        // stepping over: downloads and teardown for the reaper
+  int admitting = 0;                 // jobs the admitter is setting up
+  int stepping = 0;                  // jobs the stepper holds (mirror for the slot count)
+  std::vector<Job*> active;          // stepper only
   std::map<int, std::unique_ptr<Job>> jobs;
   int next_id = 1;
-  bool stop = false;
-  std::thread worker;
+  bool stop = false, stepper_done = false;
+  std::thread worker, reaper;
+  std::vector<std::thread> admitters;
   // stats
   std::vector<float> lat_ms;
   long long iters_done = 0;
   int jobs_done = 0, jobs_failed = 0, peak_active = 0;
   double active_sum = 0.0;
   long long active_samples = 0;
+  double admit_s = 0.0, step_s = 0.0, finish_s = 0.0;   // thread time by activity
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
-  void run();
+  void run();         // stepping thread
+  void admit_run();   // admission thread
+  void reap_run();    // teardown thread
   bool admit_one(Job* j);
   void finish(Job* j, int status, const std::string& err);
+  // with mu held: may the admitter set up the next queued job now?
+  bool can_admit() const {
+    if (stop || pending.empty()) return false;
+    if (pending.front()->bytes > budget) return true;   // refused at once
+    return stepping + (int)ready.size() + admitting < max_active && used + pending.front()->bytes <= budget;
+  }
 };
 
 void drh_server_s::finish(Job* j, int status, const std::string& err) {
@@ -134,6 +160,7 @@ void drh_server_s::finish(Job* j, int status, const std::string& err) {
   j->t_done = std::chrono::steady_clock::now();
   if (status == 1) jobs_done++; else jobs_failed++;
   cv_done.notify_all();
+  cv_help.notify_all();   // memory freed: the next queued job may fit
 }
 
 bool drh_server_s::admit_one(Job* j) {
@@ -142,6 +169,9 @@ bool drh_server_s::admit_one(Job* j) {
   if (s != DRH_OK) { finish(j, s, "drh_create failed"); return false; }
   j->h = h;
   (void)drh_set_device_sharing(h, max_active);
+  // a short job refits its commit-time tree throughout: one captured step form instead of
+  // two (capture and instantiation cost host time comparable to a short job's steps)
+  (void)drh_set_bvh_rebuild(h, j->iterations <= 64 ? j->iterations + 1 : 8);
   const drh_scene_desc d = j->desc();
   const size_t tbytes = 4 * j->targets.size();
   if ((s = drh_commit(h, &d)) != DRH_OK || (s = drh_opt_init(h, DRH_OPT_ADAM, j->lr_pos, j->lr_col)) != DRH_OK ||
@@ -154,41 +184,87 @@ bool drh_server_s::admit_one(Job* j) {
   return true;
 }
 
-void drh_server_s::run() {
+// Admission (handle, commit with its LBVH build, uploads) runs on its own thread beside the
+// stepping thread, and so does teardown (downloads, frees): neither holds up the other
+// jobs' steps, and a burst of finishing jobs does not delay the next admissions.
+void drh_server_s::admit_run() {
   (void)hipSetDevice(device);
   while (true) {
-    // admission: queued jobs enter while a slot and the HBM budget allow
-    std::vector<Job*> to_admit;
+    Job* adm = nullptr;
     {
       std::unique_lock<std::mutex> lk(mu);
+      cv_help.wait(lk, [&] { return can_admit() || stop; });
+      if (!can_admit()) break;   // stopped
+      Job* j = pending.front();
+      pending.pop_front();
+      if (j->bytes > budget) {           // can never fit
+        j->status = DRH_E_OOM;
+        j->error = "job exceeds the server's HBM budget";
+        jobs_failed++;
+        cv_done.notify_all();
+        continue;
+      }
+      used += j->bytes;
+      admitting++;
+      adm = j;
+    }
+    const auto t = std::chrono::steady_clock::now();
+    const bool ok = admit_one(adm);
+    std::lock_guard<std::mutex> lk(mu);
+    admit_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    admitting--;
+    if (ok) {
+      ready.push_back(adm);
+      cv_work.notify_all();
+    }
+  }
+}
+
+void drh_server_s::reap_run() {
+  (void)hipSetDevice(device);
+  while (true) {
+    Done fin{nullptr, 0, std::string()};
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_reap.wait(lk, [&] { return !to_finish.empty() || (stop && stepper_done); });
+      if (to_finish.empty()) break;   // stopped, and the stepper has handed over everything
+      fin = std::move(to_finish.front());
+      to_finish.pop_front();
+    }
+    const auto t = std::chrono::steady_clock::now();
+    finish(fin.j, fin.status, fin.err);
+    std::lock_guard<std::mutex> lk(mu);
+    finish_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+  }
+}
+
+void drh_server_s::run() {
+  (void)hipSetDevice(device);
+  auto hand_over = [&](size_t k, int status, const std::string& err) {
+    Job* j = active[k];
+    active.erase(active.begin() + k);
+    std::lock_guard<std::mutex> lk(mu);
+    to_finish.push_back(Done{j, status, err});
+    stepping = (int)active.size();
+    cv_reap.notify_all();
+    cv_help.notify_all();   // a slot is free
+  };
+  while (true) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      while (!ready.empty()) { active.push_back(ready.front()); ready.pop_front(); }
+      stepping = (int)active.size();
       if (stop) break;
-      if (active.empty() && pending.empty()) {
+      if (active.empty()) {
         cv_work.wait_for(lk, std::chrono::milliseconds(50));
         continue;
       }
-      while (!pending.empty() && (int)(active.size() + to_admit.size()) < max_active) {
-        Job* j = pending.front();
-        if (j->bytes > budget) {           // can never fit
-          pending.pop_front();
-          j->status = DRH_E_OOM;
-          j->error = "job exceeds the server's HBM budget";
-          jobs_failed++;
-          cv_done.notify_all();
-          continue;
-        }
-        if (used + j->bytes > budget) break;  // wait for memory to free up (FIFO)
-        used += j->bytes;
-        pending.pop_front();
-        to_admit.push_back(j);
-      }
+      peak_active = std::max(peak_active, (int)active.size());
+      active_sum += active.size();
+      active_samples++;
     }
-    for (Job* j : to_admit)
-      if (admit_one(j)) active.push_back(j);
-    peak_active = std::max(peak_active, (int)active.size());
-    active_sum += active.size();
-    active_samples++;
     // stepping: keep up to two steps in flight per job; harvest completed ones
-    bool progressed = !to_admit.empty();
+    bool progressed = false;
     for (size_t k = 0; k < active.size();) {
       Job* j = active[k];
       // harvest
@@ -208,13 +284,13 @@ void drh_server_s::run() {
         progressed = true;
       }
       if (j->completed >= j->iterations) {
-        active.erase(active.begin() + k);
-        finish(j, 1, "");
+        hand_over(k, 1, "");
         progressed = true;
         continue;
       }
       // issue
-      if (j->issued < j->iterations && j->inflight.size() < 2) {
+      if (j->issued < j->iterations && j->inflight.size() < MAX_INFLIGHT) {
+        const auto ts = std::chrono::steady_clock::now();
         hipEvent_t a, b;
         (void)hipEventCreate(&a);
         (void)hipEventCreate(&b);
@@ -226,20 +302,24 @@ void drh_server_s::run() {
         if (s != DRH_OK) {
           (void)hipEventDestroy(a);
           (void)hipEventDestroy(b);
-          active.erase(active.begin() + k);
-          finish(j, s, drh_last_error(j->h));
+          hand_over(k, s, drh_last_error(j->h));
           continue;
         }
         j->inflight.push_back({a, b});
         j->issued++;
+        std::lock_guard<std::mutex> lk(mu);
+        step_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
         progressed = true;
       }
       k++;
     }
     if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
-  for (Job* j : active) finish(j, DRH_E_INVALID, "server stopped");
-  active.clear();
+  // stopped: hand every held job to the reaper
+  while (!active.empty()) hand_over(active.size() - 1, DRH_E_INVALID, "server stopped");
+  std::lock_guard<std::mutex> lk(mu);
+  stepper_done = true;
+  cv_reap.notify_all();
 }
 
 extern "C" {
@@ -254,6 +334,8 @@ drh_status drh_server_create(int device, size_t hbm_budget_bytes, int max_active
   s->budget = hbm_budget_bytes;
   s->max_active = max_active;
   s->worker = std::thread([s] { s->run(); });
+  for (int k = 0; k < ADMITTERS; k++) s->admitters.emplace_back([s] { s->admit_run(); });
+  s->reaper = std::thread([s] { s->reap_run(); });
   *out = s;
   return DRH_OK;
 }
@@ -300,7 +382,7 @@ drh_status drh_server_submit(drh_server s, const drh_job_desc* d, int* job_id) {
   *job_id = j->id;
   s->pending.push_back(j.get());
   s->jobs[j->id] = std::move(j);
-  s->cv_work.notify_all();
+  s->cv_help.notify_all();
   return DRH_OK;
 }
 
@@ -348,6 +430,9 @@ drh_status drh_server_stats(drh_server s, drh_server_stats_t* out) {
   out->mean_active = s->active_samples ? s->active_sum / (double)s->active_samples : 0.0;
   out->hbm_used = s->used;
   out->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+  out->admit_s = s->admit_s;
+  out->step_s = s->step_s;
+  out->finish_s = s->finish_s;
   return DRH_OK;
 }
 
@@ -359,8 +444,11 @@ void drh_server_destroy(drh_server s) {
     for (Job* j : s->pending) { j->status = DRH_E_INVALID; j->error = "server stopped"; }
     s->pending.clear();
     s->cv_work.notify_all();
+    s->cv_help.notify_all();
   }
-  s->worker.join();
+  for (auto& t : s->admitters) t.join();   // they set up no new job once stopped
+  s->worker.join();     // hands its jobs (and those admitted meanwhile) to the reaper
+  s->reaper.join();     // finishes them, then exits
   delete s;
 }
 

@@ -347,7 +347,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_longlong), ("jobs_done", ctypes.c_int), ("jobs_failed", ctypes.c_int),
                 ("jobs_queued", ctypes.c_int), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("peak_active", ctypes.c_int), ("mean_active", ctypes.c_double), ("hbm_used", ctypes.c_size_t),
-                ("elapsed_s", ctypes.c_double)]
+                ("elapsed_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("step_s", ctypes.c_double),
+                ("finish_s", ctypes.c_double)]
 
 
 def settings_struct(mode, spp, rng_mode=RNG_REF_PRNG14, tracer=TRACER_BVH, device=0, light=None):

@@ -182,8 +182,11 @@ drh_status drh_set_device_sharing(drh_handle h, int handles);
 drh_status drh_get_texels(drh_handle h, float* texels);
 
 /* Device memory plumbing (no counterpart in the reference): for callers that hold no HIP
- * allocator of their own, e.g. Python tests and the benchmark.  Allocations are on the
- * handle's device; copies are ordered on its stream and complete before return. */
+ * allocator of their own, e.g. Python tests and the benchmark.  Allocations come from the
+ * device's stream-ordered pool on the handle's stream and are usable at return; drh_free
+ * waits for the handle's stream, then returns the memory to the pool without waiting for the
+ * rest of the device (free it only once no other stream still uses it).  Copies are ordered
+ * on the handle's stream and complete before return. */
 drh_status drh_malloc(drh_handle h, size_t bytes, void** d_ptr);
 drh_status drh_free(drh_handle h, void* d_ptr);
 drh_status drh_memcpy_htod(drh_handle h, void* d_dst, const void* src, size_t bytes);
@@ -230,6 +233,8 @@ typedef struct {
   double mean_active;
   size_t hbm_used;
   double elapsed_s;
+  double admit_s, step_s, finish_s;  /* worker-thread time: admitting jobs (create, commit,
+                                        uploads), issuing steps, finishing jobs (downloads, free) */
 } drh_server_stats_t;
 
 drh_status drh_server_create(int device, size_t hbm_budget_bytes, int max_active, drh_server* out);

@@ -570,23 +570,36 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
   std::vector<int2> hedges;
   std::vector<int> hemesh;
   if (!topo_same) {
-    std::vector<std::pair<uint64_t, int>> keyed;
-    keyed.reserve(3 * (size_t)nF);
-    for (int i = 0; i < n; i++) {
-      if (!df[i]) continue;
-      for (int f = nto[i]; f < nto[i + 1]; f++)
-        for (int k = 0; k < 3; k++) {
-          const uint32_t a = nvo[i] + idx_local[3 * f + k], b = nvo[i] + idx_local[3 * f + (k + 1) % 3];
-          const uint32_t v0 = std::min(a, b), v1 = std::max(a, b);
-          keyed.push_back({((uint64_t)v0 << 32) | v1, i});
+    // (v0, v1) with v0 < v1 in ascending order, duplicates once: a counting sort by v0, then
+    // each vertex's few v1 in order.  Meshes own disjoint vertex ranges, so an edge occurs
+    // in one mesh only and whichever copy is kept carries the same mesh id.
+    std::vector<int> start(nV + 1, 0);
+    auto each_edge = [&](auto&& fn) {
+      for (int i = 0; i < n; i++) {
+        if (!df[i]) continue;
+        for (int f = nto[i]; f < nto[i + 1]; f++)
+          for (int k = 0; k < 3; k++) {
+            const uint32_t a = nvo[i] + idx_local[3 * f + k], b = nvo[i] + idx_local[3 * f + (k + 1) % 3];
+            fn(std::min(a, b), std::max(a, b), i);
+          }
+      }
+    };
+    each_edge([&](uint32_t v0, uint32_t, int) { start[v0 + 1]++; });   // indices checked above
+    for (int v = 0; v < nV; v++) start[v + 1] += start[v];
+    std::vector<uint32_t> other(start[nV]);
+    std::vector<int> omesh(start[nV]);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    each_edge([&](uint32_t v0, uint32_t v1, int i) { other[fill[v0]] = v1; omesh[fill[v0]] = i; fill[v0]++; });
+    for (int v = 0; v < nV; v++) {
+      const int b = start[v], e = start[v + 1];
+      for (int x = b + 1; x < e; x++)   // insertion sort: a vertex has a handful of neighbours
+        for (int y = x; y > b && other[y] < other[y - 1]; y--) { std::swap(other[y], other[y - 1]); std::swap(omesh[y], omesh[y - 1]); }
+      for (int x = b; x < e; x++)
+        if (x == b || other[x] != other[x - 1]) {
+          hedges.push_back(make_int2(v, (int)other[x]));
+          hemesh.push_back(omesh[x]);
         }
     }
-    std::stable_sort(keyed.begin(), keyed.end(), [](const std::pair<uint64_t, int>& x, const std::pair<uint64_t, int>& y) { return x.first < y.first; });
-    for (size_t k = 0; k < keyed.size(); k++)
-      if (k == 0 || keyed[k].first != keyed[k - 1].first) {
-        hedges.push_back(make_int2((int)(keyed[k].first >> 32), (int)(keyed[k].first & 0xFFFFFFFFu)));
-        hemesh.push_back(keyed[k].second);
-      }
   }
   std::vector<uint32_t> gidx(3 * (size_t)nF);
   std::vector<int> tmesh(nF);

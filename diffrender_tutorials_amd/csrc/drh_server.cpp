@@ -110,6 +110,13 @@ struct drh_server_s {
   int admitting = 0;                 // jobs the admitter is setting up
   int stepping = 0;                  // jobs the stepper holds (mirror for the slot count)
   std::vector<Job*> active;          // stepper only
+  std::vector<hipEvent_t> spare_events;   // stepper only: recycled timing events
+  hipEvent_t take_event() {
+    if (spare_events.empty()) { hipEvent_t e = nullptr; (void)hipEventCreate(&e); return e; }
+    hipEvent_t e = spare_events.back();
+    spare_events.pop_back();
+    return e;
+  }
   std::map<int, std::unique_ptr<Job>> jobs;
   int next_id = 1;
   bool stop = false, stepper_done = false;
@@ -271,8 +278,8 @@ void drh_server_s::run() {
       while (!j->inflight.empty() && hipEventQuery(j->inflight.front().second) == hipSuccess) {
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, j->inflight.front().first, j->inflight.front().second);
-        (void)hipEventDestroy(j->inflight.front().first);
-        (void)hipEventDestroy(j->inflight.front().second);
+        spare_events.push_back(j->inflight.front().first);
+        spare_events.push_back(j->inflight.front().second);
         j->inflight.pop_front();
         j->completed++;
         j->latencies_ms.push_back(ms);
@@ -291,17 +298,15 @@ void drh_server_s::run() {
       // issue
       if (j->issued < j->iterations && j->inflight.size() < MAX_INFLIGHT) {
         const auto ts = std::chrono::steady_clock::now();
-        hipEvent_t a, b;
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
+        hipEvent_t a = take_event(), b = take_event();
         hipStream_t st = (hipStream_t)drh_stream(j->h);
         (void)hipEventRecord(a, st);
         const drh_status s = drh_opt_step(j->h, j->cams.data(), (int)j->cams.size(), (const float*)j->d_targets,
                                           j->edge_samples, (float*)j->d_loss);
         (void)hipEventRecord(b, st);
         if (s != DRH_OK) {
-          (void)hipEventDestroy(a);
-          (void)hipEventDestroy(b);
+          spare_events.push_back(a);
+          spare_events.push_back(b);
           hand_over(k, s, drh_last_error(j->h));
           continue;
         }
@@ -317,6 +322,8 @@ void drh_server_s::run() {
   }
   // stopped: hand every held job to the reaper
   while (!active.empty()) hand_over(active.size() - 1, DRH_E_INVALID, "server stopped");
+  for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
+  spare_events.clear();
   std::lock_guard<std::mutex> lk(mu);
   stepper_done = true;
   cv_reap.notify_all();

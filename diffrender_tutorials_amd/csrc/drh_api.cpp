@@ -313,6 +313,7 @@ struct drh_renderer : BufOwner {
   // keeps each sample's closest hit and the interior pass reads it back
   int interior_aligned = 0, interior_store = 0;
   int edge_order = DRH_EDGE_ORDER_AUTO;   // sample order of the edge pass
+  int edge_grad_mode = DRH_EDGE_GRAD_REFERENCE_DIAGONAL;
   DBuf bin_count, bin_rec, bin_order;     // pixel-bin order of the kept edge samples
   DBuf ecol;                              // edge samples' side colour differences (float4 each)
   DBuf hitbuf;
@@ -897,6 +898,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         ea.order = ba.order; ea.nkept = ba.bin_count + ba.nbins;
       }
       ea.single_mesh = single >= 0 ? 1 : 0;
+      ea.full_jacobian = edge_grad_mode == DRH_EDGE_GRAD_FULL_JACOBIAN ? 1 : 0;
       ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
       ea.ecol = ecol.as<float4>();
       timed("k_edge_trace", [&] { launch_edge_trace(st.mode, ea, stream); });
@@ -1272,6 +1274,13 @@ drh_status drh_set_interior_samples(drh_handle h, int aligned, int store) {
   if (aligned != h->interior_aligned || store != h->interior_store) h->drop_graph();
   h->interior_aligned = aligned;
   h->interior_store = store;
+  return DRH_OK;
+}
+
+drh_status drh_set_edge_grad_mode(drh_handle h, int mode) {
+  if (!h || (mode != DRH_EDGE_GRAD_REFERENCE_DIAGONAL && mode != DRH_EDGE_GRAD_FULL_JACOBIAN)) return DRH_E_INVALID;
+  if (mode != h->edge_grad_mode) h->drop_graph();
+  h->edge_grad_mode = mode;
   return DRH_OK;
 }
 

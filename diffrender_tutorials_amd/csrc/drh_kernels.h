@@ -197,6 +197,7 @@ struct EdgeArgs {
   const float* tr_inv0;   // 16 floats
   int lds_base, lds_n;
   int single_mesh, single_tr_off;
+  int full_jacobian;      // 1: edge_grad with the full screen Jacobian (cross terms kept)
 };
 
 // Edge samples binned by the EDGE_BIN x EDGE_BIN pixel tile of their sample point.  Only the

@@ -1245,12 +1245,20 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
     if (adjv * adjv > 0.0f) {
     const float d0x = ((1 - t) * nx) * adjv * weight, d0y = ((1 - t) * ny) * adjv * weight;
     const float d1x = (t * nx) * adjv * weight, d1y = (t * ny) * adjv * weight;
-    // edge_grad (src/drender.cpp:31-87): diagonal screen Jacobian, as the reference
+    // edge_grad (src/drender.cpp:31-87): the reference keeps the diagonal of the screen
+    // Jacobian (dSx/dVx, dSy/dVy, and both for z), exact only for views whose screen axes are
+    // world x and y; the full-Jacobian mode adds the cross terms dSy/dVx and dSx/dVy
     const F3 V0 = ld3(a.tr.s.pos + 3 * ev.x), V1 = ld3(a.tr.s.pos + 3 * ev.y);
     const F3 g0x = vs_grad(c, V0, 0), g0y = vs_grad(c, V0, 1);
     const F3 g1x = vs_grad(c, V1, 0), g1y = vs_grad(c, V1, 1);
-    const float dv0_dx = g0x.x * d0x, dv0_dy = g0y.y * d0y, dv0_dz = (g0x.z * d0x + g0y.z * d0y);
-    const float dv1_dx = g1x.x * d1x, dv1_dy = g1y.y * d1y, dv1_dz = (g1x.z * d1x + g1y.z * d1y);
+    float dv0_dx = g0x.x * d0x, dv0_dy = g0y.y * d0y;
+    float dv1_dx = g1x.x * d1x, dv1_dy = g1y.y * d1y;
+    const float dv0_dz = (g0x.z * d0x + g0y.z * d0y);
+    const float dv1_dz = (g1x.z * d1x + g1y.z * d1y);
+    if (a.full_jacobian) {
+      dv0_dx = dv0_dx + g0y.x * d0y; dv0_dy = g0x.y * d0x + dv0_dy;
+      dv1_dx = dv1_dx + g1y.x * d1y; dv1_dy = g1x.y * d1x + dv1_dy;
+    }
     const int m = a.edge_mesh[eid];
     goff = a.layout[4 * m + 0];
     if (goff >= 0) {

@@ -24,6 +24,7 @@ SILHOUETTE, VERTEX_COLOR, TEXTURE_COLOR, LAMBERT = 1, 2, 3, 4
 RNG_REF_PRNG14, RNG_QMC = 0, 1
 TRACER_BVH, TRACER_BF = 0, 1
 EDGE_ORDER_AUTO, EDGE_ORDER_SAMPLE, EDGE_ORDER_BINNED = 0, 1, 2
+EDGE_GRAD_REFERENCE_DIAGONAL, EDGE_GRAD_FULL_JACOBIAN = 0, 1
 OPT_SIMPLE, OPT_ADAM = 0, 1
 
 STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_E_HIP", -4: "DRH_E_OOM",
@@ -39,7 +40,7 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
-           "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild"]
+           "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild", "drh_set_edge_grad_mode"]
 
 
 class DrhError(RuntimeError):
@@ -214,6 +215,10 @@ class DiffRender:
     def set_edge_order(self, order):
         """EDGE_ORDER_AUTO / _SAMPLE (rnd0 slot order) / _BINNED (kept samples by 8x8 pixel bin)."""
         self._check(load().drh_set_edge_order(self.h, int(order)))
+
+    def set_edge_grad_mode(self, mode):
+        """EDGE_GRAD_REFERENCE_DIAGONAL (the reference's edge_grad) or EDGE_GRAD_FULL_JACOBIAN."""
+        self._check(load().drh_set_edge_grad_mode(self.h, int(mode)))
 
     def set_bvh_rebuild(self, every):
         """Full LBVH build every `every` fused steps (1: each step), box refits between."""

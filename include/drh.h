@@ -171,6 +171,15 @@ drh_status drh_set_edge_order(drh_handle h, int order);
  * valid tree, so hits and images are bit-identical for every k and gradients differ only by
  * float summation order, as between any two runs.  No counterpart in the reference. */
 drh_status drh_set_bvh_rebuild(drh_handle h, int every);
+/* Screen Jacobian in the edge gradient (edge_grad, src/drender.cpp:31-87).
+ *   DRH_EDGE_GRAD_REFERENCE_DIAGONAL (default): the reference's terms, dSx/dVx, dSy/dVy and
+ *     both z terms -- its parity mode, exact when the view's screen axes are world x and y.
+ *   DRH_EDGE_GRAD_FULL_JACOBIAN: adds the cross terms dSy/dVx and dSx/dVy, the derivative of
+ *     the rendered silhouette for any view (checked against finite differences).
+ * For an axis-aligned view both modes give the same gradient.  Replaces no reference entry
+ * point (the reference has only the diagonal form). */
+enum { DRH_EDGE_GRAD_REFERENCE_DIAGONAL = 0, DRH_EDGE_GRAD_FULL_JACOBIAN = 1 };
+drh_status drh_set_edge_grad_mode(drh_handle h, int mode);
 /* How many handles share the device (1 by default).  A handle alone runs view 0's edge
  * preparation on a second stream beside the render; when several share the device (the
  * server sets its slot count) that overlap comes from the other handles instead and the

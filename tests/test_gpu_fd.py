@@ -44,9 +44,10 @@ def silhouette_coords(m, cam):
     return np.sort(np.concatenate([3 * verts, 3 * verts + 1, 3 * verts + 2]))
 
 
-def gpu_grad(m, cam, adj, N):
+def gpu_grad(m, cam, adj, N, jac=drh.EDGE_GRAD_REFERENCE_DIAGONAL):
     r = drh.DiffRender(mode=drh.SILHOUETTE, spp=1)
     r.commit([m])
+    r.set_edge_grad_mode(jac)
     g = r.d_render([cam], [adj], N)
     r.close()
     return g[: 3 * m.V]
@@ -130,3 +131,36 @@ def test_config2_geometry_gradient_matches_fd_on_64_coordinates():
     say("config-2 geometry rel L2 by log2(N)", errs)
     assert errs[26] < 0.04
     assert errs[26] < 0.6 * errs[22]
+
+
+def test_full_jacobian_matches_fd_for_an_oblique_view():
+    """A view whose screen axes are not world x and y: the reference's diagonal screen Jacobian
+    drops the cross terms dSy/dVx and dSx/dVy, so its gradient is not the derivative of the
+    rendered image there; the full-Jacobian mode is."""
+    m = S.cube(0.5, seed=1)
+    cam = S.camera_look((1.6, 1.1, 2.4), 128, 128)
+    adj = smooth_adjoint(128, 128)
+    coords = np.arange(24)
+    fd = O.Scene([m], O.TRACER_BF).fd_silhouette(cam, adj, coords, 1e-6)
+    full = rel_err(gpu_grad(m, cam, adj, 1 << 20, drh.EDGE_GRAD_FULL_JACOBIAN)[coords], fd)
+    diag = rel_err(gpu_grad(m, cam, adj, 1 << 20)[coords], fd)
+    say("oblique cube rel L2: full Jacobian", full, "diagonal", diag)
+    assert full < 0.03
+    assert diag > 5 * full
+
+
+def test_full_jacobian_equals_diagonal_for_an_axis_aligned_view():
+    m = S.icosphere(3, 1.0)
+    cam = S.camera_front(128, 128)
+    adj = smooth_adjoint(128, 128)
+    r = drh.DiffRender(mode=drh.SILHOUETTE, spp=1)
+    try:
+        r.commit([m])
+        g0 = r.d_render([cam], [adj], 1 << 16)
+        r.set_edge_grad_mode(drh.EDGE_GRAD_FULL_JACOBIAN)
+        g1 = r.d_render([cam], [adj], 1 << 16)
+    finally:
+        r.close()
+    # the cross terms are exactly zero here; only the float atomics' order differs
+    assert np.abs(g1).sum() > 0
+    assert np.linalg.norm(g1 - g0) <= 1e-5 * np.linalg.norm(g0)

@@ -9,6 +9,7 @@ as a SHA-256 of the full float32/uint32 bytes plus a strided sample of values.
 """
 from __future__ import annotations
 
+import dataclasses
 import hashlib
 import os
 import sys
@@ -131,6 +132,60 @@ def adam_vectors():
     print("wrote adam_torch")
 
 
+def _vsgf_mesh():
+    """A small closed mesh with normals and texture coordinates for the VSGF fixtures."""
+    m = S.radial_displace(S.icosphere(1), 0.05, seed=4)
+    p = m.pos.astype(np.float64)
+    tc = np.stack([np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi) + 0.5,
+                   np.arccos(np.clip(p[:, 1] / np.linalg.norm(p, axis=1), -1, 1)) / np.pi], 1)
+    m.tc = tc.astype(np.float32)
+    m.nrm = S.vertex_normals(m.pos, m.idx)
+    return m
+
+
+def _fixture_image(H, W, seed):
+    """Float RGB with the tone map's edge cases: 0, tiny, exactly 1, above 1, random."""
+    rng = np.random.default_rng(seed)
+    img = rng.uniform(0.0, 1.2, (H, W, 3)).astype(np.float32)
+    img[0, 0] = (0.0, 1e-7, 1.0)
+    img[0, 1] = (0.5, 2.0, 1e6)
+    return img
+
+
+def meshio_files():
+    """The reference's VSGF and image writers and its VSGF reader (common_src/cmesh.cpp:139-205,
+    common_src/Image2d.cpp:486-735) on small inputs: meshio_files.npz holds the inputs, the
+    bytes the reference wrote and what the reference read back from this package's writer."""
+    import tempfile
+    from diffrender_tutorials_amd import meshio
+    m = _vsgf_mesh()
+    V = m.V
+    pos4 = np.concatenate([m.pos, np.ones((V, 1), np.float32)], 1)
+    nrm4 = np.concatenate([m.nrm, np.zeros((V, 1), np.float32)], 1)
+    out = {"pos4": pos4, "nrm4": nrm4, "tc": m.tc, "idx": m.idx}
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "ref.vsgf")
+        refio.save_vsgf(f, pos4, nrm4, m.tc, m.idx)
+        out["ref_vsgf"] = np.frombuffer(open(f, "rb").read(), np.uint8).copy()
+        for tag, mm in (("ours", m), ("ours_nonrm", dataclasses.replace(m, nrm=None))):
+            f = os.path.join(td, tag + ".vsgf")
+            meshio.save_vsgf(f, mm)
+            out[tag + "_vsgf_sha"] = hashlib.sha256(open(f, "rb").read()).hexdigest()
+            back = refio.load_vsgf(f)
+            for k, v in back.items():
+                out[tag + "_ref_" + k] = v
+        for tag, (H, W, seed) in (("img8", (5, 8, 11)), ("img6", (3, 6, 12))):
+            img = _fixture_image(H, W, seed)
+            out[tag] = img
+            for ext, gamma in ((".bmp", 2.2), (".ppm", 2.2), (".image3f", 2.2), (".ppm", 1.0)):
+                f = os.path.join(td, tag + ext)
+                refio.save_image(f, img, gamma)
+                key = f"{tag}_ref{ext.replace('.', '_')}" + ("" if gamma == 2.2 else "_g1")
+                out[key] = np.frombuffer(open(f, "rb").read(), np.uint8).copy()
+    np.savez_compressed(os.path.join(HERE, "meshio_files.npz"), **out)
+    print("wrote meshio_files")
+
+
 def main(only=None):
     """Write every fixture, or only those named in `only`."""
     if not os.path.exists(refio.HARNESS):
@@ -140,6 +195,8 @@ def main(only=None):
         sequences()
     if want("adam_torch"):
         adam_vectors()
+    if want("meshio_files"):
+        meshio_files()
     c1 = S.config1()
     # one optimiser step: later steps amplify ulp-level position differences through the
     # sign-like normalised update, so the step itself is what is pinned

@@ -24,64 +24,11 @@
 
 #include "../../include/drh.h"
 #include "drh_kernels.h"
+#include "drh_mat4.h"
 
 using namespace drh;
 
 namespace {
-
-// ---------------------------------------------------------------------------
-// LiteMath matrices on the host (common_src/LiteMath.h:1342-1634)
-// ---------------------------------------------------------------------------
-struct M4 {
-  float m[16];  // column-major: m[c*4 + r]
-  float operator()(int r, int c) const { return m[c * 4 + r]; }
-  float& at(int r, int c) { return m[c * 4 + r]; }
-};
-M4 m4_load(const float* p) { M4 r; std::memcpy(r.m, p, sizeof r.m); return r; }
-M4 m4_identity() { M4 r{}; r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.0f; return r; }
-void m4_vec(const M4& B, const float V[4], float R[4]) {
-  for (int r = 0; r < 4; r++) R[r] = V[0] * B.m[r] + V[1] * B.m[4 + r] + V[2] * B.m[8 + r] + V[3] * B.m[12 + r];
-}
-M4 m4_mul(const M4& a, const M4& b) {
-  M4 r;
-  for (int c = 0; c < 4; c++) m4_vec(a, b.m + 4 * c, r.m + 4 * c);
-  return r;
-}
-M4 m4_transpose(const M4& a) { M4 r; for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) r.m[j * 4 + i] = a.m[i * 4 + j]; return r; }
-
-// Cofactor inverse with pair products, term for term as inverse4x4 (LiteMath.h:1551-1634).
-M4 m4_inverse(const M4& a) {
-  float p[12];
-  M4 o;
-  auto A = [&](int r, int c) { return a(r, c); };
-  // cofactors of columns 0 and 1 from 2x2 products of rows 2, 3
-  p[0] = A(2,2) * A(3,3); p[1] = A(2,3) * A(3,2); p[2] = A(2,1) * A(3,3); p[3] = A(2,3) * A(3,1);
-  p[4] = A(2,1) * A(3,2); p[5] = A(2,2) * A(3,1); p[6] = A(2,0) * A(3,3); p[7] = A(2,3) * A(3,0);
-  p[8] = A(2,0) * A(3,2); p[9] = A(2,2) * A(3,0); p[10] = A(2,0) * A(3,1); p[11] = A(2,1) * A(3,0);
-  o.at(0,0) = p[0] * A(1,1) + p[3] * A(1,2) + p[4] * A(1,3);  o.at(0,0) -= p[1] * A(1,1) + p[2] * A(1,2) + p[5] * A(1,3);
-  o.at(1,0) = p[1] * A(1,0) + p[6] * A(1,2) + p[9] * A(1,3);  o.at(1,0) -= p[0] * A(1,0) + p[7] * A(1,2) + p[8] * A(1,3);
-  o.at(2,0) = p[2] * A(1,0) + p[7] * A(1,1) + p[10] * A(1,3); o.at(2,0) -= p[3] * A(1,0) + p[6] * A(1,1) + p[11] * A(1,3);
-  o.at(3,0) = p[5] * A(1,0) + p[8] * A(1,1) + p[11] * A(1,2); o.at(3,0) -= p[4] * A(1,0) + p[9] * A(1,1) + p[10] * A(1,2);
-  o.at(0,1) = p[1] * A(0,1) + p[2] * A(0,2) + p[5] * A(0,3);  o.at(0,1) -= p[0] * A(0,1) + p[3] * A(0,2) + p[4] * A(0,3);
-  o.at(1,1) = p[0] * A(0,0) + p[7] * A(0,2) + p[8] * A(0,3);  o.at(1,1) -= p[1] * A(0,0) + p[6] * A(0,2) + p[9] * A(0,3);
-  o.at(2,1) = p[3] * A(0,0) + p[6] * A(0,1) + p[11] * A(0,3); o.at(2,1) -= p[2] * A(0,0) + p[7] * A(0,1) + p[10] * A(0,3);
-  o.at(3,1) = p[4] * A(0,0) + p[9] * A(0,1) + p[10] * A(0,2); o.at(3,1) -= p[5] * A(0,0) + p[8] * A(0,1) + p[11] * A(0,2);
-  // columns 2 and 3 from 2x2 products of rows 0, 1
-  p[0] = A(0,2) * A(1,3); p[1] = A(0,3) * A(1,2); p[2] = A(0,1) * A(1,3); p[3] = A(0,3) * A(1,1);
-  p[4] = A(0,1) * A(1,2); p[5] = A(0,2) * A(1,1); p[6] = A(0,0) * A(1,3); p[7] = A(0,3) * A(1,0);
-  p[8] = A(0,0) * A(1,2); p[9] = A(0,2) * A(1,0); p[10] = A(0,0) * A(1,1); p[11] = A(0,1) * A(1,0);
-  o.at(0,2) = p[0] * A(3,1) + p[3] * A(3,2) + p[4] * A(3,3);  o.at(0,2) -= p[1] * A(3,1) + p[2] * A(3,2) + p[5] * A(3,3);
-  o.at(1,2) = p[1] * A(3,0) + p[6] * A(3,2) + p[9] * A(3,3);  o.at(1,2) -= p[0] * A(3,0) + p[7] * A(3,2) + p[8] * A(3,3);
-  o.at(2,2) = p[2] * A(3,0) + p[7] * A(3,1) + p[10] * A(3,3); o.at(2,2) -= p[3] * A(3,0) + p[6] * A(3,1) + p[11] * A(3,3);
-  o.at(3,2) = p[5] * A(3,0) + p[8] * A(3,1) + p[11] * A(3,2); o.at(3,2) -= p[4] * A(3,0) + p[9] * A(3,1) + p[10] * A(3,2);
-  o.at(0,3) = p[2] * A(2,2) + p[5] * A(2,3) + p[1] * A(2,1);  o.at(0,3) -= p[4] * A(2,3) + p[0] * A(2,1) + p[3] * A(2,2);
-  o.at(1,3) = p[8] * A(2,3) + p[0] * A(2,0) + p[7] * A(2,2);  o.at(1,3) -= p[6] * A(2,2) + p[9] * A(2,3) + p[1] * A(2,0);
-  o.at(2,3) = p[6] * A(2,1) + p[11] * A(2,3) + p[3] * A(2,0); o.at(2,3) -= p[10] * A(2,3) + p[2] * A(2,0) + p[7] * A(2,1);
-  o.at(3,3) = p[10] * A(2,2) + p[4] * A(2,0) + p[9] * A(2,1); o.at(3,3) -= p[8] * A(2,1) + p[11] * A(2,2) + p[5] * A(2,0);
-  const float k = 1.0f / (A(0,0) * o(0,0) + A(0,1) * o(1,0) + A(0,2) * o(2,0) + A(0,3) * o(3,0));
-  for (int i = 0; i < 16; i++) o.m[i] = o.m[i] * k;
-  return o;
-}
 
 // ---------------------------------------------------------------------------
 // sample sequences on the host
@@ -334,7 +281,7 @@ struct drh_renderer : BufOwner {
 
   // device scene
   DBuf pos_orig, pos, col, tc, nrm_in, nrm, nrm2d, texels, idx, tri_mesh, mesh_vo, mesh_to, dtex_w, dtex_h,
-      dtex_off, dmats, dlayout, dtr_inv0;
+      dtex_off, dmats, dlayout, dtr_inv0, tr_rows;
   // LBVH
   DBuf bf_a, bf_e1, bf_e2, box_lo, box_hi, cen, bounds, keys0, vals0, keys1, vals1, hist, child, node_parent,
       leaf_parent, counter, nrange, frontier, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
@@ -352,7 +299,7 @@ struct drh_renderer : BufOwner {
   std::vector<DevCam> cam_cache;
   // optimiser
   int opt_kind = -1;
-  float pos_lr = 0.1f, col_lr = 0.1f, tex_lr = 0.0f;
+  float pos_lr = 0.1f, col_lr = 0.1f, tex_lr = 0.0f, tr_lr = 0.0f;
   bool skip_tex_interior = false;   // set by the fused step: the interior pass feeds nothing
   int opt_iter = 0;
   DBuf opt_m, opt_v, opt_t;
@@ -564,6 +511,11 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     const M4 T0inv = m4_inverse(m4_load(d->transforms));
     tr_inv0.assign(T0inv.m, T0inv.m + 16);
   }
+  // rows 0-2 of each transform, row-major: the optimiser's copy (the order of d_tr)
+  std::vector<float> rows(12 * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 4; c++) rows[12 * i + 4 * r + c] = d->transforms[16 * i + 4 * c + r];
   // edge set: std::set<Edge> of differentiable meshes ordered by (v0, v1), first kept
   // (src/drender.h:26-52,274-285); depends on topology only, so cached.
   std::vector<uint32_t> idx_local(d->idx, d->idx + 3 * (size_t)nF);
@@ -624,7 +576,7 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
       (s = R(mesh_to, 4 * (size_t)(n + 1), "alloc")) || (s = R(dtex_w, 4 * (size_t)n, "alloc")) ||
       (s = R(dtex_h, 4 * (size_t)n, "alloc")) || (s = R(dtex_off, 4 * (size_t)n, "alloc")) ||
       (s = R(dmats, 4 * mats.size(), "alloc")) || (s = R(dlayout, 4 * layout.size(), "alloc")) ||
-      (s = R(dtr_inv0, 64, "alloc")))
+      (s = R(dtr_inv0, 64, "alloc")) || (s = R(tr_rows, 48 * (size_t)n, "alloc")))
     return s;
   if ((s = up(pos_orig, d->pos, v3))) return s;
   if (d->col) { if ((s = up(col, d->col, v3))) return s; }
@@ -648,7 +600,8 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
       (s = up(mesh_vo, vo.data(), 4 * (size_t)(n + 1))) || (s = up(mesh_to, to.data(), 4 * (size_t)(n + 1))) ||
       (s = up(dtex_w, tw.data(), 4 * (size_t)n)) || (s = up(dtex_h, th.data(), 4 * (size_t)n)) ||
       (s = up(dtex_off, toff.data(), 4 * (size_t)n)) || (s = up(dmats, mats.data(), 4 * mats.size())) ||
-      (s = up(dlayout, layout.data(), 4 * layout.size())) || (s = up(dtr_inv0, tr_inv0.data(), 64)))
+      (s = up(dlayout, layout.data(), 4 * layout.size())) || (s = up(dtr_inv0, tr_inv0.data(), 64)) ||
+      (s = up(tr_rows, rows.data(), 48 * (size_t)n)))
     return s;
   if (!topo_same) {
     E = (int)hedges.size();
@@ -1138,10 +1091,11 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
   if (s) return s;
   // StepDecay (src/optimizer_simple.cpp:82-90): decayPeriod 100, decay_mult 0.75 for GD_Adam,
   // applied after iterations 100, 200, ...
-  float plr = h->pos_lr, clr = h->col_lr;
+  // scales every block's rate: positions, colours, texture and transform
+  float plr = h->pos_lr, clr = h->col_lr, tlr = h->tex_lr, trlr = h->tr_lr;
   if (h->opt_kind == DRH_OPT_SIMPLE)
     for (int k = 1; k < h->opt_iter; k++)
-      if (k % 100 == 0) { plr *= 0.75f; clr *= 0.75f; }
+      if (k % 100 == 0) { plr *= 0.75f; clr *= 0.75f; tlr *= 0.75f; trlr *= 0.75f; }
   for (int i = 0; i < h->nm; i++) {
     if (!h->diff[i]) continue;
     const int off = h->layout[4 * i];
@@ -1152,8 +1106,9 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
     float seg_lr[MAX_SEG] = {0};
     float* seg_param[MAX_SEG] = {nullptr};
     int nseg = 0;
-    // OptSimple::GetLR (src/optimizer_simple.cpp:61-80) blocks; positions and colours are
-    // updated (OptUpdateScene with _OPT_POS | _OPT_COL), texture / transform blocks are not
+    // OptSimple::GetLR (src/optimizer_simple.cpp:61-80) blocks.  OptUpdateScene
+    // (src/optimizer_simple.cpp:92-186) moves positions and colours, the texture when its rate
+    // is set, and rows 0-2 of the transform when its rate is set (all of OPT_ALL)
     seg_start[nseg] = 0; seg_lr[nseg] = plr; seg_param[nseg] = h->pos_orig.as<float>() + 3ll * h->vo[i]; nseg++;
     if (colo >= 0) {
       seg_start[nseg] = colo; seg_lr[nseg] = clr; seg_param[nseg] = h->col.as<float>() + 3ll * h->vo[i]; nseg++;
@@ -1161,11 +1116,14 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
     const int texo = h->layout[4 * i + 2];
     if (texo >= 0) {
       seg_start[nseg] = texo;
-      seg_lr[nseg] = h->tex_lr;
+      seg_lr[nseg] = tlr;
       seg_param[nseg] = h->tex_lr != 0.0f ? h->texels.as<float>() + h->tex_off[i] : nullptr;
       nseg++;
     }
-    seg_start[nseg] = h->layout[4 * i + 3]; seg_lr[nseg] = 0.0f; seg_param[nseg] = nullptr; nseg++;
+    seg_start[nseg] = h->layout[4 * i + 3];
+    seg_lr[nseg] = trlr;
+    seg_param[nseg] = h->tr_lr != 0.0f ? h->tr_rows.as<float>() + 12ll * i : nullptr;
+    nseg++;
     (void)nv;
     if (h->opt_kind == DRH_OPT_SIMPLE) {
       OptApplyArgs a{};
@@ -1180,6 +1138,14 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
       for (int k = 0; k < nseg; k++) { a.seg_start[k] = seg_start[k]; a.seg_lr[k] = seg_lr[k]; a.seg_param[k] = seg_param[k]; }
       h->timed("k_adam", [&] { launch_adam(a, h->stream); });
     }
+  }
+  if (h->tr_lr != 0.0f) {
+    // the moved transforms into the matrices the next commit applies; the inverse of mesh 0's
+    // transform used by the edge gradient stays the one of the first commit, as the
+    // reference's Scene::transforms_inv is appended to and never cleared
+    // (common_src/scene_utils.cpp:110-122)
+    TransformArgs ta{h->nm, h->dlayout.as<int>(), h->tr_rows.as<float>(), h->dmats.as<float>()};
+    h->timed("k_transform_update", [&] { launch_transform_update(ta, h->stream); });
   }
   if (h->opt_kind == DRH_OPT_ADAM) h->timed("k_step_inc", [&] { launch_step_inc(h->opt_t.as<int>(), h->stream); });
   return h->hip(hipGetLastError(), "opt_step");
@@ -1258,6 +1224,27 @@ drh_status drh_get_params(drh_handle h, float* pos, float* col) {
   if (pos && (s = h->hip(hipMemcpyAsync(pos, h->pos_orig.p, 12 * (size_t)h->V, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
   if (col && (s = h->hip(hipMemcpyAsync(col, h->col.p, 12 * (size_t)h->V, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
   return h->hip(hipStreamSynchronize(h->stream), "get_params");
+}
+
+drh_status drh_get_transforms(drh_handle h, float* transforms) {
+  if (!h || !transforms) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "get_transforms: commit first");
+  (void)hipSetDevice(h->st.device);
+  std::vector<float> m(48 * (size_t)h->nm);
+  drh_status s;
+  if ((s = h->hip(hipMemcpyAsync(m.data(), h->dmats.p, 4 * m.size(), hipMemcpyDeviceToHost, h->stream), "download")) ||
+      (s = h->hip(hipStreamSynchronize(h->stream), "get_transforms")))
+    return s;
+  for (int i = 0; i < h->nm; i++) std::memcpy(transforms + 16 * (size_t)i, &m[48 * (size_t)i], 64);
+  return DRH_OK;
+}
+
+drh_status drh_opt_set_transform_lr(drh_handle h, float lr) {
+  if (!h) return DRH_E_INVALID;
+  if (!(lr >= 0.0f) || lr > 1e30f) return h->fail(DRH_E_INVALID, "opt_set_transform_lr: bad rate");
+  if (lr != h->tr_lr) h->drop_graph();   // the captured step bakes the segment table in
+  h->tr_lr = lr;
+  return DRH_OK;
 }
 
 drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr) {

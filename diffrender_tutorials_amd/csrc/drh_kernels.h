@@ -243,6 +243,16 @@ struct AdamArgs {
   float* seg_param[MAX_SEG];
 };
 
+// The optimiser's transform update: rows 0-2 of each differentiable mesh's transform
+// (row-major 3x4, the order of its 12 gradient entries) back into T, then the normal matrix
+// transpose(inverse(T)) (Scene::prepare_for_render, common_src/scene_utils.cpp:140-143).
+struct TransformArgs {
+  int n_meshes;
+  const int* layout;   // 4 per mesh; layout[4i] < 0: mesh i is not differentiable
+  const float* rows;   // 12 per mesh
+  float* mats;         // 48 per mesh: T, transpose(inverse(T)), identity (column-major)
+};
+
 struct HitsArgs {
   Tracer tr;
   const DevCam* cams;
@@ -280,6 +290,7 @@ void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s);   // bin_count must b
 void launch_optsimple(const float* grad, float* m, float* G, int off, const OptApplyArgs& a, hipStream_t s);
 void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_step_inc(int* step, hipStream_t s);
+void launch_transform_update(const TransformArgs& a, hipStream_t s);
 void launch_debug_hits(const HitsArgs& a, int tiles, hipStream_t s);
 
 }  // namespace drh

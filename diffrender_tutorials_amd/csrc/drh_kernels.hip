@@ -7,7 +7,7 @@
 //   edges    k_project, k_edge_len, k_cdf_{sums,scan,spec,walk,final},
 //            k_edge_bin, k_bin_scan, k_edge_scatter (pixel-bin order),
 //            k_edge_trace<MODE> (side colours), k_edge_grad               (edge_derivatives)
-//   optimise k_optsimple_{moments,apply}, k_adam, k_step_inc
+//   optimise k_optsimple_{moments,apply}, k_adam, k_transform_update, k_step_inc
 //
 // Wave64 everywhere.  Rays are traced a wavefront at a time (trace_bvh in drh_device.h): a
 // pixel tile or 32 edge samples (both sides) per wave, one wave-uniform LBVH walk per packet.
@@ -15,6 +15,7 @@
 
 #include "drh_device.h"
 #include "drh_kernels.h"
+#include "drh_mat4.h"
 
 namespace drh {
 
@@ -1365,6 +1366,20 @@ __global__ void k_adam(AdamArgs a) {
   const float bc1 = (float)(1.0 - pow((double)a.b1, t)), bc2 = (float)(1.0 - pow((double)a.b2, t));
   const float mh = m / bc1, vh = v / bc2;
   p[i - a.seg_start[seg]] -= a.seg_lr[seg] * mh / (sqrtf(vh) + a.eps);
+}
+
+__global__ void k_transform_update(TransformArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_meshes || a.layout[4 * i] < 0) return;
+  float* mt = a.mats + 48 * i;
+  M4 T = m4_load(mt);
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 4; c++) T.at(r, c) = a.rows[12 * i + 4 * r + c];
+  const M4 NT = m4_transpose(m4_inverse(T));
+  for (int k = 0; k < 16; k++) { mt[k] = T.m[k]; mt[16 + k] = NT.m[k]; }
+}
+void launch_transform_update(const TransformArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_transform_update, dim3((a.n_meshes + 63) / 64), dim3(64), 0, s, a);
 }
 
 __global__ void k_step_inc(int* step) {

@@ -1,0 +1,74 @@
+// drh_mat4.h -- LiteMath float4x4 arithmetic (common_src/LiteMath.h:1342-1634) for the host
+// runtime and the kernels alike: one definition, so a matrix derived on the device (the
+// transform update of the optimiser) is bit for bit the one the host derives at commit.
+// Every translation unit is built with -ffp-contract=off and IEEE division.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace drh {
+
+struct M4 {
+  float m[16];  // column-major: m[c*4 + r]
+  __host__ __device__ float operator()(int r, int c) const { return m[c * 4 + r]; }
+  __host__ __device__ float& at(int r, int c) { return m[c * 4 + r]; }
+};
+__host__ __device__ inline M4 m4_load(const float* p) {
+  M4 r;
+  for (int i = 0; i < 16; i++) r.m[i] = p[i];
+  return r;
+}
+__host__ __device__ inline M4 m4_identity() {
+  M4 r{};
+  r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.0f;
+  return r;
+}
+__host__ __device__ inline void m4_vec(const M4& B, const float V[4], float R[4]) {
+  for (int r = 0; r < 4; r++) R[r] = V[0] * B.m[r] + V[1] * B.m[4 + r] + V[2] * B.m[8 + r] + V[3] * B.m[12 + r];
+}
+__host__ __device__ inline M4 m4_mul(const M4& a, const M4& b) {
+  M4 r;
+  for (int c = 0; c < 4; c++) m4_vec(a, b.m + 4 * c, r.m + 4 * c);
+  return r;
+}
+__host__ __device__ inline M4 m4_transpose(const M4& a) {
+  M4 r;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) r.m[j * 4 + i] = a.m[i * 4 + j];
+  return r;
+}
+
+// Cofactor inverse with pair products, term for term as inverse4x4 (LiteMath.h:1551-1634).
+__host__ __device__ inline M4 m4_inverse(const M4& a) {
+  float p[12];
+  M4 o;
+  auto A = [&](int r, int c) { return a(r, c); };
+  // cofactors of columns 0 and 1 from 2x2 products of rows 2, 3
+  p[0] = A(2,2) * A(3,3); p[1] = A(2,3) * A(3,2); p[2] = A(2,1) * A(3,3); p[3] = A(2,3) * A(3,1);
+  p[4] = A(2,1) * A(3,2); p[5] = A(2,2) * A(3,1); p[6] = A(2,0) * A(3,3); p[7] = A(2,3) * A(3,0);
+  p[8] = A(2,0) * A(3,2); p[9] = A(2,2) * A(3,0); p[10] = A(2,0) * A(3,1); p[11] = A(2,1) * A(3,0);
+  o.at(0,0) = p[0] * A(1,1) + p[3] * A(1,2) + p[4] * A(1,3);  o.at(0,0) -= p[1] * A(1,1) + p[2] * A(1,2) + p[5] * A(1,3);
+  o.at(1,0) = p[1] * A(1,0) + p[6] * A(1,2) + p[9] * A(1,3);  o.at(1,0) -= p[0] * A(1,0) + p[7] * A(1,2) + p[8] * A(1,3);
+  o.at(2,0) = p[2] * A(1,0) + p[7] * A(1,1) + p[10] * A(1,3); o.at(2,0) -= p[3] * A(1,0) + p[6] * A(1,1) + p[11] * A(1,3);
+  o.at(3,0) = p[5] * A(1,0) + p[8] * A(1,1) + p[11] * A(1,2); o.at(3,0) -= p[4] * A(1,0) + p[9] * A(1,1) + p[10] * A(1,2);
+  o.at(0,1) = p[1] * A(0,1) + p[2] * A(0,2) + p[5] * A(0,3);  o.at(0,1) -= p[0] * A(0,1) + p[3] * A(0,2) + p[4] * A(0,3);
+  o.at(1,1) = p[0] * A(0,0) + p[7] * A(0,2) + p[8] * A(0,3);  o.at(1,1) -= p[1] * A(0,0) + p[6] * A(0,2) + p[9] * A(0,3);
+  o.at(2,1) = p[3] * A(0,0) + p[6] * A(0,1) + p[11] * A(0,3); o.at(2,1) -= p[2] * A(0,0) + p[7] * A(0,1) + p[10] * A(0,3);
+  o.at(3,1) = p[4] * A(0,0) + p[9] * A(0,1) + p[10] * A(0,2); o.at(3,1) -= p[5] * A(0,0) + p[8] * A(0,1) + p[11] * A(0,2);
+  // columns 2 and 3 from 2x2 products of rows 0, 1
+  p[0] = A(0,2) * A(1,3); p[1] = A(0,3) * A(1,2); p[2] = A(0,1) * A(1,3); p[3] = A(0,3) * A(1,1);
+  p[4] = A(0,1) * A(1,2); p[5] = A(0,2) * A(1,1); p[6] = A(0,0) * A(1,3); p[7] = A(0,3) * A(1,0);
+  p[8] = A(0,0) * A(1,2); p[9] = A(0,2) * A(1,0); p[10] = A(0,0) * A(1,1); p[11] = A(0,1) * A(1,0);
+  o.at(0,2) = p[0] * A(3,1) + p[3] * A(3,2) + p[4] * A(3,3);  o.at(0,2) -= p[1] * A(3,1) + p[2] * A(3,2) + p[5] * A(3,3);
+  o.at(1,2) = p[1] * A(3,0) + p[6] * A(3,2) + p[9] * A(3,3);  o.at(1,2) -= p[0] * A(3,0) + p[7] * A(3,2) + p[8] * A(3,3);
+  o.at(2,2) = p[2] * A(3,0) + p[7] * A(3,1) + p[10] * A(3,3); o.at(2,2) -= p[3] * A(3,0) + p[6] * A(3,1) + p[11] * A(3,3);
+  o.at(3,2) = p[5] * A(3,0) + p[8] * A(3,1) + p[11] * A(3,2); o.at(3,2) -= p[4] * A(3,0) + p[9] * A(3,1) + p[10] * A(3,2);
+  o.at(0,3) = p[2] * A(2,2) + p[5] * A(2,3) + p[1] * A(2,1);  o.at(0,3) -= p[4] * A(2,3) + p[0] * A(2,1) + p[3] * A(2,2);
+  o.at(1,3) = p[8] * A(2,3) + p[0] * A(2,0) + p[7] * A(2,2);  o.at(1,3) -= p[6] * A(2,2) + p[9] * A(2,3) + p[1] * A(2,0);
+  o.at(2,3) = p[6] * A(2,1) + p[11] * A(2,3) + p[3] * A(2,0); o.at(2,3) -= p[10] * A(2,3) + p[2] * A(2,0) + p[7] * A(2,1);
+  o.at(3,3) = p[10] * A(2,2) + p[4] * A(2,0) + p[9] * A(2,1); o.at(3,3) -= p[8] * A(2,1) + p[11] * A(2,2) + p[5] * A(2,0);
+  const float k = 1.0f / (A(0,0) * o(0,0) + A(0,1) * o(1,0) + A(0,2) * o(2,0) + A(0,3) * o(3,0));
+  for (int i = 0; i < 16; i++) o.m[i] = o.m[i] * k;
+  return o;
+}
+
+}  // namespace drh

@@ -34,7 +34,7 @@ STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params",
-           "drh_opt_set_texture_lr", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples",
+           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples",
            "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
@@ -167,6 +167,7 @@ class DiffRender:
         d, keep = scene_desc(meshes)
         self._check(L.drh_commit(self.h, ctypes.byref(d)))
         self.V = int(sum(m.V for m in meshes))
+        self.n_meshes = len(meshes)
         self.has_col = keep[5] is not None
 
     def grad_size(self):
@@ -226,6 +227,16 @@ class DiffRender:
 
     def opt_set_texture_lr(self, lr):
         self._check(load().drh_opt_set_texture_lr(self.h, ctypes.c_float(lr)))
+
+    def opt_set_transform_lr(self, lr):
+        """OptimizerParameters::transforms_lr with _OPT_TFM: 0 keeps the transforms fixed."""
+        self._check(load().drh_opt_set_transform_lr(self.h, ctypes.c_float(lr)))
+
+    def get_transforms(self):
+        """The current transforms, (n_meshes, 16) float32, column-major as Mesh.transform."""
+        out = np.empty(16 * self.n_meshes, np.float32)
+        self._check(load().drh_get_transforms(self.h, _fp(out)))
+        return out.reshape(-1, 16)
 
     def get_texels(self, n):
         out = np.empty(n, np.float32)

@@ -83,6 +83,22 @@ def translate(t) -> np.ndarray:
     return m
 
 
+def rotate_y(deg: float) -> np.ndarray:
+    """rotate4x4Y (common_src/LiteMath.h:1501-1509), column-major float32."""
+    phi = np.float32(np.deg2rad(deg))
+    c, s = np.cos(phi), np.sin(phi)
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = c, -s, s, c     # rows of m are the columns
+    return m.reshape(-1)
+
+
+def mat_mul(a, b) -> np.ndarray:
+    """Column-major a * b (float32)."""
+    a4 = np.asarray(a, np.float32).reshape(4, 4)
+    b4 = np.asarray(b, np.float32).reshape(4, 4)
+    return (b4 @ a4).astype(np.float32).reshape(-1)
+
+
 def _normalize(v):
     v = np.asarray(v, dtype=np.float32)
     n = f32(math.sqrt(float(f32(v[0] * v[0] + v[1] * v[1]) + f32(v[2] * v[2]))))

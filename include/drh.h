@@ -143,6 +143,17 @@ drh_status drh_get_params(drh_handle h, float* pos, float* col);
  * fixed.  With 0 in TEXTURE_COLOR / LAMBERT mode the fused step leaves out the interior pass,
  * whose only output there is the texture gradient: the step's results are unchanged. */
 drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr);
+/* Learning rate of the transform block (OptimizerParameters::transforms_lr with _OPT_TFM,
+ * src/optimizer.h:26-45; OptUpdateScene, src/optimizer_simple.cpp:113-135): each step moves
+ * rows 0-2 of every differentiable mesh's transform by its scaled d_tr entries (row-major,
+ * 4 per row), then derives the normal matrix again.  0 (the default) keeps the transforms
+ * fixed.  As in the reference, the inverse of mesh 0's transform that the edge gradient
+ * multiplies by stays the one of the last drh_commit (Scene::transforms_inv is appended to
+ * and never cleared, common_src/scene_utils.cpp:110-122).  The reference's default flags
+ * (OPT_ALL) move the transforms with transforms_lr = 0.1. */
+drh_status drh_opt_set_transform_lr(drh_handle h, float lr);
+/* Download the current (optimised) transforms: 16 floats per mesh, column-major. */
+drh_status drh_get_transforms(drh_handle h, float* transforms);
 /* Interior samples (interior_derivatives, src/drender.h:197-233).  aligned = 0 (default): the
  * reference's Hammersley offsets, so every interior sample is a fresh ray (recompute).
  * aligned = 1: the render's grid offsets (d + 0.5)/sqrt(spp) instead -- another unbiased

@@ -12,6 +12,7 @@
 // Usage: ref_harness <input.bin> <output.bin>
 // Input (little endian):
 //   int32 magic(0x44524831), mode, spp, n_meshes, n_views, edge_samples, ops, opt_iters
+//   if ops & OP_OPT_ALL: float32 position_lr, textures_lr, transforms_lr (flags OPT_ALL)
 //   per mesh: int32 nv, nf, has_col, has_tc, has_nrm, tex_w, tex_h, diff;
 //             float32 transform[16]; pos[3nv]; [col[3nv]]; [tc[2nv]]; [nrm[3nv]]; uint32 idx[3nf];
 //             float32 tex[tex_w*tex_h*3]
@@ -33,12 +34,13 @@ using namespace diff_render;
 
 enum {
   OP_RENDER = 1, OP_HITS = 2, OP_DRENDER = 4, OP_COMPARE = 8, OP_SEQ = 16, OP_EDGES = 32, OP_OPTIMIZE = 64,
+  OP_OPT_ALL = 128,
 };
 enum {
   T_IMAGES = 1, T_HIT_PRIM = 2, T_HIT_GEOM = 3, T_HIT_T = 4, T_HIT_U = 5, T_HIT_V = 6, T_GRAD = 7, T_LOSS = 8,
   T_QMC = 9, T_PRNG = 10, T_HAMM = 11, T_EDGE_V0 = 12, T_EDGE_V1 = 13, T_EDGE_PMF = 14, T_EDGE_CDF = 15,
   T_COMPARE_GRAD = 16, T_COMPARE_IMAGES = 17, T_PROJ = 18, T_OPT_POS = 19, T_OPT_COL = 20, T_OPT_ERR = 21,
-  T_PROJ_INV = 22, T_WV_INV = 23, T_WVP = 24,
+  T_PROJ_INV = 22, T_WV_INV = 23, T_WVP = 24, T_OPT_TR = 25, T_OPT_TEX = 26,
 };
 
 static FILE* g_in;
@@ -59,6 +61,8 @@ int main(int argc, char** argv) {
   if (rd<int32_t>() != 0x44524831) { fprintf(stderr, "bad magic\n"); return 1; }
   const int mode = rd<int32_t>(), spp = rd<int32_t>(), nm = rd<int32_t>(), nviews = rd<int32_t>();
   const int edge_samples = rd<int32_t>(), ops = rd<int32_t>(), opt_iters = rd<int32_t>();
+  float lr_pos = 0.1f, lr_tex = 0.1f, lr_tfm = 0.1f;
+  if (ops & OP_OPT_ALL) { lr_pos = rd<float>(); lr_tex = rd<float>(); lr_tfm = rd<float>(); }
 
   Scene scene;
   std::vector<int> diff_ids;
@@ -248,6 +252,11 @@ int main(int argc, char** argv) {
     OptimizerParameters p(OptimizerParameters::GD_Adam);
     p.verbose = false;
     p.flags = OptimizerParameters::_OPT_POS | OptimizerParameters::_OPT_COL;
+    if (ops & OP_OPT_ALL) {
+      // every block: positions, transforms, colours, textures with the given rates
+      p.flags = OptimizerParameters::OPT_ALL;
+      p.position_lr = lr_pos; p.textures_lr = lr_tex; p.transforms_lr = lr_tfm;
+    }
     p.differentiable_mesh_ids = diff_ids;
     IOptimizer* opt = CreateSimpleOptimizer();
     opt->Init(scene, R, cams.data(), given.data(), nviews, p);
@@ -259,6 +268,17 @@ int main(int argc, char** argv) {
     put(T_OPT_POS, pos.data(), pos.size() * 4);
     put(T_OPT_COL, col.data(), col.size() * 4);
     put(T_OPT_ERR, &err, 4);
+    if (ops & OP_OPT_ALL) {
+      std::vector<float> trs, tex;
+      for (int i = 0; i < nm; i++) {
+        const float4x4 t = res.get_transform(i)[0];
+        const float* f = (const float*)&t;
+        trs.insert(trs.end(), f, f + 16);
+      }
+      if (!res.get_mesh(0).textures.empty()) tex = res.get_mesh(0).textures[0].m_data;
+      put(T_OPT_TR, trs.data(), trs.size() * 4);
+      put(T_OPT_TEX, tex.data(), tex.size() * 4);
+    }
     delete opt;
   }
   fclose(g_out);

@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.dirname(HERE))   # tests/: the golden scenes
 
 import refio  # noqa: E402
 from diffrender_tutorials_amd import scenes as S  # noqa: E402
@@ -186,6 +187,26 @@ def meshio_files():
     print("wrote meshio_files")
 
 
+def opt_all_case(name):
+    """OptSimple (GD_Adam) with OPT_ALL flags: positions, transforms, colours and textures move
+    with the case's rates (src/optimizer_simple.cpp:92-186, 272-309)."""
+    import golden_cases as GC
+    _, mode, spp, iters, lrs = GC.OPT_ALL_CASES[name]
+    meshes, targets, cam = GC.opt_all_scene(name)
+    out = {"mode": mode, "spp": spp, "opt_iters": iters, "lrs": np.array(lrs, np.float32)}
+    for i, m in enumerate(meshes):
+        out[f"mesh{i}_sha"] = mesh_hash(m)
+    cam_fields(out, "cam_", cam)
+    rt = refio.run(targets, [cam], mode, spp, refio.OP_RENDER)
+    tgt = rt["images"].reshape(cam.height, cam.width, 3)
+    out["target_sha"] = sha(tgt)
+    ro = refio.run(meshes, [cam], mode, spp, refio.OP_OPTIMIZE, images=[tgt], opt_iters=iters, opt_lrs=lrs)
+    for k in ("opt_pos", "opt_col", "opt_err", "opt_tr", "opt_tex"):
+        out[k] = ro[k]
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, {k: v.shape for k, v in out.items() if isinstance(v, np.ndarray)})
+
+
 def main(only=None):
     """Write every fixture, or only those named in `only`."""
     if not os.path.exists(refio.HARNESS):
@@ -197,6 +218,10 @@ def main(only=None):
         adam_vectors()
     if want("meshio_files"):
         meshio_files()
+    import golden_cases as GC
+    for name in GC.OPT_ALL_CASES:
+        if want(name):
+            opt_all_case(name)
     c1 = S.config1()
     # one optimiser step: later steps amplify ulp-level position differences through the
     # sign-like normalised update, so the step itself is what is pinned

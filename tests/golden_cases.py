@@ -36,6 +36,34 @@ def _box():
     return box, box_t
 
 
+def tfm_scene():
+    """A differentiable sphere under a rotated, translated transform beside a static box; the
+    target moves the sphere's transform (transform optimisation, silhouette)."""
+    ball = S.icosphere(2, 0.35)
+    ball.transform = S.mat_mul(S.translate((0.15, -0.1, 0.05)), S.rotate_y(20.0))
+    box = S.cube(0.25, 3)
+    box.transform = S.translate((-0.55, 0.3, -0.3))
+    box.differentiable = False
+    ball_t = ball.copy()
+    ball_t.transform = S.mat_mul(S.translate((0.22, -0.04, 0.05)), S.rotate_y(20.0))
+    return [ball, box], [ball_t, box], S.camera_look((0.4, 0.5, 3.0), 128, 128)
+
+
+# optimiser runs with every block moving (OptimizerParameters::OPT_ALL): name ->
+# (scene, shading mode, spp, iterations, (position_lr, textures_lr, transforms_lr))
+OPT_ALL_CASES = {
+    "optall_tfm_sil_1": ("tfm", 1, 4, 1, (0.0, 0.0, 0.1)),
+    "optall_tfm_sil_3": ("tfm", 1, 4, 3, (0.0, 0.0, 0.1)),
+    "optall_tfm_pos_sil_2": ("tfm", 1, 4, 2, (0.02, 0.0, 0.05)),
+    "optall_tex_2": ("ico3_texture", 3, 4, 2, (0.0, 0.05, 0.0)),
+}
+
+
+def opt_all_scene(name):
+    key = OPT_ALL_CASES[name][0]
+    return tfm_scene() if key == "tfm" else scene(key)
+
+
 def scene(name):
     """(meshes, target_meshes, camera) of a golden case."""
     if name == "cfg1_cube_vcol":

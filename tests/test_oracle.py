@@ -154,6 +154,62 @@ def test_optsimple_run_matches_reference():
     assert np.allclose(m.col.reshape(-1), fx["opt_col"], rtol=0, atol=1e-7)
 
 
+def _rows_to_transform(T, rows):
+    t = np.asarray(T, np.float32).reshape(4, 4).copy()     # t[c, r] = T(r, c)
+    for r in range(3):
+        for c in range(4):
+            t[c, r] = rows[4 * r + c]
+    return t.reshape(-1)
+
+
+@pytest.mark.parametrize("name", ["optall_tfm_sil_1", "optall_tfm_sil_3", "optall_tex_2"])
+def test_optsimple_opt_all_matches_reference(name):
+    """OptSimple with OPT_ALL flags (transforms and textures move too), composed from oracle
+    pieces, against the reference's OptSimple::Run: OptStep scales each block by its rate and
+    OptUpdateScene subtracts the d_tr block from rows 0-2 of the transform
+    (src/optimizer_simple.cpp:92-186, 188-236).  Position rates are 0 here, so the stale
+    inverse transform the reference keeps for d_pos does not enter."""
+    from golden_cases import OPT_ALL_CASES, opt_all_scene
+    fx = load(name)
+    _, mode, spp, iters, (plr, tlr, trlr) = OPT_ALL_CASES[name]
+    meshes, targets, cam = opt_all_scene(name)
+    for i, m in enumerate(meshes):
+        assert mesh_hash(m) == str(fx[f"mesh{i}_sha"])
+    tgt = O.Scene(targets).render(mode, spp, [cam])[0]
+    assert sha(tgt) == str(fx["target_sha"])
+    ms = [m.copy() for m in meshes]
+    m0 = ms[0]
+    nv = m0.V
+    texo = 3 * nv
+    ntex = 0 if m0.tex is None or mode not in (3, 4) else m0.tex.size
+    tro = texo + ntex
+    G = tro + 12
+    mom = np.zeros(G, np.float32)
+    gsq = np.zeros(G, np.float32)
+    starts = [0] + ([texo] if ntex else []) + [tro]
+    rates = [plr] + ([tlr] if ntex else []) + [trlr]
+    for it in range(iters):
+        sc = O.Scene(ms, O.TRACER_BVH)
+        img = sc.render(mode, spp, [cam])[0]
+        _, adj = O.loss(img, tgt)
+        g = sc.d_render(mode, spp, [cam], [adj], cam.width * cam.height)
+        assert g.size == G
+        step = O.optsimple_step(g, mom, gsq, starts, rates)
+        m0.pos = (m0.pos - step[:3 * nv].reshape(-1, 3)).astype(np.float32)
+        if ntex:
+            m0.tex = (m0.tex.reshape(-1) - step[texo:tro]).astype(np.float32).reshape(m0.tex.shape)
+        rows = m0.transform.reshape(4, 4).T[:3].reshape(-1) - step[tro:tro + 12]
+        m0.transform = _rows_to_transform(m0.transform, rows)
+    assert np.allclose(m0.pos.reshape(-1), fx["opt_pos"], rtol=0, atol=1e-7)
+    assert np.allclose(m0.transform, fx["opt_tr"][:16], rtol=0, atol=1e-6)
+    for i in range(1, len(ms)):
+        assert np.array_equal(ms[i].transform, fx["opt_tr"][16 * i:16 * i + 16])
+    if ntex:
+        assert np.allclose(m0.tex.reshape(-1), fx["opt_tex"], rtol=0, atol=1e-6)
+    if trlr:
+        assert np.abs(fx["opt_tr"][:16] - meshes[0].transform).max() > 0.05   # the transform did move
+
+
 def test_adam_matches_torch_vectors():
     fx = load("adam_torch")
     p = fx["p0"].copy()

@@ -281,7 +281,12 @@ struct drh_renderer : BufOwner {
 
   // device scene
   DBuf pos_orig, pos, col, tc, nrm_in, nrm, nrm2d, texels, idx, tri_mesh, mesh_vo, mesh_to, dtex_w, dtex_h,
-      dtex_off, dmats, dlayout, dtr_inv0, tr_rows;
+      dtex_off, dmats, dlayout, dtr_inv0, tr_rows, drflag, drparams, trscr;
+  // RESTRICTED_TRANSFORM meshes: flag and the 7 parameters per mesh
+  std::vector<int> rflag;
+  std::vector<float> rparams_h;
+  bool any_restricted = false;
+  int tr_block(int i) const { return rflag[i] ? 7 : 12; }
   // LBVH
   DBuf bf_a, bf_e1, bf_e2, box_lo, box_hi, cen, bounds, keys0, vals0, keys1, vals1, hist, child, node_parent,
       leaf_parent, counter, nrange, frontier, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
@@ -480,7 +485,23 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
   }
   if ((st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) && !d->tc)
     return fail(DRH_E_UNSUPPORTED, "commit: texture coordinates required for this shading mode");
+  // RESTRICTED_TRANSFORM meshes: the transform is TransformR::to_mat of their 7 parameters
+  // (Scene::prepare_for_render, common_src/scene_utils.cpp:113-118)
+  std::vector<int> rf(n, 0);
+  std::vector<float> rp(7 * (size_t)n, 0.0f);
+  std::vector<float> eff(d->transforms, d->transforms + 16 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    rf[i] = d->restricted ? (d->restricted[i] != 0) : 0;
+    if (!rf[i]) continue;
+    if (!d->restricted_params) return fail(DRH_E_INVALID, "commit: restricted mesh %d without restricted_params", i);
+    std::memcpy(&rp[7 * (size_t)i], d->restricted_params + 7 * (size_t)i, 28);
+    const M4 T = m4_restricted(&rp[7 * (size_t)i]);
+    std::memcpy(&eff[16 * (size_t)i], T.m, 64);
+  }
   nm = n; V = nV; F = nF; vo = nvo; to = nto; tex_w = tw; tex_h = th; tex_off = toff; diff = df;
+  rflag = rf; rparams_h = rp;
+  any_restricted = false;
+  for (int i = 0; i < n; i++) any_restricted = any_restricted || (rf[i] && df[i]);
   has_nrm = d->nrm != nullptr;
   has_col = d->col != nullptr;
   // gradient layout (DMesh, src/dmesh.h:35-85)
@@ -495,12 +516,12 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     if (st.mode == DRH_VERTEX_COLOR) { colo = off; off += 3 * nv; }
     if (st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) { texo = off; off += 3 * tw[i] * th[i]; }
     layout[4 * i] = G; layout[4 * i + 1] = colo; layout[4 * i + 2] = texo; layout[4 * i + 3] = off;
-    G += off + 12;
+    G += off + (rf[i] ? 7 : 12);   // RESTRICTED_TRANSFORM_SIZE or TRANSFORM_SIZE
   }
   // matrices: T, transpose(inverse(T)), identity
   mats.assign(48 * (size_t)n, 0.0f);
   for (int i = 0; i < n; i++) {
-    const M4 T = m4_load(d->transforms + 16 * i);
+    const M4 T = m4_load(&eff[16 * (size_t)i]);
     const M4 NT = m4_transpose(m4_inverse(T));
     const M4 I = m4_identity();
     std::memcpy(&mats[48 * i], T.m, 64);
@@ -508,14 +529,14 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     std::memcpy(&mats[48 * i + 32], I.m, 64);
   }
   {
-    const M4 T0inv = m4_inverse(m4_load(d->transforms));
+    const M4 T0inv = m4_inverse(m4_load(eff.data()));
     tr_inv0.assign(T0inv.m, T0inv.m + 16);
   }
   // rows 0-2 of each transform, row-major: the optimiser's copy (the order of d_tr)
   std::vector<float> rows(12 * (size_t)n);
   for (int i = 0; i < n; i++)
     for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 4; c++) rows[12 * i + 4 * r + c] = d->transforms[16 * i + 4 * c + r];
+      for (int c = 0; c < 4; c++) rows[12 * i + 4 * r + c] = eff[16 * i + 4 * c + r];
   // edge set: std::set<Edge> of differentiable meshes ordered by (v0, v1), first kept
   // (src/drender.h:26-52,274-285); depends on topology only, so cached.
   std::vector<uint32_t> idx_local(d->idx, d->idx + 3 * (size_t)nF);
@@ -576,7 +597,9 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
       (s = R(mesh_to, 4 * (size_t)(n + 1), "alloc")) || (s = R(dtex_w, 4 * (size_t)n, "alloc")) ||
       (s = R(dtex_h, 4 * (size_t)n, "alloc")) || (s = R(dtex_off, 4 * (size_t)n, "alloc")) ||
       (s = R(dmats, 4 * mats.size(), "alloc")) || (s = R(dlayout, 4 * layout.size(), "alloc")) ||
-      (s = R(dtr_inv0, 64, "alloc")) || (s = R(tr_rows, 48 * (size_t)n, "alloc")))
+      (s = R(dtr_inv0, 64, "alloc")) || (s = R(tr_rows, 48 * (size_t)n, "alloc")) ||
+      (s = R(drflag, 4 * (size_t)n, "alloc")) || (s = R(drparams, 28 * (size_t)n, "alloc")) ||
+      (s = R(trscr, 48 * (size_t)n, "alloc")))
     return s;
   if ((s = up(pos_orig, d->pos, v3))) return s;
   if (d->col) { if ((s = up(col, d->col, v3))) return s; }
@@ -601,7 +624,8 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
       (s = up(dtex_w, tw.data(), 4 * (size_t)n)) || (s = up(dtex_h, th.data(), 4 * (size_t)n)) ||
       (s = up(dtex_off, toff.data(), 4 * (size_t)n)) || (s = up(dmats, mats.data(), 4 * mats.size())) ||
       (s = up(dlayout, layout.data(), 4 * layout.size())) || (s = up(dtr_inv0, tr_inv0.data(), 64)) ||
-      (s = up(tr_rows, rows.data(), 48 * (size_t)n)))
+      (s = up(tr_rows, rows.data(), 48 * (size_t)n)) || (s = up(drflag, rf.data(), 4 * (size_t)n)) ||
+      (s = up(drparams, rp.data(), 28 * (size_t)n)))
     return s;
   if (!topo_same) {
     E = (int)hedges.size();
@@ -773,6 +797,10 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   if ((s = ensure_rnd(N))) return s;
   if ((s = build_view_suspects(views))) return s;
   if (!accumulate) timed("k_fill_f32", [&] { launch_fill_f32(d_grad, G, 0.0f, stream); });
+  // with restricted meshes the edge pass gathers every mesh's 12 transform entries apart,
+  // then k_restricted_grad maps them into the gradient blocks
+  const bool tr_apart = any_restricted && E > 0 && N > 0;
+  if (tr_apart) timed("k_fill_f32", [&] { launch_fill_f32(trscr.as<float>(), 12ll * nm, 0.0f, stream); });
   // single differentiable mesh: position / transform gradient offsets
   int single = -1;
   for (int i = 0; i < nm; i++) if (diff[i]) { single = (single == -1) ? i : -2; }
@@ -853,10 +881,16 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ea.single_mesh = single >= 0 ? 1 : 0;
       ea.full_jacobian = edge_grad_mode == DRH_EDGE_GRAD_FULL_JACOBIAN ? 1 : 0;
       ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
+      ea.single_id = single >= 0 ? single : 0;
+      ea.trscr = tr_apart ? trscr.as<float>() : nullptr;
       ea.ecol = ecol.as<float4>();
       timed("k_edge_trace", [&] { launch_edge_trace(st.mode, ea, stream); });
       timed("k_edge_grad", [&] { launch_edge_grad(ea, stream); });
     }
+  }
+  if (tr_apart) {
+    RestrictedArgs ra{nm, dlayout.as<int>(), drflag.as<int>(), drparams.as<float>(), trscr.as<float>(), d_grad};
+    timed("k_restricted_grad", [&] { launch_restricted_grad(ra, stream); });
   }
   return hip(hipGetLastError(), "d_render kernels");
 }
@@ -1101,7 +1135,7 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
     const int off = h->layout[4 * i];
     const int nv = h->vo[i + 1] - h->vo[i];
     const int colo = h->layout[4 * i + 1];
-    const int n = h->layout[4 * i + 3] + 12;
+    const int n = h->layout[4 * i + 3] + h->tr_block(i);
     int seg_start[MAX_SEG] = {0};
     float seg_lr[MAX_SEG] = {0};
     float* seg_param[MAX_SEG] = {nullptr};
@@ -1122,7 +1156,9 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
     }
     seg_start[nseg] = h->layout[4 * i + 3];
     seg_lr[nseg] = trlr;
-    seg_param[nseg] = h->tr_lr != 0.0f ? h->tr_rows.as<float>() + 12ll * i : nullptr;
+    seg_param[nseg] = h->tr_lr == 0.0f ? nullptr
+                      : h->rflag[i] ? h->drparams.as<float>() + 7ll * i   // TransformR parameters
+                                    : h->tr_rows.as<float>() + 12ll * i;
     nseg++;
     (void)nv;
     if (h->opt_kind == DRH_OPT_SIMPLE) {
@@ -1144,7 +1180,8 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
     // transform used by the edge gradient stays the one of the first commit, as the
     // reference's Scene::transforms_inv is appended to and never cleared
     // (common_src/scene_utils.cpp:110-122)
-    TransformArgs ta{h->nm, h->dlayout.as<int>(), h->tr_rows.as<float>(), h->dmats.as<float>()};
+    TransformArgs ta{h->nm, h->dlayout.as<int>(), h->tr_rows.as<float>(), h->dmats.as<float>(), h->drflag.as<int>(),
+                     h->drparams.as<float>()};
     h->timed("k_transform_update", [&] { launch_transform_update(ta, h->stream); });
   }
   if (h->opt_kind == DRH_OPT_ADAM) h->timed("k_step_inc", [&] { launch_step_inc(h->opt_t.as<int>(), h->stream); });
@@ -1236,6 +1273,20 @@ drh_status drh_get_transforms(drh_handle h, float* transforms) {
       (s = h->hip(hipStreamSynchronize(h->stream), "get_transforms")))
     return s;
   for (int i = 0; i < h->nm; i++) std::memcpy(transforms + 16 * (size_t)i, &m[48 * (size_t)i], 64);
+  return DRH_OK;
+}
+
+drh_status drh_get_restricted(drh_handle h, float* params) {
+  if (!h || !params) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "get_restricted: commit first");
+  (void)hipSetDevice(h->st.device);
+  std::vector<float> p(7 * (size_t)h->nm);
+  drh_status s;
+  if ((s = h->hip(hipMemcpyAsync(p.data(), h->drparams.p, 4 * p.size(), hipMemcpyDeviceToHost, h->stream), "download")) ||
+      (s = h->hip(hipStreamSynchronize(h->stream), "get_restricted")))
+    return s;
+  for (int i = 0; i < h->nm; i++)
+    for (int k = 0; k < 7; k++) params[7 * (size_t)i + k] = h->rflag[i] ? p[7 * (size_t)i + k] : 0.0f;
   return DRH_OK;
 }
 

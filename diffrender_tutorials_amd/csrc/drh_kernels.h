@@ -197,6 +197,8 @@ struct EdgeArgs {
   const float* tr_inv0;   // 16 floats
   int lds_base, lds_n;
   int single_mesh, single_tr_off;
+  int single_id;          // the single differentiable mesh
+  float* trscr;           // non-NULL: transform gradients go here, 12 per mesh (restricted meshes present)
   int full_jacobian;      // 1: edge_grad with the full screen Jacobian (cross terms kept)
 };
 
@@ -251,6 +253,20 @@ struct TransformArgs {
   const int* layout;   // 4 per mesh; layout[4i] < 0: mesh i is not differentiable
   const float* rows;   // 12 per mesh
   float* mats;         // 48 per mesh: T, transpose(inverse(T)), identity (column-major)
+  const int* rflag;    // per mesh 1: T = TransformR::to_mat of rparams (NULL: none)
+  const float* rparams;  // 7 per mesh
+};
+
+// Transform gradients of restricted meshes: the 12 matrix entries gathered by the edge pass
+// (scr) through the Jacobian of TransformR's matrix into the mesh's 7-entry block
+// (src/drender.h:398-422); other differentiable meshes get their 12 entries added as they are.
+struct RestrictedArgs {
+  int n_meshes;
+  const int* layout;
+  const int* rflag;
+  const float* rparams;
+  const float* scr;    // 12 per mesh
+  float* grad;
 };
 
 struct HitsArgs {
@@ -291,6 +307,7 @@ void launch_optsimple(const float* grad, float* m, float* G, int off, const OptA
 void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_step_inc(int* step, hipStream_t s);
 void launch_transform_update(const TransformArgs& a, hipStream_t s);
+void launch_restricted_grad(const RestrictedArgs& a, hipStream_t s);
 void launch_debug_hits(const HitsArgs& a, int tiles, hipStream_t s);
 
 }  // namespace drh

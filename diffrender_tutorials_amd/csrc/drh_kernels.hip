@@ -1284,7 +1284,7 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
                             o0.z * dv0_dy + o1.z * dv1_dy, 1 * dv0_dy + 1 * dv1_dy,
                             o0.x * dv0_dz + o1.x * dv1_dz, o0.y * dv0_dz + o1.y * dv1_dz,
                             o0.z * dv0_dz + o1.z * dv1_dz, 1 * dv0_dz + 1 * dv1_dz};
-      float* gt = a.grad + goff + a.layout[4 * m + 3];
+      float* gt = a.trscr ? a.trscr + 12 * m : a.grad + goff + a.layout[4 * m + 3];
 #pragma unroll
       for (int q = 0; q < 12; q++) atomicAdd(gt + q, tv[q]);
     }
@@ -1318,7 +1318,7 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
   __syncthreads();
   if (a.single_mesh && threadIdx.x < 12) {
     const float v = ltr[0][threadIdx.x] + ltr[1][threadIdx.x] + ltr[2][threadIdx.x] + ltr[3][threadIdx.x];
-    if (v != 0.0f) atomicAdd(a.grad + a.single_tr_off + threadIdx.x, v);
+    if (v != 0.0f) atomicAdd((a.trscr ? a.trscr + 12 * a.single_id : a.grad + a.single_tr_off) + threadIdx.x, v);
   }
   if (use_lds)
     for (int i = threadIdx.x; i < a.lds_n; i += 256)
@@ -1373,10 +1373,34 @@ __global__ void k_transform_update(TransformArgs a) {
   if (i >= a.n_meshes || a.layout[4 * i] < 0) return;
   float* mt = a.mats + 48 * i;
   M4 T = m4_load(mt);
-  for (int r = 0; r < 3; r++)
-    for (int c = 0; c < 4; c++) T.at(r, c) = a.rows[12 * i + 4 * r + c];
+  if (a.rflag && a.rflag[i]) {
+    T = m4_restricted(a.rparams + 7 * i);
+  } else {
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 4; c++) T.at(r, c) = a.rows[12 * i + 4 * r + c];
+  }
   const M4 NT = m4_transpose(m4_inverse(T));
   for (int k = 0; k < 16; k++) { mt[k] = T.m[k]; mt[16 + k] = NT.m[k]; }
+}
+__global__ void k_restricted_grad(RestrictedArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_meshes || a.layout[4 * i] < 0) return;
+  const float* d = a.scr + 12 * i;
+  float* g = a.grad + a.layout[4 * i] + a.layout[4 * i + 3];
+  if (a.rflag[i]) {
+    float J[7][12];
+    restricted_jacobian(a.rparams + 7 * i, J);
+    for (int k = 0; k < 7; k++) {
+      float acc = 0.0f;
+      for (int l = 0; l < 12; l++) acc += J[k][l] * d[l];
+      g[k] += acc;
+    }
+  } else {
+    for (int l = 0; l < 12; l++) g[l] += d[l];
+  }
+}
+void launch_restricted_grad(const RestrictedArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_restricted_grad, dim3((a.n_meshes + 63) / 64), dim3(64), 0, s, a);
 }
 void launch_transform_update(const TransformArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_transform_update, dim3((a.n_meshes + 63) / 64), dim3(64), 0, s, a);

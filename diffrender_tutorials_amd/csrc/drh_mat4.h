@@ -71,4 +71,54 @@ __host__ __device__ inline M4 m4_inverse(const M4& a) {
   return o;
 }
 
+// TransformR::to_mat (common_src/scene.h:99-104): translate4x4(t) * rotate4x4Z(rz) *
+// rotate4x4Y(ry) * rotate4x4X(rx) * scale4x4(s), multiplied left to right, for
+// x = (tx, ty, tz, rx, ry, rz, s) (LiteMath.h:1474-1519).
+__host__ __device__ inline M4 m4_restricted(const float* x) {
+  M4 T = m4_identity(), Rx = m4_identity(), Ry = m4_identity(), Rz = m4_identity(), S = m4_identity();
+  T.m[12] = x[0]; T.m[13] = x[1]; T.m[14] = x[2];
+  Rx.m[5] = cosf(x[3]); Rx.m[6] = sinf(x[3]); Rx.m[9] = -sinf(x[3]); Rx.m[10] = cosf(x[3]);
+  Ry.m[0] = cosf(x[4]); Ry.m[2] = -sinf(x[4]); Ry.m[8] = sinf(x[4]); Ry.m[10] = cosf(x[4]);
+  Rz.m[0] = cosf(x[5]); Rz.m[1] = sinf(x[5]); Rz.m[4] = -sinf(x[5]); Rz.m[5] = cosf(x[5]);
+  S.m[0] = x[6]; S.m[5] = x[6]; S.m[10] = x[6];
+  return m4_mul(m4_mul(m4_mul(m4_mul(T, Rz), Ry), Rx), S);
+}
+
+// Jacobian of the 3x4 row-major matrix (the order of the 12 transform gradient entries) with
+// respect to x, closed form: the matrix is s * Rz Ry Rx in columns 0-2 and t in column 3
+// (the function src/drender_autodiff.cpp:78-92 differentiates with Enzyme).  J[k][4r + c].
+__host__ __device__ inline void restricted_jacobian(const float* x, float J[7][12]) {
+  const float cx = cosf(x[3]), sx = sinf(x[3]), cy = cosf(x[4]), sy = sinf(x[4]);
+  const float cz = cosf(x[5]), sz = sinf(x[5]), s = x[6];
+  const float Rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
+  const float Ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
+  const float Rz[3][3] = {{cz, -sz, 0}, {sz, cz, 0}, {0, 0, 1}};
+  const float dRx[3][3] = {{0, 0, 0}, {0, -sx, -cx}, {0, cx, -sx}};
+  const float dRy[3][3] = {{-sy, 0, cy}, {0, 0, 0}, {-cy, 0, -sy}};
+  const float dRz[3][3] = {{-sz, -cz, 0}, {cz, -sz, 0}, {0, 0, 0}};
+  auto mul3 = [](const float (*a)[3], const float (*b)[3], float (*o)[3]) {
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) o[r][c] = a[r][0] * b[0][c] + a[r][1] * b[1][c] + a[r][2] * b[2][c];
+  };
+  float YX[3][3], dYX[3][3], YdX[3][3], R[3][3], Drx[3][3], Dry[3][3], Drz[3][3];
+  mul3(Ry, Rx, YX);
+  mul3(Rz, YX, R);
+  mul3(dRy, Rx, dYX);
+  mul3(Ry, dRx, YdX);
+  mul3(Rz, YdX, Drx);
+  mul3(Rz, dYX, Dry);
+  mul3(dRz, YX, Drz);
+  for (int k = 0; k < 7; k++)
+    for (int l = 0; l < 12; l++) J[k][l] = 0.0f;
+  for (int r = 0; r < 3; r++) {
+    J[r][4 * r + 3] = 1.0f;
+    for (int c = 0; c < 3; c++) {
+      J[3][4 * r + c] = s * Drx[r][c];
+      J[4][4 * r + c] = s * Dry[r][c];
+      J[5][4 * r + c] = s * Drz[r][c];
+      J[6][4 * r + c] = R[r][c];
+    }
+  }
+}
+
 }  // namespace drh

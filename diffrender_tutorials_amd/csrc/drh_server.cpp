@@ -352,6 +352,10 @@ drh_status drh_server_submit(drh_server s, const drh_job_desc* d, int* job_id) {
     return DRH_E_INVALID;
   const drh_scene_desc& sc = d->scene;
   if (sc.n_meshes <= 0 || !sc.vcount || !sc.fcount || !sc.transforms || !sc.pos || !sc.idx) return DRH_E_INVALID;
+  // jobs fit vertex positions and colours; RESTRICTED_TRANSFORM meshes go through a handle
+  if (sc.restricted)
+    for (int i = 0; i < sc.n_meshes; i++)
+      if (sc.restricted[i]) return DRH_E_UNSUPPORTED;
   auto j = std::make_unique<Job>();
   j->st = d->settings;
   j->n_meshes = sc.n_meshes;

@@ -34,7 +34,7 @@ STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params",
-           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples",
+           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_restricted", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples",
            "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
@@ -68,7 +68,8 @@ class SceneDesc(ctypes.Structure):
                 ("tc", ctypes.POINTER(ctypes.c_float)), ("nrm", ctypes.POINTER(ctypes.c_float)),
                 ("idx", ctypes.POINTER(ctypes.c_uint32)), ("tex_w", ctypes.POINTER(ctypes.c_int)),
                 ("tex_h", ctypes.POINTER(ctypes.c_int)), ("texels", ctypes.POINTER(ctypes.c_float)),
-                ("differentiable", ctypes.POINTER(ctypes.c_int))]
+                ("differentiable", ctypes.POINTER(ctypes.c_int)), ("restricted", ctypes.POINTER(ctypes.c_int)),
+                ("restricted_params", ctypes.POINTER(ctypes.c_float))]
 
 
 _lib = None
@@ -238,6 +239,12 @@ class DiffRender:
         self._check(load().drh_get_transforms(self.h, _fp(out)))
         return out.reshape(-1, 16)
 
+    def get_restricted(self):
+        """The current TransformR parameters, (n_meshes, 7): translate, rotate, scale."""
+        out = np.empty(7 * self.n_meshes, np.float32)
+        self._check(load().drh_get_restricted(self.h, _fp(out)))
+        return out.reshape(-1, 7)
+
     def get_texels(self, n):
         out = np.empty(n, np.float32)
         self._check(load().drh_get_texels(self.h, _fp(out)))
@@ -341,9 +348,16 @@ def scene_desc(meshes):
     texs = [np.asarray(m.tex, np.float32).reshape(-1) for m in meshes if m.tex is not None]
     tex = np.ascontiguousarray(np.concatenate(texs)) if texs else None
     df = np.array([1 if m.differentiable else 0 for m in meshes], np.int32)
+    rs = getattr  # RESTRICTED_TRANSFORM meshes carry 7 parameters in Mesh.restricted
+    rf = np.array([0 if rs(m, "restricted", None) is None else 1 for m in meshes], np.int32)
+    rp = np.ascontiguousarray(np.concatenate([np.zeros(7, np.float32) if rs(m, "restricted", None) is None
+                                              else np.asarray(m.restricted, np.float32).reshape(7) for m in meshes]))
+    if not rf.any():
+        rf, rp = None, None
     d = SceneDesc(n, _ip(vc), _ip(fc), _fp(tr), _fp(pos), _fp(col), _fp(tc), _fp(nrm),
-                  idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ip(tw), _ip(th), _fp(tex), _ip(df))
-    return d, (vc, fc, tr, pos, col, tc, nrm, idx, tw, th, tex, df)
+                  idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _ip(tw), _ip(th), _fp(tex), _ip(df),
+                  _ip(rf), _fp(rp))
+    return d, (vc, fc, tr, pos, col, tc, nrm, idx, tw, th, tex, df, rf, rp)
 
 
 class JobDesc(ctypes.Structure):

@@ -27,6 +27,10 @@ class Mesh:
     transform: np.ndarray = dataclasses.field(
         default_factory=lambda: np.eye(4, dtype=np.float32).reshape(-1))  # 16 floats, column-major
     differentiable: bool = True
+    # RESTRICTED_TRANSFORM mesh (TransformR, common_src/scene.h:91-116): 7 float32 parameters
+    # translate xyz, rotate xyz (radians), scale; the transform is their matrix and the
+    # gradient holds their 7 derivatives.  None: a plain 4x4 transform.
+    restricted: np.ndarray | None = None
 
     @property
     def V(self) -> int:

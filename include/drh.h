@@ -85,6 +85,14 @@ typedef struct {
   const int* tex_h;
   const float* texels;
   const int* differentiable; /* per mesh 0/1 (the DScene's differentiable mesh ids) */
+  /* Optional (NULL: none).  restricted[i] != 0 makes mesh i a RESTRICTED_TRANSFORM mesh
+   * (Scene::add_mesh with TransformR, common_src/scene.h:91-145): its transform is
+   * TransformR::to_mat of the 7 values restricted_params[7i..7i+6] = translate xyz, rotate xyz
+   * (radians), scale (its transforms[] entry is ignored), and its gradient block holds those
+   * 7 parameters' derivatives instead of the 12 matrix entries (DMesh, src/dmesh.h:35-85;
+   * src/drender.h:398-422). */
+  const int* restricted;
+  const float* restricted_params;
 } drh_scene_desc;
 
 /* MakeDifferentialRenderer + IDiffRender::init (src/drender.cpp:89-147, src/drender.h:79-88). */
@@ -154,6 +162,9 @@ drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr);
 drh_status drh_opt_set_transform_lr(drh_handle h, float lr);
 /* Download the current (optimised) transforms: 16 floats per mesh, column-major. */
 drh_status drh_get_transforms(drh_handle h, float* transforms);
+/* Download the current (optimised) restricted-transform parameters: 7 floats per mesh
+ * (translate, rotate, scale); zeros for meshes that are not RESTRICTED_TRANSFORM meshes. */
+drh_status drh_get_restricted(drh_handle h, float* params);
 /* Interior samples (interior_derivatives, src/drender.h:197-233).  aligned = 0 (default): the
  * reference's Hammersley offsets, so every interior sample is a fresh ray (recompute).
  * aligned = 1: the render's grid offsets (d + 0.5)/sqrt(spp) instead -- another unbiased

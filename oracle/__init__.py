@@ -26,6 +26,11 @@ Parity status of each function (how it is pinned):
   * orc_silhouette_functional / orc_fd_silhouette (FP64 box-filtered silhouette and its
     central differences): pinned by the closed-form area of a cube's front face and by
     analytic derivatives of that area (tests/test_oracle.py).
+  * restricted_matrix / restricted_jacobian (TransformR::to_mat and the Jacobian the
+    reference takes with Enzyme, which this image lacks, so the reference cannot be run
+    there): the matrix is pinned against LiteMath's product of its factors at special
+    angles and the Jacobian against FP64 central differences of the matrix
+    (tests/test_oracle.py).
 """
 from __future__ import annotations
 
@@ -243,3 +248,40 @@ def optsimple_step(grad, m, G, block_start, block_lr):
 def adam_step(param, grad, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     lib().orc_adam_step(param.size, _p(param), _p(np.ascontiguousarray(grad, np.float32)), _p(m), _p(v),
                         step, ctypes.c_float(lr), ctypes.c_float(b1), ctypes.c_float(b2), ctypes.c_float(eps))
+
+
+def restricted_matrix(x):
+    """TransformR::to_mat (common_src/scene.h:99-104) in FP64: translate(t) * rotZ(rz) *
+    rotY(ry) * rotX(rx) * scale(s) with LiteMath's rotation conventions (LiteMath.h:1491-1519);
+    returns the 4x4 matrix (row-major)."""
+    x = np.asarray(x, np.float64)
+    cx, sx, cy, sy, cz, sz = np.cos(x[3]), np.sin(x[3]), np.cos(x[4]), np.sin(x[4]), np.cos(x[5]), np.sin(x[5])
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    M = np.eye(4)
+    M[:3, :3] = Rz @ Ry @ Rx * x[6]
+    M[:3, 3] = x[:3]
+    return M
+
+
+def restricted_jacobian(x):
+    """d(rows 0-2 of the matrix, row-major 12)/dx, 7x12 in FP64: the Jacobian
+    dmat_dtransforms_jac (src/drender_autodiff.cpp:78-107) computes with Enzyme, written out:
+    rows 0-2 of column 3 are t, columns 0-2 are s * Rz Ry Rx."""
+    x = np.asarray(x, np.float64)
+    cx, sx, cy, sy, cz, sz = np.cos(x[3]), np.sin(x[3]), np.cos(x[4]), np.sin(x[4]), np.cos(x[5]), np.sin(x[5])
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    dRx = np.array([[0, 0, 0], [0, -sx, -cx], [0, cx, -sx]])
+    dRy = np.array([[-sy, 0, cy], [0, 0, 0], [-cy, 0, -sy]])
+    dRz = np.array([[-sz, -cz, 0], [cz, -sz, 0], [0, 0, 0]])
+    J = np.zeros((7, 3, 4))
+    for r in range(3):
+        J[r, r, 3] = 1.0
+    J[3, :, :3] = x[6] * (Rz @ Ry @ dRx)
+    J[4, :, :3] = x[6] * (Rz @ dRy @ Rx)
+    J[5, :, :3] = x[6] * (dRz @ Ry @ Rx)
+    J[6, :, :3] = Rz @ Ry @ Rx
+    return J.reshape(7, 12)

@@ -13,6 +13,7 @@
 // Input (little endian):
 //   int32 magic(0x44524831), mode, spp, n_meshes, n_views, edge_samples, ops, opt_iters
 //   if ops & OP_OPT_ALL: float32 position_lr, textures_lr, transforms_lr (flags OPT_ALL)
+//   if ops & OP_TRANSFORMR: int32 n, float32 params[7n] (TransformR::to_mat of each)
 //   per mesh: int32 nv, nf, has_col, has_tc, has_nrm, tex_w, tex_h, diff;
 //             float32 transform[16]; pos[3nv]; [col[3nv]]; [tc[2nv]]; [nrm[3nv]]; uint32 idx[3nf];
 //             float32 tex[tex_w*tex_h*3]
@@ -34,13 +35,13 @@ using namespace diff_render;
 
 enum {
   OP_RENDER = 1, OP_HITS = 2, OP_DRENDER = 4, OP_COMPARE = 8, OP_SEQ = 16, OP_EDGES = 32, OP_OPTIMIZE = 64,
-  OP_OPT_ALL = 128,
+  OP_OPT_ALL = 128, OP_TRANSFORMR = 256,
 };
 enum {
   T_IMAGES = 1, T_HIT_PRIM = 2, T_HIT_GEOM = 3, T_HIT_T = 4, T_HIT_U = 5, T_HIT_V = 6, T_GRAD = 7, T_LOSS = 8,
   T_QMC = 9, T_PRNG = 10, T_HAMM = 11, T_EDGE_V0 = 12, T_EDGE_V1 = 13, T_EDGE_PMF = 14, T_EDGE_CDF = 15,
   T_COMPARE_GRAD = 16, T_COMPARE_IMAGES = 17, T_PROJ = 18, T_OPT_POS = 19, T_OPT_COL = 20, T_OPT_ERR = 21,
-  T_PROJ_INV = 22, T_WV_INV = 23, T_WVP = 24, T_OPT_TR = 25, T_OPT_TEX = 26,
+  T_PROJ_INV = 22, T_WV_INV = 23, T_WVP = 24, T_OPT_TR = 25, T_OPT_TEX = 26, T_TRANSFORMR = 27,
 };
 
 static FILE* g_in;
@@ -63,6 +64,19 @@ int main(int argc, char** argv) {
   const int edge_samples = rd<int32_t>(), ops = rd<int32_t>(), opt_iters = rd<int32_t>();
   float lr_pos = 0.1f, lr_tex = 0.1f, lr_tfm = 0.1f;
   if (ops & OP_OPT_ALL) { lr_pos = rd<float>(); lr_tex = rd<float>(); lr_tfm = rd<float>(); }
+  if (ops & OP_TRANSFORMR) {
+    // TransformR::to_mat of each 7-parameter set (translate, rotate, scale), float4x4 memory
+    const int n = rd<int32_t>();
+    std::vector<float> out;
+    for (int k = 0; k < n; k++) {
+      std::vector<float> x;
+      rdv(x, 7);
+      const TransformR t(float3(x[0], x[1], x[2]), float3(x[3], x[4], x[5]), x[6]);
+      const float4x4 m = t.to_mat();
+      out.insert(out.end(), (const float*)&m, (const float*)&m + 16);
+    }
+    put(T_TRANSFORMR, out.data(), out.size() * 4);
+  }
 
   Scene scene;
   std::vector<int> diff_ids;

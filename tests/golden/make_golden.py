@@ -207,6 +207,20 @@ def opt_all_case(name):
     print("wrote", name, {k: v.shape for k, v in out.items() if isinstance(v, np.ndarray)})
 
 
+def transformr_vectors():
+    """TransformR::to_mat (common_src/scene.h:99-104) of the reference on a few parameter sets:
+    identity, the initial transform of the reference's restricted-transform test
+    (src/tests.cpp:350-361), quarter turns and random sets."""
+    rng = np.random.default_rng(17)
+    x = [[0, 0, 0, 0, 0, 0, 1], [0, 0, 0, 0.0, 0.2, 0.2, 1], [0.1, -0.2, 0.3, 1.5707964, 0, 0, 1],
+         [0, 0, 0, 0, 1.5707964, 0, 2], [0, 0, 0, 0, 0, 1.5707964, 0.5]]
+    x += [list(rng.uniform(-1, 1, 6)) + [float(rng.uniform(0.5, 2))] for _ in range(6)]
+    x = np.array(x, np.float32)
+    r = refio.run([], [], 1, 1, 0, transformr=x)
+    np.savez_compressed(os.path.join(HERE, "transformr.npz"), params=x, to_mat=r["transformr"].reshape(-1, 16))
+    print("wrote transformr")
+
+
 def main(only=None):
     """Write every fixture, or only those named in `only`."""
     if not os.path.exists(refio.HARNESS):
@@ -218,6 +232,8 @@ def main(only=None):
         adam_vectors()
     if want("meshio_files"):
         meshio_files()
+    if want("transformr"):
+        transformr_vectors()
     import golden_cases as GC
     for name in GC.OPT_ALL_CASES:
         if want(name):

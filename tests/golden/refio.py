@@ -14,21 +14,28 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 HARNESS = os.path.join(HERE, "..", "..", "oracle", "_ref", "ref_harness")
 
 OP_RENDER, OP_HITS, OP_DRENDER, OP_COMPARE, OP_SEQ, OP_EDGES, OP_OPTIMIZE, OP_OPT_ALL = 1, 2, 4, 8, 16, 32, 64, 128
+OP_TRANSFORMR = 256
 TAGS = {1: "images", 2: "hit_prim", 3: "hit_geom", 4: "hit_t", 5: "hit_u", 6: "hit_v", 7: "grad", 8: "loss",
         9: "qmc", 10: "prng", 11: "hamm", 12: "edge_v0", 13: "edge_v1", 14: "edge_pmf", 15: "edge_cdf",
         16: "compare_grad", 17: "compare_images", 18: "proj", 19: "opt_pos", 20: "opt_col", 21: "opt_err",
-        22: "proj_inv", 23: "wv_inv", 24: "wvp", 25: "opt_tr", 26: "opt_tex"}
+        22: "proj_inv", 23: "wv_inv", 24: "wvp", 25: "opt_tr", 26: "opt_tex", 27: "transformr"}
 INT_TAGS = {"hit_prim", "hit_geom", "qmc", "edge_v0", "edge_v1"}
 
 
-def run(meshes, cams, mode, spp, ops, edge_samples=0, images=None, opt_iters=0, opt_lrs=None):
-    """opt_lrs = (position_lr, textures_lr, transforms_lr) runs the optimiser with OPT_ALL."""
+def run(meshes, cams, mode, spp, ops, edge_samples=0, images=None, opt_iters=0, opt_lrs=None, transformr=None):
+    """opt_lrs = (position_lr, textures_lr, transforms_lr) runs the optimiser with OPT_ALL;
+    transformr = (n, 7) parameter sets returns TransformR::to_mat of each."""
     buf = bytearray()
     if opt_lrs is not None:
         ops |= OP_OPT_ALL
     buf += struct.pack("<8i", 0x44524831, mode, spp, len(meshes), len(cams), edge_samples, ops, opt_iters)
     if opt_lrs is not None:
         buf += struct.pack("<3f", *opt_lrs)
+    if transformr is not None:
+        # the harness reads this block right after the header (and the rates)
+        buf[24:28] = struct.pack("<i", ops | OP_TRANSFORMR)
+        x = np.asarray(transformr, np.float32).reshape(-1, 7)
+        buf += struct.pack("<i", x.shape[0]) + x.tobytes()
     for m in meshes:
         tw = 0 if m.tex is None else m.tex.shape[1]
         th = 0 if m.tex is None else m.tex.shape[0]

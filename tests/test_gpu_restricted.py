@@ -1,0 +1,160 @@
+"""RESTRICTED_TRANSFORM meshes (TransformR: translate, rotate, scale; common_src/scene.h:91-116).
+
+The reference differentiates TransformR's matrix with Enzyme (src/drender_autodiff.cpp:78-107,
+src/drender.h:398-422), which this image lacks, so its restricted-transform gradient cannot be
+run here.  What is pinned instead:
+  * the matrix: the commit's TransformR::to_mat equals the reference's, bit for bit, on golden
+    parameter sets (tests/golden/transformr.npz);
+  * the 7-entry gradient: J(x)^T times the 12 matrix-entry gradient of the same scene committed
+    with the plain matrix, where J is the oracle's closed-form Jacobian (itself pinned against
+    FP64 central differences in tests/test_oracle.py) and the 12 entries come from the oracle's
+    d_render (1e-3 relative L2) and from this library's own plain-matrix pass (1e-5);
+  * the optimiser: OptSimple steps on the 7 parameters (transforms_lr) against the same steps
+    composed on the host from oracle gradients.
+No convergence test: the reference's brute-force tracer applies a mesh's transform to the
+already transformed vertices again (raytrace_src/raytrace3d_bf.cpp:116-125), kept here for
+parity, so the image of a non-identity transform T shows T*T while the edge gradient follows
+T, and a fit of the reference's restricted-transform test (src/tests.cpp:350-361) need not
+converge."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from diffrender_tutorials_amd import drh, scenes as S
+from golden_cases import load, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+X0 = np.array([0.0, 0.0, 0.0, 0.0, 0.2, 0.2, 1.0], np.float32)   # src/tests.cpp:356
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pyramid():
+    """A square pyramid (the shape of the reference's restricted-transform test)."""
+    pos = np.array([[-0.5, -0.4, -0.5], [0.5, -0.4, -0.5], [0.5, -0.4, 0.5], [-0.5, -0.4, 0.5], [0.0, 0.5, 0.0]],
+                   np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2], [0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4]], np.uint32)
+    return S.Mesh(pos=pos, idx=idx)
+
+
+def _restricted(m, x):
+    out = m.copy()
+    out.restricted = np.asarray(x, np.float32).copy()
+    return out
+
+
+def _plain(m, T):
+    out = m.copy()
+    out.restricted = None
+    out.transform = np.asarray(T, np.float32).reshape(16).copy()
+    return out
+
+
+def _to_mat(x):
+    """The commit's matrix of x (TransformR::to_mat, float)."""
+    r = drh.DiffRender(mode=drh.SILHOUETTE, spp=1)
+    try:
+        r.commit([_restricted(_pyramid(), x)])
+        return r.get_transforms()[0]
+    finally:
+        r.close()
+
+
+def test_commit_matrix_is_the_reference_to_mat():
+    fx = load("transformr")
+    for x, m in zip(fx["params"], fx["to_mat"]):
+        T = _to_mat(x)
+        assert np.array_equal(T, m), (x, np.abs(T - m).max())
+
+
+@pytest.mark.parametrize("x", [X0, np.array([0.05, -0.05, 0.02, 0.1, -0.15, 0.12, 1.1], np.float32)])
+def test_restricted_gradient_is_the_jacobian_of_the_matrix_gradient(x):
+    cam = S.camera_look((0.6, 0.8, 2.6), 128, 128)
+    base = _pyramid()
+    T = _to_mat(x)
+    tgt = O.Scene([_plain(base, S.translate((0.05, 0.02, 0)))]).render(1, 4, [cam])[0]
+    N = 1 << 15
+    out = {}
+    for name, m in (("restricted", _restricted(base, x)), ("plain", _plain(base, T))):
+        r = drh.DiffRender(mode=drh.SILHOUETTE, spp=4)
+        try:
+            r.commit([m])
+            _, g, _ = r.d_render_and_compare([cam], [tgt], N, images=False)
+            out[name] = g
+        finally:
+            r.close()
+    nv3 = 3 * base.V
+    gr, gp = out["restricted"], out["plain"]
+    assert gr.size == nv3 + 7 and gp.size == nv3 + 12
+    J = O.restricted_jacobian(x)
+    want_self = J @ gp[nv3:].astype(np.float64)
+    sc = O.Scene([_plain(base, T)], O.TRACER_BVH)
+    img = sc.render(1, 4, [cam])[0]
+    _, adj = O.loss(img, tgt)
+    go = sc.d_render(1, 4, [cam], [adj], N)
+    want_oracle = J @ go[nv3:].astype(np.float64)
+    e_self = rel_l2(gr[nv3:], want_self)
+    e_oracle = rel_l2(gr[nv3:], want_oracle)
+    say("x", x, "restricted grad", gr[nv3:], "rel L2 vs J^T plain", e_self, "vs J^T oracle", e_oracle)
+    assert np.abs(want_oracle).sum() > 0
+    assert e_self < 1e-5
+    assert e_oracle < 1e-3
+    assert rel_l2(gr[:nv3], gp[:nv3]) < 1e-5
+
+
+def _host_steps(base, x, tgt, cam, iters, lr):
+    """OptSimple (GD_Adam) on the 7 parameters composed from oracle gradients (flags with
+    _OPT_TFM only: position rate 0)."""
+    x = np.asarray(x, np.float32).copy()
+    nv3 = 3 * base.V
+    G = nv3 + 7
+    mom = np.zeros(G, np.float32)
+    gsq = np.zeros(G, np.float32)
+    for _ in range(iters):
+        T = _to_mat(x)
+        sc = O.Scene([_plain(base, T)], O.TRACER_BVH)
+        img = sc.render(1, 4, [cam])[0]
+        _, adj = O.loss(img, tgt)
+        g12 = sc.d_render(1, 4, [cam], [adj], cam.width * cam.height)
+        g = np.concatenate([g12[:nv3], (O.restricted_jacobian(x) @ g12[nv3:].astype(np.float64)).astype(np.float32)])
+        step = O.optsimple_step(g, mom, gsq, [0, nv3], [0.0, lr])
+        x = (x - step[nv3:]).astype(np.float32)
+    return x
+
+
+def test_optimiser_moves_the_restricted_parameters_as_the_reference_step():
+    cam = S.camera_look((0.6, 0.8, 2.6), 96, 96)
+    base = _pyramid()
+    tgt = O.Scene([base]).render(1, 4, [cam])[0]
+    lr = 0.05
+    r = drh.DiffRender(mode=drh.SILHOUETTE, spp=4)
+    xs = []
+    try:
+        r.commit([_restricted(base, X0)])
+        r.opt_init(drh.OPT_SIMPLE, 0.0, 0.1)
+        r.opt_set_transform_lr(lr)
+        dt = r.device_array(tgt.reshape(-1))
+        dl = r.device_array(np.zeros(1, np.float32))
+        cs = drh.cameras_array([cam])
+        for _ in range(2):
+            r.opt_step_dev(cs, 1, dt.ptr, cam.width * cam.height, dl.ptr)
+            xs.append(r.get_restricted()[0].copy())
+        T = r.get_transforms()[0]
+        pos, _ = r.get_params()
+    finally:
+        r.close()
+    for k, xg in enumerate(xs):
+        xh = _host_steps(base, X0, tgt, cam, k + 1, lr)
+        say("step", k + 1, "device", xg, "host", xh)
+        assert np.abs(xg - xh).max() < 1e-5 * (k + 1)
+    assert np.abs(xs[0] - X0).max() > 0.5 * lr * 0.75        # the parameters did move
+    assert np.array_equal(pos, base.pos)                       # position rate 0
+    assert np.abs(T - _to_mat(xs[-1])).max() < 1e-6            # the device matrix follows them

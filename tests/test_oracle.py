@@ -210,6 +210,28 @@ def test_optsimple_opt_all_matches_reference(name):
         assert np.abs(fx["opt_tr"][:16] - meshes[0].transform).max() > 0.05   # the transform did move
 
 
+def test_restricted_matrix_matches_reference_to_mat():
+    """TransformR::to_mat of the reference (float) on golden parameter sets."""
+    fx = load("transformr")
+    for x, m in zip(fx["params"], fx["to_mat"]):
+        M = O.restricted_matrix(x)
+        assert np.abs(M - m.reshape(4, 4).T).max() <= 3e-7 * (1 + np.abs(x).max()), x
+
+
+def test_restricted_jacobian_matches_central_differences():
+    """The closed-form Jacobian against FP64 central differences of the matrix."""
+    rng = np.random.default_rng(3)
+    xs = [np.array([0, 0, 0, 0, 0.2, 0.2, 1.0])] + [np.r_[rng.uniform(-2, 2, 6), rng.uniform(0.3, 3)] for _ in range(8)]
+    h = 1e-6
+    for x in xs:
+        J = O.restricted_jacobian(x)
+        for k in range(7):
+            e = np.zeros(7)
+            e[k] = h
+            fd = (O.restricted_matrix(x + e)[:3] - O.restricted_matrix(x - e)[:3]).reshape(-1) / (2 * h)
+            assert np.abs(J[k] - fd).max() < 1e-8, (x, k)
+
+
 def test_adam_matches_torch_vectors():
     fx = load("adam_torch")
     p = fx["p0"].copy()

@@ -158,3 +158,37 @@ def test_optimiser_moves_the_restricted_parameters_as_the_reference_step():
     assert np.abs(xs[0] - X0).max() > 0.5 * lr * 0.75        # the parameters did move
     assert np.array_equal(pos, base.pos)                       # position rate 0
     assert np.abs(T - _to_mat(xs[-1])).max() < 1e-6            # the device matrix follows them
+
+
+def test_restricted_gradient_matches_fd_of_the_silhouette():
+    """At the identity parameters (where the tracer's doubled transform is still the identity)
+    the 7-entry gradient against FP64 central differences, over the parameters, of the
+    box-filtered silhouette of the transformed cube (the functional of src/fin_diff.cpp's check,
+    exact per-pixel coverage from the oracle); axis-aligned camera, so the reference's diagonal
+    screen Jacobian is exact."""
+    m = S.cube(0.5, seed=1)
+    cam = S.camera_front(128, 128)
+    yy, xx = np.mgrid[0:128, 0:128].astype(np.float32)
+    a = (1.0 + 0.5 * np.sin(xx * (7.0 / 128)) * np.cos(yy * (11.0 / 128))).astype(np.float32)
+    adj = np.repeat(a[..., None] / 3.0, 3, axis=2)
+    x0 = np.array([0, 0, 0, 0, 0, 0, 1], np.float64)
+
+    def L(x):
+        M = O.restricted_matrix(x)
+        mm = m.copy()
+        mm.pos = (np.c_[m.pos.astype(np.float64), np.ones(m.V)] @ M.T)[:, :3].astype(np.float32)
+        return O.Scene([mm], O.TRACER_BF).silhouette_functional(cam, adj)
+
+    h = 1e-3
+    fd = np.array([(L(x0 + h * e) - L(x0 - h * e)) / (2 * h) for e in np.eye(7)])
+    r = drh.DiffRender(mode=drh.SILHOUETTE, spp=1)
+    try:
+        r.commit([_restricted(m, x0)])
+        g = r.d_render([cam], [adj], 1 << 20)
+    finally:
+        r.close()
+    gr = g[3 * m.V:]
+    e = rel_l2(gr, fd)
+    say("restricted grad", gr, "fd", fd, "rel L2", e)
+    assert np.abs(fd).max() > 0
+    assert e < 0.02

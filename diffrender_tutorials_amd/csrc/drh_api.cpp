@@ -281,7 +281,8 @@ struct drh_renderer : BufOwner {
 
   // device scene
   DBuf pos_orig, pos, col, tc, nrm_in, nrm, nrm2d, texels, idx, tri_mesh, mesh_vo, mesh_to, dtex_w, dtex_h,
-      dtex_off, dmats, dlayout, dtr_inv0, tr_rows, drflag, drparams, trscr;
+      dtex_off, dmats, dlayout, dtr_inv0, tr_rows, drflag, drparams, trscr, dacc;
+  bool deterministic = false;   // fixed-point gradient sums (drh_set_deterministic)
   // RESTRICTED_TRANSFORM meshes: flag and the 7 parameters per mesh
   std::vector<int> rflag;
   std::vector<float> rparams_h;
@@ -801,6 +802,15 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   // then k_restricted_grad maps them into the gradient blocks
   const bool tr_apart = any_restricted && E > 0 && N > 0;
   if (tr_apart) timed("k_fill_f32", [&] { launch_fill_f32(trscr.as<float>(), 12ll * nm, 0.0f, stream); });
+  // deterministic mode: every gradient term into 32.32 fixed-point integer sums (G floats,
+  // then the 12 transform entries per mesh gathered apart), converted once at the end
+  unsigned long long* dacc_p = nullptr;
+  if (deterministic && G > 0) {
+    const size_t nacc = (size_t)G + 12 * (size_t)nm;
+    if ((s = hip(dacc.reserve(8 * nacc), "alloc"))) return s;
+    dacc_p = dacc.as<unsigned long long>();
+    timed("k_fill_u32", [&] { launch_fill_u32(dacc.as<uint32_t>(), 2ll * (long long)nacc, 0u, stream); });
+  }
   // single differentiable mesh: position / transform gradient offsets
   int single = -1;
   for (int i = 0; i < nm; i++) if (diff[i]) { single = (single == -1) ? i : -2; }
@@ -847,6 +857,11 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         blocks = num_regions(sq, c.w, c.h, ia.region);
       }
       if (ia.region == 0) ia.region = 1;
+      if (dacc_p) {
+        // no LDS sums or hash tables (their float atomics would make the order matter)
+        ia.dacc = dacc_p; ia.lds_base = 0; ia.lds_n = 0; ia.hash = 0; ia.region = 1;
+        if (!ia.uniform) blocks = tiles;
+      }
       timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
     }
     if (E > 0 && N > 0) {
@@ -883,11 +898,17 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
       ea.single_id = single >= 0 ? single : 0;
       ea.trscr = tr_apart ? trscr.as<float>() : nullptr;
+      ea.dacc = dacc_p; ea.dG = G;
+      if (dacc_p) { ea.lds_base = 0; ea.lds_n = 0; }
       ea.ecol = ecol.as<float4>();
       timed("k_edge_trace", [&] { launch_edge_trace(st.mode, ea, stream); });
       timed("k_edge_grad", [&] { launch_edge_grad(ea, stream); });
     }
   }
+  if (dacc_p)
+    timed("k_det_finish", [&] {
+      launch_det_finish(dacc_p, G, d_grad, tr_apart ? trscr.as<float>() : nullptr, tr_apart ? 12 * nm : 0, stream);
+    });
   if (tr_apart) {
     RestrictedArgs ra{nm, dlayout.as<int>(), drflag.as<int>(), drparams.as<float>(), trscr.as<float>(), d_grad};
     timed("k_restricted_grad", [&] { launch_restricted_grad(ra, stream); });
@@ -1273,6 +1294,13 @@ drh_status drh_get_transforms(drh_handle h, float* transforms) {
       (s = h->hip(hipStreamSynchronize(h->stream), "get_transforms")))
     return s;
   for (int i = 0; i < h->nm; i++) std::memcpy(transforms + 16 * (size_t)i, &m[48 * (size_t)i], 64);
+  return DRH_OK;
+}
+
+drh_status drh_set_deterministic(drh_handle h, int on) {
+  if (!h || (on != 0 && on != 1)) return DRH_E_INVALID;
+  if ((on != 0) != h->deterministic) h->drop_graph();
+  h->deterministic = on != 0;
   return DRH_OK;
 }
 

@@ -165,6 +165,7 @@ struct InteriorArgs {
   const float4* hits;     // non-NULL (aligned only): the render's stored hits instead of tracing
   int hash;               // entries of the per-wave LDS hash table (0: atomics straight to grad)
   int region;             // tiles per side of a wave's region (1: one tile)
+  unsigned long long* dacc;  // deterministic mode: 32.32 fixed-point sums per gradient float (else NULL)
 };
 
 struct ProjectArgs {
@@ -199,6 +200,8 @@ struct EdgeArgs {
   int single_mesh, single_tr_off;
   int single_id;          // the single differentiable mesh
   float* trscr;           // non-NULL: transform gradients go here, 12 per mesh (restricted meshes present)
+  unsigned long long* dacc;  // deterministic mode: 32.32 fixed-point sums, G for grad then 12 per mesh for trscr
+  int dG;                 // G: where trscr's entries start in dacc
   int full_jacobian;      // 1: edge_grad with the full screen Jacobian (cross terms kept)
 };
 
@@ -308,6 +311,8 @@ void launch_adam(const AdamArgs& a, hipStream_t s);
 void launch_step_inc(int* step, hipStream_t s);
 void launch_transform_update(const TransformArgs& a, hipStream_t s);
 void launch_restricted_grad(const RestrictedArgs& a, hipStream_t s);
+// Deterministic mode: grad[i] += dacc[i] / 2^32 (i < G); trscr[j] = dacc[G + j] / 2^32.
+void launch_det_finish(const unsigned long long* dacc, int G, float* grad, float* trscr, int ntr, hipStream_t s);
 void launch_debug_hits(const HitsArgs& a, int tiles, hipStream_t s);
 
 }  // namespace drh

@@ -567,6 +567,26 @@ __device__ __forceinline__ void hash_add(int* hkey, float* hval, int nh, float* 
   atomicAdd(grad + key, x); atomicAdd(grad + key + 1, y); atomicAdd(grad + key + 2, z);
 }
 
+// Deterministic mode: a gradient term as a 32.32 fixed-point integer added with a 64-bit
+// integer atomic.  Integer addition is associative, so the sums do not depend on the order
+// the waves arrive in and repeated runs agree bit for bit.  v * 2^32 is exact in float and its
+// conversion exact for |v| >= 2^-9; smaller terms are rounded to a multiple of 2^-32.  Every
+// partial sum must stay below 2^31 in magnitude.
+__device__ __forceinline__ void dadd(unsigned long long* acc, int off, float v) {
+  atomicAdd(acc + off, (unsigned long long)__float2ll_rn(v * 4294967296.0f));
+}
+
+__global__ void k_det_finish(const unsigned long long* acc, int G, float* grad, float* trscr, int ntr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G + ntr) return;
+  const float v = (float)((double)(long long)acc[i] * (1.0 / 4294967296.0));
+  if (i < G) grad[i] += v;
+  else trscr[i - G] = v;
+}
+void launch_det_finish(const unsigned long long* dacc, int G, float* grad, float* trscr, int ntr, hipStream_t s) {
+  if (G + ntr > 0) hipLaunchKernelGGL(k_det_finish, dim3((G + ntr + 255) / 256), dim3(256), 0, s, dacc, G, grad, trscr, ntr);
+}
+
 // ---------------------------------------------------------------------------
 // interior_derivatives (src/drender.h:197-233): Hammersley subpixel offsets,
 // shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74) / <TEXTURE_COLOR> (:132-162, LAMBERT too)
@@ -687,6 +707,10 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
               hash_add(hkey, hval, nh, a.grad, o1, g[3], g[4], g[5]);
               hash_add(hkey, hval, nh, a.grad, o2, g[6], g[7], g[8]);
               if (MODE != 2) hash_add(hkey, hval, nh, a.grad, o3, g[9], g[10], g[11]);
+            } else if (a.dacc) {
+              const int o[4] = {o0, o1, o2, o3};
+              for (int q = 0; q < (MODE == 2 ? 3 : 4); q++)
+                for (int c3 = 0; c3 < 3; c3++) dadd(a.dacc, o[q] + c3, g[3 * q + c3]);
             } else {
               float* G = use_lds ? lgrad : a.grad;
               atomicAdd(G + o0, g[0]); atomicAdd(G + o0 + 1, g[1]); atomicAdd(G + o0 + 2, g[2]);
@@ -713,7 +737,9 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   }
   if (a.uniform) {
     uni.x = wave_sum(uni.x); uni.y = wave_sum(uni.y); uni.z = wave_sum(uni.z);
-    if (lane == 0) {
+    if (lane == 0 && a.dacc) {
+      dadd(a.dacc, a.uniform_off, uni.x); dadd(a.dacc, a.uniform_off + 1, uni.y); dadd(a.dacc, a.uniform_off + 2, uni.z);
+    } else if (lane == 0) {
       atomicAdd(a.grad + a.uniform_off, uni.x);
       atomicAdd(a.grad + a.uniform_off + 1, uni.y);
       atomicAdd(a.grad + a.uniform_off + 2, uni.z);
@@ -1284,9 +1310,14 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
                             o0.z * dv0_dy + o1.z * dv1_dy, 1 * dv0_dy + 1 * dv1_dy,
                             o0.x * dv0_dz + o1.x * dv1_dz, o0.y * dv0_dz + o1.y * dv1_dz,
                             o0.z * dv0_dz + o1.z * dv1_dz, 1 * dv0_dz + 1 * dv1_dz};
-      float* gt = a.trscr ? a.trscr + 12 * m : a.grad + goff + a.layout[4 * m + 3];
+      if (a.dacc) {
+        const int to = a.trscr ? a.dG + 12 * m : goff + a.layout[4 * m + 3];
+        for (int q = 0; q < 12; q++) dadd(a.dacc, to + q, tv[q]);
+      } else {
+        float* gt = a.trscr ? a.trscr + 12 * m : a.grad + goff + a.layout[4 * m + 3];
 #pragma unroll
-      for (int q = 0; q < 12; q++) atomicAdd(gt + q, tv[q]);
+        for (int q = 0; q < 12; q++) atomicAdd(gt + q, tv[q]);
+      }
     }
     }  // differentiable mesh
     }  // adj != 0
@@ -1296,10 +1327,17 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
     if (a.single_mesh) tail = seg_reduce<12>(key, gsum);
     else tail = seg_reduce<6>(key, gsum);
     if (tail && key >= 0) {
+      if (a.dacc) {
+        for (int c3 = 0; c3 < 3; c3++) {
+          dadd(a.dacc, goff + 3 * vn0 + c3, gsum[c3]);
+          dadd(a.dacc, goff + 3 * vn1 + c3, gsum[3 + c3]);
+        }
+      } else {
       float* G = use_lds ? lpos : a.grad;
       const int gb = use_lds ? goff - a.lds_base : goff;
       atomicAdd(G + gb + 3 * vn0, gsum[0]); atomicAdd(G + gb + 3 * vn0 + 1, gsum[1]); atomicAdd(G + gb + 3 * vn0 + 2, gsum[2]);
       atomicAdd(G + gb + 3 * vn1, gsum[3]); atomicAdd(G + gb + 3 * vn1 + 1, gsum[4]); atomicAdd(G + gb + 3 * vn1 + 2, gsum[5]);
+      }
       if (a.single_mesh) {
         // the run's transform gradient: orig_vertices of its two (shared) vertices times the
         // run's summed screen-chain terms (src/drender.cpp:73-86), into the wave's LDS row
@@ -1307,10 +1345,13 @@ __global__ __launch_bounds__(256) void k_edge_grad(EdgeArgs a) {
 #pragma unroll
         for (int r = 0; r < 3; r++) {
           const float d0 = gsum[6 + r], d1 = gsum[9 + r];
-          atomicAdd(&ltr[wave][4 * r + 0], o0.x * d0 + o1.x * d1);
-          atomicAdd(&ltr[wave][4 * r + 1], o0.y * d0 + o1.y * d1);
-          atomicAdd(&ltr[wave][4 * r + 2], o0.z * d0 + o1.z * d1);
-          atomicAdd(&ltr[wave][4 * r + 3], 1 * d0 + 1 * d1);
+          const float t4[4] = {o0.x * d0 + o1.x * d1, o0.y * d0 + o1.y * d1, o0.z * d0 + o1.z * d1, 1 * d0 + 1 * d1};
+          if (a.dacc) {
+            const int to = a.trscr ? a.dG + 12 * a.single_id : a.single_tr_off;
+            for (int c4 = 0; c4 < 4; c4++) dadd(a.dacc, to + 4 * r + c4, t4[c4]);
+          } else {
+            for (int c4 = 0; c4 < 4; c4++) atomicAdd(&ltr[wave][4 * r + c4], t4[c4]);
+          }
         }
       }
     }

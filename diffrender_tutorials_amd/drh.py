@@ -40,7 +40,8 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
-           "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild", "drh_set_edge_grad_mode"]
+           "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild", "drh_set_edge_grad_mode",
+           "drh_set_deterministic"]
 
 
 class DrhError(RuntimeError):
@@ -228,6 +229,10 @@ class DiffRender:
 
     def opt_set_texture_lr(self, lr):
         self._check(load().drh_opt_set_texture_lr(self.h, ctypes.c_float(lr)))
+
+    def set_deterministic(self, on):
+        """Fixed-point gradient sums, bit-reproducible run to run (1) or float atomics (0, default)."""
+        self._check(load().drh_set_deterministic(self.h, int(on)))
 
     def opt_set_transform_lr(self, lr):
         """OptimizerParameters::transforms_lr with _OPT_TFM: 0 keeps the transforms fixed."""

@@ -162,6 +162,15 @@ drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr);
 drh_status drh_opt_set_transform_lr(drh_handle h, float lr);
 /* Download the current (optimised) transforms: 16 floats per mesh, column-major. */
 drh_status drh_get_transforms(drh_handle h, float* transforms);
+/* Deterministic gradients (off by default): d_render, d_render_and_compare and the fused step
+ * add every gradient term as a 32.32 fixed-point integer with 64-bit integer atomics, whose
+ * sums do not depend on the order the GPU's waves finish in, and convert once at the end, so
+ * repeated runs give the same gradient bit for bit.  Terms of magnitude >= 2^-9 enter
+ * exactly; smaller ones are rounded to a multiple of 2^-32; every partial sum must stay below
+ * 2^31 in magnitude.  Slower (no LDS sums, more atomics).  The build counterpart of the
+ * reference's serial sum of its per-thread gradient copies (src/drender.h:229-232, 380-425),
+ * which is ordered; no counterpart in the reference's interface. */
+drh_status drh_set_deterministic(drh_handle h, int on);
 /* Download the current (optimised) restricted-transform parameters: 7 floats per mesh
  * (translate, rotate, scale); zeros for meshes that are not RESTRICTED_TRANSFORM meshes. */
 drh_status drh_get_restricted(drh_handle h, float* params);

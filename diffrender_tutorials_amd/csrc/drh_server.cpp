@@ -63,7 +63,8 @@ struct Job {
   drh_scene_desc desc() const {
     return drh_scene_desc{n_meshes, vcount.data(), fcount.data(), transforms.data(), pos.data(),
                           has_col ? col.data() : nullptr, has_tc ? tc.data() : nullptr, has_nrm ? nrm.data() : nullptr,
-                          idx.data(), tex_w.data(), tex_h.data(), texels.empty() ? nullptr : texels.data(), diff.data()};
+                          idx.data(), tex_w.data(), tex_h.data(), texels.empty() ? nullptr : texels.data(), diff.data(),
+                          nullptr, nullptr};   // no RESTRICTED_TRANSFORM meshes in jobs
   }
 };
 

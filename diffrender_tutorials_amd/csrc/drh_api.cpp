@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/drh.h"
+#include "drh_internal.h"
 #include "drh_kernels.h"
 #include "drh_mat4.h"
 
@@ -1294,6 +1295,15 @@ drh_status drh_get_transforms(drh_handle h, float* transforms) {
       (s = h->hip(hipStreamSynchronize(h->stream), "get_transforms")))
     return s;
   for (int i = 0; i < h->nm; i++) std::memcpy(transforms + 16 * (size_t)i, &m[48 * (size_t)i], 64);
+  return DRH_OK;
+}
+
+drh_status drh_prepare_steps(drh_handle h, const drh_camera* cams, int views, int edge_samples) {
+  if (!h || !cams || views <= 0 || edge_samples < 0) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  long long total;
+  drh_status s;
+  if ((s = h->set_cams(cams, views, &total)) || (s = h->ensure_rnd(edge_samples))) return s;
   return DRH_OK;
 }
 

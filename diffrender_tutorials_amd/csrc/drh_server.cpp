@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/drh.h"
+#include "drh_internal.h"
 
 namespace {
 
@@ -182,9 +183,14 @@ bool drh_server_s::admit_one(Job* j) {
   (void)drh_set_bvh_rebuild(h, j->iterations <= 64 ? j->iterations + 1 : 8);
   const drh_scene_desc d = j->desc();
   const size_t tbytes = 4 * j->targets.size();
-  if ((s = drh_commit(h, &d)) != DRH_OK || (s = drh_opt_init(h, DRH_OPT_ADAM, j->lr_pos, j->lr_col)) != DRH_OK ||
-      (s = drh_malloc(h, tbytes, &j->d_targets)) != DRH_OK || (s = drh_malloc(h, 16, &j->d_loss)) != DRH_OK ||
-      (s = drh_memcpy_htod(h, j->d_targets, j->targets.data(), tbytes)) != DRH_OK) {
+  // every upload first, while the handle's stream is still empty (their synchronisations then
+  // wait for nothing): targets, cameras, edge-sample table; the commit's LBVH build and the
+  // optimiser set-up are left queued behind them, so neither admission nor the job's first
+  // step waits for kernels that queue behind the other jobs' steps
+  if ((s = drh_malloc(h, tbytes, &j->d_targets)) != DRH_OK || (s = drh_malloc(h, 16, &j->d_loss)) != DRH_OK ||
+      (s = drh_memcpy_htod(h, j->d_targets, j->targets.data(), tbytes)) != DRH_OK ||
+      (s = drh_prepare_steps(h, j->cams.data(), (int)j->cams.size(), j->edge_samples)) != DRH_OK ||
+      (s = drh_commit(h, &d)) != DRH_OK || (s = drh_opt_init(h, DRH_OPT_ADAM, j->lr_pos, j->lr_col)) != DRH_OK) {
     finish(j, s, drh_last_error(h));
     return false;
   }

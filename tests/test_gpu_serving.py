@@ -143,13 +143,17 @@ def test_server_result_alone_equals_result_in_mix():
     r_alone = srv.wait(j)
     assert r_alone["status"] == 0 and r_alone["iterations"] == iters
     say("mix")
-    ids = [srv.submit([m], c, tg, iters, N, mode=drh.LAMBERT, spp=spp, light=lam)
-           for (m, _, c), tg in zip(jobs, tgts)]
+    # the other jobs run longer and go in first, so job 0 steps while they are on the GPU
+    long_iters = 10 * iters
+    ids = [srv.submit([m], c, tg, long_iters, N, mode=drh.LAMBERT, spp=spp, light=lam)
+           for (m, _, c), tg in zip(jobs[1:], tgts[1:])]
+    ids.insert(0, srv.submit([m0], c0, tgts[0], iters, N, mode=drh.LAMBERT, spp=spp, light=lam))
     res = [srv.wait(i) for i in ids]
     st = srv.stats()
     srv.close()
     say("stats", st)
-    assert all(r["status"] == 0 and r["iterations"] == iters for r in res)
+    assert all(r["status"] == 0 for r in res)
+    assert res[0]["iterations"] == iters and all(r["iterations"] == long_iters for r in res[1:])
     assert st["peak_active"] >= 2
     for r in (r_alone, res[0]):
         assert_same_fit(r["pos"].reshape(-1, 3), p_alone, m0.pos)

@@ -245,6 +245,10 @@ def main(only=None):
         case("cfg1_cube_vcol", c1["meshes"], c1["target"], c1["cams"][0], 2, 1, 1 << 16, opt_iters=1)
     if not only:
         _rest(c1)
+    elif "tfm_lambert_2mesh" in only:
+        import golden_cases as GC
+        ms, ts, cam_t = GC.scene("tfm_lambert_2mesh")
+        case("tfm_lambert_2mesh", ms, ts, cam_t, 4, 4, 1 << 14)
 
 
 def _rest(c1):
@@ -266,6 +270,9 @@ def _rest(c1):
     box_t = box.copy()
     box_t.pos = (box.pos + np.array([0.05, -0.03, 0.0], np.float32)).astype(np.float32)
     case("ico2_vcol_2mesh", [ico_v, box], [ico_vt, box_t], S.camera_look((0.3, 0.4, 3.0), 96, 96), 2, 1, 1 << 13)
+    import golden_cases as GC
+    ms, ts, cam_t = GC.scene("tfm_lambert_2mesh")
+    case("tfm_lambert_2mesh", ms, ts, cam_t, 4, 4, 1 << 14)
     # config 2 geometry at full resolution: hits, image and the edge sampler (no gradient)
     c2 = S.config2()
     case("cfg2_ico5_lambert", c2["meshes"], c2["target"], c2["cams"][0], 4, 4, 0)

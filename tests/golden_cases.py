@@ -84,13 +84,21 @@ def scene(name):
         return ([S.with_random_colors(S.icosphere(2), 7), box],
                 [S.with_random_colors(S.radial_displace(S.icosphere(2), 0.1, seed=8), 9), box_t],
                 S.camera_look((0.3, 0.4, 3.0), 96, 96))
+    if name == "tfm_lambert_2mesh":
+        # both meshes under non-identity transforms, both differentiable: the tracer's doubled
+        # instance transform and mesh 0's inverse transform in every mesh's edge gradient
+        (ball, box), (ball_t, _), cam = tfm_scene()
+        ball, box, ball_t = S.with_albedo(ball, 0.8), S.with_albedo(box, 0.6), S.with_albedo(ball_t, 0.8)
+        box.differentiable = True
+        return [ball, box], [ball_t, box], cam
     if name == "cfg2_ico5_lambert":
         c = S.config2()
         return c["meshes"], c["target"], c["cams"][0]
     raise KeyError(name)
 
 
-SMALL_CASES = ["cfg1_cube_vcol", "cube_rot_vcol", "cube_rot_sil", "ico3_lambert", "ico3_texture", "ico2_vcol_2mesh"]
+SMALL_CASES = ["cfg1_cube_vcol", "cube_rot_vcol", "cube_rot_sil", "ico3_lambert", "ico3_texture", "ico2_vcol_2mesh",
+               "tfm_lambert_2mesh"]
 ALL_CASES = SMALL_CASES + ["cfg2_ico5_lambert"]
 
 

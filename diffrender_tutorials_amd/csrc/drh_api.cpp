@@ -284,6 +284,7 @@ struct drh_renderer : BufOwner {
   DBuf pos_orig, pos, col, tc, nrm_in, nrm, nrm2d, texels, idx, tri_mesh, mesh_vo, mesh_to, dtex_w, dtex_h,
       dtex_off, dmats, dlayout, dtr_inv0, tr_rows, drflag, drparams, trscr, dacc;
   bool deterministic = false;   // fixed-point gradient sums (drh_set_deterministic)
+  bool interior_pos = false;    // interior position term in VERTEX_COLOR (drh_set_interior_pos_grad)
   // RESTRICTED_TRANSFORM meshes: flag and the 7 parameters per mesh
   std::vector<int> rflag;
   std::vector<float> rparams_h;
@@ -841,6 +842,10 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
       ia.aligned = interior_aligned;
       ia.hits = (interior_store && hits_stored) ? hitbuf.as<float4>() : nullptr;
+      if (interior_pos && st.mode == DRH_VERTEX_COLOR) {
+        ia.pos_grad = 1; ia.tr_inv0 = dtr_inv0.as<float>();
+        ia.hits = nullptr;   // the position term needs each sample's ray: traced again
+      }
       int blocks = tiles;
       if ((st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) && nm == 1 && diff[0] && tex_w[0] * tex_h[0] == 1) {
         ia.uniform = 1; ia.uniform_off = layout[2];
@@ -1304,6 +1309,15 @@ drh_status drh_prepare_steps(drh_handle h, const drh_camera* cams, int views, in
   long long total;
   drh_status s;
   if ((s = h->set_cams(cams, views, &total)) || (s = h->ensure_rnd(edge_samples))) return s;
+  return DRH_OK;
+}
+
+drh_status drh_set_interior_pos_grad(drh_handle h, int on) {
+  if (!h || (on != 0 && on != 1)) return DRH_E_INVALID;
+  if (on && h->st.mode != DRH_VERTEX_COLOR)
+    return h->fail(DRH_E_UNSUPPORTED, "set_interior_pos_grad: VERTEX_COLOR only");
+  if ((on != 0) != h->interior_pos) h->drop_graph();
+  h->interior_pos = on != 0;
   return DRH_OK;
 }
 

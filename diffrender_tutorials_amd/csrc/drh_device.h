@@ -315,6 +315,27 @@ __device__ __forceinline__ bool covered(const Tracer& T, F3 o, F3 d, int* wst) {
   return h.gtri != NO_HIT;
 }
 
+// Derivatives of the Moller-Trumbore barycentrics u (weight of C) and v (weight of B) of the
+// ray (o, d) against triangle (A, A + e1, A + e2) with respect to A, B and C, closed form:
+// with tvec = o - A, pvec = d x e2, det = e1.pvec, u = d.(tvec x e1) / det and
+// v = tvec.pvec / det, so du/dB = (d x tvec - u pvec) / det, du/dC = -u (e1 x d) / det,
+// dv/dB = -v pvec / det, dv/dC = (tvec x d - v (e1 x d)) / det, and the A terms are minus
+// the sum of the tvec, B and C ones (what BarU_grad / BarV_grad compute, src/functions.cpp:
+// 226-470).  False (and no output) where det is clamped (|det| < 1e-6).
+__device__ __forceinline__ bool bary_grad(F3 o, F3 d, F3 A, F3 e1, F3 e2, float u, float v, F3 du[3], F3 dv[3]) {
+  const F3 tvec = sub(o, A), pvec = cross(d, e2), e1xd = cross(e1, d);
+  const float det = dot(e1, pvec);
+  if (!(fabsf(det) >= 1e-6f)) return false;
+  const float id = 1.0f / det;
+  du[1] = mul(id, sub(cross(d, tvec), mul(u, pvec)));
+  du[2] = mul(-u * id, e1xd);
+  du[0] = mul(-1.0f, add(add(mul(id, e1xd), du[1]), du[2]));
+  dv[1] = mul(-v * id, pvec);
+  dv[2] = mul(id, sub(cross(tvec, d), mul(v, e1xd)));
+  dv[0] = mul(-1.0f, add(add(mul(id, pvec), dv[1]), dv[2]));
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Shading (src/shade_simple.cpp, src/shade_lambert.cpp, src/shade_common.cpp)
 // ---------------------------------------------------------------------------

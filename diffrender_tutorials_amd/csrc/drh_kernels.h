@@ -166,6 +166,8 @@ struct InteriorArgs {
   int hash;               // entries of the per-wave LDS hash table (0: atomics straight to grad)
   int region;             // tiles per side of a wave's region (1: one tile)
   unsigned long long* dacc;  // deterministic mode: 32.32 fixed-point sums per gradient float (else NULL)
+  int pos_grad;           // VERTEX_COLOR: add the interior position term (k_interior<2, true>)
+  const float* tr_inv0;   // mesh 0's inverse transform (for pos_grad)
 };
 
 struct ProjectArgs {

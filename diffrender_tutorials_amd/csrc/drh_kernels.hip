@@ -591,7 +591,10 @@ void launch_det_finish(const unsigned long long* dacc, int G, float* grad, float
 // interior_derivatives (src/drender.h:197-233): Hammersley subpixel offsets,
 // shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74) / <TEXTURE_COLOR> (:132-162, LAMBERT too)
 // ---------------------------------------------------------------------------
-template <int MODE>
+// POS (VERTEX_COLOR only): also the position gradient of the interior term, dF/du du/dV +
+// dF/dv dv/dV through the barycentrics of each sample's hit (bary_grad), the term the
+// reference's shade_grad<VERTEX_COLOR> has written out but disabled (src/shade_simple.cpp:75-108).
+template <int MODE, bool POS = false>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
   __shared__ int wst[64];
@@ -630,8 +633,13 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
       for (int si = s_begin; si < s_end; si++) {
         const F3 val = divs(adj, (float)a.nsamp);
         long long key = -1 - lane;   // lanes without a contribution form runs of their own
-        float g[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        constexpr int NG = POS ? 18 : 12;
+        float g[NG];
+#pragma unroll
+        for (int q = 0; q < NG; q++) g[q] = 0.0f;
         int o0 = 0, o1 = 0, o2 = 0, o3 = 0;   // offsets (in G) the four float3 terms go to
+        int p0 = 0, p1 = 0, p2 = 0;           // POS: offsets of the three vertex positions
+        F3 ro = {0, 0, 0}, rd = {0, 0, 1};   // this sample's camera ray
         do {
           if (!inside) break;
           Hit h;
@@ -643,8 +651,8 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
           } else {
             const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
             const float yo = a.aligned ? ((si / sq) + 0.5f) / float(sq) : a.hamm[2 * si + 1];
-            F3 o, d;
-            eye_ray(c, x + xo, y + yo, o, d);
+            eye_ray(c, x + xo, y + yo, ro, rd);
+            const F3 o = ro, d = rd;
             if (a.coverage) {
               // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
               // hit, so the sample adds exactly `val` if the ray hits anything
@@ -669,6 +677,25 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
             const int gb = use_lds ? base - a.lds_base : base;
             o0 = gb + 3 * ((int)I[0] - vo); o1 = gb + 3 * ((int)I[1] - vo); o2 = gb + 3 * ((int)I[2] - vo);
             key = h.gtri;
+            if (POS) {
+              // dF/du = (c_C - c_A).val, dF/dv = (c_B - c_A).val; world-space du/dV, dv/dV,
+              // then mesh 0's inverse transform as in edge_grad (src/drender.cpp:60-71)
+              const F3 cAv = ld3(a.tr.s.col + 3 * I[0]), cBv = ld3(a.tr.s.col + 3 * I[1]), cCv = ld3(a.tr.s.col + 3 * I[2]);
+              const float dFu = dot(sub(cCv, cAv), val), dFv = dot(sub(cBv, cAv), val);
+              F3 du[3], dv[3];
+              const bool ok = bary_grad(ro, rd, f4xyz(T.bf_a[h.gtri]), f4xyz(T.bf_e1[h.gtri]), f4xyz(T.bf_e2[h.gtri]),
+                                        h.u, h.v, du, dv);
+              const float* Ti = a.tr_inv0;
+#pragma unroll
+              for (int k = 0; k < 3; k++) {
+                const F3 wgt = ok ? add(mul(dFu, du[k]), mul(dFv, dv[k])) : F3{0, 0, 0};
+                g[9 + 3 * k + 0] = Ti[0] * wgt.x + Ti[4] * wgt.y + Ti[8] * wgt.z;
+                g[9 + 3 * k + 1] = Ti[1] * wgt.x + Ti[5] * wgt.y + Ti[9] * wgt.z;
+                g[9 + 3 * k + 2] = Ti[2] * wgt.x + Ti[6] * wgt.y + Ti[10] * wgt.z;
+              }
+              const int pb = use_lds ? goff - a.lds_base : goff;
+              p0 = pb + 3 * ((int)I[0] - vo); p1 = pb + 3 * ((int)I[1] - vo); p2 = pb + 3 * ((int)I[2] - vo);
+            }
           } else {
             const int w = a.tr.s.tex_w[m], hh = a.tr.s.tex_h[m];
             if (w == 0 || hh == 0) break;
@@ -700,7 +727,19 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
           }
         } while (false);
         if (runs) {
-          const bool tail = seg_reduce<MODE == 2 ? 9 : 12>(key, g);
+          const bool tail = seg_reduce<MODE == 2 ? (POS ? 18 : 9) : 12>(key, g);
+          if (POS && tail && key >= 0) {
+            const int op[3] = {p0, p1, p2};
+            for (int q = 0; q < 3; q++) {
+              const float* gv = g + 9 + 3 * q;
+              if (nh > 0) hash_add(hkey, hval, nh, a.grad, op[q], gv[0], gv[1], gv[2]);
+              else if (a.dacc) { for (int c3 = 0; c3 < 3; c3++) dadd(a.dacc, op[q] + c3, gv[c3]); }
+              else {
+                float* G = use_lds ? lgrad : a.grad;
+                atomicAdd(G + op[q], gv[0]); atomicAdd(G + op[q] + 1, gv[1]); atomicAdd(G + op[q] + 2, gv[2]);
+              }
+            }
+          }
           if (tail && key >= 0) {
             if (nh > 0) {
               hash_add(hkey, hval, nh, a.grad, o0, g[0], g[1], g[2]);
@@ -1541,7 +1580,9 @@ void launch_loss_final(const double* part, int n, float* out, double denom, hipS
 }
 void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s) {
   const size_t lds = sizeof(float) * (size_t)std::max(a.lds_n, 4 * a.hash);
-  if (mode == 2) {
+  if (mode == 2 && a.pos_grad) {
+    hipLaunchKernelGGL((k_interior<2, true>), dim3(blocks), dim3(64), lds, s, a);
+  } else if (mode == 2) {
     hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), lds, s, a);
   } else {
     hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), lds, s, a);

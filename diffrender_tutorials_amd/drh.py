@@ -41,7 +41,7 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
            "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild", "drh_set_edge_grad_mode",
-           "drh_set_deterministic"]
+           "drh_set_deterministic", "drh_set_interior_pos_grad"]
 
 
 class DrhError(RuntimeError):
@@ -229,6 +229,10 @@ class DiffRender:
 
     def opt_set_texture_lr(self, lr):
         self._check(load().drh_opt_set_texture_lr(self.h, ctypes.c_float(lr)))
+
+    def set_interior_pos_grad(self, on):
+        """VERTEX_COLOR: add the interior term's vertex position gradient (1) or not (0, default)."""
+        self._check(load().drh_set_interior_pos_grad(self.h, int(on)))
 
     def set_deterministic(self, on):
         """Fixed-point gradient sums, bit-reproducible run to run (1) or float atomics (0, default)."""

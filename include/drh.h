@@ -171,6 +171,15 @@ drh_status drh_get_transforms(drh_handle h, float* transforms);
  * reference's serial sum of its per-thread gradient copies (src/drender.h:229-232, 380-425),
  * which is ordered; no counterpart in the reference's interface. */
 drh_status drh_set_deterministic(drh_handle h, int on);
+/* VERTEX_COLOR: also the interior term's vertex position gradient (off by default, as in the
+ * reference, whose shade_grad<VERTEX_COLOR> has it written out but disabled,
+ * src/shade_simple.cpp:75-108): each interior sample adds dF/du du/dV + dF/dv dv/dV for the
+ * three vertices of its hit, with dF/du = (c_C - c_A).val and dF/dv = (c_B - c_A).val and the
+ * closed-form derivatives of the ray's barycentrics (BarU_grad / BarV_grad,
+ * src/functions.cpp:226-470), through mesh 0's inverse transform like the edge term.  With it
+ * the position gradient of a vertex-coloured mesh is the derivative of the image away from
+ * silhouettes too.  Other modes: DRH_E_UNSUPPORTED. */
+drh_status drh_set_interior_pos_grad(drh_handle h, int on);
 /* Download the current (optimised) restricted-transform parameters: 7 floats per mesh
  * (translate, rotate, scale); zeros for meshes that are not RESTRICTED_TRANSFORM meshes. */
 drh_status drh_get_restricted(drh_handle h, float* params);

@@ -26,6 +26,9 @@ Parity status of each function (how it is pinned):
   * orc_silhouette_functional / orc_fd_silhouette (FP64 box-filtered silhouette and its
     central differences): pinned by the closed-form area of a cube's front face and by
     analytic derivatives of that area (tests/test_oracle.py).
+  * mt_barycentrics / barycentric_grad (the MT barycentrics of a ray and their derivatives
+    with respect to the triangle's vertices, the reference's BarU_grad / BarV_grad): pinned
+    against FP64 central differences of mt_barycentrics (tests/test_oracle.py).
   * restricted_matrix / restricted_jacobian (TransformR::to_mat and the Jacobian the
     reference takes with Enzyme, which this image lacks, so the reference cannot be run
     there): the matrix is pinned against LiteMath's product of its factors at special
@@ -285,3 +288,33 @@ def restricted_jacobian(x):
     J[5, :, :3] = x[6] * (dRz @ Ry @ Rx)
     J[6, :, :3] = Rz @ Ry @ Rx
     return J.reshape(7, 12)
+
+
+def mt_barycentrics(o, d, A, B, C):
+    """Moller-Trumbore barycentrics in FP64 (raytrace_src/raytrace3d_bf.cpp:123-150 without the
+    determinant clamp): u = weight of C, v = weight of B."""
+    o, d, A, B, C = (np.asarray(x, np.float64) for x in (o, d, A, B, C))
+    e1, e2 = B - A, C - A
+    pvec = np.cross(d, e2)
+    det = e1 @ pvec
+    tvec = o - A
+    return float(np.cross(tvec, e1) @ d / det), float(tvec @ pvec / det)
+
+
+def barycentric_grad(o, d, A, B, C):
+    """(du, dv): each 3x3, row k the derivative with respect to vertex k (A, B, C), closed form
+    (the quantities BarU_grad / BarV_grad compute, src/functions.cpp:226-470)."""
+    o, d, A, B, C = (np.asarray(x, np.float64) for x in (o, d, A, B, C))
+    e1, e2 = B - A, C - A
+    tvec, pvec, e1xd = o - A, np.cross(d, e2), np.cross(e1, d)
+    det = e1 @ pvec
+    u, v = mt_barycentrics(o, d, A, B, C)
+    du = np.zeros((3, 3))
+    dv = np.zeros((3, 3))
+    du[1] = (np.cross(d, tvec) - u * pvec) / det
+    du[2] = -u * e1xd / det
+    du[0] = -(e1xd / det + du[1] + du[2])
+    dv[1] = -v * pvec / det
+    dv[2] = (np.cross(tvec, d) - v * e1xd) / det
+    dv[0] = -(pvec / det + dv[1] + dv[2])
+    return du, dv

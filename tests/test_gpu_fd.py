@@ -164,3 +164,64 @@ def test_full_jacobian_equals_diagonal_for_an_axis_aligned_view():
     # the cross terms are exactly zero here; only the float atomics' order differs
     assert np.abs(g1).sum() > 0
     assert np.linalg.norm(g1 - g0) <= 1e-5 * np.linalg.norm(g0)
+
+
+def interior_coords(m, cam):
+    """Coordinates of visible vertices all of whose triangles face the camera (no silhouette)."""
+    wv = np.asarray(cam.world_view, np.float64).reshape(4, 4).T
+    eye = np.linalg.inv(wv)[:3, 3]
+    P = m.pos.astype(np.float64)
+    A, B, C = P[m.idx[:, 0]], P[m.idx[:, 1]], P[m.idx[:, 2]]
+    front = np.einsum("ij,ij->i", np.cross(B - A, C - A), A - eye) < 0
+    fv, bv = np.zeros(m.V, bool), np.zeros(m.V, bool)
+    fv[m.idx[front].ravel()] = True
+    bv[m.idx[~front].ravel()] = True
+    verts = np.nonzero(fv & ~bv)[0]
+    return np.sort(np.concatenate([3 * verts, 3 * verts + 1, 3 * verts + 2]))
+
+
+def test_vertex_colour_interior_position_gradient_matches_fd():
+    """VERTEX_COLOR with drh_set_interior_pos_grad: at vertices away from the silhouette the
+    image changes only through the interpolated colours (continuous across edges), so the
+    position gradient is the interior term dF/du du/dV + dF/dv dv/dV.  Against FP64 central
+    differences of sum(adjoint * image) of the oracle's own render (the reference's
+    arithmetic, 4 samples per pixel; h = 1e-3); without the term the gradient there is ~0."""
+    m = S.with_random_colors(S.icosphere(3, 1.0), 6)
+    cam = S.camera_front(128, 128)
+    adj = smooth_adjoint(128, 128)
+    rng = np.random.default_rng(5)
+    coords = rng.choice(interior_coords(m, cam), 48, replace=False)
+
+    def L(pos):
+        mm = m.copy()
+        mm.pos = pos.astype(np.float32)
+        img = O.Scene([mm], O.TRACER_BVH).render(2, 4, [cam])[0]
+        return float(np.sum(img.astype(np.float64) * adj))
+
+    h = 1e-3
+    fd = np.empty(len(coords))
+    for k, cidx in enumerate(coords):
+        p, q = m.pos.astype(np.float64).copy(), m.pos.astype(np.float64).copy()
+        p.reshape(-1)[cidx] += h
+        q.reshape(-1)[cidx] -= h
+        fd[k] = (L(p) - L(q)) / (2 * h)
+    out = {}
+    for on in (1, 0):
+        r = drh.DiffRender(mode=drh.VERTEX_COLOR, spp=4)
+        try:
+            r.commit([m])
+            r.set_interior_pos_grad(on)
+            out[on] = r.d_render([cam], [adj], 1 << 16)[: 3 * m.V][coords]
+        finally:
+            r.close()
+    e_on, e_off = rel_err(out[1], fd), rel_err(out[0], fd)
+    say("interior position term: rel L2 vs FD", e_on, "without the term", e_off)
+    assert np.abs(fd).max() > 0
+    assert e_on < 0.05
+    assert e_off > 0.9
+    with pytest.raises(drh.DrhError):
+        r2 = drh.DiffRender(mode=drh.SILHOUETTE, spp=1)
+        try:
+            r2.set_interior_pos_grad(1)
+        finally:
+            r2.close()

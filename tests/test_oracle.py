@@ -232,6 +232,28 @@ def test_restricted_jacobian_matches_central_differences():
             assert np.abs(J[k] - fd).max() < 1e-8, (x, k)
 
 
+def test_barycentric_grad_matches_central_differences():
+    """The closed-form derivatives of the MT barycentrics against FP64 central differences."""
+    rng = np.random.default_rng(11)
+    h = 1e-7
+    for _ in range(20):
+        A, B, C = rng.normal(size=(3, 3))
+        o = rng.normal(size=3) * 0.2 + np.array([0.0, 0.0, 4.0])
+        P = (A + B + C) / 3 + rng.normal(size=3) * 0.2
+        d = (P - o) / np.linalg.norm(P - o)
+        du, dv = O.barycentric_grad(o, d, A, B, C)
+        V = [A, B, C]
+        for k in range(3):
+            for c in range(3):
+                p, m = [x.copy() for x in V], [x.copy() for x in V]
+                p[k][c] += h
+                m[k][c] -= h
+                up, vp = O.mt_barycentrics(o, d, *p)
+                um, vm = O.mt_barycentrics(o, d, *m)
+                assert abs(du[k, c] - (up - um) / (2 * h)) < 1e-6 * (1 + abs(du[k, c]))
+                assert abs(dv[k, c] - (vp - vm) / (2 * h)) < 1e-6 * (1 + abs(dv[k, c]))
+
+
 def test_adam_matches_torch_vectors():
     fx = load("adam_torch")
     p = fx["p0"].copy()

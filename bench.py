@@ -277,7 +277,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path runs one job "
-              "per GPU; see DESIGN.md for how it shards)", file=sys.stderr)
+              "per GPU; jobs spread over GPUs without communication)", file=sys.stderr)
         sys.exit(2)
 
     from diffrender_tutorials_amd import drh, scenes as S

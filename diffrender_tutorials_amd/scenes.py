@@ -231,7 +231,7 @@ def vertex_normals(pos: np.ndarray, idx: np.ndarray) -> np.ndarray:
 
 def radial_displace(mesh: Mesh, sigma: float, seed: int, smooth: bool = True) -> Mesh:
     """Move every vertex along its radius by sigma*N(0,1), optionally averaged over the
-    one-ring (BASELINE config 2's target); normals recomputed from the new geometry."""
+    one-ring (the icosphere-5 fitting target); normals recomputed from the new geometry."""
     rng = np.random.default_rng(seed)
     d = rng.standard_normal(mesh.V)
     if smooth:
@@ -247,7 +247,7 @@ def radial_displace(mesh: Mesh, sigma: float, seed: int, smooth: bool = True) ->
 
 
 def perturb(mesh: Mesh, sigma: float, seed: int) -> Mesh:
-    """Vertices + N(0, sigma^2) (BASELINE config 1's target)."""
+    """Vertices + N(0, sigma^2) (the unit-cube fitting target)."""
     rng = np.random.default_rng(seed)
     out = mesh.copy()
     out.pos = (mesh.pos + rng.normal(0.0, sigma, size=mesh.pos.shape)).astype(np.float32)

@@ -2,7 +2,7 @@
 golden vectors written from the reference compiled from its own sources, and the
 oracle (CPU restatement) on the same inputs.
 
-Bars (BASELINE north star): hit triangle ids bit-exact against the brute-force tracer;
+Bars: hit triangle ids bit-exact against the brute-force tracer;
 images within 1e-5 absolute (bit-exact is asserted where the hash matches); gradients
 within 1e-3 relative L2 of the CPU d_render with identical samples."""
 from __future__ import annotations

@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--serve-jobs", type=int, default=4, help="concurrent config-2 jobs in the served leg")
+    ap.add_argument("--serve-jobs", type=int, default=4, help="concurrent config-2 jobs in the served leg (0: skip the plain and served legs)")
     ap.add_argument("--serve-iters", type=int, default=50, help="Adam iterations per served job")
     ap.add_argument("--plain-iters", type=int, default=20, help="iterations of the plain host-driven loop")
     ap.add_argument("--no-serve", action="store_true", help="skip the plain and served legs")
@@ -381,12 +381,13 @@ def main():
     }
 
     if not args.no_serve:
-        plain = plain_leg(drh, meshes, cam, target, N, spp, lam, args.plain_iters)
-        served = served_leg(drh, S, meshes, cam, N, spp, lam, args.serve_jobs, args.serve_iters)
-        result["plain_one_job_iters_per_s"] = round(plain, 3)
-        result["served"] = served
-        result["served_over_plain"] = round(served["iters_per_s"] / plain, 3)
-        result["p50_ms"], result["p99_ms"] = served["p50_ms"], served["p99_ms"]
+        if args.serve_jobs > 0:
+            plain = plain_leg(drh, meshes, cam, target, N, spp, lam, args.plain_iters)
+            served = served_leg(drh, S, meshes, cam, N, spp, lam, args.serve_jobs, args.serve_iters)
+            result["plain_one_job_iters_per_s"] = round(plain, 3)
+            result["served"] = served
+            result["served_over_plain"] = round(served["iters_per_s"] / plain, 3)
+            result["p50_ms"], result["p99_ms"] = served["p50_ms"], served["p99_ms"]
         if args.mix_jobs > 0:
             result["mix"] = mix_legs(drh, S, lam, args.mix_jobs, args.mix_iters, args.mix_active)
         if args.multiview_steps > 0:

@@ -6,7 +6,10 @@
   float2 texture coordinates, uint32 indices and one uint32 material index per triangle.
 * 8-bit images with the reference's tone map (common_src/Image2d.cpp:580-665): each channel
   int(x^(1/gamma) * 255 + 0.5) clamped to [0, 255], gamma 2.2 by default; ``.bmp`` (24-bit,
-  rows bottom-up) and ``.ppm`` (ASCII P3).  The BMP rows are padded to 4 bytes and the file
+  rows bottom-up), ``.ppm`` (ASCII P3) and ``.png`` (8-bit RGBA with alpha 255, rows top-down:
+  the pixels the reference hands to stb's PNG writer, common_src/Image2d.cpp:629-646; the
+  stream is written here with zlib, so the compressed bytes differ from stb's, the pixels
+  do not).  The BMP rows are padded to 4 bytes and the file
   size field holds the whole file, as the format specifies (the reference writes neither).
 * ``.image3f``: uint32 width, height, then the float RGB pixels (the reference's raw dump).
 
@@ -16,6 +19,7 @@ returns.
 from __future__ import annotations
 
 import struct
+import zlib
 
 import numpy as np
 
@@ -106,6 +110,13 @@ def save_image(path: str, img: np.ndarray, gamma: float = 2.2) -> None:
             f.write(" ".join(str(int(v)) for v in px.reshape(-1)))
             f.write(" ")
         return
+    if ext == ".png":
+        rgba = np.empty((H, W, 4), np.uint8)
+        rgba[:, :, :3] = px
+        rgba[:, :, 3] = 255
+        with open(path, "wb") as f:
+            f.write(png_bytes(rgba))
+        return
     if ext == ".bmp":
         row = 3 * W
         pad = (-row) % 4
@@ -118,7 +129,88 @@ def save_image(path: str, img: np.ndarray, gamma: float = 2.2) -> None:
         with open(path, "wb") as f:
             f.write(header + info + data)
         return
-    raise ValueError(f"unsupported image extension {ext!r} (.bmp, .ppm, .image3f)")
+    raise ValueError(f"unsupported image extension {ext!r} (.bmp, .ppm, .png, .image3f)")
+
+
+_PNG_SIG = b"\x89PNG\r\n\x1a\n"
+
+
+def _png_chunk(kind: bytes, data: bytes) -> bytes:
+    return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+
+def png_bytes(px: np.ndarray) -> bytes:
+    """An (H, W, 3 or 4) uint8 array as a PNG file (8-bit RGB or RGBA, no interlace): every
+    scanline with filter type 0, the whole image in one zlib-compressed IDAT chunk."""
+    px = np.ascontiguousarray(px, np.uint8)
+    if px.ndim != 3 or px.shape[2] not in (3, 4):
+        raise ValueError("expected an (H, W, 3) or (H, W, 4) uint8 array")
+    H, W, C = px.shape
+    raw = np.zeros((H, 1 + W * C), np.uint8)           # filter byte 0, then the row
+    raw[:, 1:] = px.reshape(H, W * C)
+    ihdr = struct.pack(">IIBBBBB", W, H, 8, 6 if C == 4 else 2, 0, 0, 0)
+    return (_PNG_SIG + _png_chunk(b"IHDR", ihdr) + _png_chunk(b"IDAT", zlib.compress(raw.tobytes(), 6))
+            + _png_chunk(b"IEND", b""))
+
+
+def load_png(path: str) -> np.ndarray:
+    """Read an 8-bit, non-interlaced RGB / RGBA / grey PNG as an (H, W, C) uint8 array, undoing
+    each scanline's filter (none, sub, up, average, Paeth; PNG specification section 9)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:8] != _PNG_SIG:
+        raise ValueError(f"{path}: not a PNG")
+    off, idat, hdr = 8, [], None
+    while off + 8 <= len(data):
+        n, kind = struct.unpack_from(">I4s", data, off)
+        body = data[off + 8: off + 8 + n]
+        if len(body) != n or struct.unpack_from(">I", data, off + 8 + n)[0] != zlib.crc32(kind + body) & 0xFFFFFFFF:
+            raise ValueError(f"{path}: bad {kind!r} chunk")
+        if kind == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif kind == b"IDAT":
+            idat.append(body)
+        elif kind == b"IEND":
+            break
+        off += 12 + n
+    if hdr is None:
+        raise ValueError(f"{path}: no IHDR")
+    W, H, depth, ctype, _, _, interlace = hdr
+    C = {0: 1, 2: 3, 4: 2, 6: 4}.get(ctype)
+    if depth != 8 or C is None or interlace:
+        raise ValueError(f"{path}: only 8-bit non-interlaced grey / RGB / RGBA PNG is read")
+    raw = np.frombuffer(zlib.decompress(b"".join(idat)), np.uint8)
+    stride = W * C
+    if raw.size != H * (stride + 1):
+        raise ValueError(f"{path}: image data size")
+    rows = raw.reshape(H, stride + 1)
+    out = np.zeros((H, stride), np.uint8)
+    prev = np.zeros(stride, np.int32)
+    for y in range(H):
+        ft, line = int(rows[y, 0]), rows[y, 1:].astype(np.int32)
+        cur = np.zeros(stride, np.int32)
+        if ft == 0:
+            cur = line
+        elif ft == 2:
+            cur = (line + prev) & 255
+        else:
+            for x in range(stride):
+                a = cur[x - C] if x >= C else 0
+                b = int(prev[x])
+                c = int(prev[x - C]) if x >= C else 0
+                if ft == 1:
+                    p = a
+                elif ft == 3:
+                    p = (a + b) >> 1
+                elif ft == 4:
+                    pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                    p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+                else:
+                    raise ValueError(f"{path}: filter type {ft}")
+                cur[x] = (int(line[x]) + p) & 255
+        out[y] = cur
+        prev = cur
+    return out.reshape(H, W, C)
 
 
 def load_bmp(path: str) -> np.ndarray:

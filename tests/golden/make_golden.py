@@ -178,7 +178,7 @@ def meshio_files():
         for tag, (H, W, seed) in (("img8", (5, 8, 11)), ("img6", (3, 6, 12))):
             img = _fixture_image(H, W, seed)
             out[tag] = img
-            for ext, gamma in ((".bmp", 2.2), (".ppm", 2.2), (".image3f", 2.2), (".ppm", 1.0)):
+            for ext, gamma in ((".bmp", 2.2), (".ppm", 2.2), (".image3f", 2.2), (".ppm", 1.0), (".png", 2.2)):
                 f = os.path.join(td, tag + ext)
                 refio.save_image(f, img, gamma)
                 key = f"{tag}_ref{ext.replace('.', '_')}" + ("" if gamma == 2.2 else "_g1")

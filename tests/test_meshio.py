@@ -4,7 +4,8 @@ from common_src/cmesh.cpp:139-205 and common_src/Image2d.cpp:486-735 compiled fr
 
 Pinned there: the VSGF reader on a file the reference wrote (tangent block present), the
 reference's reader on this package's writer (with and without normals), the tone map and the
-PPM text byte for byte, the raw float dump byte for byte, and the BMP pixel bytes.  The BMP
+PPM text byte for byte, the raw float dump byte for byte, the BMP pixel bytes and the PNG
+pixels (decoded from the file the reference's stb writer produced).  The BMP
 header differs from the reference's in the fields the reference leaves wrong or empty (file
 size, image size, resolution); rows of widths that are not a multiple of 4 are padded here
 and not in the reference's writer."""
@@ -150,8 +151,66 @@ def test_bmp_rows_are_padded_for_any_width(tmp_path):
     assert np.array_equal(meshio.load_bmp(str(f)), meshio.tonemap(img))
 
 
+@pytest.mark.parametrize("tag", ["img8", "img6"])
+def test_png_pixels_match_the_reference(tmp_path, tag):
+    """The reference writes .png through stb (common_src/Image2d.cpp:629-646): tone-mapped RGBA
+    with alpha 255, rows top-down.  Its file, decoded, holds exactly the pixels of ours; the
+    compressed streams differ (zlib here, stb's own deflate there)."""
+    img = GOLD[tag]
+    ref = tmp_path / "ref.png"
+    ref.write_bytes(GOLD[tag + "_ref_png"].tobytes())
+    f = tmp_path / "x.png"
+    meshio.save_image(str(f), img)
+    ours, theirs = meshio.load_png(str(f)), meshio.load_png(str(ref))
+    H, W, _ = img.shape
+    assert ours.shape == theirs.shape == (H, W, 4)
+    assert np.array_equal(ours, theirs)
+    assert np.array_equal(ours[:, :, :3], meshio.tonemap(img)) and (ours[:, :, 3] == 255).all()
+
+
+def test_png_reader_undoes_every_filter_type(tmp_path):
+    """Rows filtered by hand with each of the five filter types (PNG spec section 9) decode to
+    the original pixels; a corrupted CRC is refused."""
+    rng = np.random.default_rng(3)
+    px = rng.integers(0, 256, (5, 7, 3), dtype=np.uint8)
+    H, W, C = px.shape
+    stride = W * C
+    rows = px.reshape(H, stride).astype(np.int32)
+    raw = bytearray()
+    for y in range(H):
+        prev = rows[y - 1] if y else np.zeros(stride, np.int32)
+        cur = rows[y]
+        left = np.concatenate([np.zeros(C, np.int32), cur[:-C]])
+        ul = np.concatenate([np.zeros(C, np.int32), prev[:-C]])
+        ft = y % 5
+        if ft == 0:
+            f = cur
+        elif ft == 1:
+            f = cur - left
+        elif ft == 2:
+            f = cur - prev
+        elif ft == 3:
+            f = cur - (left + prev) // 2
+        else:
+            pa, pb, pc = np.abs(prev - ul), np.abs(left - ul), np.abs(left + prev - 2 * ul)
+            pred = np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, prev, ul))
+            f = cur - pred
+        raw += bytes([ft]) + (f & 255).astype(np.uint8).tobytes()
+    import zlib
+    body = (meshio._PNG_SIG + meshio._png_chunk(b"IHDR", struct.pack(">IIBBBBB", W, H, 8, 2, 0, 0, 0))
+            + meshio._png_chunk(b"IDAT", zlib.compress(bytes(raw))) + meshio._png_chunk(b"IEND", b""))
+    f = tmp_path / "f.png"
+    f.write_bytes(body)
+    assert np.array_equal(meshio.load_png(str(f)), px)
+    bad = bytearray(body)
+    bad[29] ^= 1                                          # inside IHDR: its CRC no longer holds
+    f.write_bytes(bytes(bad))
+    with pytest.raises(ValueError):
+        meshio.load_png(str(f))
+
+
 def test_save_image_rejects_bad_input(tmp_path):
     with pytest.raises(ValueError):
-        meshio.save_image(str(tmp_path / "x.png"), np.zeros((2, 2, 3), np.float32))
+        meshio.save_image(str(tmp_path / "x.jpg"), np.zeros((2, 2, 3), np.float32))
     with pytest.raises(ValueError):
         meshio.save_image(str(tmp_path / "x.bmp"), np.zeros((2, 2), np.float32))

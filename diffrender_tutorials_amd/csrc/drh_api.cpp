@@ -818,7 +818,10 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   for (int i = 0; i < nm; i++) if (diff[i]) { single = (single == -1) ? i : -2; }
   // Edge samples in pixel-bin order when there are few samples per edge (rnd0 order then
   // scatters a wavefront's rays over distant edges); buffers sized before any launch.
-  const bool binned = E > 0 && N > 0 &&
+  // Deterministic mode keeps the rnd0 slot order whatever was asked: a sample's rank inside
+  // its bin comes from an atomic counter (wave arrival order), which would regroup the float
+  // run sums of k_edge_grad from run to run.
+  const bool binned = E > 0 && N > 0 && !deterministic &&
                       (edge_order == DRH_EDGE_ORDER_BINNED ||
                        (edge_order == DRH_EDGE_ORDER_AUTO && N < (long long)EDGE_BIN_AUTO_PER_EDGE * E));
   if (E > 0 && N > 0 && (s = hip(ecol.reserve(16 * (size_t)N), "alloc"))) return s;

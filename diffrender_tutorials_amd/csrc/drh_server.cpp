@@ -227,10 +227,8 @@ void drh_server_s::admit_run() {
     std::lock_guard<std::mutex> lk(mu);
     admit_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
     admitting--;
-    if (ok) {
-      ready.push_back(adm);
-      cv_work.notify_all();
-    }
+    if (ok) ready.push_back(adm);
+    cv_work.notify_all();   // the stepper also waits here for admissions to drain at stop
   }
 }
 
@@ -327,7 +325,16 @@ void drh_server_s::run() {
     }
     if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
-  // stopped: hand every held job to the reaper
+  // Stopped.  An admitter may still be setting up a job it took before the stop (outside the
+  // lock); it would push that job into `ready` after this thread had left, where nothing
+  // drains it any more (its handle, pooled HBM and budget share would never be released).  So
+  // wait for every admission in progress to end, then hand the held jobs and whatever the
+  // admitters left in `ready` to the reaper.
+  {
+    std::unique_lock<std::mutex> lk(mu);
+    cv_work.wait(lk, [&] { return admitting == 0; });
+    while (!ready.empty()) { active.push_back(ready.front()); ready.pop_front(); }
+  }
   while (!active.empty()) hand_over(active.size() - 1, DRH_E_INVALID, "server stopped");
   for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
   spare_events.clear();

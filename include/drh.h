@@ -167,7 +167,9 @@ drh_status drh_get_transforms(drh_handle h, float* transforms);
  * sums do not depend on the order the GPU's waves finish in, and convert once at the end, so
  * repeated runs give the same gradient bit for bit.  Terms of magnitude >= 2^-9 enter
  * exactly; smaller ones are rounded to a multiple of 2^-32; every partial sum must stay below
- * 2^31 in magnitude.  Slower (no LDS sums, more atomics).  The build counterpart of the
+ * 2^31 in magnitude.  Slower (no LDS sums, more atomics); the edge pass keeps the rnd0 slot
+ * order whatever drh_set_edge_order asks (pixel-bin ranks come from atomic counters).  The
+ * build counterpart of the
  * reference's serial sum of its per-thread gradient copies (src/drender.h:229-232, 380-425),
  * which is ordered; no counterpart in the reference's interface. */
 drh_status drh_set_deterministic(drh_handle h, int on);

@@ -83,6 +83,30 @@ def test_deterministic_gradient_repeats_bit_for_bit(name):
     assert e < 1e-4
 
 
+@pytest.mark.parametrize("order", [drh.EDGE_ORDER_AUTO, drh.EDGE_ORDER_BINNED])
+def test_deterministic_gradient_repeats_with_fewer_samples_than_edges(order):
+    """N < E makes the automatic edge order pick the pixel-bin order, whose in-bin ranks come
+    from atomic counters; deterministic mode keeps the rnd0 slot order instead (for an explicit
+    EDGE_ORDER_BINNED too), so three runs agree bit for bit."""
+    m = S.with_random_colors(S.icosphere(4), 3)            # E = 7680 edges
+    cam = S.camera_look((0.5, 1.0, 2.8), 128, 128)
+    tgt = O.Scene([_moved(m, (0.03, -0.02, 0.01))], O.TRACER_BVH).render(drh.VERTEX_COLOR, 1, [cam])[0]
+    N = 1 << 12
+    gs = []
+    for _ in range(3):
+        r = drh.DiffRender(mode=drh.VERTEX_COLOR, spp=1)
+        try:
+            r.commit([m])
+            assert N < len(r.edges()[0])
+            r.set_edge_order(order)
+            r.set_deterministic(1)
+            gs.append(r.d_render_and_compare([cam], [tgt], N, images=False)[1])
+        finally:
+            r.close()
+    assert np.abs(gs[0][:3 * m.V]).sum() > 0
+    assert np.array_equal(gs[0], gs[1]) and np.array_equal(gs[0], gs[2])
+
+
 def test_deterministic_gradient_matches_oracle():
     meshes, mode, lam, cam = _case("vcol_two_meshes")
     tg = [_moved(m, (0.03, -0.02, 0.01)) for m in meshes]

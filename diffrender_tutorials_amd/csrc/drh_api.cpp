@@ -304,6 +304,20 @@ struct drh_renderer : BufOwner {
   long long rnd_n = -1;
   // per call
   DBuf cams, img, adj, tgt, loss_part, loss, grad;
+  DBuf fault;   // the kernels' fault word (DevScene::fault), zero while all is well
+  // After a synchronisation: a set fault word fails the call with DRH_E_DEVICE (and is
+  // cleared, so the handle stays usable).
+  drh_status check_fault() {
+    uint32_t f = 0;
+    drh_status s;
+    if ((s = hip(hipMemcpyAsync(&f, fault.p, 4, hipMemcpyDeviceToHost, stream), "fault word")) ||
+        (s = hip(hipStreamSynchronize(stream), "fault word")))
+      return s;
+    if (!f) return DRH_OK;
+    (void)hipMemsetAsync(fault.p, 0, 4, stream);
+    return fail(DRH_E_DEVICE, "device check failed (fault word 0x%x: %s)", f,
+                (f & FAULT_STACK_OVERFLOW) ? "a traversal stack overflowed" : "unknown");
+  }
   std::vector<DevCam> cam_cache;
   // optimiser
   int opt_kind = -1;
@@ -387,6 +401,7 @@ struct drh_renderer : BufOwner {
     s.nrm = nrm.as<float>(); s.nrm2d = nrm2d.as<float>(); s.texels = texels.as<float>();
     s.tex_w = dtex_w.as<int>(); s.tex_h = dtex_h.as<int>(); s.tex_off = dtex_off.as<int>();
     s.n_meshes = nm;
+    s.fault = fault.as<int>();
     t.bf_a = bf_a.as<float4>(); t.bf_e1 = bf_e1.as<float4>(); t.bf_e2 = bf_e2.as<float4>();
     t.use_bf = st.tracer == DRH_TRACER_BF;
     t.cam = {nullptr, nullptr};
@@ -966,7 +981,9 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   hammersley(settings->spp, hm);
   if (r->hamm.reserve(4 * hm.size()) != hipSuccess) return DRH_E_OOM;
   if (hipMemcpyAsync(r->hamm.p, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice, r->stream) != hipSuccess ||
-      hipStreamSynchronize(r->stream) != hipSuccess || r->loss.reserve(16) != hipSuccess)
+      hipStreamSynchronize(r->stream) != hipSuccess || r->loss.reserve(16) != hipSuccess ||
+      r->fault.reserve(16) != hipSuccess || hipMemsetAsync(r->fault.p, 0, 16, r->stream) != hipSuccess ||
+      hipStreamSynchronize(r->stream) != hipSuccess)
     return DRH_E_HIP;
   *out = r.release();
   return DRH_OK;
@@ -1027,7 +1044,7 @@ drh_status drh_render(drh_handle h, const drh_camera* cams, int views, float* im
   if ((s = h->hip(h->img.reserve(4 * (size_t)total), "alloc"))) return s;
   if ((s = h->run_render(views, h->img.as<float>(), nullptr, nullptr, nullptr))) return s;
   if ((s = h->hip(hipMemcpyAsync(images, h->img.p, 4 * (size_t)total, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
-  return h->hip(hipStreamSynchronize(h->stream), "render");
+  return h->check_fault();
 }
 
 drh_status drh_d_render_dev(drh_handle h, const drh_camera* cams, int views, const float* d_adj, int edge_samples,
@@ -1060,7 +1077,7 @@ drh_status drh_d_render(drh_handle h, const drh_camera* cams, int views, const f
   if ((s = h->run_d_render(views, h->adj.as<float>(), edge_samples, h->grad.as<float>(), accumulate != 0))) return s;
   if (h->G > 0 && (s = h->hip(hipMemcpyAsync(grad, h->grad.p, 4 * (size_t)h->G, hipMemcpyDeviceToHost, h->stream), "download")))
     return s;
-  return h->hip(hipStreamSynchronize(h->stream), "d_render");
+  return h->check_fault();
 }
 
 drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, int views, const float* d_targets,
@@ -1113,7 +1130,7 @@ drh_status drh_d_render_and_compare(drh_handle h, const drh_camera* cams, int vi
     return s;
   float l = 0.0f;
   if ((s = h->hip(hipMemcpyAsync(&l, h->loss.p, 4, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
-  if ((s = h->hip(hipStreamSynchronize(h->stream), "d_render_and_compare"))) return s;
+  if ((s = h->check_fault())) return s;
   if (loss) *loss = l;
   return DRH_OK;
 }
@@ -1412,7 +1429,7 @@ void* drh_stream(drh_handle h) { return h ? (void*)h->stream : nullptr; }
 
 drh_status drh_sync(drh_handle h) {
   if (!h) return DRH_E_INVALID;
-  return h->hip(hipStreamSynchronize(h->stream), "sync");
+  return h->check_fault();
 }
 
 drh_status drh_profile_enable(drh_handle h, int on) {
@@ -1452,7 +1469,7 @@ drh_status drh_debug_hits(drh_handle h, const drh_camera* cam, uint32_t* prim, u
   if ((s = h->hip(hipGetLastError(), "debug_hits"))) return s;
   std::vector<uint32_t> hb(5 * (size_t)n);
   if ((s = h->hip(hipMemcpyAsync(hb.data(), b.p, 20 * (size_t)n, hipMemcpyDeviceToHost, h->stream), "download"))) return s;
-  if ((s = h->hip(hipStreamSynchronize(h->stream), "debug_hits"))) return s;
+  if ((s = h->check_fault())) return s;
   std::memcpy(prim, hb.data(), 4 * (size_t)n);
   std::memcpy(geom, hb.data() + n, 4 * (size_t)n);
   std::memcpy(t, hb.data() + 2 * n, 4 * (size_t)n);
@@ -1562,3 +1579,4 @@ extern "C" drh_status drh_debug_cdf(drh_handle h, const float* pmf_raw, int E, f
     return s;
   return h->hip(hipStreamSynchronize(h->stream), "debug_cdf");
 }
+

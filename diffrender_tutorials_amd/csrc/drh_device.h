@@ -107,7 +107,11 @@ struct DevScene {
   const int* __restrict__ tex_h;
   const int* __restrict__ tex_off;
   int n_meshes;
+  // Device-side fault word (bit 0: a traversal stack overflowed and a subtree was skipped);
+  // read back by the host at its next synchronisation and reported as DRH_E_DEVICE.
+  int* fault;
 };
+constexpr int FAULT_STACK_OVERFLOW = 1;
 
 struct Hit { uint32_t gtri; float t, u, v; };
 
@@ -165,8 +169,13 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
 // ballots pick the children any lane needs (nearer child first by majority), and the
 // traversal stack is wave-uniform too: 64 ints of LDS per wave (`wst`), written by every
 // active lane with the same value (no exec-mask juggling for a single writer), read as a
-// broadcast; 64 entries cover the Karras tree's depth (<= 30 + ceil(log2 F)
-// levels) for F < 2^34.  A lane whose ray is NaN, or an any-hit lane that has its hit, gets
+// broadcast.  The stack holds at most one far child per level of the path to the current
+// node; a Karras tree over unique keys (30-bit Morton code, then the 32-bit triangle index
+// as tie-break) splits at a strictly deeper key bit on every level, so it is at most 63
+// levels deep and 64 entries always suffice.  Should a push ever find the stack full, the
+// far child is dropped and the scene's fault word records it (the host then fails the call
+// with DRH_E_DEVICE rather than return a hit that may be wrong).
+// A lane whose ray is NaN, or an any-hit lane that has its hit, gets
 // lim = -1 and no box passes for it any more.
 // A lane tests every leaf its own slab test opens and possibly more (leaves opened for
 // other lanes): the closest hit with the (t, triangle id) tie rule is the same over any
@@ -218,6 +227,7 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
       const bool lfirst = both ? 2 * __popcll(lf) >= __popcll(both) : __popcll(bl) >= __popcll(br);
       const int nearc = lfirst ? cl : cr, farc = lfirst ? cr : cl;
       if (sp < 64) { wst[sp] = farc; sp++; }   // every active lane writes the same value
+      else if (s.fault) atomicOr(s.fault, FAULT_STACK_OVERFLOW);
       node = nearc;
     } else if (goL) {
       node = cl;

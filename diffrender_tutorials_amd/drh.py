@@ -28,7 +28,7 @@ EDGE_GRAD_REFERENCE_DIAGONAL, EDGE_GRAD_FULL_JACOBIAN = 0, 1
 OPT_SIMPLE, OPT_ADAM = 0, 1
 
 STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_E_HIP", -4: "DRH_E_OOM",
-          -5: "DRH_E_UNSUPPORTED"}
+          -5: "DRH_E_UNSUPPORTED", -6: "DRH_E_DEVICE"}
 
 # every symbol include/drh.h declares
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",

@@ -37,6 +37,8 @@ typedef enum {
   DRH_E_HIP = -3,            /* HIP runtime error; see drh_last_error */
   DRH_E_OOM = -4,            /* device allocation failed */
   DRH_E_UNSUPPORTED = -5,    /* a scene feature outside this path (see drh_last_error) */
+  DRH_E_DEVICE = -6,         /* a kernel's own check failed (a traversal stack overflowed); the
+                                outputs of the calls since the last synchronisation are not valid */
 } drh_status;
 
 /* SHADING_MODEL numbering (src/dmesh.h:16-23); PHONG / GGX / PATH_TEST are not on this path. */

@@ -259,7 +259,21 @@ struct drh_renderer : BufOwner {
   int sharing = 1;          // handles sharing the device (drh_set_device_sharing)
   // interior samples: Hammersley (reference) or the render's grid (aligned); store: the render
   // keeps each sample's closest hit and the interior pass reads it back
-  int interior_aligned = 0, interior_store = 0;
+  int interior_aligned = 0, interior_store = 0;   // interior_store: DRH_HITS_*
+  size_t hit_limit = ~(size_t)0;                  // DRH_HITS_AUTO stores only within this many bytes
+  // Free a buffer unless the handle's stream is being captured (the free would become a node
+  // of the captured step); the eager step before any capture has freed it already.
+  void release_uncaptured(DBuf& b) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(stream, &cs);
+    if (cs == hipStreamCaptureStatusNone) b.release();
+  }
+  // bytes of stored hits a d_render_and_compare of `views` views needs
+  size_t hit_bytes(int views) const {
+    size_t px = 0;
+    for (int v = 0; v < views; v++) px += (size_t)cam_cache[v].w * cam_cache[v].h;
+    return 16 * px * (size_t)nsamp;
+  }
   int edge_order = DRH_EDGE_ORDER_AUTO;   // sample order of the edge pass
   int edge_grad_mode = DRH_EDGE_GRAD_REFERENCE_DIAGONAL;
   DBuf bin_count, bin_rec, bin_order;     // pixel-bin order of the kept edge samples
@@ -531,7 +545,7 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     const int nv = vo[i + 1] - vo[i];
     int off = 3 * nv;
     int colo = -1, texo = -1;
-    if (st.mode == DRH_VERTEX_COLOR) { colo = off; off += 3 * nv; }
+    if (st.mode == DRH_VERTEX_COLOR || st.mode == DRH_LAMBERT_VERTEX_COLOR) { colo = off; off += 3 * nv; }
     if (st.mode == DRH_TEXTURE_COLOR || st.mode == DRH_LAMBERT) { texo = off; off += 3 * tw[i] * th[i]; }
     layout[4 * i] = G; layout[4 * i + 1] = colo; layout[4 * i + 2] = texo; layout[4 * i + 3] = off;
     G += off + (rf[i] ? 7 : 12);   // RESTRICTED_TRANSFORM_SIZE or TRANSFORM_SIZE
@@ -681,7 +695,7 @@ drh_status drh_renderer::build_bvh(bool full) {
       (s = R(leaf_hi, f16)) || (s = R(node_lo, f16)) || (s = R(node_hi, f16)) || (s = R(nodes, 4 * f16)) ||
       (s = R(tri_a, f16)) || (s = R(tri_e1, f16)) || (s = R(tri_e2, f16)))
     return s;
-  light_sus = st.mode == DRH_LAMBERT && st.tracer != DRH_TRACER_BF;
+  light_sus = (st.mode == DRH_LAMBERT || st.mode == DRH_LAMBERT_VERTEX_COLOR) && st.tracer != DRH_TRACER_BF;
   if (light_sus && ((s = R(sus_light, f16)) || (s = R(sus_light_cnt, 16)))) return s;
   // k_prepare also empties the centroid bounds and the light's suspect count, and
   // k_tri_setup clears the refit counters
@@ -768,9 +782,14 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   if ((s0 = build_view_suspects(views))) return s0;
   hits_stored = false;
   if (store_hits) {
-    long long px = 0;
-    for (int v = 0; v < views; v++) px += (long long)cam_cache[v].w * cam_cache[v].h;
-    if ((s0 = hip(hitbuf.reserve(16 * (size_t)px * nsamp), "alloc"))) return s0;
+    const hipError_t e = hitbuf.reserve(hit_bytes(views));
+    if (e != hipSuccess) {
+      // the automatic policy recomputes when the memory is not there; a forced store fails
+      if (interior_store != DRH_HITS_AUTO || e != hipErrorOutOfMemory) return hip(e, "alloc");
+      (void)hipGetLastError();
+      release_uncaptured(hitbuf);
+      store_hits = false;
+    }
   }
   int part = 0;
   std::vector<int> tiles(views);
@@ -859,6 +878,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
       ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
       ia.aligned = interior_aligned;
+      ia.light = light;
       ia.hits = (interior_store && hits_stored) ? hitbuf.as<float4>() : nullptr;
       if (interior_pos && st.mode == DRH_VERTEX_COLOR) {
         ia.pos_grad = 1; ia.tr_inv0 = dtr_inv0.as<float>();
@@ -948,7 +968,8 @@ extern "C" {
 drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   if (!settings || !out) return DRH_E_INVALID;
   *out = nullptr;
-  if (settings->mode < DRH_SILHOUETTE || settings->mode > DRH_LAMBERT || settings->spp < 1 ||
+  if (settings->mode < DRH_SILHOUETTE || (settings->mode > DRH_LAMBERT && settings->mode != DRH_LAMBERT_VERTEX_COLOR) ||
+      settings->spp < 1 ||
       (settings->rng_mode != DRH_RNG_REF_PRNG14 && settings->rng_mode != DRH_RNG_QMC) ||
       (settings->tracer != DRH_TRACER_BVH && settings->tracer != DRH_TRACER_BF))
     return DRH_E_INVALID;
@@ -1100,7 +1121,11 @@ drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, in
   }
   float* loss = d_loss ? d_loss : h->loss.as<float>();
   if ((s = h->ensure_rnd(edge_samples)) || (s = h->fork_prep0(edge_samples))) return s;
-  const bool store = h->interior_store && h->interior_aligned && !h->skip_tex_interior && h->st.mode != DRH_SILHOUETTE;
+  bool store = h->interior_store && h->interior_aligned && !h->skip_tex_interior && h->st.mode != DRH_SILHOUETTE;
+  if (store && h->interior_store == DRH_HITS_AUTO && h->hit_bytes(views) > h->hit_limit) {
+    store = false;
+    h->release_uncaptured(h->hitbuf);   // recompute: hold no hit buffer beyond the limit
+  }
   if ((s = h->run_render(views, img, d_targets, h->adj.as<float>(), loss, store))) return s;
   s = h->run_d_render(views, h->adj.as<float>(), edge_samples, d_grad, false);
   h->prep0_forked = false;
@@ -1379,11 +1404,24 @@ drh_status drh_opt_set_texture_lr(drh_handle h, float tex_lr) {
 }
 
 drh_status drh_set_interior_samples(drh_handle h, int aligned, int store) {
-  if (!h || (aligned != 0 && aligned != 1) || (store != 0 && store != 1)) return DRH_E_INVALID;
+  if (!h || (aligned != 0 && aligned != 1) || store < DRH_HITS_RECOMPUTE || store > DRH_HITS_AUTO) return DRH_E_INVALID;
   if (store && !aligned) return h->fail(DRH_E_INVALID, "set_interior_samples: storing hits needs aligned samples");
   if (aligned != h->interior_aligned || store != h->interior_store) h->drop_graph();
   h->interior_aligned = aligned;
   h->interior_store = store;
+  return DRH_OK;
+}
+
+drh_status drh_set_hit_buffer_limit(drh_handle h, size_t bytes) {
+  if (!h) return DRH_E_INVALID;
+  if (bytes != h->hit_limit) h->drop_graph();   // the captured step bakes the choice in
+  h->hit_limit = bytes;
+  return DRH_OK;
+}
+
+drh_status drh_get_hit_buffer_bytes(drh_handle h, size_t* bytes) {
+  if (!h || !bytes) return DRH_E_INVALID;
+  *bytes = h->hitbuf.bytes;
   return DRH_OK;
 }
 
@@ -1579,4 +1617,3 @@ extern "C" drh_status drh_debug_cdf(drh_handle h, const float* pmf_raw, int E, f
     return s;
   return h->hip(hipStreamSynchronize(h->stream), "debug_cdf");
 }
-

@@ -382,11 +382,49 @@ __device__ __forceinline__ F2 blend2(const float* tc, uint32_t A, uint32_t B, ui
   return {tc[2 * A] * w + tc[2 * B] * v + u * tc[2 * C], tc[2 * A + 1] * w + tc[2 * B + 1] * v + u * tc[2 * C + 1]};
 }
 
-// shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT.  `nrm` is the 3D scene's normals
-// for render and the projected scene's for edge samples (src/drender.h:353-354).
+// Shading modes: the reference's SHADING_MODEL numbers (src/dmesh.h:16-23) and one of this
+// build's: LAMBERT with the interpolated vertex colour as albedo instead of a texture (the
+// multi-view fitting workload's shading; no counterpart in the reference).
+constexpr int MODE_LAMBERT_VCOL = 8;
+
+// LAMBERT's light terms at a closest hit (src/shade_lambert.cpp:26-34): ndl = max(0, n.(-L))
+// with the interpolated, un-normalised normal n, and shadow = 1 unless the ray from
+// pos + (t - 1e-5) dir toward -L hits anything.  The lit colour is (shadow * Lc) * ndl.  With
+// ndl == +0 and a finite light colour the shadow ray cannot change that product (bit for bit),
+// so it is only cast where ndl > 0 (shadow stays 1 otherwise).
+__device__ __forceinline__ void lambert_terms(const Tracer& T, const Light& L, F3 o, F3 d, const Hit& h,
+                                              uint32_t A, uint32_t B, uint32_t C, const float* nrm, int* wst,
+                                              float& shadow, float& ndl) {
+  const F3 n = blend3(ld3(nrm + 3 * A), ld3(nrm + 3 * B), ld3(nrm + 3 * C), h.u, h.v);
+  const float BIAS = 1e-5f;
+  const F3 negL = mul(-1.0f, L.L);
+  ndl = max_std(0.0f, dot(n, negL));   // +0 or positive
+  shadow = 1.0f;
+  if (ndl != 0.0f || !(fabsf(L.Lc.x) <= 3.4e38f && fabsf(L.Lc.y) <= 3.4e38f && fabsf(L.Lc.z) <= 3.4e38f)) {
+    const F3 surf = add(o, mul(h.t - BIAS, d));
+    const Hit sh = trace<true>(T, surf, negL, 0.0f, 1e9f, wst);
+    shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
+  }
+}
+// (shadow * Lc) * ndl + ambient: the factor LAMBERT multiplies the albedo by
+__device__ __forceinline__ F3 lambert_factor(const Light& L, float shadow, float ndl) {
+  return add(mul(mul(shadow, L.Lc), ndl), L.amb);
+}
+// The pair (shadow, ndl) in one float, for the stored hits of MODE_LAMBERT_VCOL: ndl when the
+// point is lit, -ndl in shadow.  ndl == +0 always comes with shadow = 1 (no ray cast), so the
+// sign decides the shadow and the magnitude is ndl.
+__device__ __forceinline__ float lit_encode(float shadow, float ndl) { return shadow != 0.0f ? ndl : -ndl; }
+__device__ __forceinline__ void lit_decode(float w, float& shadow, float& ndl) {
+  shadow = w < 0.0f ? 0.0f : 1.0f;
+  ndl = fabsf(w);
+}
+
+// shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT / LAMBERT_VCOL.  `nrm` is the 3D
+// scene's normals for render and the projected scene's for edge samples (src/drender.h:353-354).
+// lit_out (LAMBERT_VCOL, may be NULL) receives the hit's light terms, lit_encode'd.
 template <int MODE>
 __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
-                                    const float* nrm, int* wst, Hit* hit_out = nullptr) {
+                                    const float* nrm, int* wst, Hit* hit_out = nullptr, float* lit_out = nullptr) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
   const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
@@ -396,24 +434,19 @@ __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCa
   const DevScene& s = T.s;
   const uint32_t A = s.idx[3 * h.gtri], B = s.idx[3 * h.gtri + 1], C = s.idx[3 * h.gtri + 2];
   if (MODE == 2) return blend3(ld3(s.col + 3 * A), ld3(s.col + 3 * B), ld3(s.col + 3 * C), h.u, h.v);
-  const int m = s.tri_mesh[h.gtri];
-  const F2 tcv = blend2(s.tc, A, B, C, h.u, h.v);
-  const F3 diffuse = bilinear(s, m, tcv);
-  if (MODE == 3) return diffuse;
-  const F3 n = blend3(ld3(nrm + 3 * A), ld3(nrm + 3 * B), ld3(nrm + 3 * C), h.u, h.v);
-  const float BIAS = 1e-5f;
-  const F3 negL = mul(-1.0f, L.L);
-  const float ndl = max_std(0.0f, dot(n, negL));   // +0 or positive
-  // lit = (shadow * Lc) * ndl: with ndl == +0 and a finite light colour the shadow ray
-  // cannot change the result (bit for bit), so it is only cast where ndl > 0
-  float shadow = 1.0f;
-  if (ndl != 0.0f || !(fabsf(L.Lc.x) <= 3.4e38f && fabsf(L.Lc.y) <= 3.4e38f && fabsf(L.Lc.z) <= 3.4e38f)) {
-    const F3 surf = add(o, mul(h.t - BIAS, d));
-    const Hit sh = trace<true>(T, surf, negL, 0.0f, 1e9f, wst);
-    shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
+  F3 diffuse;
+  if (MODE == MODE_LAMBERT_VCOL) {
+    diffuse = blend3(ld3(s.col + 3 * A), ld3(s.col + 3 * B), ld3(s.col + 3 * C), h.u, h.v);
+  } else {
+    const int m = s.tri_mesh[h.gtri];
+    const F2 tcv = blend2(s.tc, A, B, C, h.u, h.v);
+    diffuse = bilinear(s, m, tcv);
+    if (MODE == 3) return diffuse;
   }
-  const F3 lit = mul(mul(shadow, L.Lc), ndl);
-  return mulv(add(lit, L.amb), diffuse);
+  float shadow, ndl;
+  lambert_terms(T, L, o, d, h, A, B, C, nrm, wst, shadow, ndl);
+  if (MODE == MODE_LAMBERT_VCOL && lit_out) *lit_out = lit_encode(shadow, ndl);
+  return mulv(lambert_factor(L, shadow, ndl), diffuse);
 }
 
 // ---------------------------------------------------------------------------

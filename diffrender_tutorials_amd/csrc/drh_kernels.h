@@ -146,7 +146,8 @@ struct RenderArgs {
   double* loss_part;
   int part_off;
   float4* hits;           // non-NULL: store each sample's closest hit (id bits, u, v, t) for the
-                          // interior pass, at ((pixel) * sq*sq + sample) of the views' pixels
+                          // interior pass, at ((pixel) * sq*sq + sample) of the views' pixels;
+                          // LAMBERT_VCOL stores lit_encode(shadow, ndl) in place of t
 };
 
 struct InteriorArgs {
@@ -168,6 +169,7 @@ struct InteriorArgs {
   unsigned long long* dacc;  // deterministic mode: 32.32 fixed-point sums per gradient float (else NULL)
   int pos_grad;           // VERTEX_COLOR: add the interior position term (k_interior<2, true>)
   const float* tr_inv0;   // mesh 0's inverse transform (for pos_grad)
+  Light light;            // LAMBERT_VCOL: the light of the colour adjoint's factor
 };
 
 struct ProjectArgs {

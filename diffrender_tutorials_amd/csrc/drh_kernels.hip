@@ -464,9 +464,11 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
       const float xo = (dx + 0.5f) / float(sq);
       const float yo = (dy + 0.5f) / float(sq);
       Hit h;
-      col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h);
+      float lit = 0.0f;
+      col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
       if (a.hits)
-        a.hits[(c.img_off / 3 + (long long)y * c.w + x) * tm.S + tm.s] = make_float4(__uint_as_float(h.gtri), h.u, h.v, h.t);
+        a.hits[(c.img_off / 3 + (long long)y * c.w + x) * tm.S + tm.s] =
+            make_float4(__uint_as_float(h.gtri), h.u, h.v, MODE == MODE_LAMBERT_VCOL ? lit : h.t);
     }
     const int base = lane - tm.s;
     for (int k = 0; k < tm.S; k++) {
@@ -479,10 +481,11 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
         const float xo = (dx + 0.5f) / float(sq);
         const float yo = (dy + 0.5f) / float(sq);
         Hit h;
-        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h);
+        float lit = 0.0f;
+        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
         if (a.hits)
           a.hits[(c.img_off / 3 + (long long)y * c.w + x) * (sq * sq) + dy * sq + dx] =
-              make_float4(__uint_as_float(h.gtri), h.u, h.v, h.t);
+              make_float4(__uint_as_float(h.gtri), h.u, h.v, MODE == MODE_LAMBERT_VCOL ? lit : h.t);
         acc = add(acc, divs(col, ns));
       }
   }
@@ -594,8 +597,12 @@ void launch_det_finish(const unsigned long long* dacc, int G, float* grad, float
 // POS (VERTEX_COLOR only): also the position gradient of the interior term, dF/du du/dV +
 // dF/dv dv/dV through the barycentrics of each sample's hit (bary_grad), the term the
 // reference's shade_grad<VERTEX_COLOR> has written out but disabled (src/shade_simple.cpp:75-108).
+// LAMBERT_VCOL: the colour gradient of VERTEX_COLOR (shade_simple.cpp:39-74) times the light
+// factor (shadow * Lc) * ndl + ambient of the hit (the derivative of its shade), which needs the
+// hit's normal and shadow ray (traced again, or decoded from the render's stored hit).
 template <int MODE, bool POS = false>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
+  constexpr bool VCOL_LIKE = MODE == 2 || MODE == MODE_LAMBERT_VCOL;
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
   __shared__ int wst[64];
   const Tracer& T = a.tr;
@@ -618,7 +625,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   // A wave's samples cover one compact pixel tile, so neighbouring lanes mostly hit the same
   // triangle (vertex colours) or texel quad (texture): the per-sample terms are summed over
   // runs of equal target in the wave and each run issues one set of atomics.
-  const bool runs = !a.coverage && (MODE == 2 || !a.uniform);
+  const bool runs = !a.coverage && (VCOL_LIKE || !a.uniform);
   for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
     for (int j = 0; j < rt * rt; j++) {
       const int tx = (unit % units_x) * rt + j % rt, ty = (unit / units_x) * rt + j / rt;
@@ -640,6 +647,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
         int o0 = 0, o1 = 0, o2 = 0, o3 = 0;   // offsets (in G) the four float3 terms go to
         int p0 = 0, p1 = 0, p2 = 0;           // POS: offsets of the three vertex positions
         F3 ro = {0, 0, 0}, rd = {0, 0, 1};   // this sample's camera ray
+        float lit_stored = 0.0f;             // LAMBERT_VCOL with stored hits: lit_encode'd terms
         do {
           if (!inside) break;
           Hit h;
@@ -647,6 +655,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
             // the render's stored closest hit of this (grid) sample
             const float4 r = a.hits[(c.img_off / 3 + (long long)y * c.w + x) * a.nsamp + si];
             h = Hit{__float_as_uint(r.x), r.w, r.y, r.z};
+            lit_stored = r.w;
             if (a.coverage) { if (h.gtri != NO_HIT) uni = add(uni, val); break; }
           } else {
             const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
@@ -666,9 +675,17 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
           const int goff = a.layout[4 * m + 0];
           if (goff < 0) break;
           const int vo = a.tr.s.mesh_vo[m];
-          if (MODE == 2) {
+          if (VCOL_LIKE) {
+            F3 vs = val;
+            if (MODE == MODE_LAMBERT_VCOL) {
+              const uint32_t* I3 = a.tr.s.idx + 3 * h.gtri;
+              float shadow, ndl;
+              if (a.hits) lit_decode(lit_stored, shadow, ndl);
+              else lambert_terms(T, a.light, ro, rd, h, I3[0], I3[1], I3[2], a.tr.s.nrm, wst, shadow, ndl);
+              vs = mulv(val, lambert_factor(a.light, shadow, ndl));
+            }
             const float w = 1.0f - h.u - h.v;
-            const F3 cA = mul(w, val), cB = mul(h.v, val), cC = mul(h.u, val);
+            const F3 cA = mul(w, vs), cB = mul(h.v, vs), cC = mul(h.u, vs);
             g[0] = cA.x; g[1] = cA.y; g[2] = cA.z;
             g[3] = cB.x; g[4] = cB.y; g[5] = cB.z;
             g[6] = cC.x; g[7] = cC.y; g[8] = cC.z;
@@ -727,7 +744,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
           }
         } while (false);
         if (runs) {
-          const bool tail = seg_reduce<MODE == 2 ? (POS ? 18 : 9) : 12>(key, g);
+          const bool tail = seg_reduce<VCOL_LIKE ? (POS ? 18 : 9) : 12>(key, g);
           if (POS && tail && key >= 0) {
             const int op[3] = {p0, p1, p2};
             for (int q = 0; q < 3; q++) {
@@ -745,17 +762,17 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
               hash_add(hkey, hval, nh, a.grad, o0, g[0], g[1], g[2]);
               hash_add(hkey, hval, nh, a.grad, o1, g[3], g[4], g[5]);
               hash_add(hkey, hval, nh, a.grad, o2, g[6], g[7], g[8]);
-              if (MODE != 2) hash_add(hkey, hval, nh, a.grad, o3, g[9], g[10], g[11]);
+              if (!VCOL_LIKE) hash_add(hkey, hval, nh, a.grad, o3, g[9], g[10], g[11]);
             } else if (a.dacc) {
               const int o[4] = {o0, o1, o2, o3};
-              for (int q = 0; q < (MODE == 2 ? 3 : 4); q++)
+              for (int q = 0; q < (VCOL_LIKE ? 3 : 4); q++)
                 for (int c3 = 0; c3 < 3; c3++) dadd(a.dacc, o[q] + c3, g[3 * q + c3]);
             } else {
               float* G = use_lds ? lgrad : a.grad;
               atomicAdd(G + o0, g[0]); atomicAdd(G + o0 + 1, g[1]); atomicAdd(G + o0 + 2, g[2]);
               atomicAdd(G + o1, g[3]); atomicAdd(G + o1 + 1, g[4]); atomicAdd(G + o1 + 2, g[5]);
               atomicAdd(G + o2, g[6]); atomicAdd(G + o2 + 1, g[7]); atomicAdd(G + o2 + 2, g[8]);
-              if (MODE != 2) { atomicAdd(G + o3, g[9]); atomicAdd(G + o3 + 1, g[10]); atomicAdd(G + o3 + 2, g[11]); }
+              if (!VCOL_LIKE) { atomicAdd(G + o3, g[9]); atomicAdd(G + o3 + 1, g[10]); atomicAdd(G + o3 + 2, g[11]); }
             }
           }
         }
@@ -1520,6 +1537,7 @@ __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
     case 1: { constexpr int M = 1; BODY; } break;  \
     case 2: { constexpr int M = 2; BODY; } break;  \
     case 3: { constexpr int M = 3; BODY; } break;  \
+    case MODE_LAMBERT_VCOL: { constexpr int M = MODE_LAMBERT_VCOL; BODY; } break; \
     default: { constexpr int M = 4; BODY; } break; \
   }
 void launch_fill_f32(float* p, long long n, float v, hipStream_t s) {
@@ -1584,6 +1602,8 @@ void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s)
     hipLaunchKernelGGL((k_interior<2, true>), dim3(blocks), dim3(64), lds, s, a);
   } else if (mode == 2) {
     hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), lds, s, a);
+  } else if (mode == MODE_LAMBERT_VCOL) {
+    hipLaunchKernelGGL((k_interior<MODE_LAMBERT_VCOL>), dim3(blocks), dim3(64), lds, s, a);
   } else {
     hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), lds, s, a);
   }
@@ -1608,6 +1628,7 @@ void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work,
   hipLaunchKernelGGL(k_cdf_walk, dim3(1), dim3(1024), 0, s, pmf_raw, E, w);
   hipLaunchKernelGGL(k_cdf_final, dim3(blocks), dim3(CDF_STAGE_T), 0, s, pmf_raw, E, w, pmf, cdf);
 }
+
 void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s) {
   if (a.N <= 0) return;
   const int blocks = (int)std::max<long long>(1, std::min<long long>((a.N + 255) / 256, 4096));

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -44,7 +45,16 @@ struct Job {
   std::vector<float> targets;
   int edge_samples = 0, iterations = 0;
   float lr_pos = 1e-3f, lr_col = 1e-3f;
+  int flags = 0;     // DRH_JOB_*
   size_t bytes = 0;  // admission estimate
+  size_t hit_bytes = 0;   // DRH_JOB_ALIGNED_INTERIOR: stored hits charged at admission (0: recompute)
+  // bytes of the render's stored hits (16 per sample of every view)
+  size_t hits_needed() const {
+    const int sq = (int)std::sqrt((float)std::max(st.spp, 1));
+    size_t px = 0;
+    for (auto& c : cams) px += (size_t)c.width * (size_t)c.height;
+    return 16 * px * (size_t)(sq * sq);
+  }
   // run state
   drh_handle h = nullptr;
   void* d_targets = nullptr;
@@ -163,7 +173,7 @@ void drh_server_s::finish(Job* j, int status, const std::string& err) {
     j->h = nullptr;
   }
   std::lock_guard<std::mutex> lk(mu);
-  used -= std::min(used, j->bytes);
+  used -= std::min(used, j->bytes + j->hit_bytes);
   j->status = status;
   j->error = err;
   j->t_done = std::chrono::steady_clock::now();
@@ -181,6 +191,13 @@ bool drh_server_s::admit_one(Job* j) {
   // a short job refits its commit-time tree throughout: one captured step form instead of
   // two (capture and instantiation cost host time comparable to a short job's steps)
   (void)drh_set_bvh_rebuild(h, j->iterations <= 64 ? j->iterations + 1 : 8);
+  if (j->flags & DRH_JOB_DETERMINISTIC) (void)drh_set_deterministic(h, 1);
+  if (j->flags & DRH_JOB_FULL_JACOBIAN) (void)drh_set_edge_grad_mode(h, DRH_EDGE_GRAD_FULL_JACOBIAN);
+  if (j->flags & DRH_JOB_ALIGNED_INTERIOR) {
+    // the hit buffer was charged to the job at admission if it fitted the budget; else recompute
+    (void)drh_set_interior_samples(h, 1, DRH_HITS_AUTO);
+    (void)drh_set_hit_buffer_limit(h, j->hit_bytes);
+  }
   const drh_scene_desc d = j->desc();
   const size_t tbytes = 4 * j->targets.size();
   // every upload first, while the handle's stream is still empty (their synchronisations then
@@ -219,6 +236,13 @@ void drh_server_s::admit_run() {
         continue;
       }
       used += j->bytes;
+      // store the render's hits when they fit what is left of the budget (multi-view
+      // adjoint: storing saves the interior pass's rays, recomputing saves the HBM)
+      j->hit_bytes = 0;
+      if ((j->flags & DRH_JOB_ALIGNED_INTERIOR) && used + j->hits_needed() <= budget) {
+        j->hit_bytes = j->hits_needed();
+        used += j->hit_bytes;
+      }
       admitting++;
       adm = j;
     }
@@ -400,6 +424,7 @@ drh_status drh_server_submit(drh_server s, const drh_job_desc* d, int* job_id) {
   j->iterations = d->iterations;
   j->lr_pos = d->lr_pos;
   j->lr_col = d->lr_col;
+  j->flags = d->flags;
   j->bytes = estimate_bytes(*j);
   j->t_submit = std::chrono::steady_clock::now();
   std::lock_guard<std::mutex> lk(s->mu);
@@ -431,6 +456,8 @@ drh_status drh_server_wait(drh_server s, int job_id, double timeout_s, drh_job_r
       const int n = std::min<int>(r->latencies_cap, (int)j->latencies_ms.size());
       std::memcpy(r->latencies_ms, j->latencies_ms.data(), 4 * (size_t)n);
     }
+    r->hbm_bytes = j->bytes + j->hit_bytes;
+    r->hit_bytes = j->hit_bytes;
   }
   return j->status == 1 ? DRH_OK : (drh_status)j->status;
 }

@@ -21,11 +21,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DRH_LIB") or os.path.join(HERE, "libdrh.so")
 
 SILHOUETTE, VERTEX_COLOR, TEXTURE_COLOR, LAMBERT = 1, 2, 3, 4
+LAMBERT_VERTEX_COLOR = 8   # LAMBERT with the vertex colour as albedo (no counterpart in the reference)
 RNG_REF_PRNG14, RNG_QMC = 0, 1
 TRACER_BVH, TRACER_BF = 0, 1
 EDGE_ORDER_AUTO, EDGE_ORDER_SAMPLE, EDGE_ORDER_BINNED = 0, 1, 2
 EDGE_GRAD_REFERENCE_DIAGONAL, EDGE_GRAD_FULL_JACOBIAN = 0, 1
 OPT_SIMPLE, OPT_ADAM = 0, 1
+HITS_RECOMPUTE, HITS_STORE, HITS_AUTO = 0, 1, 2
+JOB_DETERMINISTIC, JOB_ALIGNED_INTERIOR, JOB_FULL_JACOBIAN = 1, 2, 4
 
 STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_E_HIP", -4: "DRH_E_OOM",
           -5: "DRH_E_UNSUPPORTED", -6: "DRH_E_DEVICE"}
@@ -41,7 +44,8 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
            "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild", "drh_set_edge_grad_mode",
-           "drh_set_deterministic", "drh_set_interior_pos_grad"]
+           "drh_set_deterministic", "drh_set_interior_pos_grad",
+           "drh_set_hit_buffer_limit", "drh_get_hit_buffer_bytes"]
 
 
 class DrhError(RuntimeError):
@@ -104,6 +108,8 @@ def load():
         L.drh_memcpy_htod.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.drh_memcpy_dtoh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.drh_debug_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
+        L.drh_set_hit_buffer_limit.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        L.drh_get_hit_buffer_bytes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
 
@@ -213,7 +219,16 @@ class DiffRender:
                                         ctypes.c_void_p(d_loss_ptr) if d_loss_ptr else None))
 
     def set_interior_samples(self, aligned, store=False):
+        """aligned: render-grid interior samples; store: HITS_RECOMPUTE / HITS_STORE / HITS_AUTO."""
         self._check(load().drh_set_interior_samples(self.h, int(aligned), int(store)))
+
+    def set_hit_buffer_limit(self, nbytes):
+        self._check(load().drh_set_hit_buffer_limit(self.h, int(nbytes)))
+
+    def hit_buffer_bytes(self):
+        n = ctypes.c_size_t()
+        self._check(load().drh_get_hit_buffer_bytes(self.h, ctypes.byref(n)))
+        return n.value
 
     def set_edge_order(self, order):
         """EDGE_ORDER_AUTO / _SAMPLE (rnd0 slot order) / _BINNED (kept samples by 8x8 pixel bin)."""
@@ -372,14 +387,15 @@ def scene_desc(meshes):
 class JobDesc(ctypes.Structure):
     _fields_ = [("settings", Settings), ("scene", SceneDesc), ("cams", ctypes.POINTER(CameraC)), ("views", ctypes.c_int),
                 ("targets", ctypes.POINTER(ctypes.c_float)), ("edge_samples", ctypes.c_int),
-                ("iterations", ctypes.c_int), ("lr_pos", ctypes.c_float), ("lr_col", ctypes.c_float)]
+                ("iterations", ctypes.c_int), ("lr_pos", ctypes.c_float), ("lr_col", ctypes.c_float),
+                ("flags", ctypes.c_int)]
 
 
 class JobResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("final_loss", ctypes.c_float),
                 ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("pos", ctypes.POINTER(ctypes.c_float)),
                 ("col", ctypes.POINTER(ctypes.c_float)), ("latencies_ms", ctypes.POINTER(ctypes.c_float)),
-                ("latencies_cap", ctypes.c_int)]
+                ("latencies_cap", ctypes.c_int), ("hbm_bytes", ctypes.c_size_t), ("hit_bytes", ctypes.c_size_t)]
 
 
 class ServerStats(ctypes.Structure):
@@ -414,12 +430,12 @@ class Server:
         self.V = {}
 
     def submit(self, meshes, cams, targets, iterations, edge_samples, mode=LAMBERT, spp=1, light=None,
-               lr_pos=1e-3, lr_col=1e-3, rng_mode=RNG_REF_PRNG14):
+               lr_pos=1e-3, lr_col=1e-3, rng_mode=RNG_REF_PRNG14, flags=0):
         d, keep = scene_desc(meshes)
         cs = cameras_array(cams)
         tg = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float32).reshape(-1) for t in targets]))
         j = JobDesc(settings_struct(mode, spp, rng_mode, light=light), d, cs, len(cams), _fp(tg), int(edge_samples),
-                    int(iterations), float(lr_pos), float(lr_col))
+                    int(iterations), float(lr_pos), float(lr_col), int(flags))
         self.last_job_bytes = load().drh_server_job_bytes(ctypes.byref(j))
         jid = ctypes.c_int()
         st = load().drh_server_submit(self.h, ctypes.byref(j), ctypes.byref(jid))
@@ -433,11 +449,11 @@ class Server:
         pos = np.empty(3 * V, np.float32)
         col = np.empty(3 * V, np.float32)
         lat = np.empty(iters, np.float32)
-        r = JobResult(0, 0, 0.0, 0.0, 0.0, _fp(pos), _fp(col), _fp(lat), iters)
+        r = JobResult(0, 0, 0.0, 0.0, 0.0, _fp(pos), _fp(col), _fp(lat), iters, 0, 0)
         st = load().drh_server_wait(self.h, job_id, float(timeout), ctypes.byref(r))
         return {"status": st, "iterations": r.iterations, "final_loss": r.final_loss, "queue_ms": r.queue_ms,
                 "run_ms": r.run_ms, "pos": pos.reshape(-1, 3), "col": col.reshape(-1, 3),
-                "latencies_ms": lat[:r.iterations]}
+                "latencies_ms": lat[:r.iterations], "hbm_bytes": r.hbm_bytes, "hit_bytes": r.hit_bytes}
 
     def stats(self):
         s = ServerStats()

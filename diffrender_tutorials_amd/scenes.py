@@ -209,6 +209,51 @@ def icosphere(subdiv: int, radius: float = 1.0) -> Mesh:
     return Mesh(pos=pos, idx=np.array(faces, dtype=np.uint32), nrm=nrm)
 
 
+def cube_sphere(n: int, radius: float = 1.0) -> Mesh:
+    """A cube whose faces are n x n grids of squares (two triangles each), every vertex
+    pushed onto the sphere: F = 12 n^2 triangles, V = 6 n^2 + 2 vertices (n = 1 is the
+    12-triangle cube, n = 81 gives 78,732 triangles, n = 129 gives 199,692).  Outward winding,
+    normals = positions / radius."""
+    n = int(n)
+    if n < 1:
+        raise ValueError("n >= 1")
+    g = np.arange(n + 1)
+    I, J = np.meshgrid(g, g, indexing="ij")
+    pts, quads = [], []
+    for axis in range(3):
+        u, v = [a for a in range(3) if a != axis]
+        for side in (0, n):
+            P = np.zeros((n + 1, n + 1, 3), np.int64)
+            P[..., axis] = side
+            P[..., u] = I
+            P[..., v] = J
+            base = sum(len(p) for p in pts)
+            pts.append(P.reshape(-1, 3))
+            k = np.arange((n + 1) * (n + 1)).reshape(n + 1, n + 1) + base
+            a, b, c, d = k[:-1, :-1], k[1:, :-1], k[1:, 1:], k[:-1, 1:]
+            quads.append(np.stack([a, b, c, d], -1).reshape(-1, 4))
+    P = np.concatenate(pts)
+    key = (P[:, 0] * (n + 1) + P[:, 1]) * (n + 1) + P[:, 2]
+    uniq, inv = np.unique(key, return_inverse=True)
+    Q = inv[np.concatenate(quads)]
+    tris = np.concatenate([Q[:, [0, 1, 2]], Q[:, [0, 2, 3]]])
+    grid = np.stack([uniq // ((n + 1) * (n + 1)), (uniq // (n + 1)) % (n + 1), uniq % (n + 1)], -1)
+    p = grid.astype(np.float64) * (2.0 / n) - 1.0
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    e1, e2 = p[tris[:, 1]] - p[tris[:, 0]], p[tris[:, 2]] - p[tris[:, 0]]
+    inward = np.sum(np.cross(e1, e2) * (p[tris[:, 0]] + p[tris[:, 1]] + p[tris[:, 2]]), axis=1) < 0
+    tris[inward] = tris[inward][:, [0, 2, 1]]
+    return Mesh(pos=(p * radius).astype(np.float32), idx=tris.astype(np.uint32), nrm=p.astype(np.float32))
+
+
+def cube_sphere_for(F: int, max_triangles: int) -> Mesh:
+    """The cube_sphere whose 12 n^2 triangles come closest to F, at most max_triangles."""
+    n = max(1, int(round(math.sqrt(F / 12.0))))
+    while n > 1 and 12 * n * n > max_triangles:
+        n -= 1
+    return cube_sphere(n)
+
+
 def one_ring(mesh: Mesh) -> list[np.ndarray]:
     nb = [set() for _ in range(mesh.V)]
     for a, b, c in mesh.idx:
@@ -311,38 +356,44 @@ def lambert_config2():
 
 def config3_jobs(n: int = 64, seed: int = 5):
     """Config 3's uneven batch: per job a triangle count F = round(exp(U(ln 12, ln 80000)))
-    (seed 5) met by the closest icosphere (20 * 4^k triangles) or the 12-triangle cube,
-    resolution uniform in {128, 256, 512}, vertex-colour or Lambert shading at random, edge
-    samples = 4 x pixels; the target is the mesh with radial noise 0.05.  Returns dicts with
-    meshes, target, cams, mode (2 VCOL / 4 LAMBERT), spp, edge_samples."""
+    (seed 5) met by the closest subdivided cube on the sphere (cube_sphere, 12 n^2 triangles,
+    at most 80,000: from the 12-triangle cube to 78,732), resolution uniform in {128, 256,
+    512}, vertex-colour or Lambert shading at random, edge samples = 4 x pixels; the target is
+    the mesh with radial noise 0.05.  Returns dicts with meshes, target, cams, mode (2 VCOL /
+    4 LAMBERT), spp, edge_samples and the drawn triangle count F."""
     rng = np.random.default_rng(seed)
     jobs = []
     for k in range(n):
         F = int(round(math.exp(rng.uniform(math.log(12), math.log(80000)))))
         res = int(rng.choice([128, 256, 512]))
         mode = 2 if rng.uniform() < 0.5 else 4
-        if F < 16:
-            base = cube(0.5, seed=100 + k)
-        else:
-            lvl = int(np.clip(round(math.log(F / 20.0) / math.log(4.0)), 0, 6))
-            base = icosphere(lvl, 1.0)
-        target = radial_displace(base, 0.05, seed=200 + k, smooth=False)
-        if mode == 2:
-            m, t = with_random_colors(base, 300 + k), with_random_colors(target, 400 + k)
-        else:
-            m, t = with_albedo(base, 0.8), with_albedo(target, 0.8)
         az = rng.uniform(0, 2 * math.pi)
-        eye = (3.0 * math.sin(az), 0.8, 3.0 * math.cos(az))
-        cam = camera_look(eye, res, res)
-        jobs.append(dict(meshes=[m], target=[t], cams=[cam], mode=mode, spp=1, edge_samples=4 * res * res))
+        jobs.append(config3_job(k, F, res, mode, az))
     return jobs
 
 
+def config3_job(k: int, F: int, res: int, mode: int, az: float) -> dict:
+    """One config-3 job: the cube_sphere closest to F triangles (at most 80,000), a target
+    with radial noise 0.05 (seed 200 + k), vertex colours (mode 2, seeds 300 + k / 400 + k) or
+    albedo 0.8 (mode 4), one camera at azimuth `az` and distance 3, res x res, spp 1, edge
+    samples = 4 x pixels."""
+    base = cube_sphere_for(F, 80000)
+    target = radial_displace(base, 0.05, seed=200 + k, smooth=False)
+    if mode == 2:
+        m, t = with_random_colors(base, 300 + k), with_random_colors(target, 400 + k)
+    else:
+        m, t = with_albedo(base, 0.8), with_albedo(target, 0.8)
+    eye = (3.0 * math.sin(az), 0.8, 3.0 * math.cos(az))
+    cam = camera_look(eye, res, res)
+    return dict(meshes=[m], target=[t], cams=[cam], mode=mode, spp=1, edge_samples=4 * res * res, F=F)
+
+
 def config5_jobs(n: int = 32, seed: int = 7):
-    """Config 5's serving stream: per job a triangle count log-uniform in [12, 200k] (closest
-    icosphere or the cube), resolution uniform in {128, 256, 512, 1024}, 1-8 views on a ring,
-    50-500 Adam iterations, Lambert shading, radial-noise-0.05 targets, N = 4 x pixels of one
-    view.  `n` jobs with seed 7; arrival times are drawn by the caller."""
+    """Config 5's serving stream: per job a triangle count log-uniform in [12, 200k] (the
+    closest cube_sphere, at most 200,000 triangles: from the 12-triangle cube to 199,692),
+    resolution uniform in {128, 256, 512, 1024}, 1-8 views on a ring, 50-500 Adam iterations,
+    Lambert shading, radial-noise-0.05 targets, N = 4 x pixels of one view.  `n` jobs with
+    seed 7; arrival times are drawn by the caller."""
     rng = np.random.default_rng(seed)
     jobs = []
     for k in range(n):
@@ -350,16 +401,19 @@ def config5_jobs(n: int = 32, seed: int = 7):
         res = int(rng.choice([128, 256, 512, 1024]))
         views = int(rng.integers(1, 9))
         iters = int(rng.integers(50, 501))
-        if F < 16:
-            base = cube(0.5, seed=500 + k)
-        else:
-            lvl = int(np.clip(round(math.log(F / 20.0) / math.log(4.0)), 0, 6))   # <= 81920 triangles
-            base = icosphere(lvl, 1.0)
-        target = radial_displace(base, 0.05, seed=600 + k, smooth=False)
-        cams = cameras_ring(views, res, res)
-        jobs.append(dict(meshes=[with_albedo(base, 0.8)], target=[with_albedo(target, 0.8)], cams=cams, mode=4,
-                         spp=1, edge_samples=4 * res * res, iterations=iters))
+        jobs.append(config5_job(k, F, res, views, iters))
     return jobs
+
+
+def config5_job(k: int, F: int, res: int, views: int, iters: int) -> dict:
+    """One config-5 job: the cube_sphere closest to F triangles (at most 200,000) with albedo
+    0.8, a target with radial noise 0.05 (seed 600 + k), `views` ring cameras at res x res,
+    Lambert, spp 1, edge samples = 4 x pixels of one view, `iters` Adam iterations."""
+    base = cube_sphere_for(F, 200000)
+    target = radial_displace(base, 0.05, seed=600 + k, smooth=False)
+    cams = cameras_ring(views, res, res)
+    return dict(meshes=[with_albedo(base, 0.8)], target=[with_albedo(target, 0.8)], cams=cams, mode=4,
+                spp=1, edge_samples=4 * res * res, iterations=iters, F=F)
 
 
 def uv_sphere(n_lat: int, n_lon: int, radius: float = 1.0) -> Mesh:

@@ -41,8 +41,15 @@ typedef enum {
                                 outputs of the calls since the last synchronisation are not valid */
 } drh_status;
 
-/* SHADING_MODEL numbering (src/dmesh.h:16-23); PHONG / GGX / PATH_TEST are not on this path. */
-enum { DRH_SILHOUETTE = 1, DRH_VERTEX_COLOR = 2, DRH_TEXTURE_COLOR = 3, DRH_LAMBERT = 4 };
+/* SHADING_MODEL numbering (src/dmesh.h:16-23); PHONG / GGX / PATH_TEST are not on this path.
+ * DRH_LAMBERT_VERTEX_COLOR has no counterpart in the reference: shade<LAMBERT>
+ * (src/shade_lambert.cpp:6-35) with the interpolated vertex colour as the albedo instead of
+ * texture 0, the shading of multi-view mesh fitting with per-vertex colours.  Its gradient
+ * block is VERTEX_COLOR's ([pos 3V][colour 3V][transform 12]); the interior term's colour
+ * adjoint is shade_grad<VERTEX_COLOR>'s (src/shade_simple.cpp:39-74) times the hit's light
+ * factor (shadow * Lc) * max(0, n.(-L)) + ambient, the derivative of the shaded colour. */
+enum { DRH_SILHOUETTE = 1, DRH_VERTEX_COLOR = 2, DRH_TEXTURE_COLOR = 3, DRH_LAMBERT = 4,
+       DRH_LAMBERT_VERTEX_COLOR = 8 };
 /* Edge-sample random numbers: the reference's 14 OpenMP PRNG streams (src/drender.h:306-331),
  * or index-addressed Niederreiter QMC (common_src/qmc.cpp:190-197). */
 enum { DRH_RNG_REF_PRNG14 = 0, DRH_RNG_QMC = 1 };
@@ -190,11 +197,23 @@ drh_status drh_get_restricted(drh_handle h, float* params);
 /* Interior samples (interior_derivatives, src/drender.h:197-233).  aligned = 0 (default): the
  * reference's Hammersley offsets, so every interior sample is a fresh ray (recompute).
  * aligned = 1: the render's grid offsets (d + 0.5)/sqrt(spp) instead -- another unbiased
- * estimate of the same pixel integral, not the reference's samples.  store = 1 (aligned only):
- * d_render_and_compare and the fused step keep each render sample's closest hit (16 B per
- * sample in HBM) and the interior pass reads it back instead of tracing; d_render alone
- * recomputes.  No counterpart in the reference. */
+ * estimate of the same pixel integral, not the reference's samples.  store (aligned only):
+ *   DRH_HITS_RECOMPUTE (0)  the interior pass traces every sample again;
+ *   DRH_HITS_STORE (1)      d_render_and_compare and the fused step keep each render sample's
+ *                           closest hit (16 B per sample in HBM) and the interior pass reads it
+ *                           back instead of tracing (d_render alone recomputes);
+ *   DRH_HITS_AUTO (2)       store when the buffer (16 B x pixels x samples of all views) fits
+ *                           the handle's hit-buffer limit (drh_set_hit_buffer_limit) and can be
+ *                           allocated, recompute otherwise (holding no buffer).
+ * Stored and recomputed hits give the same gradient up to float summation order.  No
+ * counterpart in the reference. */
+enum { DRH_HITS_RECOMPUTE = 0, DRH_HITS_STORE = 1, DRH_HITS_AUTO = 2 };
 drh_status drh_set_interior_samples(drh_handle h, int aligned, int store);
+/* The most HBM DRH_HITS_AUTO may use for stored hits (default: no limit but the device's free
+ * memory); the server sets it from its HBM budget.  No counterpart in the reference. */
+drh_status drh_set_hit_buffer_limit(drh_handle h, size_t bytes);
+/* Bytes the handle holds for stored hits now (0 while it recomputes). */
+drh_status drh_get_hit_buffer_bytes(drh_handle h, size_t* bytes);
 /* Order in which the edge pass (edge_derivatives, src/drender.h:235-425) visits its samples.
  * The samples, their edges and their contributions are the reference's in every order; only
  * the order of the gradient sums (float rounding) differs.
@@ -265,7 +284,18 @@ typedef struct {
   int edge_samples;            /* per view per iteration */
   int iterations;
   float lr_pos, lr_col;        /* Adam learning rates */
+  int flags;                   /* DRH_JOB_* (0: the defaults of a fresh handle) */
 } drh_job_desc;
+
+/* drh_job_desc.flags:
+ *   DRH_JOB_DETERMINISTIC     drh_set_deterministic(1): a job's results do not depend on what
+ *                             else the server runs beside it, bit for bit;
+ *   DRH_JOB_ALIGNED_INTERIOR  drh_set_interior_samples(1, DRH_HITS_AUTO): the render's hits are
+ *                             stored for the interior pass when they fit what is left of the
+ *                             server's HBM budget at admission (charged to the job), and
+ *                             recomputed otherwise;
+ *   DRH_JOB_FULL_JACOBIAN     drh_set_edge_grad_mode(DRH_EDGE_GRAD_FULL_JACOBIAN). */
+enum { DRH_JOB_DETERMINISTIC = 1, DRH_JOB_ALIGNED_INTERIOR = 2, DRH_JOB_FULL_JACOBIAN = 4 };
 
 typedef struct {
   drh_status status;
@@ -276,6 +306,8 @@ typedef struct {
   float* col;                  /* out (may be NULL): 3V colours */
   float* latencies_ms;         /* out (may be NULL): per-iteration latency */
   int latencies_cap;
+  size_t hbm_bytes;            /* HBM charged against the budget (footprint + stored hits) */
+  size_t hit_bytes;            /* of which the stored hits (0: the interior pass recomputed) */
 } drh_job_result;
 
 typedef struct {

@@ -47,6 +47,7 @@ LIB_PATH = os.path.join(_HERE, "liboracle.so")
 REF_HARNESS = os.path.join(_HERE, "_ref", "ref_harness")
 
 SILHOUETTE, VERTEX_COLOR, TEXTURE_COLOR, LAMBERT = 1, 2, 3, 4
+LAMBERT_VERTEX_COLOR = 8   # LAMBERT with the vertex colour as albedo (no counterpart in the reference)
 RNG_REF_PRNG14, RNG_QMC = 0, 1
 TRACER_BF, TRACER_BVH = 0, 1
 
@@ -159,12 +160,15 @@ class Scene:
             off += c.width * c.height * 3
         return res
 
-    def d_render(self, mode, spp, cams, adjoints, edge_samples, rng_mode=RNG_REF_PRNG14, lam=None, grad=None):
+    def d_render(self, mode, spp, cams, adjoints, edge_samples, rng_mode=RNG_REF_PRNG14, lam=None, grad=None,
+                 aligned=False):
+        """aligned: interior samples at the render's grid offsets instead of Hammersley points."""
         cs = (_Cam * len(cams))(*[cam_struct(c) for c in cams])
         adj = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1) for a in adjoints]))
         if grad is None:
             grad = np.zeros(self.grad_size(mode), np.float32)
-        lib().orc_d_render(self.h, mode, spp, lam_struct(lam), cs, len(cams), _p(adj), edge_samples, rng_mode, _p(grad))
+        lib().orc_d_render_ex(self.h, mode, spp, lam_struct(lam), cs, len(cams), _p(adj), edge_samples, rng_mode,
+                              1 if aligned else 0, _p(grad))
         return grad
 
     def silhouette_functional(self, cam, adjoint):

@@ -331,8 +331,8 @@ int grad_layout(const Scene& s, int mode, std::vector<int>* mesh_off) {
     if (mesh_off) (*mesh_off)[i] = off;
     const int nv = s.vo[i + 1] - s.vo[i];
     off += 3 * nv;                                   // positions
-    if (mode == ORC_VERTEX_COLOR) off += 3 * nv;     // colours
-    if (mode != ORC_SILHOUETTE && mode != ORC_VERTEX_COLOR) off += 3 * s.tex_w[i] * s.tex_h[i];
+    if (mode == ORC_VERTEX_COLOR || mode == ORC_LAMBERT_VERTEX_COLOR) off += 3 * nv;    // colours
+    if (mode == ORC_TEXTURE_COLOR || mode == ORC_LAMBERT) off += 3 * s.tex_w[i] * s.tex_h[i];
     off += 12;                                       // one instance transform (DMesh::TRANSFORM_SIZE)
   }
   return off;
@@ -581,8 +581,24 @@ inline f2 blend2(f2 a, f2 b, f2 c, float u, float v) {
   return {a.x * w + b.x * v + u * c.x, a.y * w + b.y * v + u * c.y};
 }
 
+// LAMBERT's light factor at a closest hit (shade_lambert.cpp:26-34):
+// (shadow * Lc) * max(0, n.(-L)) + ambient with the interpolated, un-normalised normal n and
+// shadow = 1 unless the ray from pos + (t - 1e-5) dir toward -L hits anything.
+f3 lambert_light(const Scene& s, const Lam& lam, f3 o, f3 d, const Hit& h, const std::vector<f3>& nrm) {
+  const uint32_t g = h.gtri;
+  const uint32_t A = s.idx[3 * g + 0], B = s.idx[3 * g + 1], C = s.idx[3 * g + 2];
+  const f3 n = blend3(nrm[A], nrm[B], nrm[C], h.u, h.v);
+  const float BIAS = 1e-5;
+  const f3 surf = add(o, mul(h.t - BIAS, d));
+  const f3 negL = mul(-1.0f, lam.L);
+  const float shadow = trace(s, surf, negL, 0.0f, 1e9f, true).prim == 0xFFFFFFFFu ? 1.0f : 0.0f;
+  const f3 lit = mul(mul(shadow, lam.Lc), fmax_std(0.0f, dot(n, negL)));
+  return add(lit, lam.amb);
+}
+
 // shade<m> (shade_simple.cpp:6-37,113-130; shade_lambert.cpp:6-35). `nrm` selects the
 // normal array: the 3D scene's for render, the projected scene's for edge samples.
+// ORC_LAMBERT_VERTEX_COLOR: LAMBERT with the interpolated vertex colour as albedo.
 f3 shade(const Scene& s, int mode, const Lam& lam, const Cam& c, f2 sp, const std::vector<f3>& nrm) {
   f3 o, d;
   eye_ray(c, sp.x, sp.y, o, d);
@@ -592,17 +608,15 @@ f3 shade(const Scene& s, int mode, const Lam& lam, const Cam& c, f2 sp, const st
   const uint32_t g = h.gtri;
   const uint32_t A = s.idx[3 * g + 0], B = s.idx[3 * g + 1], C = s.idx[3 * g + 2];
   if (mode == ORC_VERTEX_COLOR) return blend3(s.col[A], s.col[B], s.col[C], h.u, h.v);
-  const f2 tc = blend2(s.tc[A], s.tc[B], s.tc[C], h.u, h.v);
-  const f3 diffuse = bilinear(s, h.geom, tc);
-  if (mode == ORC_TEXTURE_COLOR) return diffuse;
-  // LAMBERT
-  const f3 n = blend3(nrm[A], nrm[B], nrm[C], h.u, h.v);
-  const float BIAS = 1e-5;
-  const f3 surf = add(o, mul(h.t - BIAS, d));
-  const f3 negL = mul(-1.0f, lam.L);
-  const float shadow = trace(s, surf, negL, 0.0f, 1e9f, true).prim == 0xFFFFFFFFu ? 1.0f : 0.0f;
-  const f3 lit = mul(mul(shadow, lam.Lc), fmax_std(0.0f, dot(n, negL)));
-  return mulv(add(lit, lam.amb), diffuse);
+  f3 diffuse;
+  if (mode == ORC_LAMBERT_VERTEX_COLOR) {
+    diffuse = blend3(s.col[A], s.col[B], s.col[C], h.u, h.v);   // the vertex colour as albedo
+  } else {
+    const f2 tc = blend2(s.tc[A], s.tc[B], s.tc[C], h.u, h.v);
+    diffuse = bilinear(s, h.geom, tc);
+    if (mode == ORC_TEXTURE_COLOR) return diffuse;
+  }
+  return mulv(lambert_light(s, lam, o, d, h, nrm), diffuse);
 }
 
 // ----------------------------------------------------------------------------
@@ -844,8 +858,14 @@ float orc_loss(const float* image, const float* target, int npix, float* adjoint
 
 void orc_d_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc_camera* cams, int views,
                   const float* adjoints, int N, int rng_mode, float* grad) {
+  orc_d_render_ex(sp, mode, spp, lam, cams, views, adjoints, N, rng_mode, 0, grad);
+}
+
+void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const orc_camera* cams, int views,
+                     const float* adjoints, int N, int rng_mode, int flags, float* grad) {
   Scene& s = *(Scene*)sp;
   const Lam L = make_lam(lam);
+  const bool aligned = (flags & ORC_INTERIOR_ALIGNED) != 0;
   if (s.tracer == ORC_TRACER_BVH) suspects_direction(s, mul(-1.0f, L.L));
   std::vector<int> moff;
   const int G = grad_layout(s, mode, &moff);
@@ -884,7 +904,10 @@ void orc_d_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc
         float* g = part[tid].data();
         for (int x = 0; x < W; x++)
           for (int si = 0; si < nsamp; si++) {
-            const float xo = hamm[2 * si], yo = hamm[2 * si + 1];
+            // Hammersley offsets (drender.h:217-218), or with `aligned` the render's grid
+            // offsets (d + 0.5)/sq (drender.h:124-128)
+            const float xo = aligned ? ((si % sq) + 0.5f) / float(sq) : hamm[2 * si];
+            const float yo = aligned ? ((si / sq) + 0.5f) / float(sq) : hamm[2 * si + 1];
             const f3 val = divs(adj_at(x, y), (float)nsamp);
             f3 o, d;
             eye_ray(c, x + xo, y + yo, o, d);
@@ -894,10 +917,12 @@ void orc_d_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc
             if (moff[m] < 0) continue;
             const uint32_t gt = h.gtri;
             const int nv = s.vo[m + 1] - s.vo[m];
-            if (mode == ORC_VERTEX_COLOR) {
-              // shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74)
+            if (mode == ORC_VERTEX_COLOR || mode == ORC_LAMBERT_VERTEX_COLOR) {
+              // shade_grad<VERTEX_COLOR> (shade_simple.cpp:39-74); with the light factor of the
+              // hit for LAMBERT_VERTEX_COLOR (the derivative of its shade in the colours)
+              const f3 vs = mode == ORC_VERTEX_COLOR ? val : mulv(val, lambert_light(s, L, o, d, h, s.nrm));
               const float w = 1.0f - h.u - h.v;
-              const f3 cA = mul(w, val), cB = mul(h.v, val), cC = mul(h.u, val);
+              const f3 cA = mul(w, vs), cB = mul(h.v, vs), cC = mul(h.u, vs);
               float* gc = g + moff[m] + 3 * nv;
               const uint32_t A = s.idx_local[3 * gt], B = s.idx_local[3 * gt + 1], C = s.idx_local[3 * gt + 2];
               gc[3 * A] += cA.x; gc[3 * A + 1] += cA.y; gc[3 * A + 2] += cA.z;
@@ -974,8 +999,8 @@ void orc_d_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc
         gp[e.vn1 * 3 + 2] += T(2,0) * dv1_dx + T(2,1) * dv1_dy + T(2,2) * dv1_dz;
         const int nvm = s.vo[e.mesh + 1] - s.vo[e.mesh];
         int trs = 3 * nvm;
-        if (mode == ORC_VERTEX_COLOR) trs += 3 * nvm;
-        if (mode != ORC_SILHOUETTE && mode != ORC_VERTEX_COLOR) trs += 3 * s.tex_w[e.mesh] * s.tex_h[e.mesh];
+        if (mode == ORC_VERTEX_COLOR || mode == ORC_LAMBERT_VERTEX_COLOR) trs += 3 * nvm;
+        if (mode == ORC_TEXTURE_COLOR || mode == ORC_LAMBERT) trs += 3 * s.tex_w[e.mesh] * s.tex_h[e.mesh];
         float* gt = gp + trs + 12 * e.inst;
         const f3 o0 = s.pos_orig[e.v0], o1 = s.pos_orig[e.v1];
         gt[0] += o0.x * dv0_dx + o1.x * dv1_dx; gt[1] += o0.y * dv0_dx + o1.y * dv1_dx;

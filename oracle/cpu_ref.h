@@ -19,7 +19,9 @@ extern "C" {
 #endif
 
 /* Shading models: same numbering as SHADING_MODEL (src/dmesh.h:16-23). */
-enum { ORC_SILHOUETTE = 1, ORC_VERTEX_COLOR = 2, ORC_TEXTURE_COLOR = 3, ORC_LAMBERT = 4 };
+enum { ORC_SILHOUETTE = 1, ORC_VERTEX_COLOR = 2, ORC_TEXTURE_COLOR = 3, ORC_LAMBERT = 4,
+       /* LAMBERT with the interpolated vertex colour as albedo (no counterpart in the reference) */
+       ORC_LAMBERT_VERTEX_COLOR = 8 };
 /* Edge-sample random numbers: the reference's 14 OpenMP PRNG streams
  * (src/drender.h:306-331) or index-addressed Niederreiter QMC (common_src/qmc.cpp:190-197). */
 enum { ORC_RNG_REF_PRNG14 = 0, ORC_RNG_QMC = 1 };
@@ -60,6 +62,12 @@ void  orc_render(void* s, int mode, int spp, const orc_lambert* lam, const orc_c
 /* DiffRender::d_render (src/drender.h:144-167); accumulates into grad. */
 void  orc_d_render(void* s, int mode, int spp, const orc_lambert* lam, const orc_camera* cams, int views,
                    const float* adjoints, int edge_samples, int rng_mode, float* grad);
+/* The same with flags: ORC_INTERIOR_ALIGNED puts the interior samples at the render's grid
+ * offsets (d + 0.5)/sqrt(spp) (src/drender.h:124-128) instead of the reference's Hammersley
+ * offsets (src/drender.h:217-218). */
+enum { ORC_INTERIOR_ALIGNED = 1 };
+void  orc_d_render_ex(void* s, int mode, int spp, const orc_lambert* lam, const orc_camera* cams, int views,
+                      const float* adjoints, int edge_samples, int rng_mode, int flags, float* grad);
 /* LossAndDiffLoss (src/utils.cpp:15-27). Returns the float-cast double sum. */
 float orc_loss(const float* image, const float* target, int npixels, float* adjoint);
 

@@ -59,6 +59,10 @@ drh_status drh_commit(drh_handle h, const drh_scene_desc* d) {
 }
 drh_status drh_set_device_sharing(drh_handle h, int) { live(h, "drh_set_device_sharing"); return DRH_OK; }
 drh_status drh_set_bvh_rebuild(drh_handle h, int) { live(h, "drh_set_bvh_rebuild"); return DRH_OK; }
+drh_status drh_set_deterministic(drh_handle h, int) { live(h, "drh_set_deterministic"); return DRH_OK; }
+drh_status drh_set_edge_grad_mode(drh_handle h, int) { live(h, "drh_set_edge_grad_mode"); return DRH_OK; }
+drh_status drh_set_interior_samples(drh_handle h, int, int) { live(h, "drh_set_interior_samples"); return DRH_OK; }
+drh_status drh_set_hit_buffer_limit(drh_handle h, size_t) { live(h, "drh_set_hit_buffer_limit"); return DRH_OK; }
 drh_status drh_malloc(drh_handle h, size_t bytes, void** p) {
   live(h, "drh_malloc");
   *p = calloc(1, bytes ? bytes : 4);

@@ -1,0 +1,170 @@
+"""The serving workloads at their real size ranges, against the oracle and alone-vs-mix.
+
+Config 3 (uneven batch): six jobs spanning the range -- the 12-triangle cube to 78,732
+triangles, 128 / 256 / 512 pixels, vertex colour and Lambert -- each at its full edge-sample
+count (4 x pixels).  Each job's first fused step (render + loss + d_render) matches the oracle
+on the same samples (images within 1e-5, gradient within 1e-3 relative L2), and with
+DRH_JOB_DETERMINISTIC every job's fitted positions, colours and last loss are bit-identical
+served alone and served in the mix of all six.
+
+Config 5 (serving stream): jobs of 12 to 199,692 triangles, 1 to 8 ring views and 128 to
+1024 pixels arrive at Poisson times (rate 20/s, seed 7) and are admitted under an HBM budget;
+every job completes, the server reports p50 / p99 per-iteration latency, and with
+DRH_JOB_DETERMINISTIC the largest and the smallest job give bit-identical results alone and in
+the stream."""
+from __future__ import annotations
+
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from diffrender_tutorials_amd import drh, scenes as S
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL, GRAD_TOL = 1e-5, 1e-3
+LAM = S.lambert_config2()
+
+# (k, triangles, resolution, mode): the config-3 subset, spanning its triangle-count range
+CFG3 = [(0, 12, 128, drh.VERTEX_COLOR), (1, 300, 256, drh.LAMBERT), (2, 5000, 512, drh.VERTEX_COLOR),
+        (3, 20000, 128, drh.LAMBERT), (4, 45000, 256, drh.VERTEX_COLOR), (5, 80000, 512, drh.LAMBERT)]
+# (k, triangles, resolution, views, iterations): the config-5 stream, spanning its ranges
+CFG5 = [(0, 200000, 512, 8, 4), (1, 12, 1024, 4, 6), (2, 30000, 1024, 3, 3), (3, 1000, 256, 6, 8),
+        (4, 5000, 128, 1, 10), (5, 80000, 1024, 2, 3), (6, 300, 512, 5, 6)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _cfg3_jobs():
+    return [S.config3_job(k, F, res, mode, 0.7 * k + 0.3) for k, F, res, mode in CFG3]
+
+
+def _targets(job):
+    r = drh.DiffRender(mode=job["mode"], spp=job["spp"], light=LAM)
+    r.commit(job["target"])
+    t = r.render(job["cams"])
+    r.close()
+    return t
+
+
+def test_config3_first_steps_match_oracle():
+    jobs = _cfg3_jobs()
+    assert min(j["meshes"][0].F for j in jobs) == 12 and max(j["meshes"][0].F for j in jobs) == 78732
+    for j in jobs:
+        m, cams, mode, N = j["meshes"], j["cams"], j["mode"], j["edge_samples"]
+        tg = _targets(j)
+        say("config 3 job", m[0].F, "triangles", cams[0].width, "px mode", mode, "N", N)
+        r = drh.DiffRender(mode=mode, spp=1, light=LAM)
+        try:
+            r.commit(m)
+            loss, g, imgs = r.d_render_and_compare(cams, tg, N)
+        finally:
+            r.close()
+        ref = O.Scene(m, O.TRACER_BVH)
+        img_o = ref.render(mode, 1, cams, lam=LAM)[0]
+        l_o, adj = O.loss(img_o, tg[0])
+        go = ref.d_render(mode, 1, cams, [adj], N, lam=LAM)
+        assert np.abs(imgs[0] - img_o).max() <= IMG_TOL
+        assert np.isclose(loss, l_o / (cams[0].width * cams[0].height), rtol=1e-5)
+        e = rel_l2(g, go)
+        say("   rel L2", e)
+        assert np.abs(go).sum() > 0 and e < GRAD_TOL
+
+
+def _serve(jobs, targets, iters, max_active, flags, gaps=None, budget=64 << 30):
+    srv = drh.Server(hbm_budget=budget, max_active=max_active)
+    try:
+        ids = []
+        t0 = time.perf_counter()
+        due = 0.0
+        for k, (j, tg) in enumerate(zip(jobs, targets)):
+            if gaps is not None:
+                due += gaps[k]
+                while time.perf_counter() - t0 < due:
+                    time.sleep(0.0005)
+            ids.append(srv.submit(j["meshes"], j["cams"], tg, iters[k], j["edge_samples"], mode=j["mode"],
+                                  spp=j["spp"], light=LAM, flags=flags))
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    return res, st
+
+
+def test_config3_jobs_alone_equal_jobs_in_the_mix_bit_for_bit():
+    jobs = _cfg3_jobs()
+    targets = [_targets(j) for j in jobs]
+    iters = [12] * len(jobs)
+    flags = drh.JOB_DETERMINISTIC
+    alone = []
+    for k, (j, tg) in enumerate(zip(jobs, targets)):
+        say("alone", j["meshes"][0].F)
+        r, _ = _serve([j], [tg], [iters[k]], len(jobs), flags)
+        assert r[0]["status"] == 0 and r[0]["iterations"] == iters[k]
+        alone.append(r[0])
+    say("mix")
+    mix, st = _serve(jobs, targets, iters, len(jobs), flags)
+    say("stats", {k: st[k] for k in ("peak_active", "mean_active", "p50_ms", "p99_ms")})
+    assert st["peak_active"] >= 2 and st["p99_ms"] >= st["p50_ms"] > 0
+    for j, a, b in zip(jobs, alone, mix):
+        assert b["status"] == 0
+        assert np.abs(a["pos"] - j["meshes"][0].pos).max() > 0          # the fit moved
+        assert np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["col"], b["col"]), j["meshes"][0].F
+        assert a["final_loss"] == b["final_loss"]
+
+
+def test_config5_poisson_stream_over_the_full_size_range():
+    jobs = [S.config5_job(k, F, res, views, it) for k, F, res, views, it in CFG5]
+    assert max(j["meshes"][0].F for j in jobs) == 199692 and min(j["meshes"][0].F for j in jobs) == 12
+    assert {len(j["cams"]) for j in jobs} >= {1, 8}
+    targets = [_targets(j) for j in jobs]
+    iters = [j["iterations"] for j in jobs]
+    gaps = np.random.default_rng(7).exponential(1.0 / 20.0, size=len(jobs))
+    flags = drh.JOB_DETERMINISTIC | drh.JOB_FULL_JACOBIAN
+    say("stream of", len(jobs), "jobs")
+    res, st = _serve(jobs, targets, iters, 4, flags, gaps=gaps, budget=288 << 30)
+    say("stats", {k: st[k] for k in ("jobs_done", "peak_active", "mean_active", "p50_ms", "p99_ms", "elapsed_s")})
+    assert all(r["status"] == 0 for r in res) and st["jobs_done"] == len(jobs)
+    assert [r["iterations"] for r in res] == iters
+    assert st["peak_active"] >= 2 and st["p99_ms"] >= st["p50_ms"] > 0
+    for k in (0, 1):   # the largest (8 views) and the smallest job, each alone
+        a, _ = _serve([jobs[k]], [targets[k]], [iters[k]], 4, flags, budget=288 << 30)
+        say("alone", jobs[k]["meshes"][0].F, "loss", a[0]["final_loss"], res[k]["final_loss"])
+        assert np.array_equal(a[0]["pos"], res[k]["pos"]) and a[0]["final_loss"] == res[k]["final_loss"]
+
+
+def test_server_hit_buffer_follows_the_budget():
+    """DRH_JOB_ALIGNED_INTERIOR jobs store the render's hits when they fit what is left of
+    the budget and recompute otherwise; both fits agree within summation order."""
+    j = S.config5_job(9, 5000, 256, 4, 3)
+    j = dict(j, meshes=[S.with_random_colors(j["meshes"][0], 3)], target=[S.with_random_colors(j["target"][0], 4)],
+             mode=drh.LAMBERT_VERTEX_COLOR)
+    tg = _targets(j)
+    hits = 16 * 4 * 256 * 256
+    out = {}
+    for name in ("roomy", "tight"):
+        budget = 64 << 30
+        if name == "tight":                 # the job's footprint fits, its hit buffer does not
+            srv = drh.Server(hbm_budget=1 << 40, max_active=1)
+            srv.submit(j["meshes"], j["cams"], tg, 1, j["edge_samples"], mode=j["mode"], light=LAM)
+            budget = srv.last_job_bytes + hits - 16
+            srv.close()
+        r, st = _serve([j], [tg], [3], 1, drh.JOB_ALIGNED_INTERIOR | drh.JOB_DETERMINISTIC, budget=budget)
+        assert r[0]["status"] == 0
+        out[name] = r[0]
+    assert out["roomy"]["hit_bytes"] == hits and out["tight"]["hit_bytes"] == 0
+    assert out["roomy"]["hbm_bytes"] == out["tight"]["hbm_bytes"] + hits
+    d = np.abs(out["roomy"]["col"] - out["tight"]["col"]).max()
+    say("hit store vs recompute under the budget: colour diff", d)
+    assert d < 1e-5

@@ -978,6 +978,8 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   r->st = *settings;
   if (hipSetDevice(settings->device) != hipSuccess) return DRH_E_HIP;
   if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return DRH_E_HIP;
+  // the allocation-free CDF's scratch address on this device (looked up once per device)
+  if (noalloc_scratch_init(settings->device) != hipSuccess) return DRH_E_HIP;
   keep_pool_memory(settings->device);
   r->pool_stream = r->stream;
   if (hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
@@ -1616,4 +1618,20 @@ extern "C" drh_status drh_debug_cdf(drh_handle h, const float* pmf_raw, int E, f
       (s = h->hip(hipMemcpyAsync(cdf_out, c.p, 4 * (size_t)(E + 1), hipMemcpyDeviceToHost, h->stream), "download")))
     return s;
   return h->hip(hipStreamSynchronize(h->stream), "debug_cdf");
+}
+
+// drh_debug_cdf's parallel exact CDF with device inputs / outputs and the work buffer in the
+// device's static scratch array: no allocation, no synchronisation (capturable).
+extern "C" drh_status drh_build_edge_sampler_noalloc(drh_handle h, const float* d_pmf_raw, int E, float* d_pmf,
+                                                     float* d_cdf) {
+  if (!h) return DRH_E_INVALID;
+  if (E <= 0 || !d_pmf_raw || !d_pmf || !d_cdf) return h->fail(DRH_E_INVALID, "build_edge_sampler_noalloc: bad arguments");
+  if (E > NOALLOC_MAX_EDGES)
+    return h->fail(DRH_E_INVALID, "build_edge_sampler_noalloc: %d edges exceed the maximum of %d (scratch of %zu bytes)",
+                   E, NOALLOC_MAX_EDGES, NOALLOC_SCRATCH_BYTES);
+  (void)hipSetDevice(h->st.device);
+  void* scr = noalloc_scratch(h->st.device);
+  if (!scr) return h->fail(DRH_E_HIP, "build_edge_sampler_noalloc: scratch address not set up on device %d", h->st.device);
+  h->timed("k_cdf_par", [&] { launch_cdf(d_pmf_raw, E, d_pmf, d_cdf, scr, h->stream, true); });
+  return h->hip(hipGetLastError(), "build_edge_sampler_noalloc");
 }

@@ -307,6 +307,15 @@ void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_ra
 // exact CDF; `work` holds cdf_work_bytes(E) of device memory (parallel form only)
 size_t cdf_work_bytes(int E);
 void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work, hipStream_t s, bool parallel = true);
+// The allocation-free CDF's work buffer: a static device array of NOALLOC_SCRATCH_BYTES per
+// GPU, large enough for NOALLOC_MAX_EDGES edges.  noalloc_scratch_init(device) looks its
+// address up once per device (at handle creation); noalloc_scratch(device) returns it (NULL
+// before that).
+constexpr int NOALLOC_MAX_EDGES = 1 << 24;
+constexpr size_t NOALLOC_SCRATCH_BYTES = 4u << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+hipError_t noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
 void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s);   // side colour differences into ecol
 void launch_edge_grad(const EdgeArgs& a, hipStream_t s);              // gradients from ecol
 void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s);   // bin_count must be zero

@@ -13,6 +13,8 @@
 // pixel tile or 32 edge samples (both sides) per wave, one wave-uniform LBVH walk per packet.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "drh_device.h"
 #include "drh_kernels.h"
 #include "drh_mat4.h"
@@ -1627,6 +1629,41 @@ void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work,
   hipLaunchKernelGGL(k_cdf_spec, dim3(blocks), dim3(CDF_STAGE_T), 0, s, pmf_raw, E, w);
   hipLaunchKernelGGL(k_cdf_walk, dim3(1), dim3(1024), 0, s, pmf_raw, E, w);
   hipLaunchKernelGGL(k_cdf_final, dim3(blocks), dim3(CDF_STAGE_T), 0, s, pmf_raw, E, w, pmf, cdf);
+}
+
+// The exact CDF's work buffer for callers that cannot allocate (drh_build_edge_sampler_noalloc):
+// a file-scope device array, one copy per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in for the symbol): the
+// device address comes from hipGetSymbolAddress, once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static_assert(8 * ((NOALLOC_MAX_EDGES + CDF_CH - 1) / CDF_CH) + 12 * ((NOALLOC_MAX_EDGES + CDF_CH - 1) / CDF_CH) +
+                      4 * ((NOALLOC_MAX_EDGES + CDF_CH - 1) / CDF_CH + 1) + 64 <= NOALLOC_SCRATCH_BYTES,
+              "the CDF work buffer of the largest edge count fits the scratch array");
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+  void* p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[device] = p;
+  if (cur != device) (void)hipSetDevice(cur);
+  return e;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s) {

@@ -44,7 +44,7 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_debug_copy", "drh_debug_cdf", "drh_opt_use_graph",
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
            "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild", "drh_set_edge_grad_mode",
-           "drh_set_deterministic", "drh_set_interior_pos_grad",
+           "drh_set_deterministic", "drh_set_interior_pos_grad", "drh_build_edge_sampler_noalloc",
            "drh_set_hit_buffer_limit", "drh_get_hit_buffer_bytes"]
 
 
@@ -110,6 +110,8 @@ def load():
         L.drh_debug_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
         L.drh_set_hit_buffer_limit.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
         L.drh_get_hit_buffer_bytes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+        L.drh_build_edge_sampler_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                      ctypes.c_void_p]
         _lib = L
     return _lib
 

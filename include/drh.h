@@ -351,6 +351,15 @@ drh_status drh_debug_edges(drh_handle h, int* n_edges, int* v0, int* v1, float* 
  * cdf_out[0..E] = sequential float running sum / total, by the parallel exact kernel
  * (parallel != 0) or the single-lane chain. */
 drh_status drh_debug_cdf(drh_handle h, const float* pmf_raw, int E, float* pmf_out, float* cdf_out, int parallel);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  The edge sampler's exact CDF (build_edge_sampler,
+ * src/drender.cpp:5-22; drh_debug_cdf with parallel != 0) over device arrays: d_pmf = d_pmf_raw /
+ * total, d_cdf[0..E] = the sequential float running sum / total.  It allocates nothing: its
+ * work buffer is a static 4 MiB device array per GPU, so E is limited to 2^24 edges (larger
+ * E: DRH_E_INVALID).  Work is queued on the handle's stream and not waited for.  Every call on
+ * one GPU shares that GPU's array (it keeps the last call's contents; every word a call reads
+ * it writes first): calls that overlap on one GPU (other handles' streams) race. */
+drh_status drh_build_edge_sampler_noalloc(drh_handle h, const float* d_pmf_raw, int E, float* d_pmf, float* d_cdf);
 /* A plain device-to-device copy kernel (k_copy_f32) of n floats: the known-size reference
  * that hardware traffic counters (FETCH_SIZE / WRITE_SIZE) are calibrated against. */
 drh_status drh_debug_copy(drh_handle h, float* d_dst, const float* d_src, long long n);

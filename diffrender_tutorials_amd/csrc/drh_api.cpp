@@ -319,6 +319,13 @@ struct drh_renderer : BufOwner {
   // per call
   DBuf cams, img, adj, tgt, loss_part, loss, grad;
   DBuf fault;   // the kernels' fault word (DevScene::fault), zero while all is well
+  // Ray counting (drh_count_rays): the traversal kernels' counting instantiations add their
+  // rays to rays_buf: [0, 1] render closest / any-hit, [2, 3] interior, [4, 5] edge trace.
+  bool count_on = false;
+  DBuf rays_buf;
+  unsigned long long* ray_slot(int kernel) const {
+    return count_on ? rays_buf.as<unsigned long long>() + 2 * kernel : nullptr;
+  }
   // After a synchronisation: a set fault word fails the call with DRH_E_DEVICE (and is
   // cleared, so the handle stays usable).
   drh_status check_fault() {
@@ -802,6 +809,7 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   for (int v = 0; v < views; v++) {
     RenderArgs ra{tracer_view(v), light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off,
                   store_hits ? hitbuf.as<float4>() : nullptr};
+    ra.tr.rays = ray_slot(0);
     timed("k_render", [&] { launch_render(st.mode, ra, tiles[v], stream); });
     part_off += tiles[v];
   }
@@ -874,7 +882,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
     const int tiles = num_tiles(sq, c.w, c.h);   // interior samples: nsamp = sq*sq Hammersley points
     if (st.mode != DRH_SILHOUETTE && G > 0 && !skip_tex_interior) {
       InteriorArgs ia{};
-      ia.tr = tracer_view(v); ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.sq = sq; ia.hamm = hamm.as<float>();
+      ia.tr = tracer_view(v); ia.tr.rays = ray_slot(1); ia.cams = cams.as<DevCam>(); ia.view = v; ia.nsamp = nsamp; ia.sq = sq; ia.hamm = hamm.as<float>();
       ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
       ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
       ia.aligned = interior_aligned;
@@ -915,7 +923,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         edge_prep(v, stream);
       }
       EdgeArgs ea{};
-      ea.tr = tracer_view(v); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
+      ea.tr = tracer_view(v); ea.tr.rays = ray_slot(2); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
       ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
       ea.proj = proj.as<float4>();
       ea.rnd = rnd.as<float2>();
@@ -1277,7 +1285,7 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
   std::memcpy(sig.data(), cams, sizeof(drh_camera) * views);
   const void* ptrs[4] = {d_targets, d_loss, (const void*)(intptr_t)edge_samples, (const void*)(intptr_t)views};
   std::memcpy(sig.data() + sizeof(drh_camera) * views, ptrs, sizeof ptrs);
-  const bool graphable = h->use_graph && h->opt_kind == DRH_OPT_ADAM && h->opt_iter > 0 && !h->prof;
+  const bool graphable = h->use_graph && h->opt_kind == DRH_OPT_ADAM && h->opt_iter > 0 && !h->prof && !h->count_on;
   if (graphable) {
     // the graph reads the camera block and the sample table: make sure they hold this call's
     // (uploads only if another call changed them; a reallocation shows in the epoch)
@@ -1419,6 +1427,28 @@ drh_status drh_set_hit_buffer_limit(drh_handle h, size_t bytes) {
   if (bytes != h->hit_limit) h->drop_graph();   // the captured step bakes the choice in
   h->hit_limit = bytes;
   return DRH_OK;
+}
+
+drh_status drh_count_rays(drh_handle h, int on) {
+  if (!h || (on != 0 && on != 1)) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  h->drop_graph();   // counting runs other kernel instantiations, eagerly
+  h->count_on = on != 0;
+  if (!on) return DRH_OK;
+  drh_status s;
+  if ((s = h->hip(h->rays_buf.reserve(64), "alloc")) || (s = h->hip(hipMemsetAsync(h->rays_buf.p, 0, 64, h->stream), "zero")))
+    return s;
+  return h->hip(hipStreamSynchronize(h->stream), "count_rays");
+}
+
+drh_status drh_read_ray_counts(drh_handle h, unsigned long long counts[6]) {
+  if (!h || !counts) return DRH_E_INVALID;
+  if (!h->rays_buf.p) { std::memset(counts, 0, 6 * sizeof(unsigned long long)); return DRH_OK; }
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  if ((s = h->hip(hipMemcpyAsync(counts, h->rays_buf.p, 48, hipMemcpyDeviceToHost, h->stream), "download")))
+    return s;
+  return h->check_fault();
 }
 
 drh_status drh_get_hit_buffer_bytes(drh_handle h, size_t* bytes) {

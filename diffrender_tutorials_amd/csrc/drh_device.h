@@ -283,7 +283,19 @@ struct Tracer {
   int use_bf;
   Suspects cam;     // for closest-hit (camera) rays of the current view
   Suspects light;   // for any-hit (shadow) rays
+  // Ray counters of the counting instantiations (COUNT = true) only: [0] closest-hit rays,
+  // [1] any-hit rays traced by the kernel this tracer is passed to.
+  unsigned long long* rays;
 };
+
+// Counting instantiations: add the wave's active lanes whose ray is cast (one atomic per wave).
+template <bool COUNT>
+__device__ __forceinline__ void count_rays(unsigned long long* c, bool cast) {
+  if (!COUNT || !c) return;
+  const uint64_t b = __ballot(cast);
+  const int leader = __ffsll((long long)__ballot(1)) - 1;
+  if ((int)(threadIdx.x & 63) == leader && b) atomicAdd(c, (unsigned long long)__popcll(b));
+}
 
 template <bool ANY>
 __device__ __forceinline__ void test_suspects(const Tracer& T, const Suspects& S, F3 o, F3 d, float tNear, float tFar,
@@ -303,8 +315,9 @@ __device__ __forceinline__ void test_suspects(const Tracer& T, const Suspects& S
   }
 }
 
-template <bool ANY>
+template <bool ANY, bool COUNT = false>
 __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* wst) {
+  count_rays<COUNT>(T.rays + (ANY ? 1 : 0), !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z));
   if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
   Hit h = trace_bvh<ANY>(T.s, o, d, tNear, tFar, wst);
   if (ANY && h.gtri != NO_HIT) return h;
@@ -316,7 +329,9 @@ __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, f
 // Does a camera ray hit anything?  Any-hit traversal with the view's suspect list: the same
 // yes / no as the closest-hit query (both accept the first triangle whose MT test passes with
 // 0 < t < tFar; the closest-hit query only keeps looking for a nearer one).
+template <bool COUNT = false>
 __device__ __forceinline__ bool covered(const Tracer& T, F3 o, F3 d, int* wst) {
+  count_rays<COUNT>(T.rays + 1, !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z));
   if (T.use_bf) return trace_bf<true>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, 0.0f, 1e9f).gtri != NO_HIT;
   Hit h = trace_bvh<true>(T.s, o, d, 0.0f, 1e9f, wst);
   if (h.gtri != NO_HIT) return true;
@@ -392,6 +407,7 @@ constexpr int MODE_LAMBERT_VCOL = 8;
 // pos + (t - 1e-5) dir toward -L hits anything.  The lit colour is (shadow * Lc) * ndl.  With
 // ndl == +0 and a finite light colour the shadow ray cannot change that product (bit for bit),
 // so it is only cast where ndl > 0 (shadow stays 1 otherwise).
+template <bool COUNT = false>
 __device__ __forceinline__ void lambert_terms(const Tracer& T, const Light& L, F3 o, F3 d, const Hit& h,
                                               uint32_t A, uint32_t B, uint32_t C, const float* nrm, int* wst,
                                               float& shadow, float& ndl) {
@@ -402,7 +418,7 @@ __device__ __forceinline__ void lambert_terms(const Tracer& T, const Light& L, F
   shadow = 1.0f;
   if (ndl != 0.0f || !(fabsf(L.Lc.x) <= 3.4e38f && fabsf(L.Lc.y) <= 3.4e38f && fabsf(L.Lc.z) <= 3.4e38f)) {
     const F3 surf = add(o, mul(h.t - BIAS, d));
-    const Hit sh = trace<true>(T, surf, negL, 0.0f, 1e9f, wst);
+    const Hit sh = trace<true, COUNT>(T, surf, negL, 0.0f, 1e9f, wst);
     shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
   }
 }
@@ -422,12 +438,12 @@ __device__ __forceinline__ void lit_decode(float w, float& shadow, float& ndl) {
 // shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT / LAMBERT_VCOL.  `nrm` is the 3D
 // scene's normals for render and the projected scene's for edge samples (src/drender.h:353-354).
 // lit_out (LAMBERT_VCOL, may be NULL) receives the hit's light terms, lit_encode'd.
-template <int MODE>
+template <int MODE, bool COUNT = false>
 __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
                                     const float* nrm, int* wst, Hit* hit_out = nullptr, float* lit_out = nullptr) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
-  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
+  const Hit h = trace<false, COUNT>(T, o, d, 0.0f, 1e9f, wst);
   if (hit_out) *hit_out = h;
   if (h.gtri == NO_HIT) return {0, 0, 0};
   if (MODE == 1) return {1, 1, 1};
@@ -444,7 +460,7 @@ __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCa
     if (MODE == 3) return diffuse;
   }
   float shadow, ndl;
-  lambert_terms(T, L, o, d, h, A, B, C, nrm, wst, shadow, ndl);
+  lambert_terms<COUNT>(T, L, o, d, h, A, B, C, nrm, wst, shadow, ndl);
   if (MODE == MODE_LAMBERT_VCOL && lit_out) *lit_out = lit_encode(shadow, ndl);
   return mulv(lambert_factor(L, shadow, ndl), diffuse);
 }

@@ -445,7 +445,7 @@ __device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
   return tile_map_xy(sq, tile % tiles_x, tile / tiles_x, lane);
 }
 
-template <int MODE>
+template <int MODE, bool COUNT = false>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   __shared__ int wst[64];   // the wave's traversal stack
   const Tracer& T = a.tr;
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
       const float yo = (dy + 0.5f) / float(sq);
       Hit h;
       float lit = 0.0f;
-      col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
+      col = shade<MODE, COUNT>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
       if (a.hits)
         a.hits[(c.img_off / 3 + (long long)y * c.w + x) * tm.S + tm.s] =
             make_float4(__uint_as_float(h.gtri), h.u, h.v, MODE == MODE_LAMBERT_VCOL ? lit : h.t);
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
         const float yo = (dy + 0.5f) / float(sq);
         Hit h;
         float lit = 0.0f;
-        const F3 col = shade<MODE>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
+        const F3 col = shade<MODE, COUNT>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
         if (a.hits)
           a.hits[(c.img_off / 3 + (long long)y * c.w + x) * (sq * sq) + dy * sq + dx] =
               make_float4(__uint_as_float(h.gtri), h.u, h.v, MODE == MODE_LAMBERT_VCOL ? lit : h.t);
@@ -602,7 +602,7 @@ void launch_det_finish(const unsigned long long* dacc, int G, float* grad, float
 // LAMBERT_VCOL: the colour gradient of VERTEX_COLOR (shade_simple.cpp:39-74) times the light
 // factor (shadow * Lc) * ndl + ambient of the hit (the derivative of its shade), which needs the
 // hit's normal and shadow ray (traced again, or decoded from the render's stored hit).
-template <int MODE, bool POS = false>
+template <int MODE, bool POS = false, bool COUNT = false>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   constexpr bool VCOL_LIKE = MODE == 2 || MODE == MODE_LAMBERT_VCOL;
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
@@ -667,10 +667,10 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
             if (a.coverage) {
               // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
               // hit, so the sample adds exactly `val` if the ray hits anything
-              if (covered(T, o, d, wst)) uni = add(uni, val);
+              if (covered<COUNT>(T, o, d, wst)) uni = add(uni, val);
               break;
             }
-            h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
+            h = trace<false, COUNT>(T, o, d, 0.0f, 1e9f, wst);
           }
           if (h.gtri == NO_HIT) break;
           const int m = a.tr.s.tri_mesh[h.gtri];
@@ -683,7 +683,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
               const uint32_t* I3 = a.tr.s.idx + 3 * h.gtri;
               float shadow, ndl;
               if (a.hits) lit_decode(lit_stored, shadow, ndl);
-              else lambert_terms(T, a.light, ro, rd, h, I3[0], I3[1], I3[2], a.tr.s.nrm, wst, shadow, ndl);
+              else lambert_terms<COUNT>(T, a.light, ro, rd, h, I3[0], I3[1], I3[2], a.tr.s.nrm, wst, shadow, ndl);
               vs = mulv(val, lambert_factor(a.light, shadow, ndl));
             }
             const float w = 1.0f - h.u - h.v;
@@ -1231,7 +1231,7 @@ __global__ void k_edge_scatter(EdgeBinArgs a) {
 // 32 samples with both sides in adjacent lanes, one ray (and shadow ray) per lane, so its
 // packet covers 32 nearby points and the kernel stays small enough for 8 waves per SIMD.
 // ecol[i] = (colour inside - colour outside, edge id bits; -1 for a rejected sample).
-template <int MODE>
+template <int MODE, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
   __shared__ int wstack[4][64];
   const Tracer& T = a.tr;
@@ -1266,7 +1266,7 @@ __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
       const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
       const float qx = sd == 0 ? p.px - 1e-3f * nx : p.px + 1e-3f * nx;
       const float qy = sd == 0 ? p.py - 1e-3f * ny : p.py + 1e-3f * ny;
-      col = shade<MODE>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
+      col = shade<MODE, COUNT>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
     }
     // both lanes of a sample are here together (same item, same `live`)
     const float ox = __shfl_xor(col.x, 1, 64), oy = __shfl_xor(col.y, 1, 64), oz = __shfl_xor(col.z, 1, 64);
@@ -1593,21 +1593,27 @@ void launch_reorder(const ReorderArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_render(int mode, const RenderArgs& a, int tiles, hipStream_t s) {
-  DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_render<M>), dim3(tiles), dim3(64), 0, s, a))
+  if (a.tr.rays) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_render<M, true>), dim3(tiles), dim3(64), 0, s, a))
+  else DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_render<M>), dim3(tiles), dim3(64), 0, s, a))
 }
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s) {
   hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1024), 0, s, part, n, out, denom);
 }
 void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s) {
   const size_t lds = sizeof(float) * (size_t)std::max(a.lds_n, 4 * a.hash);
+  const bool cnt = a.tr.rays != nullptr;   // the ray-counting instantiations
   if (mode == 2 && a.pos_grad) {
-    hipLaunchKernelGGL((k_interior<2, true>), dim3(blocks), dim3(64), lds, s, a);
+    if (cnt) hipLaunchKernelGGL((k_interior<2, true, true>), dim3(blocks), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((k_interior<2, true>), dim3(blocks), dim3(64), lds, s, a);
   } else if (mode == 2) {
-    hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), lds, s, a);
+    if (cnt) hipLaunchKernelGGL((k_interior<2, false, true>), dim3(blocks), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), lds, s, a);
   } else if (mode == MODE_LAMBERT_VCOL) {
-    hipLaunchKernelGGL((k_interior<MODE_LAMBERT_VCOL>), dim3(blocks), dim3(64), lds, s, a);
+    if (cnt) hipLaunchKernelGGL((k_interior<MODE_LAMBERT_VCOL, false, true>), dim3(blocks), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((k_interior<MODE_LAMBERT_VCOL>), dim3(blocks), dim3(64), lds, s, a);
   } else {
-    hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), lds, s, a);
+    if (cnt) hipLaunchKernelGGL((k_interior<3, false, true>), dim3(blocks), dim3(64), lds, s, a);
+    else hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), lds, s, a);
   }
 }
 void launch_project(const ProjectArgs& a, hipStream_t s) {
@@ -1675,7 +1681,8 @@ void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s) {
 }
 void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s) {
   const int blocks = (int)std::max<long long>(1, std::min<long long>((2 * a.N + 255) / 256, 8192));
-  DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M>), dim3(blocks), dim3(256), 0, s, a))
+  if (a.tr.rays) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, true>), dim3(blocks), dim3(256), 0, s, a))
+  else DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M>), dim3(blocks), dim3(256), 0, s, a))
 }
 void launch_edge_grad(const EdgeArgs& a, hipStream_t s) {
   const int blocks = (int)std::max<long long>(1, std::min<long long>((a.N + 255) / 256, 2048));

@@ -45,7 +45,7 @@ EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_gra
            "drh_server_create", "drh_server_submit", "drh_server_wait", "drh_server_stats", "drh_server_destroy",
            "drh_server_job_bytes", "drh_set_edge_order", "drh_set_bvh_rebuild", "drh_set_edge_grad_mode",
            "drh_set_deterministic", "drh_set_interior_pos_grad", "drh_build_edge_sampler_noalloc",
-           "drh_set_hit_buffer_limit", "drh_get_hit_buffer_bytes"]
+           "drh_set_hit_buffer_limit", "drh_get_hit_buffer_bytes", "drh_count_rays", "drh_read_ray_counts"]
 
 
 class DrhError(RuntimeError):
@@ -305,6 +305,18 @@ class DiffRender:
         pmf = np.empty(n.value, np.float32); cdf = np.empty(n.value + 1, np.float32)
         self._check(load().drh_debug_edges(self.h, ctypes.byref(n), _ip(v0), _ip(v1), _fp(pmf), _fp(cdf)))
         return v0, v1, pmf, cdf
+
+    def count_rays(self, on=True):
+        """Run the counting copies of the traversal kernels (eagerly) from now on (zeroes the counts)."""
+        self._check(load().drh_count_rays(self.h, int(on)))
+
+    def ray_counts(self):
+        """Rays cast since counting was turned on: dict of render camera / shadow, interior
+        closest-hit / any-hit, edge side / shadow rays."""
+        c = (ctypes.c_ulonglong * 6)()
+        self._check(load().drh_read_ray_counts(self.h, c))
+        keys = ("render_camera", "render_shadow", "interior_closest", "interior_any", "edge_side", "edge_shadow")
+        return dict(zip(keys, [int(x) for x in c]))
 
     def profile(self, on=True):
         self._check(load().drh_profile_enable(self.h, int(on)))

@@ -334,6 +334,15 @@ void       drh_server_destroy(drh_server s);
 void*      drh_stream(drh_handle h);
 drh_status drh_sync(drh_handle h);
 
+/* Ray counts (no counterpart in the reference): with counting on, render / d_render /
+ * d_render_and_compare / the fused step run counting copies of the traversal kernels (eagerly,
+ * not from a graph) that add up the rays they cast; drh_count_rays(h, 1) also zeroes the
+ * counts.  drh_read_ray_counts returns [0] render camera rays, [1] render shadow rays,
+ * [2] interior closest-hit rays, [3] interior any-hit rays (coverage or shadow rays),
+ * [4] edge-sample side rays, [5] edge-sample shadow rays, summed since counting was turned on. */
+drh_status drh_count_rays(drh_handle h, int on);
+drh_status drh_read_ray_counts(drh_handle h, unsigned long long counts[6]);
+
 /* Per-kernel timing with HIP events recorded on the handle's stream around each launch.
  * drh_profile_read returns the summed milliseconds and launch count of kernel `name`
  * (names as in the rocprof trace) since the last drh_profile_enable. */

@@ -19,7 +19,9 @@
 //             float32 tex[tex_w*tex_h*3]
 //   per view: float32 world_view[16], proj[16], width, height
 //   if ops & (OP_DRENDER|OP_COMPARE|OP_OPTIMIZE): per view float32 image[H*W*3] (adjoint or target)
-// Output: records {int32 tag, int32 nbytes, bytes}.
+// Output: records {int32 tag, int32 nbytes, bytes}.  OP_OPTIMIZE also records the wall time of
+// the optimisation loop (float64 seconds) and the OpenMP thread count.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -42,6 +44,7 @@ enum {
   T_QMC = 9, T_PRNG = 10, T_HAMM = 11, T_EDGE_V0 = 12, T_EDGE_V1 = 13, T_EDGE_PMF = 14, T_EDGE_CDF = 15,
   T_COMPARE_GRAD = 16, T_COMPARE_IMAGES = 17, T_PROJ = 18, T_OPT_POS = 19, T_OPT_COL = 20, T_OPT_ERR = 21,
   T_PROJ_INV = 22, T_WV_INV = 23, T_WVP = 24, T_OPT_TR = 25, T_OPT_TEX = 26, T_TRANSFORMR = 27,
+  T_OPT_SECONDS = 28, T_OMP_THREADS = 29,
 };
 
 static FILE* g_in;
@@ -275,7 +278,14 @@ int main(int argc, char** argv) {
     IOptimizer* opt = CreateSimpleOptimizer();
     opt->Init(scene, R, cams.data(), given.data(), nviews, p);
     float err = 0.0f;
+    // wall time of the reference's whole optimisation loop (commit + d_render_and_compare +
+    // OptStep per iteration), for the CPU baseline
+    const auto t0 = std::chrono::steady_clock::now();
     Scene res = opt->Run(opt_iters, err);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    put(T_OPT_SECONDS, &secs, sizeof secs);
+    const int threads = omp_get_max_threads();
+    put(T_OMP_THREADS, &threads, sizeof threads);
     std::vector<float> pos, col;
     for (auto& v : res.get_mesh(0).vertices) { pos.push_back(v.x); pos.push_back(v.y); pos.push_back(v.z); }
     for (auto& c : res.get_mesh(0).colors) { col.push_back(c.x); col.push_back(c.y); col.push_back(c.z); }

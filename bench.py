@@ -204,7 +204,10 @@ def mix_legs(drh, S, lam, njobs, iters, max_active):
             "served_over_plain": round(plain_s / served_s, 3),
             "served_over_sequential_fused": round(seq_s / served_s, 3),
             "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4), "peak_active": st["peak_active"],
-            "mean_active": round(st["mean_active"], 3), "triangles": tris}
+            "mean_active": round(st["mean_active"], 3), "served_wall_s": round(served_s, 4),
+            "server_thread_s": {"admit": round(st["admit_s"], 4), "step": round(st["step_s"], 4),
+                                "finish": round(st["finish_s"], 4)},
+            "admitters": int(os.environ.get("DRH_SERVER_ADMITTERS", "1")), "triangles": tris}
 
 
 def poisson_leg(drh, S, lam, njobs, rate, max_active, iter_scale):

@@ -278,6 +278,7 @@ struct drh_renderer : BufOwner {
   int edge_grad_mode = DRH_EDGE_GRAD_REFERENCE_DIAGONAL;
   DBuf bin_count, bin_rec, bin_order;     // pixel-bin order of the kept edge samples
   DBuf ecol;                              // edge samples' side colour differences (float4 each)
+  DBuf cov_part;                          // coverage interior pass: per-tile partial sums
   DBuf hitbuf;
   bool hits_stored = false;
   std::string err;
@@ -867,6 +868,11 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
                       (edge_order == DRH_EDGE_ORDER_BINNED ||
                        (edge_order == DRH_EDGE_ORDER_AUTO && N < (long long)EDGE_BIN_AUTO_PER_EDGE * E));
   if (E > 0 && N > 0 && (s = hip(ecol.reserve(16 * (size_t)N), "alloc"))) return s;
+  {
+    int max_tiles = 1;
+    for (int v = 0; v < views; v++) max_tiles = std::max(max_tiles, num_tiles(sq, cam_cache[v].w, cam_cache[v].h));
+    if ((s = hip(cov_part.reserve(16 * (size_t)max_tiles), "alloc"))) return s;
+  }
   if (binned) {
     int max_bins = 1;
     for (int v = 0; v < views; v++) {
@@ -914,7 +920,10 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         ia.dacc = dacc_p; ia.lds_base = 0; ia.lds_n = 0; ia.hash = 0; ia.region = 1;
         if (!ia.uniform) blocks = tiles;
       }
-      timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
+      if (ia.uniform && ia.coverage)
+        timed("k_interior", [&] { launch_coverage(ia, tiles, cov_part.as<float4>(), stream); });
+      else
+        timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
     }
     if (E > 0 && N > 0) {
       if (v == 0 && prep0_forked) {

@@ -302,6 +302,10 @@ void launch_suspects(const SuspectArgs& a, hipStream_t s);
 void launch_render(int mode, const RenderArgs& a, int tiles, hipStream_t s);
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s);
 void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s);
+// The interior pass of a single 1x1 albedo with all texture coordinates 0 (a.uniform and
+// a.coverage): one wave per pixel tile, partial sums into part (16 B per tile), then a fixed-
+// order sum added to grad[uniform_off .. +3).
+void launch_coverage(const InteriorArgs& a, int tiles, float4* part, hipStream_t s);
 void launch_project(const ProjectArgs& a, hipStream_t s);
 void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s);
 // exact CDF; `work` holds cdf_work_bytes(E) of device memory (parallel form only)

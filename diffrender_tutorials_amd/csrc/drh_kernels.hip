@@ -810,6 +810,63 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   }
 }
 
+// The interior pass when its only output is a 1x1 albedo reached through texture
+// coordinates that are all 0 (one differentiable mesh; TEXTURE_COLOR / LAMBERT): the bilinear
+// weights are (1, 0, 0, 0) at any hit, so a sample adds exactly adj / nsamp to the albedo's
+// gradient iff its ray hits anything (an any-hit ray, shade_grad<TEXTURE_COLOR>,
+// src/shade_simple.cpp:132-162 with src/shade_common.cpp:23-45).  One wave per pixel tile as
+// the render (one sample per lane), the wave's sum of its covered samples into part[tile];
+// k_coverage_sum adds the partial sums in a fixed order, so the result does not depend on
+// scheduling.  Small in registers, it keeps as many waves resident as the render.
+template <bool COUNT = false>
+__global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restrict__ part) {
+  __shared__ int wst[64];
+  const Tracer& T = a.tr;
+  const DevCam& c = a.cams[a.view];
+  const int lane = threadIdx.x;
+  const int sq = a.sq;
+  const TileMap tm = tile_map(sq, c.w, blockIdx.x, lane);
+  const bool inside = tm.x < c.w && tm.y < c.h;
+  F3 acc = {0, 0, 0};
+  if (inside) {
+    const F3 val = divs(ld3(a.adj + c.img_off + 3ll * ((long long)tm.y * c.w + tm.x)), (float)a.nsamp);
+    const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
+    for (int si = s_begin; si < s_end; si++) {
+      bool hit;
+      if (a.hits) {
+        hit = __float_as_uint(a.hits[(c.img_off / 3 + (long long)tm.y * c.w + tm.x) * a.nsamp + si].x) != NO_HIT;
+      } else {
+        const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
+        const float yo = a.aligned ? ((si / sq) + 0.5f) / float(sq) : a.hamm[2 * si + 1];
+        F3 o, d;
+        eye_ray(c, tm.x + xo, tm.y + yo, o, d);
+        hit = covered<COUNT>(T, o, d, wst);
+      }
+      if (hit) acc = add(acc, val);
+    }
+  }
+  acc.x = wave_sum(acc.x); acc.y = wave_sum(acc.y); acc.z = wave_sum(acc.z);
+  if (lane == 0) part[blockIdx.x] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+}
+
+// grad[0..2] += the sum of n partial float3 (part[i].xyz), in a fixed order, in double.
+__global__ __launch_bounds__(1024) void k_coverage_sum(const float4* __restrict__ part, int n, float* grad) {
+  __shared__ double s[3][1024];
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    const float4 p = part[i];
+    x += p.x; y += p.y; z += p.z;
+  }
+  s[0][threadIdx.x] = x; s[1][threadIdx.x] = y; s[2][threadIdx.x] = z;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int k = 0; k < 3; k++) s[k][threadIdx.x] += s[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) grad[threadIdx.x] += (float)s[threadIdx.x][0];
+}
+
 // ---------------------------------------------------------------------------
 // edge_derivatives (src/drender.h:235-425)
 // ---------------------------------------------------------------------------
@@ -1615,6 +1672,12 @@ void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s)
     if (cnt) hipLaunchKernelGGL((k_interior<3, false, true>), dim3(blocks), dim3(64), lds, s, a);
     else hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), lds, s, a);
   }
+}
+void launch_coverage(const InteriorArgs& a, int tiles, float4* part, hipStream_t s) {
+  if (tiles <= 0) return;
+  if (a.tr.rays) hipLaunchKernelGGL((k_coverage<true>), dim3(tiles), dim3(64), 0, s, a, part);
+  else hipLaunchKernelGGL((k_coverage<false>), dim3(tiles), dim3(64), 0, s, a, part);
+  hipLaunchKernelGGL(k_coverage_sum, dim3(1), dim3(1024), 0, s, (const float4*)part, tiles, a.grad + a.uniform_off);
 }
 void launch_project(const ProjectArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_project, dim3((a.V + 255) / 256), dim3(256), 0, s, a);

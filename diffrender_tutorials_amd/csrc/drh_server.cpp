@@ -18,6 +18,7 @@
 #include <cmath>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -82,8 +83,14 @@ struct Job {
 // Steps a job keeps queued on its stream (graph replays back to back; more than two keeps
 // the device fed while the stepping thread serves the other jobs).
 constexpr size_t MAX_INFLIGHT = 4;
-// Threads setting jobs up (each job's set-up is serial host work: handle, uploads, LBVH build).
+// Threads setting jobs up (each job's set-up is serial host work: handle, uploads, edge set,
+// LBVH build); DRH_SERVER_ADMITTERS (1..16) overrides the default when the server starts.
 constexpr int ADMITTERS = 1;
+int admitter_threads() {
+  const char* e = std::getenv("DRH_SERVER_ADMITTERS");
+  const int n = e ? std::atoi(e) : ADMITTERS;
+  return n >= 1 && n <= 16 ? n : ADMITTERS;
+}
 
 // HBM a job holds while active (the buffers drh_renderer sizes for it), with 25% headroom.
 size_t estimate_bytes(const Job& j) {
@@ -379,7 +386,7 @@ drh_status drh_server_create(int device, size_t hbm_budget_bytes, int max_active
   s->budget = hbm_budget_bytes;
   s->max_active = max_active;
   s->worker = std::thread([s] { s->run(); });
-  for (int k = 0; k < ADMITTERS; k++) s->admitters.emplace_back([s] { s->admit_run(); });
+  for (int k = 0, n = admitter_threads(); k < n; k++) s->admitters.emplace_back([s] { s->admit_run(); });
   s->reaper = std::thread([s] { s->reap_run(); });
   *out = s;
   return DRH_OK;

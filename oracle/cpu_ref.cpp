@@ -553,18 +553,20 @@ f3 bilinear(const Scene& s, int mesh, f2 tc) {
     r[i] = (1 - dx) * (1 - dy) * P00[i] + (1 - dx) * dy * P01[i] + dx * (1 - dy) * P10[i] + dx * dy * P11[i];
   return {r[0], r[1], r[2]};
 }
-// sample_bilinear_clamp_3f_grad (shade_common.cpp:23-45): tc is NOT clamped here
-void bilinear_grad(const Scene& s, int mesh, f3 val, f2 tc, float* g) {
+// sample_bilinear_clamp_3f_grad (shade_common.cpp:23-45): tc is NOT clamped here.  The terms
+// are float products as in the reference; Acc is the type they are summed in.
+template <class Acc>
+void bilinear_grad(const Scene& s, int mesh, f3 val, f2 tc, Acc* g) {
   const int w = s.tex_w[mesh], h = s.tex_h[mesh];
   const float px = tc.x * (float)w, py = tc.y * (float)h;
   const int ix = (int)px, iy = (int)py;
   const int x0 = clampi(ix, 0, w - 1), y0 = clampi(iy, 0, h - 1);
   const int x1 = clampi(ix + 1, 0, w - 1), y1 = clampi(iy + 1, 0, h - 1);
   const float dx = px - (float)x0, dy = py - (float)y0;
-  float* d00 = g + 3 * (y0 * w + x0);
-  float* d01 = g + 3 * (y1 * w + x0);
-  float* d10 = g + 3 * (y0 * w + x1);
-  float* d11 = g + 3 * (y1 * w + x1);
+  Acc* d00 = g + 3 * (y0 * w + x0);
+  Acc* d01 = g + 3 * (y1 * w + x0);
+  Acc* d10 = g + 3 * (y0 * w + x1);
+  Acc* d11 = g + 3 * (y1 * w + x1);
   d00[0] += (1 - dx) * (1 - dy) * val.x; d00[1] += (1 - dx) * (1 - dy) * val.y; d00[2] += (1 - dx) * (1 - dy) * val.z;
   d01[0] += (1 - dx) * dy * val.x;       d01[1] += (1 - dx) * dy * val.y;       d01[2] += (1 - dx) * dy * val.z;
   d10[0] += dx * (1 - dy) * val.x;       d10[1] += dx * (1 - dy) * val.y;       d10[2] += dx * (1 - dy) * val.z;
@@ -892,7 +894,10 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
     const int W = (int)cams[vi].width, H = (int)cams[vi].height;
     const float* adj = adjoints + img_off;
     auto adj_at = [&](int x, int y) { const float* p = adj + 3 * ((size_t)y * W + x); return f3{p[0], p[1], p[2]}; };
-    std::vector<std::vector<float>> part(nthreads, std::vector<float>(G, 0.0f));
+    // Per-thread partial sums in double: the result then does not depend on the thread count
+    // or the OpenMP schedule (the reference sums float partials per thread, so its order
+    // varies with both; what remains between the two is its float rounding).
+    std::vector<std::vector<double>> part(nthreads, std::vector<double>(G, 0.0));
     // ---- interior_derivatives (drender.h:197-233) ----
     if (mode != ORC_SILHOUETTE) {
 #pragma omp parallel for schedule(dynamic, 2)
@@ -901,7 +906,7 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
 #ifdef _OPENMP
         tid = omp_get_thread_num();
 #endif
-        float* g = part[tid].data();
+        double* g = part[tid].data();
         for (int x = 0; x < W; x++)
           for (int si = 0; si < nsamp; si++) {
             // Hammersley offsets (drender.h:217-218), or with `aligned` the render's grid
@@ -923,7 +928,7 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
               const f3 vs = mode == ORC_VERTEX_COLOR ? val : mulv(val, lambert_light(s, L, o, d, h, s.nrm));
               const float w = 1.0f - h.u - h.v;
               const f3 cA = mul(w, vs), cB = mul(h.v, vs), cC = mul(h.u, vs);
-              float* gc = g + moff[m] + 3 * nv;
+              double* gc = g + moff[m] + 3 * nv;
               const uint32_t A = s.idx_local[3 * gt], B = s.idx_local[3 * gt + 1], C = s.idx_local[3 * gt + 2];
               gc[3 * A] += cA.x; gc[3 * A + 1] += cA.y; gc[3 * A + 2] += cA.z;
               gc[3 * B] += cB.x; gc[3 * B + 1] += cB.y; gc[3 * B + 2] += cB.z;
@@ -949,7 +954,7 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
 #ifdef _OPENMP
         tid = omp_get_thread_num();
 #endif
-        float* g = part[tid].data();
+        double* g = part[tid].data();
         const float rnd0 = rnd[2 * i], rnd1 = rnd[2 * i + 1];
         const int eid = sample_edge(E.cdf, rnd0);
         const EdgeRec& e = E.edges[eid];
@@ -990,7 +995,7 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
         const float dv1_dz = (g1x[2] * d1.x + g1y[2] * d1.y);
         const M44& T = s.tr_inv[0];  // mesh 0's inverse transform for every mesh (drender.cpp:60)
         if (moff[e.mesh] < 0) continue;
-        float* gp = g + moff[e.mesh];
+        double* gp = g + moff[e.mesh];
         gp[e.vn0 * 3 + 0] += T(0,0) * dv0_dx + T(0,1) * dv0_dy + T(0,2) * dv0_dz;
         gp[e.vn0 * 3 + 1] += T(1,0) * dv0_dx + T(1,1) * dv0_dy + T(1,2) * dv0_dz;
         gp[e.vn0 * 3 + 2] += T(2,0) * dv0_dx + T(2,1) * dv0_dy + T(2,2) * dv0_dz;
@@ -1001,7 +1006,7 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
         int trs = 3 * nvm;
         if (mode == ORC_VERTEX_COLOR || mode == ORC_LAMBERT_VERTEX_COLOR) trs += 3 * nvm;
         if (mode == ORC_TEXTURE_COLOR || mode == ORC_LAMBERT) trs += 3 * s.tex_w[e.mesh] * s.tex_h[e.mesh];
-        float* gt = gp + trs + 12 * e.inst;
+        double* gt = gp + trs + 12 * e.inst;
         const f3 o0 = s.pos_orig[e.v0], o1 = s.pos_orig[e.v1];
         gt[0] += o0.x * dv0_dx + o1.x * dv1_dx; gt[1] += o0.y * dv0_dx + o1.y * dv1_dx;
         gt[2] += o0.z * dv0_dx + o1.z * dv1_dx; gt[3] += 1 * dv0_dx + 1 * dv1_dx;
@@ -1011,8 +1016,11 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
         gt[10] += o0.z * dv0_dz + o1.z * dv1_dz; gt[11] += 1 * dv0_dz + 1 * dv1_dz;
       }
     }
-    for (int t = 0; t < nthreads; t++)
-      for (int j = 0; j < G; j++) grad[j] += part[t][j];
+    for (int j = 0; j < G; j++) {
+      double sum = 0.0;
+      for (int t = 0; t < nthreads; t++) sum += part[t][j];
+      grad[j] += (float)sum;
+    }
     img_off += (size_t)3 * W * H;
   }
 }

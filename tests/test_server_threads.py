@@ -23,15 +23,17 @@ SERVER = os.path.join(HERE, "..", "diffrender_tutorials_amd", "csrc", "drh_serve
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
+@pytest.mark.parametrize("admitters", [1, 4])
 @pytest.mark.parametrize("sanitizer", ["thread", "address"])
-def test_server_threading_under_sanitizer(tmp_path, sanitizer):
+def test_server_threading_under_sanitizer(tmp_path, sanitizer, admitters):
     exe = str(tmp_path / f"stress_{sanitizer}")
     cmd = ["g++", "-std=c++17", "-O1", "-g", f"-fsanitize={sanitizer}", "-fno-omit-frame-pointer",
            "-include", os.path.join(STUB, "tsan_cv.h"), "-I", STUB,
            os.path.join(STUB, "stress.cpp"), os.path.join(STUB, "drh_stub.cpp"), SERVER, "-o", exe, "-lpthread"]
     b = subprocess.run(cmd, capture_output=True, text=True)
     assert b.returncode == 0, b.stderr[-4000:]
-    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 second_deadlock_stack=1", ASAN_OPTIONS="detect_leaks=1")
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 second_deadlock_stack=1", ASAN_OPTIONS="detect_leaks=1",
+               DRH_SERVER_ADMITTERS=str(admitters))
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0 and "server stress ok" in r.stdout, (r.stdout[-2000:], r.stderr[-6000:])
     assert "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[-6000:]

@@ -260,6 +260,13 @@ struct drh_renderer : BufOwner {
   // interior samples: Hammersley (reference) or the render's grid (aligned); store: the render
   // keeps each sample's closest hit and the interior pass reads it back
   int interior_aligned = 0, interior_store = 0;   // interior_store: DRH_HITS_*
+  // pixel-sample jitter (drh_set_pixel_jitter): the pattern of seed + step, where step is 0
+  // outside the fused step, the host's step number for OptSimple steps and the device's Adam
+  // step counter for Adam steps (so a captured step draws a fresh pattern on every replay)
+  int jit_on = 0;
+  uint32_t jit_seed = 0, jit_host_step = 0;
+  const uint32_t* jit_dev_step = nullptr;
+  Jitter jitter() const { return Jitter{jit_seed + jit_host_step, jit_on, jit_on ? jit_dev_step : nullptr}; }
   size_t hit_limit = ~(size_t)0;                  // DRH_HITS_AUTO stores only within this many bytes
   // Free a buffer unless the handle's stream is being captured (the free would become a node
   // of the captured step); the eager step before any capture has freed it already.
@@ -809,7 +816,7 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   int part_off = 0;
   for (int v = 0; v < views; v++) {
     RenderArgs ra{tracer_view(v), light, cams.as<DevCam>(), v, sq, d_img, d_target, d_adj, loss_part.as<double>(), part_off,
-                  store_hits ? hitbuf.as<float4>() : nullptr};
+                  store_hits ? hitbuf.as<float4>() : nullptr, jitter()};
     ra.tr.rays = ray_slot(0);
     timed("k_render", [&] { launch_render(st.mode, ra, tiles[v], stream); });
     part_off += tiles[v];
@@ -892,6 +899,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ia.adj = d_adj; ia.grad = d_grad; ia.layout = dlayout.as<int>();
       ia.lds_base = 0; ia.lds_n = 0; ia.uniform = 0; ia.uniform_off = 0;
       ia.aligned = interior_aligned;
+      ia.jit = jitter();
       ia.light = light;
       ia.hits = (interior_store && hits_stored) ? hitbuf.as<float4>() : nullptr;
       if (interior_pos && st.mode == DRH_VERTEX_COLOR) {
@@ -1211,7 +1219,12 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
   // TEXTURE / LAMBERT interior adjoints reach only the texture block (shade_grad,
   // src/shade_lambert.cpp:37-42): with the texture frozen they feed nothing
   h->skip_tex_interior = (h->st.mode == DRH_TEXTURE_COLOR || h->st.mode == DRH_LAMBERT) && h->tex_lr == 0.0f;
+  // step t (from 1) samples the jitter pattern of seed + t
+  if (h->opt_kind == DRH_OPT_ADAM) h->jit_dev_step = h->opt_t.as<uint32_t>();
+  else h->jit_host_step = (uint32_t)h->opt_iter + 1u;
   s = drh_d_render_and_compare_dev(h, cams, views, d_targets, edge_samples, h->grad.as<float>(), nullptr, d_loss);
+  h->jit_dev_step = nullptr;
+  h->jit_host_step = 0;
   h->skip_tex_interior = false;
   if (s) return s;
   // StepDecay (src/optimizer_simple.cpp:82-90): decayPeriod 100, decay_mult 0.75 for GD_Adam,
@@ -1334,6 +1347,14 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
   }
   if ((s = opt_step_body(h, cams, views, d_targets, edge_samples, d_loss))) return s;
   h->opt_iter++;
+  return DRH_OK;
+}
+
+drh_status drh_set_pixel_jitter(drh_handle h, int on, unsigned seed) {
+  if (!h) return DRH_E_INVALID;
+  if ((on != 0) != (h->jit_on != 0) || seed != h->jit_seed) h->drop_graph();   // baked into the captured step
+  h->jit_on = on != 0;
+  h->jit_seed = seed;
   return DRH_OK;
 }
 

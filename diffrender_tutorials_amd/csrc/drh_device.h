@@ -77,6 +77,34 @@ __device__ __forceinline__ void eye_ray(const DevCam& c, float x, float y, F3& o
   d = normalize(F3{b[0] - a[0], b[1] - a[1], b[2] - a[2]});
 }
 
+// Pixel-sample positions.  The reference puts sample (dx, dy) of a pixel at the centre of its
+// cell of the sqrt(spp) x sqrt(spp) grid, ((dx + 0.5)/sq, (dy + 0.5)/sq) (src/drender.h:124-128).
+// With jitter on, the sample moves to a uniform point of that cell, ((dx + ux)/sq, (dy + uy)/sq)
+// with ux, uy in [0, 1) multiples of 2^-24 drawn from an integer hash of (seed + step, view,
+// x, y, sample): stratified jitter, a fresh pattern per seed and per optimiser step (step is
+// read from the device when the caller passes a counter).  Off, ux = uy = 0.5 exactly.
+// The hash is Wellons' "lowbias32" (hash-prospector, 2020) chained over the keys.
+struct Jitter { uint32_t seed; int on; const uint32_t* step; };
+__host__ __device__ __forceinline__ uint32_t jitter_hash(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ void sample_offset(const Jitter& j, int view, int x, int y, int si, int sq,
+                                              float& xo, float& yo) {
+  float ux = 0.5f, uy = 0.5f;
+  if (j.on) {
+    const uint32_t base = j.seed + (j.step ? *j.step : 0u);
+    uint32_t k = jitter_hash(base ^ jitter_hash((uint32_t)view + 0x9e3779b9U));
+    k = jitter_hash(k ^ (uint32_t)x);
+    k = jitter_hash(k ^ (uint32_t)y);
+    k = jitter_hash(k ^ (uint32_t)si);
+    ux = (float)(k >> 8) * (1.0f / 16777216.0f);
+    uy = (float)(jitter_hash(k) >> 8) * (1.0f / 16777216.0f);
+  }
+  xo = ((si % sq) + ux) / float(sq);
+  yo = ((si / sq) + uy) / float(sq);
+}
+
 // ---------------------------------------------------------------------------
 // Scene data on the device
 // ---------------------------------------------------------------------------

@@ -148,6 +148,7 @@ struct RenderArgs {
   float4* hits;           // non-NULL: store each sample's closest hit (id bits, u, v, t) for the
                           // interior pass, at ((pixel) * sq*sq + sample) of the views' pixels;
                           // LAMBERT_VCOL stores lit_encode(shadow, ndl) in place of t
+  Jitter jit;             // sample positions in the pixel (off: the grid's centres)
 };
 
 struct InteriorArgs {
@@ -170,6 +171,7 @@ struct InteriorArgs {
   int pos_grad;           // VERTEX_COLOR: add the interior position term (k_interior<2, true>)
   const float* tr_inv0;   // mesh 0's inverse transform (for pos_grad)
   Light light;            // LAMBERT_VCOL: the light of the colour adjoint's factor
+  Jitter jit;             // aligned: the render's sample positions
 };
 
 struct ProjectArgs {

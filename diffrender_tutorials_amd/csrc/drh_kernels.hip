@@ -462,9 +462,8 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
     // base + k, so the gather below adds col/spp in the reference's dy-major order
     F3 col = {0, 0, 0};
     if (inside) {
-      const int dx = tm.s % sq, dy = tm.s / sq;
-      const float xo = (dx + 0.5f) / float(sq);
-      const float yo = (dy + 0.5f) / float(sq);
+      float xo, yo;
+      sample_offset(a.jit, a.view, x, y, tm.s, sq, xo, yo);
       Hit h;
       float lit = 0.0f;
       col = shade<MODE, COUNT>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
@@ -480,8 +479,8 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   } else if (inside) {
     for (int dy = 0; dy < sq; dy++)
       for (int dx = 0; dx < sq; dx++) {
-        const float xo = (dx + 0.5f) / float(sq);
-        const float yo = (dy + 0.5f) / float(sq);
+        float xo, yo;
+        sample_offset(a.jit, a.view, x, y, dy * sq + dx, sq, xo, yo);
         Hit h;
         float lit = 0.0f;
         const F3 col = shade<MODE, COUNT>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
@@ -660,8 +659,8 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
             lit_stored = r.w;
             if (a.coverage) { if (h.gtri != NO_HIT) uni = add(uni, val); break; }
           } else {
-            const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
-            const float yo = a.aligned ? ((si / sq) + 0.5f) / float(sq) : a.hamm[2 * si + 1];
+            float xo = a.hamm[2 * si], yo = a.hamm[2 * si + 1];
+            if (a.aligned) sample_offset(a.jit, a.view, x, y, si, sq, xo, yo);
             eye_ray(c, x + xo, y + yo, ro, rd);
             const F3 o = ro, d = rd;
             if (a.coverage) {
@@ -836,8 +835,8 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
       if (a.hits) {
         hit = __float_as_uint(a.hits[(c.img_off / 3 + (long long)tm.y * c.w + tm.x) * a.nsamp + si].x) != NO_HIT;
       } else {
-        const float xo = a.aligned ? ((si % sq) + 0.5f) / float(sq) : a.hamm[2 * si];
-        const float yo = a.aligned ? ((si / sq) + 0.5f) / float(sq) : a.hamm[2 * si + 1];
+        float xo = a.hamm[2 * si], yo = a.hamm[2 * si + 1];
+        if (a.aligned) sample_offset(a.jit, a.view, tm.x, tm.y, si, sq, xo, yo);
         F3 o, d;
         eye_ray(c, tm.x + xo, tm.y + yo, o, d);
         hit = covered<COUNT>(T, o, d, wst);

@@ -37,7 +37,7 @@ STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params",
-           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_restricted", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples",
+           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_restricted", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples", "drh_set_pixel_jitter",
            "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
@@ -223,6 +223,11 @@ class DiffRender:
     def set_interior_samples(self, aligned, store=False):
         """aligned: render-grid interior samples; store: HITS_RECOMPUTE / HITS_STORE / HITS_AUTO."""
         self._check(load().drh_set_interior_samples(self.h, int(aligned), int(store)))
+
+    def set_pixel_jitter(self, on, seed=0):
+        """Stratified jitter of the pixel samples with pattern `seed` (the fused step's t-th step
+        uses seed + t); off: the reference's cell centres."""
+        self._check(load().drh_set_pixel_jitter(self.h, int(bool(on)), ctypes.c_uint(seed & 0xFFFFFFFF)))
 
     def set_hit_buffer_limit(self, nbytes):
         self._check(load().drh_set_hit_buffer_limit(self.h, int(nbytes)))

@@ -209,6 +209,14 @@ drh_status drh_get_restricted(drh_handle h, float* params);
  * counterpart in the reference. */
 enum { DRH_HITS_RECOMPUTE = 0, DRH_HITS_STORE = 1, DRH_HITS_AUTO = 2 };
 drh_status drh_set_interior_samples(drh_handle h, int aligned, int store);
+/* Jittered pixel samples (off by default; no counterpart in the reference, whose render puts
+ * sample (dx, dy) at the centre of its cell of the sqrt(spp) x sqrt(spp) grid,
+ * src/drender.h:124-128).  On: the sample moves to a uniform point of its cell (stratified
+ * jitter) drawn from an integer hash of (seed + step, view, x, y, sample), step = 0 for
+ * render / d_render / d_render_and_compare and t = 1, 2, ... for the fused step's t-th step, so
+ * every optimiser step draws a fresh pattern.  The interior pass with aligned samples uses
+ * the same points; the Hammersley interior samples are unchanged. */
+drh_status drh_set_pixel_jitter(drh_handle h, int on, unsigned seed);
 /* The most HBM DRH_HITS_AUTO may use for stored hits (default: no limit but the device's free
  * memory); the server sets it from its HBM budget.  No counterpart in the reference. */
 drh_status drh_set_hit_buffer_limit(drh_handle h, size_t bytes);

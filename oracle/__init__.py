@@ -139,6 +139,10 @@ class Scene:
     def set_tracer(self, kind):
         lib().orc_scene_set_tracer(self.h, kind)
 
+    def set_jitter(self, on, seed=0):
+        """Jittered pixel samples of `seed` (the device's drh_set_pixel_jitter pattern)."""
+        lib().orc_set_jitter(self.h, 1 if on else 0, ctypes.c_uint32(seed & 0xFFFFFFFF))
+
     def grad_size(self, mode):
         return lib().orc_grad_size(self.h, mode)
 
@@ -196,6 +200,14 @@ class Scene:
         proj = np.empty(3 * sum(m.V for m in self.meshes), np.float32)
         lib().orc_edge_sampler(self.h, ctypes.byref(c), _p(v0, ctypes.c_int), _p(v1, ctypes.c_int), _p(pmf), _p(cdf), _p(proj))
         return v0, v1, pmf, cdf, proj.reshape(-1, 3)
+
+
+def pixel_sample(on, seed, view, x, y, si, sq):
+    """Offsets (xo, yo) in the pixel of sample si (orc_pixel_sample)."""
+    xo, yo = ctypes.c_float(), ctypes.c_float()
+    lib().orc_pixel_sample(1 if on else 0, ctypes.c_uint32(seed & 0xFFFFFFFF), view, x, y, si, sq,
+                           ctypes.byref(xo), ctypes.byref(yo))
+    return xo.value, yo.value
 
 
 def loss(image, target):

@@ -290,7 +290,32 @@ struct Scene {
   // after BVH traversal: for camera rays of the current view (shared origin) and for shadow
   // rays (shared direction).  Set before the parallel loops.
   std::vector<int> sus_cam, sus_light;
+  // jittered pixel samples (orc_set_jitter)
+  int jit_on = 0;
+  uint32_t jit_seed = 0;
 };
+
+// Pixel-sample positions.  The reference: sample (dx, dy) at the centre of its cell of the
+// sq x sq grid (src/drender.h:124-128).  Jittered (the build's own option, no counterpart in the
+// reference): a uniform point of that cell, offsets ux, uy = (k >> 8) * 2^-24 from a chain of
+// Wellons' "lowbias32" integer hash (hash-prospector, 2020) over (seed, view, x, y, sample).
+static uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+static void pixel_sample(const Scene& s, int view, int x, int y, int si, int sq, float& xo, float& yo) {
+  float ux = 0.5f, uy = 0.5f;
+  if (s.jit_on) {
+    uint32_t k = lowbias32(s.jit_seed ^ lowbias32((uint32_t)view + 0x9e3779b9U));
+    k = lowbias32(k ^ (uint32_t)x);
+    k = lowbias32(k ^ (uint32_t)y);
+    k = lowbias32(k ^ (uint32_t)si);
+    ux = (float)(k >> 8) * (1.0f / 16777216.0f);
+    uy = (float)(lowbias32(k) >> 8) * (1.0f / 16777216.0f);
+  }
+  xo = ((si % sq) + ux) / float(sq);
+  yo = ((si / sq) + uy) / float(sq);
+}
 
 // Clamped-determinant suspects.  BF's invDet = sign/max(|det|, 1e-6) lets a nearly parallel
 // ray accept a triangle it never touches.  With d = a*e1 + b*e2 + c*(o-A), acceptance needs
@@ -777,6 +802,19 @@ void* orc_scene_create(int n_meshes, const int* vcount, const int* fcount, const
 
 void orc_scene_destroy(void* s) { delete (Scene*)s; }
 
+void orc_set_jitter(void* sp, int on, uint32_t seed) {
+  Scene* s = (Scene*)sp;
+  s->jit_on = on != 0;
+  s->jit_seed = seed;
+}
+
+void orc_pixel_sample(int on, uint32_t seed, int view, int x, int y, int si, int sq, float* xo, float* yo) {
+  Scene s;
+  s.jit_on = on != 0;
+  s.jit_seed = seed;
+  pixel_sample(s, view, x, y, si, sq, *xo, *yo);
+}
+
 void orc_scene_set_tracer(void* sp, int kind) {
   Scene* s = (Scene*)sp;
   s->tracer = kind;
@@ -833,8 +871,8 @@ void orc_render(void* sp, int mode, int spp, const orc_lambert* lam, const orc_c
         f3 acc = {0, 0, 0};
         for (int dy = 0; dy < sq; dy++)
           for (int dx = 0; dx < sq; dx++) {
-            const float xo = (dx + 0.5f) / float(sq);
-            const float yo = (dy + 0.5f) / float(sq);
+            float xo, yo;
+            pixel_sample(s, vi, x, y, dy * sq + dx, sq, xo, yo);
             const f3 col = shade(s, mode, L, c, f2{x + xo, y + yo}, s.nrm);
             acc = add(acc, divs(col, (float)nsamp));
           }
@@ -909,10 +947,10 @@ void orc_d_render_ex(void* sp, int mode, int spp, const orc_lambert* lam, const 
         double* g = part[tid].data();
         for (int x = 0; x < W; x++)
           for (int si = 0; si < nsamp; si++) {
-            // Hammersley offsets (drender.h:217-218), or with `aligned` the render's grid
-            // offsets (d + 0.5)/sq (drender.h:124-128)
-            const float xo = aligned ? ((si % sq) + 0.5f) / float(sq) : hamm[2 * si];
-            const float yo = aligned ? ((si / sq) + 0.5f) / float(sq) : hamm[2 * si + 1];
+            // Hammersley offsets (drender.h:217-218), or with `aligned` the render's sample
+            // positions (drender.h:124-128, jittered when jitter is on)
+            float xo = hamm[2 * si], yo = hamm[2 * si + 1];
+            if (aligned) pixel_sample(s, vi, x, y, si, sq, xo, yo);
             const f3 val = divs(adj_at(x, y), (float)nsamp);
             f3 o, d;
             eye_ray(c, x + xo, y + yo, o, d);

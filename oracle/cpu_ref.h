@@ -51,6 +51,11 @@ void* orc_scene_create(int n_meshes, const int* vcount, const int* fcount, const
                        const int* differentiable);
 void  orc_scene_destroy(void* s);
 void  orc_scene_set_tracer(void* s, int kind);
+/* Jittered pixel samples for orc_render and the aligned interior samples of orc_d_render_ex
+ * (the device's drh_set_pixel_jitter pattern of `seed`); off: the reference's cell centres. */
+void  orc_set_jitter(void* s, int on, uint32_t seed);
+/* The sample position of sample si of pixel (x, y) in view `view` (offsets in the pixel). */
+void  orc_pixel_sample(int on, uint32_t seed, int view, int x, int y, int si, int sq, float* xo, float* yo);
 /* Size of the flat gradient (DScene full_data of every differentiable mesh, in mesh order). */
 int   orc_grad_size(void* s, int mode);
 

@@ -90,6 +90,9 @@ def fused_rate(drh, r, cams, views, d_target, N, d_loss, steps, warmup):
     return time.perf_counter() - t0
 
 
+JITTER_SEED = 2024   # pixel-sample jitter pattern of the config-2 legs
+
+
 def plain_leg(drh, meshes, cam, target, N, spp, lam, iters):
     """The plain one-job-at-a-time GPU path: the reference's optimiser loop
     (src/optimizer_simple.cpp:272-309) driving the drop-in through its host interface --
@@ -99,6 +102,7 @@ def plain_leg(drh, meshes, cam, target, N, spp, lam, iters):
     m = meshes[0].copy()
     V = m.V
     r.commit([m])
+    r.set_pixel_jitter(True, JITTER_SEED)
     P = r.grad_size()
     mom, vel = np.zeros(P, np.float32), np.zeros(P, np.float32)
     lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
@@ -363,6 +367,7 @@ def cpu_port_leg(meshes, target, cam, lam, N, spp, seconds):
     c0 = time.perf_counter()
     while True:
         ref = O.Scene([m], O.TRACER_BVH)
+        ref.set_jitter(True, JITTER_SEED + iters + 1)   # the GPU step's pattern of step t
         img = ref.render(4, spp, [cam], lam=lam)[0]
         _, adj = O.loss(img, target)
         g = ref.d_render(4, spp, [cam], [adj], N, lam=lam)
@@ -384,7 +389,8 @@ def cpu_port_leg(meshes, target, cam, lam, N, spp, seconds):
             "cpu_model": cpu_model(),
             "sample": f"{iters} full config-2 iterations of the oracle (its C++ port of the reference loop, -O3, "
                       f"OpenMP) with its CPU BVH tracer as the Embree stand-in (Embree binary absent): commit with "
-                      f"BVH build, render, loss, d_render (interior + edges), Adam on positions and albedo"}
+                      f"BVH build, render (4 spp jittered), loss, d_render (interior + edges), Adam on positions "
+                      f"and albedo"}
 
 
 def main():
@@ -399,7 +405,7 @@ def main():
     ap.add_argument("--plain-iters", type=int, default=20, help="iterations of the plain host-driven loop")
     ap.add_argument("--no-serve", action="store_true", help="skip the plain, served, multi-view and stream legs")
     ap.add_argument("--mix-jobs", type=int, default=64, help="jobs in the config-3 uneven mix (0: skip)")
-    ap.add_argument("--mix-iters", type=int, default=30, help="Adam iterations per mix job")
+    ap.add_argument("--mix-iters", type=int, default=100, help="Adam iterations per mix job (config 3: 100 steps)")
     ap.add_argument("--mix-active", type=int, default=16, help="server slots for the mix")
     ap.add_argument("--multiview-steps", type=int, default=5, help="timed config-4 steps per interior mode (0: skip)")
     ap.add_argument("--stream-jobs", type=int, default=24, help="jobs in the config-5 Poisson stream (0: skip)")
@@ -443,6 +449,7 @@ def main():
     def handle(tex_lr):
         r = drh.DiffRender(mode=drh.LAMBERT, spp=spp, light=lam)
         r.commit(meshes)
+        r.set_pixel_jitter(True, JITTER_SEED)   # 4 spp jittered: a fresh stratified pattern per step
         r.opt_init(drh.OPT_ADAM, 1e-3, 1e-3)
         r.opt_set_texture_lr(tex_lr)
         return r
@@ -528,10 +535,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "config2: icosphere-5 (10242 V / 20480 F), Lambert, 1024x1024, spp 4, 2^20 edge samples,"
+        "config": {"workload": "config2: icosphere-5 (10242 V / 20480 F), Lambert, 1024x1024, 4 spp jittered, 2^20 edge samples,"
                                " fused commit (LBVH build / refit) + render + loss + d_render (interior texture adjoint"
                                " + edge sampling) + Adam on the vertex positions and the albedo per step",
                    "triangles": F, "vertices": V, "resolution": [W, H], "spp": spp, "edge_samples": N,
+                   "pixel_samples": f"stratified jitter, seed {JITTER_SEED} + step",
                    "parallelism": "single GPU"},
         "rays_per_step": rays,
         "rays_per_s": round(rays["total"] * value, 1),

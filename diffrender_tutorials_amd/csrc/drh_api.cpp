@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -327,6 +328,100 @@ struct drh_renderer : BufOwner {
   // per call
   DBuf cams, img, adj, tgt, loss_part, loss, grad;
   DBuf fault;   // the kernels' fault word (DevScene::fault), zero while all is well
+  // Host staging: what a commit, a camera change or a handle's own edge-sample table sends to
+  // the device is first copied into a pinned buffer of the handle and sent from there by DMA
+  // on the stream; the call returns without waiting for the transfer (the caller's arrays
+  // are free at once).  Two buffers alternate; a staging waits (an event) only for the
+  // transfers of the staging before the previous one, long done in practice.
+  struct Stage { char* p = nullptr; size_t cap = 0; hipEvent_t done = nullptr; bool pending = false; };
+  Stage stg[2];
+  int stg_cur = 0;
+  size_t stg_used = 0;
+  drh_status stage_begin(size_t need) {
+    stg_cur ^= 1;
+    Stage& b = stg[stg_cur];
+    if (b.pending) {
+      if (hipEventSynchronize(b.done) != hipSuccess) return hip(hipGetLastError(), "staging");
+      b.pending = false;
+    }
+    if (need > b.cap) {
+      if (b.p) (void)hipHostFree(b.p);
+      b.p = nullptr;
+      b.cap = 0;
+      const size_t cap = std::max<size_t>(need + need / 4, 1 << 16);
+      if (hipHostMalloc((void**)&b.p, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DRH_E_OOM, "pinned staging buffer of %zu bytes", cap);
+      }
+      b.cap = cap;
+    }
+    if (!b.done && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) return hip(hipGetLastError(), "staging");
+    stg_used = 0;
+    segs.clear();
+    return DRH_OK;
+  }
+  static size_t stage_round(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+  // copy src into the current staging buffer (stage_begin reserved room for it) and send it
+  drh_status stage_send(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return DRH_OK;
+    if (stg_used + bytes > stg[stg_cur].cap) return fail(DRH_E_INVALID, "staging: %zu bytes beyond the reserved room", bytes);
+    char* q = stg[stg_cur].p + stg_used;
+    std::memcpy(q, src, bytes);
+    stg_used += stage_round(bytes);
+    return hip(hipMemcpyAsync(dst, q, bytes, hipMemcpyHostToDevice, stream), "upload");
+  }
+  // Many arrays at once: stage_put collects them (copied into the staging buffer now),
+  // stage_flush sends the staged block with one copy into `arena` and scatters it to the
+  // arrays' buffers with one kernel (instead of a copy call per array).
+  struct Seg { void* dst; size_t off, bytes; };
+  std::vector<Seg> segs;
+  DBuf arena;
+  drh_status stage_put(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return DRH_OK;
+    if (stg_used + bytes > stg[stg_cur].cap || bytes % 4) {
+      segs.clear();
+      return fail(DRH_E_INVALID, "staging: %zu bytes beyond the reserved room", bytes);
+    }
+    char* q = stg[stg_cur].p + stg_used;
+    std::memcpy(q, src, bytes);
+    segs.push_back(Seg{dst, stg_used, bytes});
+    stg_used += stage_round(bytes);
+    return DRH_OK;
+  }
+  drh_status stage_flush() {
+    drh_status s;
+    if (segs.empty()) return DRH_OK;
+    if ((s = hip(arena.reserve(stg_used), "alloc")) ||
+        (s = hip(hipMemcpyAsync(arena.p, stg[stg_cur].p, stg_used, hipMemcpyHostToDevice, stream), "upload")))
+      return s;
+    for (size_t b = 0; b < segs.size(); b += SCATTER_MAX) {
+      ScatterArgs a{};
+      a.src = arena.as<uint32_t>();
+      for (size_t k = b; k < segs.size() && a.n < SCATTER_MAX; k++, a.n++) {
+        a.dst[a.n] = (uint32_t*)segs[k].dst;
+        a.off[a.n] = (long long)(segs[k].off / 4);
+        a.words[a.n] = (long long)(segs[k].bytes / 4);
+      }
+      launch_scatter(a, stream);
+    }
+    segs.clear();
+    return hip(hipGetLastError(), "upload");
+  }
+  drh_status stage_end() {
+    Stage& b = stg[stg_cur];
+    if (hipEventRecord(b.done, stream) != hipSuccess) return hip(hipGetLastError(), "staging");
+    b.pending = true;
+    return DRH_OK;
+  }
+  void stage_free() {
+    for (Stage& b : stg) {
+      if (b.pending) (void)hipEventSynchronize(b.done);
+      if (b.done) (void)hipEventDestroy(b.done);
+      if (b.p) (void)hipHostFree(b.p);
+      b = Stage{};
+    }
+  }
+  const float2* rnd_ptr = nullptr;   // the edge-sample table (shared, or this handle's rnd)
   // Ray counting (drh_count_rays): the traversal kernels' counting instantiations add their
   // rays to rays_buf: [0, 1] render closest / any-hit, [2, 3] interior, [4, 5] edge trace.
   bool count_on = false;
@@ -361,14 +456,70 @@ struct drh_renderer : BufOwner {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<char> graph_sig[2];
   unsigned long long graph_epoch[2] = {0, 0};
-  // a graph may still run from earlier launches on the stream: wait before destroying it
+  // A dropped form's executable is kept (idle) and updated in place from the next capture of
+  // that form when the topology matches (hipGraphExecUpdate), which costs far less host time
+  // than instantiating again: a handle re-used for another scene of the same shape, or
+  // re-captured after new arguments, pays only the capture.  Dropping does not wait for the
+  // stream: an event recorded behind the form's last launch is waited for only when the idle
+  // executable is updated or destroyed (by then long passed).
+  hipGraphExec_t stale[2] = {nullptr, nullptr};
+  hipGraph_t stale_graph[2] = {nullptr, nullptr};
+  hipEvent_t stale_ev[2] = {nullptr, nullptr};
+  void retire(int k) {
+    if (!stale[k] && !stale_graph[k]) return;
+    (void)hipEventSynchronize(stale_ev[k]);
+    if (stale[k]) (void)hipGraphExecDestroy(stale[k]);
+    if (stale_graph[k]) (void)hipGraphDestroy(stale_graph[k]);
+    stale[k] = nullptr;
+    stale_graph[k] = nullptr;
+  }
   void drop_form(int k) {
-    if (gexec[k] || graph[k]) (void)hipStreamSynchronize(stream);
-    if (gexec[k]) { (void)hipGraphExecDestroy(gexec[k]); gexec[k] = nullptr; }
-    if (graph[k]) { (void)hipGraphDestroy(graph[k]); graph[k] = nullptr; }
     graph_sig[k].clear();
+    if (!gexec[k] && !graph[k]) return;
+    retire(k);
+    if (!stale_ev[k] && hipEventCreateWithFlags(&stale_ev[k], hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(stream);   // no event: wait here instead
+      if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+      if (graph[k]) (void)hipGraphDestroy(graph[k]);
+      gexec[k] = nullptr;
+      graph[k] = nullptr;
+      return;
+    }
+    (void)hipEventRecord(stale_ev[k], stream);   // behind every launch of this form so far
+    stale[k] = gexec[k];
+    stale_graph[k] = graph[k];
+    gexec[k] = nullptr;
+    graph[k] = nullptr;
   }
   void drop_graph() { drop_form(0); drop_form(1); }
+  void destroy_graphs() {
+    drop_graph();
+    for (int k = 0; k < 2; k++) {
+      retire(k);
+      if (stale_ev[k]) (void)hipEventDestroy(stale_ev[k]);
+      stale_ev[k] = nullptr;
+    }
+  }
+  // executable for captured graph g of form k: the idle one updated in place, else a new one
+  bool instantiate(int k, hipGraph_t g) {
+    if (stale[k]) {
+      (void)hipEventSynchronize(stale_ev[k]);   // its last launch has finished
+      hipGraphNode_t bad = nullptr;
+      hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+      const hipError_t e = hipGraphExecUpdate(stale[k], g, &bad, &res);
+      if (e == hipSuccess && res == hipGraphExecUpdateSuccess) {
+        gexec[k] = stale[k];
+        stale[k] = nullptr;
+        if (stale_graph[k]) (void)hipGraphDestroy(stale_graph[k]);
+        stale_graph[k] = nullptr;
+        return true;
+      }
+      (void)hipGetLastError();
+    }
+    retire(k);
+    return hipGraphInstantiate(&gexec[k], g, nullptr, nullptr, 0) == hipSuccess;
+  }
   // profiling
   bool prof = false;
   std::vector<ProfEvent> pending;
@@ -631,12 +782,17 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     }
 
   auto R = [&](DBuf& b, size_t bytes, const char* what) -> drh_status { return hip(b.reserve(std::max<size_t>(bytes, 16)), what); };
-  auto up = [&](DBuf& b, const void* src, size_t bytes) -> drh_status {
-    if (!bytes) return DRH_OK;
-    return hip(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream), "commit upload");
-  };
+  auto up = [&](DBuf& b, const void* src, size_t bytes) -> drh_status { return stage_put(b.p, src, bytes); };
   drh_status s;
   const size_t v3 = 12 * (size_t)V;
+  {
+    auto q = [](size_t b) { return stage_round(b); };
+    size_t need = q(v3) + q(v3) + q(8 * (size_t)V) + 2 * q(v3) + q(4 * (size_t)tsum) + q(12 * (size_t)F) +
+                  q(4 * (size_t)F) + 2 * q(4 * (size_t)(n + 1)) + 3 * q(4 * (size_t)n) + q(4 * mats.size()) +
+                  q(4 * layout.size()) + q(64) + q(48 * (size_t)n) + q(4 * (size_t)n) + q(28 * (size_t)n);
+    if (!topo_same) need += q(8 * hedges.size()) + q(4 * hemesh.size());
+    if ((s = stage_begin(need))) return s;
+  }
   if ((s = R(pos_orig, v3, "alloc")) || (s = R(pos, v3, "alloc")) || (s = R(col, v3, "alloc")) ||
       (s = R(tc, 8 * (size_t)V, "alloc")) || (s = R(nrm_in, v3, "alloc")) || (s = R(nrm, v3, "alloc")) ||
       (s = R(nrm2d, v3, "alloc")) || (s = R(texels, 4 * (size_t)tsum, "alloc")) || (s = R(idx, 12 * (size_t)F, "alloc")) ||
@@ -663,7 +819,6 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     std::vector<float> def(3 * (size_t)V, 0.0f);
     for (int k = 0; k < V; k++) def[3 * k] = 1.0f;  // prepared default normal (1,0,0)
     if ((s = up(nrm, def.data(), v3)) || (s = up(nrm2d, def.data(), v3))) return s;
-    if ((s = hip(hipStreamSynchronize(stream), "commit upload"))) return s;
   }
   if (tsum && d->texels) { if ((s = up(texels, d->texels, 4 * (size_t)tsum))) return s; }
   if ((s = up(idx, gidx.data(), 12 * (size_t)F)) || (s = up(tri_mesh, tmesh.data(), 4 * (size_t)F)) ||
@@ -685,9 +840,8 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     diff_cache = df;
   }
   if ((s = R(proj, 16 * (size_t)V, "alloc"))) return s;
-  // everything uploaded from this call's host arrays must land before they may change
-  if ((s = hip(hipStreamSynchronize(stream), "commit sync"))) return s;
-  return DRH_OK;
+  if ((s = stage_flush())) return s;
+  return stage_end();   // the caller's arrays were copied into the staging buffer
 }
 
 drh_status drh_renderer::build_bvh(bool full) {
@@ -783,8 +937,9 @@ drh_status drh_renderer::set_cams(const drh_camera* c, int views, long long* tot
   if (!same) {
     drh_status s;
     if ((s = hip(cams.reserve(sizeof(DevCam) * views), "alloc"))) return s;
-    if ((s = hip(hipMemcpyAsync(cams.p, dc.data(), sizeof(DevCam) * views, hipMemcpyHostToDevice, stream), "camera upload"))) return s;
-    if ((s = hip(hipStreamSynchronize(stream), "camera upload"))) return s;
+    if ((s = stage_begin(stage_round(sizeof(DevCam) * views))) ||
+        (s = stage_send(cams.p, dc.data(), sizeof(DevCam) * views)) || (s = stage_end()))
+      return s;
     cam_cache = dc;
     views_built = -1;
   }
@@ -830,16 +985,45 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   return hip(hipGetLastError(), "render kernels");
 }
 
+// Edge-sample tables on the device, shared read-only by every handle of the device (a table
+// depends only on the random-number mode and N): uploaded once, kept until the process ends,
+// SHARED_RND_BYTES of them at most per device (beyond that a handle uploads its own).
+constexpr size_t SHARED_RND_BYTES = 512ull << 20;
+const float2* shared_sample_table(int device, int mode, long long N) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, long long>, float2*> tables;
+  static std::map<int, size_t> used;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_tuple(device, mode, N);
+  auto it = tables.find(key);
+  if (it != tables.end()) return it->second;
+  const size_t bytes = 8 * (size_t)N;
+  if (used[device] + bytes > SHARED_RND_BYTES) return nullptr;
+  const auto t = sorted_sample_table(mode, N);
+  float2* p = nullptr;
+  if (hipMalloc((void**)&p, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  if (hipMemcpy(p, t->data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(p);
+    return nullptr;
+  }
+  used[device] += bytes;
+  tables.emplace(key, p);
+  return p;
+}
+
 drh_status drh_renderer::ensure_rnd(long long N) {
   if (N == rnd_n) return DRH_OK;
   if (N > 0x7FFFFFFFll) return fail(DRH_E_INVALID, "edge_samples too large");
   drh_status s;
-  if ((s = hip(rnd.reserve(std::max<size_t>(8 * (size_t)N, 16)), "alloc"))) return s;
-  if (N > 0) {
+  rnd_ptr = nullptr;
+  if (N > 0 && (rnd_ptr = shared_sample_table(st.device, st.rng_mode, N)) == nullptr) {
+    if ((s = hip(rnd.reserve(8 * (size_t)N), "alloc"))) return s;
     const auto t = sorted_sample_table(st.rng_mode, N);
-    if ((s = hip(hipMemcpyAsync(rnd.p, t->data(), 8 * (size_t)N, hipMemcpyHostToDevice, stream), "rng upload")) ||
-        (s = hip(hipStreamSynchronize(stream), "rng upload")))
+    if ((s = stage_begin(stage_round(8 * (size_t)N))) || (s = stage_send(rnd.p, t->data(), 8 * (size_t)N)) ||
+        (s = stage_end()))
       return s;
+    rnd_ptr = rnd.as<float2>();
   }
   rnd_n = N;
   return DRH_OK;
@@ -943,7 +1127,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       ea.tr = tracer_view(v); ea.tr.rays = ray_slot(2); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
       ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
       ea.proj = proj.as<float4>();
-      ea.rnd = rnd.as<float2>();
+      ea.rnd = rnd_ptr;
       ea.adj = d_adj; ea.grad = d_grad; ea.layout = dlayout.as<int>();
       ea.tr_inv0 = dtr_inv0.as<float>();
       ea.lds_base = 0; ea.lds_n = 0;
@@ -954,7 +1138,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         EdgeBinArgs ba{};
         ba.cams = cams.as<DevCam>(); ba.view = v; ba.N = N; ba.E = E;
         ba.edges = edges.as<int2>(); ba.cdf = cdf.as<float>(); ba.proj = proj.as<float4>();
-        ba.rnd = rnd.as<float2>(); ba.adj = d_adj;
+        ba.rnd = rnd_ptr; ba.adj = d_adj;
         ba.bins_x = (c.w + EDGE_BIN - 1) / EDGE_BIN;
         ba.nbins = ba.bins_x * ((c.h + EDGE_BIN - 1) / EDGE_BIN);
         ba.bin_count = bin_count.as<int>(); ba.rec = bin_rec.as<int4>(); ba.order = bin_order.as<int2>();
@@ -1041,7 +1225,8 @@ void drh_destroy(drh_handle h) {
   if (!h) return;
   (void)hipSetDevice(h->st.device);
   (void)hipStreamSynchronize(h->stream);
-  h->drop_graph();
+  h->destroy_graphs();
+  h->stage_free();
   for (auto& p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto e : h->event_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(h->side);
@@ -1330,8 +1515,7 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
     const drh_status body = opt_step_body(h, cams, views, d_targets, edge_samples, d_loss);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(h->stream, &g);
-    if (body == DRH_OK && ec == hipSuccess && g &&
-        hipGraphInstantiate(&h->gexec[form], g, nullptr, nullptr, 0) == hipSuccess) {
+    if (body == DRH_OK && ec == hipSuccess && g && h->instantiate(form, g)) {
       h->graph[form] = g;
       h->graph_sig[form] = sig;
       h->graph_epoch[form] = h->realloc_epoch.load();

@@ -291,6 +291,15 @@ struct HitsArgs {
 void launch_fill_f32(float* p, long long n, float v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, long long n, uint32_t v, hipStream_t s);
 void launch_copy_f32(float* d, const float* src, long long n, hipStream_t s);
+// up to SCATTER_MAX arrays copied out of one staged block in one launch
+constexpr int SCATTER_MAX = 24;
+struct ScatterArgs {
+  int n;
+  const uint32_t* src;
+  uint32_t* dst[SCATTER_MAX];
+  long long off[SCATTER_MAX], words[SCATTER_MAX];
+};
+void launch_scatter(const ScatterArgs& a, hipStream_t s);
 void launch_prepare(const PrepareArgs& a, hipStream_t s);
 void launch_tri_setup(const TriSetupArgs& a, hipStream_t s);
 void launch_morton(const MortonArgs& a, hipStream_t s);

@@ -46,6 +46,16 @@ __global__ void k_fill_f32(float* p, long long n, float v) {
 __global__ void k_copy_f32(float4* __restrict__ d, const float4* __restrict__ s, long long n4) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x) d[i] = s[i];
 }
+// One upload of many arrays: segment blockIdx.y of the staged block `src` (4-byte words at
+// word offset off[y], words[y] of them) to its destination buffer.
+__global__ void k_scatter(ScatterArgs a) {
+  const int g = blockIdx.y;
+  const uint32_t* __restrict__ src = a.src + a.off[g];
+  uint32_t* __restrict__ dst = a.dst[g];
+  const long long n = a.words[g];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
 __global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1602,6 +1612,13 @@ void launch_fill_f32(float* p, long long n, float v, hipStream_t s) {
   if (n <= 0) return;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+void launch_scatter(const ScatterArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  long long most = 0;
+  for (int g = 0; g < a.n; g++) most = std::max(most, a.words[g]);
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((most + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_scatter, dim3(blocks, a.n), dim3(256), 0, s, a);
 }
 void launch_copy_f32(float* d, const float* src, long long n, hipStream_t s) {
   // n is a multiple of 4 floats (16 B) for the calibration buffers

@@ -4,6 +4,9 @@
 // drh_server_submit (inputs copied).  Threads serve them: admitters set queued jobs up while
 // a slot and the HBM budget allow (handle, commit with its LBVH build, uploads); the reaper
 // tears finished ones down (downloads, frees), which admits the next.
+// A finished job's handle is kept idle, with its device buffers, streams and graph
+// executables, for the next job with the same settings (its set-up then re-commits and
+// re-captures instead of creating and destroying HIP objects).
 // The stepper steps the admitted jobs round-robin: every job keeps up to four fused
 // fwd+bwd+Adam steps in flight on its own HIP stream (hipGraph replays after its first
 // step), so the device overlaps the jobs' kernels while set-up and teardown of others go on
@@ -18,6 +21,7 @@
 #include <cmath>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -59,6 +63,7 @@ struct Job {
   // run state
   drh_handle h = nullptr;
   void* d_targets = nullptr;
+  size_t targets_cap = 0;   // bytes allocated at d_targets (an idle handle's may be larger)
   void* d_loss = nullptr;
   int issued = 0, completed = 0;
   std::deque<std::pair<hipEvent_t, hipEvent_t>> inflight;
@@ -90,6 +95,30 @@ int admitter_threads() {
   const char* e = std::getenv("DRH_SERVER_ADMITTERS");
   const int n = e ? std::atoi(e) : ADMITTERS;
   return n >= 1 && n <= 16 ? n : ADMITTERS;
+}
+
+bool same_settings(const drh_settings& a, const drh_settings& b) {
+  return a.mode == b.mode && a.spp == b.spp && a.rng_mode == b.rng_mode && a.tracer == b.tracer &&
+         a.device == b.device && a.light_custom == b.light_custom &&
+         std::memcmp(a.light_dir, b.light_dir, sizeof a.light_dir) == 0 &&
+         std::memcmp(a.light_color, b.light_color, sizeof a.light_color) == 0 &&
+         std::memcmp(a.ambient, b.ambient, sizeof a.ambient) == 0;
+}
+
+// A finished job's handle waiting for the next job with the same settings.
+struct Idle {
+  drh_handle h;
+  drh_settings st;
+  void* d_targets;
+  size_t targets_cap;
+  void* d_loss;
+  size_t bytes;   // charged to the budget while idle: the footprint of its last job
+};
+
+void destroy_idle(const Idle& e) {
+  (void)drh_free(e.h, e.d_targets);
+  (void)drh_free(e.h, e.d_loss);
+  drh_destroy(e.h);
 }
 
 // HBM a job holds while active (the buffers drh_renderer sizes for it), with 25% headroom.
@@ -129,6 +158,10 @@ struct drh_server_s {
   int admitting = 0;                 // jobs the admitter is setting up
   int stepping = 0;                  // jobs the stepper holds (mirror for the slot count)
   std::vector<Job*> active;          // stepper only
+  // idle handles (at most max_active), oldest first; their bytes stay in `used` (idle_bytes
+  // of it) until a job takes the handle over or admission evicts it to make room
+  std::vector<Idle> idle;
+  size_t idle_bytes = 0;
   std::vector<hipEvent_t> spare_events;   // stepper only: recycled timing events
   hipEvent_t take_event() {
     if (spare_events.empty()) { hipEvent_t e = nullptr; (void)hipEventCreate(&e); return e; }
@@ -148,6 +181,10 @@ struct drh_server_s {
   double active_sum = 0.0;
   long long active_samples = 0;
   double admit_s = 0.0, step_s = 0.0, finish_s = 0.0;   // thread time by activity
+  // admission by phase (handle and options, targets, cameras + sample table, commit,
+  // optimiser set-up); printed at destroy when DRH_SERVER_TRACE is set
+  double phase_s[5] = {0, 0, 0, 0, 0};
+  std::chrono::steady_clock::time_point phase_t;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   void run();         // stepping thread
@@ -159,11 +196,13 @@ struct drh_server_s {
   bool can_admit() const {
     if (stop || pending.empty()) return false;
     if (pending.front()->bytes > budget) return true;   // refused at once
-    return stepping + (int)ready.size() + admitting < max_active && used + pending.front()->bytes <= budget;
+    return stepping + (int)ready.size() + admitting < max_active &&
+           used - idle_bytes + pending.front()->bytes <= budget;   // idle handles can be evicted
   }
 };
 
 void drh_server_s::finish(Job* j, int status, const std::string& err) {
+  bool kept = false;
   if (j->h) {
     if (status == 1) {
       const int V = j->V();
@@ -174,13 +213,31 @@ void drh_server_s::finish(Job* j, int status, const std::string& err) {
     }
     for (auto& ev : j->inflight) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     j->inflight.clear();
-    (void)drh_free(j->h, j->d_targets);
-    (void)drh_free(j->h, j->d_loss);
-    drh_destroy(j->h);
+    // a healthy handle (its last step completed: the downloads above waited for it) goes idle
+    // for the next job with the same settings; the oldest idle one makes room if need be
+    const Idle e{j->h, j->st, j->d_targets, j->targets_cap, j->d_loss, j->bytes + j->hit_bytes};
+    std::vector<Idle> drop;
+    if (status == 1 && drh_sync(j->h) == DRH_OK) {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!stop) {
+        if ((int)idle.size() >= max_active) {
+          drop.push_back(idle.front());
+          idle_bytes -= idle.front().bytes;
+          used -= std::min(used, idle.front().bytes);
+          idle.erase(idle.begin());
+        }
+        idle.push_back(e);
+        idle_bytes += e.bytes;
+        kept = true;
+      }
+    }
+    if (!kept) drop.push_back(e);
+    for (const Idle& d : drop) destroy_idle(d);
     j->h = nullptr;
+    j->d_targets = j->d_loss = nullptr;
   }
   std::lock_guard<std::mutex> lk(mu);
-  used -= std::min(used, j->bytes + j->hit_bytes);
+  if (!kept) used -= std::min(used, j->bytes + j->hit_bytes);   // a kept handle's charge stays
   j->status = status;
   j->error = err;
   j->t_done = std::chrono::steady_clock::now();
@@ -190,34 +247,56 @@ void drh_server_s::finish(Job* j, int status, const std::string& err) {
 }
 
 bool drh_server_s::admit_one(Job* j) {
-  drh_handle h = nullptr;
-  drh_status s = drh_create(&j->st, &h);
-  if (s != DRH_OK) { finish(j, s, "drh_create failed"); return false; }
-  j->h = h;
+  drh_status s = DRH_OK;
+  phase_t = std::chrono::steady_clock::now();
+  drh_handle h = j->h;   // an idle handle taken over at admission, or none
+  if (!h) {
+    if ((s = drh_create(&j->st, &h)) != DRH_OK) { finish(j, s, "drh_create failed"); return false; }
+    j->h = h;
+  }
+  // every option the server sets, set for this job (an idle handle keeps its last job's)
   (void)drh_set_device_sharing(h, max_active);
+  (void)drh_opt_use_graph(h, 1);
   // a short job refits its commit-time tree throughout: one captured step form instead of
   // two (capture and instantiation cost host time comparable to a short job's steps)
   (void)drh_set_bvh_rebuild(h, j->iterations <= 64 ? j->iterations + 1 : 8);
-  if (j->flags & DRH_JOB_DETERMINISTIC) (void)drh_set_deterministic(h, 1);
-  if (j->flags & DRH_JOB_FULL_JACOBIAN) (void)drh_set_edge_grad_mode(h, DRH_EDGE_GRAD_FULL_JACOBIAN);
+  (void)drh_set_deterministic(h, (j->flags & DRH_JOB_DETERMINISTIC) ? 1 : 0);
+  (void)drh_set_edge_grad_mode(h, (j->flags & DRH_JOB_FULL_JACOBIAN) ? DRH_EDGE_GRAD_FULL_JACOBIAN
+                                                                       : DRH_EDGE_GRAD_REFERENCE_DIAGONAL);
   if (j->flags & DRH_JOB_ALIGNED_INTERIOR) {
     // the hit buffer was charged to the job at admission if it fitted the budget; else recompute
     (void)drh_set_interior_samples(h, 1, DRH_HITS_AUTO);
     (void)drh_set_hit_buffer_limit(h, j->hit_bytes);
+  } else {
+    (void)drh_set_interior_samples(h, 0, DRH_HITS_RECOMPUTE);
+    (void)drh_set_hit_buffer_limit(h, ~(size_t)0);
   }
   const drh_scene_desc d = j->desc();
   const size_t tbytes = 4 * j->targets.size();
+  if (j->d_targets && j->targets_cap < tbytes) { (void)drh_free(h, j->d_targets); j->d_targets = nullptr; }
+  if (!j->d_targets) {
+    if ((s = drh_malloc(h, tbytes, &j->d_targets)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
+    j->targets_cap = tbytes;
+  }
+  if (!j->d_loss && (s = drh_malloc(h, 16, &j->d_loss)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
   // every upload first, while the handle's stream is still empty (their synchronisations then
   // wait for nothing): targets, cameras, edge-sample table; the commit's LBVH build and the
   // optimiser set-up are left queued behind them, so neither admission nor the job's first
   // step waits for kernels that queue behind the other jobs' steps
-  if ((s = drh_malloc(h, tbytes, &j->d_targets)) != DRH_OK || (s = drh_malloc(h, 16, &j->d_loss)) != DRH_OK ||
-      (s = drh_memcpy_htod(h, j->d_targets, j->targets.data(), tbytes)) != DRH_OK ||
-      (s = drh_prepare_steps(h, j->cams.data(), (int)j->cams.size(), j->edge_samples)) != DRH_OK ||
-      (s = drh_commit(h, &d)) != DRH_OK || (s = drh_opt_init(h, DRH_OPT_ADAM, j->lr_pos, j->lr_col)) != DRH_OK) {
-    finish(j, s, drh_last_error(h));
-    return false;
-  }
+  auto lap = [&](int k) {
+    const auto now = std::chrono::steady_clock::now();
+    phase_s[k] += std::chrono::duration<double>(now - phase_t).count();
+    phase_t = now;
+  };
+  lap(0);
+  if ((s = drh_memcpy_htod(h, j->d_targets, j->targets.data(), tbytes)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
+  lap(1);
+  if ((s = drh_prepare_steps(h, j->cams.data(), (int)j->cams.size(), j->edge_samples)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
+  lap(2);
+  if ((s = drh_commit(h, &d)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
+  lap(3);
+  if ((s = drh_opt_init(h, DRH_OPT_ADAM, j->lr_pos, j->lr_col)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
+  lap(4);
   j->t_admit = std::chrono::steady_clock::now();
   return true;
 }
@@ -229,6 +308,7 @@ void drh_server_s::admit_run() {
   (void)hipSetDevice(device);
   while (true) {
     Job* adm = nullptr;
+    std::vector<Idle> evict;
     {
       std::unique_lock<std::mutex> lk(mu);
       cv_help.wait(lk, [&] { return can_admit() || stop; });
@@ -242,7 +322,29 @@ void drh_server_s::admit_run() {
         cv_done.notify_all();
         continue;
       }
+      // take over an idle handle with the same settings if its charge fits, then evict idle
+      // handles (oldest first) until the job fits the budget
+      for (size_t k = 0; k < idle.size(); k++) {
+        if (!same_settings(idle[k].st, j->st)) continue;
+        if (used - idle_bytes + std::max(j->bytes, idle[k].bytes) > budget) continue;
+        const Idle e = idle[k];
+        idle.erase(idle.begin() + k);
+        idle_bytes -= e.bytes;
+        used -= std::min(used, e.bytes);
+        j->h = e.h;
+        j->d_targets = e.d_targets;
+        j->targets_cap = e.targets_cap;
+        j->d_loss = e.d_loss;
+        j->bytes = std::max(j->bytes, e.bytes);   // its buffers are at least the last job's
+        break;
+      }
       used += j->bytes;
+      while (used > budget && !idle.empty()) {
+        evict.push_back(idle.front());
+        idle_bytes -= idle.front().bytes;
+        used -= std::min(used, idle.front().bytes);
+        idle.erase(idle.begin());
+      }
       // store the render's hits when they fit what is left of the budget (multi-view
       // adjoint: storing saves the interior pass's rays, recomputing saves the HBM)
       j->hit_bytes = 0;
@@ -254,6 +356,7 @@ void drh_server_s::admit_run() {
       adm = j;
     }
     const auto t = std::chrono::steady_clock::now();
+    for (const Idle& e : evict) destroy_idle(e);
     const bool ok = admit_one(adm);
     std::lock_guard<std::mutex> lk(mu);
     admit_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
@@ -508,6 +611,12 @@ void drh_server_destroy(drh_server s) {
   for (auto& t : s->admitters) t.join();   // they set up no new job once stopped
   s->worker.join();     // hands its jobs (and those admitted meanwhile) to the reaper
   s->reaper.join();     // finishes them, then exits
+  if (std::getenv("DRH_SERVER_TRACE"))
+    std::fprintf(stderr, "drh_server admission s: handle %.4f targets %.4f cameras+table %.4f commit %.4f opt_init %.4f\n",
+                 s->phase_s[0], s->phase_s[1], s->phase_s[2], s->phase_s[3], s->phase_s[4]);
+  (void)hipSetDevice(s->device);
+  for (const Idle& e : s->idle) destroy_idle(e);   // no thread is left to take them
+  s->idle.clear();
   delete s;
 }
 

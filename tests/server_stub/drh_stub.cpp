@@ -102,6 +102,12 @@ drh_status drh_opt_step(drh_handle h, const drh_camera*, int, const float* d_tar
   return DRH_OK;
 }
 void* drh_stream(drh_handle h) { return &live(h, "drh_stream")->stream; }
+drh_status drh_sync(drh_handle h) {
+  live(h, "drh_sync");
+  while (stub_now_ns() < h->stream.busy_until.load()) host_work(20);   // hipStreamSynchronize
+  return DRH_OK;
+}
+drh_status drh_opt_use_graph(drh_handle h, int) { live(h, "drh_opt_use_graph"); return DRH_OK; }
 drh_status drh_get_params(drh_handle h, float* pos, float* col) {
   live(h, "drh_get_params");
   while (stub_now_ns() < h->stream.busy_until.load()) host_work(20);

@@ -100,6 +100,35 @@ int main() {
     drh_server_destroy(s);
     CHECK(g_live_handles.load() == 0);   // every admitted job was torn down
   }
+  // 4. idle handles: jobs of one setting take the last job's handle over; alternating
+  // settings under a budget of one and a half jobs evict the idle handle to fit
+  {
+    drh_job_desc d = tj.desc(4, 64);
+    const size_t need = drh_server_job_bytes(&d);
+    drh_server s;
+    CHECK(drh_server_create(0, need + need / 2, 4, &s) == DRH_OK);
+    std::vector<int> ids;
+    for (int k = 0; k < 16; k++) {
+      drh_job_desc dk = tj.desc(2 + k % 3, 64);
+      if (k >= 8) dk.settings.spp = (k % 2) ? 1 : 4;
+      int id;
+      CHECK(drh_server_submit(s, &dk, &id) == DRH_OK);
+      ids.push_back(id);
+    }
+    for (int k = 0; k < 16; k++) {
+      float pos[9];
+      drh_job_result r{};
+      r.pos = pos;
+      CHECK(drh_server_wait(s, ids[k], 60.0, &r) == DRH_OK);
+      CHECK(r.iterations == 2 + k % 3 && pos[0] == (float)r.iterations);
+    }
+    drh_server_stats_t stt;
+    CHECK(drh_server_stats(s, &stt) == DRH_OK);
+    CHECK(stt.hbm_used <= need + need / 2);
+    CHECK(g_live_handles.load() == 1);   // the last job's handle, idle
+    drh_server_destroy(s);
+    CHECK(g_live_handles.load() == 0);
+  }
   printf("server stress ok\n");
   return 0;
 }

@@ -184,13 +184,12 @@ struct drh_server_s {
   // admission by phase (handle and options, targets, cameras + sample table, commit,
   // optimiser set-up); printed at destroy when DRH_SERVER_TRACE is set
   double phase_s[5] = {0, 0, 0, 0, 0};
-  std::chrono::steady_clock::time_point phase_t;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   void run();         // stepping thread
   void admit_run();   // admission thread
   void reap_run();    // teardown thread
-  bool admit_one(Job* j);
+  bool admit_one(Job* j, double phase[5]);
   void finish(Job* j, int status, const std::string& err);
   // with mu held: may the admitter set up the next queued job now?
   bool can_admit() const {
@@ -246,9 +245,9 @@ void drh_server_s::finish(Job* j, int status, const std::string& err) {
   cv_help.notify_all();   // memory freed: the next queued job may fit
 }
 
-bool drh_server_s::admit_one(Job* j) {
+bool drh_server_s::admit_one(Job* j, double phase[5]) {
   drh_status s = DRH_OK;
-  phase_t = std::chrono::steady_clock::now();
+  auto phase_t = std::chrono::steady_clock::now();
   drh_handle h = j->h;   // an idle handle taken over at admission, or none
   if (!h) {
     if ((s = drh_create(&j->st, &h)) != DRH_OK) { finish(j, s, "drh_create failed"); return false; }
@@ -285,7 +284,7 @@ bool drh_server_s::admit_one(Job* j) {
   // step waits for kernels that queue behind the other jobs' steps
   auto lap = [&](int k) {
     const auto now = std::chrono::steady_clock::now();
-    phase_s[k] += std::chrono::duration<double>(now - phase_t).count();
+    phase[k] += std::chrono::duration<double>(now - phase_t).count();
     phase_t = now;
   };
   lap(0);
@@ -357,8 +356,10 @@ void drh_server_s::admit_run() {
     }
     const auto t = std::chrono::steady_clock::now();
     for (const Idle& e : evict) destroy_idle(e);
-    const bool ok = admit_one(adm);
+    double phase[5] = {0, 0, 0, 0, 0};
+    const bool ok = admit_one(adm, phase);
     std::lock_guard<std::mutex> lk(mu);
+    for (int k = 0; k < 5; k++) phase_s[k] += phase[k];
     admit_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
     admitting--;
     if (ok) ready.push_back(adm);

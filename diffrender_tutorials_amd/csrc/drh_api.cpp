@@ -284,6 +284,7 @@ struct drh_renderer : BufOwner {
   }
   int edge_order = DRH_EDGE_ORDER_AUTO;   // sample order of the edge pass
   int edge_grad_mode = DRH_EDGE_GRAD_REFERENCE_DIAGONAL;
+  int side_test = DRH_SIDE_TEST_REFERENCE;   // ray/triangle test of the edge samples' side rays
   DBuf bin_count, bin_rec, bin_order;     // pixel-bin order of the kept edge samples
   DBuf ecol;                              // edge samples' side colour differences (float4 each)
   DBuf cov_part;                          // coverage interior pass: per-tile partial sums
@@ -1117,6 +1118,16 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       else
         timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
     }
+    if (interior_pos && (st.mode == DRH_LAMBERT || st.mode == DRH_LAMBERT_VERTEX_COLOR) && G > 0) {
+      // the Lambert interior position term: a pass of its own over the same samples
+      InteriorArgs pa{};
+      pa.tr = tracer_view(v); pa.tr.rays = ray_slot(1); pa.cams = cams.as<DevCam>(); pa.view = v;
+      pa.nsamp = nsamp; pa.sq = sq; pa.hamm = hamm.as<float>(); pa.adj = d_adj; pa.grad = d_grad;
+      pa.layout = dlayout.as<int>(); pa.aligned = interior_aligned; pa.jit = jitter(); pa.light = light;
+      pa.hits = (interior_store && hits_stored) ? hitbuf.as<float4>() : nullptr;
+      pa.tr_inv0 = dtr_inv0.as<float>(); pa.dacc = dacc_p;
+      timed("k_interior_pos", [&] { launch_interior_pos(st.mode, pa, tiles, stream); });
+    }
     if (E > 0 && N > 0) {
       if (v == 0 && prep0_forked) {
         if ((s = hip(hipStreamWaitEvent(stream, ev_join, 0), "join"))) return s;   // prepared beside the render
@@ -1148,6 +1159,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       }
       ea.single_mesh = single >= 0 ? 1 : 0;
       ea.full_jacobian = edge_grad_mode == DRH_EDGE_GRAD_FULL_JACOBIAN ? 1 : 0;
+      ea.watertight = side_test == DRH_SIDE_TEST_WATERTIGHT ? 1 : 0;
       ea.single_tr_off = single >= 0 ? layout[4 * single] + layout[4 * single + 3] : 0;
       ea.single_id = single >= 0 ? single : 0;
       ea.trscr = tr_apart ? trscr.as<float>() : nullptr;
@@ -1542,6 +1554,13 @@ drh_status drh_set_pixel_jitter(drh_handle h, int on, unsigned seed) {
   return DRH_OK;
 }
 
+drh_status drh_set_edge_side_test(drh_handle h, int test) {
+  if (!h || (test != DRH_SIDE_TEST_REFERENCE && test != DRH_SIDE_TEST_WATERTIGHT)) return DRH_E_INVALID;
+  if (test != h->side_test) h->drop_graph();   // the captured step bakes the choice in
+  h->side_test = test;
+  return DRH_OK;
+}
+
 drh_status drh_opt_use_graph(drh_handle h, int on) {
   if (!h) return DRH_E_INVALID;
   h->use_graph = on != 0;
@@ -1583,8 +1602,8 @@ drh_status drh_prepare_steps(drh_handle h, const drh_camera* cams, int views, in
 
 drh_status drh_set_interior_pos_grad(drh_handle h, int on) {
   if (!h || (on != 0 && on != 1)) return DRH_E_INVALID;
-  if (on && h->st.mode != DRH_VERTEX_COLOR)
-    return h->fail(DRH_E_UNSUPPORTED, "set_interior_pos_grad: VERTEX_COLOR only");
+  if (on && h->st.mode != DRH_VERTEX_COLOR && h->st.mode != DRH_LAMBERT && h->st.mode != DRH_LAMBERT_VERTEX_COLOR)
+    return h->fail(DRH_E_UNSUPPORTED, "set_interior_pos_grad: VERTEX_COLOR, LAMBERT and LAMBERT_VERTEX_COLOR only");
   if ((on != 0) != h->interior_pos) h->drop_graph();
   h->interior_pos = on != 0;
   return DRH_OK;

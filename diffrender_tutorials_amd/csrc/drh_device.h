@@ -159,6 +159,58 @@ __device__ __forceinline__ bool mt_test(float4 A4, float4 E14, float4 E24, F3 o,
   return v > 0.0f && u > 0.0f && (u + v < 1.0f) && t > tNear;
 }
 
+// Watertight ray/triangle test (S. Woop, C. Benthin, I. Wald, "Watertight Ray/Triangle
+// Intersection", JCGT 2(1), 2013), the opt-in test of the edge pass's side rays: the ray's
+// dominant axis becomes z, the triangle is sheared into the ray's frame and the signs of the
+// 2D edge functions U, V, W decide (recomputed in double when one is exactly 0).  Unlike
+// Moller-Trumbore's divide by a tiny determinant, a nearly edge-on triangle cannot accept a
+// ray that misses it by more than the edge functions' rounding, and no ray slips between two
+// triangles that share an edge.  Barycentrics as the reference's: u = weight of C, v of B.
+struct WtRay { int kx, ky, kz; float Sx, Sy, Sz; };
+__device__ __forceinline__ float comp(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ WtRay wt_ray(F3 d) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  WtRay r;
+  r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  r.kx = r.kz == 2 ? 0 : r.kz + 1;
+  r.ky = r.kx == 2 ? 0 : r.kx + 1;
+  const float dz = comp(d, r.kz);
+  if (dz < 0.0f) { const int t = r.kx; r.kx = r.ky; r.ky = t; }   // keep the winding
+  r.Sx = comp(d, r.kx) / dz;
+  r.Sy = comp(d, r.ky) / dz;
+  r.Sz = 1.0f / dz;
+  return r;
+}
+__device__ __forceinline__ bool wt_test(const WtRay& r, F3 o, F3 A, F3 B, F3 C, float tNear, float tFar,
+                                        float& t, float& u, float& v) {
+  const F3 a = sub(A, o), b = sub(B, o), c = sub(C, o);
+  const float az = comp(a, r.kz), bz = comp(b, r.kz), cz = comp(c, r.kz);
+  const float Ax = comp(a, r.kx) - r.Sx * az, Ay = comp(a, r.ky) - r.Sy * az;
+  const float Bx = comp(b, r.kx) - r.Sx * bz, By = comp(b, r.ky) - r.Sy * bz;
+  const float Cx = comp(c, r.kx) - r.Sx * cz, Cy = comp(c, r.ky) - r.Sy * cz;
+  float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float T = U * (r.Sz * az) + V * (r.Sz * bz) + W * (r.Sz * cz);
+  const float tn = tNear * det, tf = tFar * det;
+  if (det > 0.0f ? (T <= tn || T >= tf) : (T >= tn || T <= tf)) return false;   // t in (tNear, tFar)
+  const float rd = 1.0f / det;
+  t = T * rd;
+  v = V * rd;
+  u = W * rd;
+  return true;
+}
+__device__ __forceinline__ F3 vertex_of(const DevScene& s, uint32_t g, int k) {
+  const uint32_t i = s.idx[3 * g + k];
+  return F3{s.pos[3 * i], s.pos[3 * i + 1], s.pos[3 * i + 2]};
+}
+
 // Read-only scene data through the constant address space: a wave-uniform index then
 // compiles to a scalar load into SGPRs, shared by the 64 rays of the wavefront.
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -209,19 +261,30 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
 // other lanes): the closest hit with the (t, triangle id) tie rule is the same over any
 // superset of the triangles a lane's own walk would test, so the result is the brute-force
 // tracer's answer for every lane.
-template <bool ANY>
+// WT: the watertight test on the triangles' shared vertex positions instead of Moller-Trumbore.
+template <bool ANY, bool WT = false>
 __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar, int* wst) {
   if (s.F == 0) return Hit{NO_HIT, tFar, 0.0f, 0.0f};
   const bool retired = o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z;
   Hit h{NO_HIT, tFar, 0.0f, 0.0f};
   const F3 id = {slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z)};
   const F3 nod = {-(o.x * id.x), -(o.y * id.y), -(o.z * id.z)};
+  const WtRay wr = WT ? wt_ray(d) : WtRay{};
   float lim = retired ? -1.0f : tFar * 1.00002f + 1e-6f;
   bool live = !retired;   // any-hit: still looking
   auto leaf = [&](int slot, bool mine) {
     const float4 A = uload(s.tri_a, slot), E1 = uload(s.tri_e1, slot), E2 = uload(s.tri_e2, slot);
     float t, u, v;
-    if (mine && mt_test(A, E1, E2, o, d, tNear, t, u, v)) {
+    bool hit = false;
+    if (mine) {
+      if (WT) {
+        const uint32_t gg = __float_as_uint(A.w);
+        hit = wt_test(wr, o, vertex_of(s, gg, 0), vertex_of(s, gg, 1), vertex_of(s, gg, 2), tNear, tFar, t, u, v);
+      } else {
+        hit = mt_test(A, E1, E2, o, d, tNear, t, u, v);
+      }
+    }
+    if (hit) {
       const uint32_t g = __float_as_uint(A.w);
       if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; lim = -1.0f; live = false; } }
       else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) {
@@ -343,9 +406,30 @@ __device__ __forceinline__ void test_suspects(const Tracer& T, const Suspects& S
   }
 }
 
-template <bool ANY, bool COUNT = false>
+// Brute force with the watertight test (the WT form of trace_bf, original triangle order).
+template <bool ANY>
+__device__ __forceinline__ Hit trace_bf_wt(const DevScene& s, F3 o, F3 d, float tNear, float tFar) {
+  Hit h{NO_HIT, tFar, 0.0f, 0.0f};
+  if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
+  const WtRay wr = wt_ray(d);
+  for (int g = 0; g < s.F; g++) {
+    float t, u, v;
+    if (wt_test(wr, o, vertex_of(s, g, 0), vertex_of(s, g, 1), vertex_of(s, g, 2), tNear, tFar, t, u, v) && t < h.t) {
+      h.gtri = (uint32_t)g; h.t = t; h.u = u; h.v = v;
+      if (ANY) return h;
+    }
+  }
+  return h;
+}
+
+template <bool ANY, bool COUNT = false, bool WT = false>
 __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* wst) {
   count_rays<COUNT>(T.rays + (ANY ? 1 : 0), !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z));
+  if (WT) {
+    // no clamped determinants to emulate: no suspect lists
+    if (T.use_bf) return trace_bf_wt<ANY>(T.s, o, d, tNear, tFar);
+    return trace_bvh<ANY, true>(T.s, o, d, tNear, tFar, wst);
+  }
   if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
   Hit h = trace_bvh<ANY>(T.s, o, d, tNear, tFar, wst);
   if (ANY && h.gtri != NO_HIT) return h;
@@ -435,7 +519,7 @@ constexpr int MODE_LAMBERT_VCOL = 8;
 // pos + (t - 1e-5) dir toward -L hits anything.  The lit colour is (shadow * Lc) * ndl.  With
 // ndl == +0 and a finite light colour the shadow ray cannot change that product (bit for bit),
 // so it is only cast where ndl > 0 (shadow stays 1 otherwise).
-template <bool COUNT = false>
+template <bool COUNT = false, bool WT = false>
 __device__ __forceinline__ void lambert_terms(const Tracer& T, const Light& L, F3 o, F3 d, const Hit& h,
                                               uint32_t A, uint32_t B, uint32_t C, const float* nrm, int* wst,
                                               float& shadow, float& ndl) {
@@ -446,7 +530,7 @@ __device__ __forceinline__ void lambert_terms(const Tracer& T, const Light& L, F
   shadow = 1.0f;
   if (ndl != 0.0f || !(fabsf(L.Lc.x) <= 3.4e38f && fabsf(L.Lc.y) <= 3.4e38f && fabsf(L.Lc.z) <= 3.4e38f)) {
     const F3 surf = add(o, mul(h.t - BIAS, d));
-    const Hit sh = trace<true, COUNT>(T, surf, negL, 0.0f, 1e9f, wst);
+    const Hit sh = trace<true, COUNT, WT>(T, surf, negL, 0.0f, 1e9f, wst);
     shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
   }
 }
@@ -466,12 +550,12 @@ __device__ __forceinline__ void lit_decode(float w, float& shadow, float& ndl) {
 // shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT / LAMBERT_VCOL.  `nrm` is the 3D
 // scene's normals for render and the projected scene's for edge samples (src/drender.h:353-354).
 // lit_out (LAMBERT_VCOL, may be NULL) receives the hit's light terms, lit_encode'd.
-template <int MODE, bool COUNT = false>
+template <int MODE, bool COUNT = false, bool WT = false>
 __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
                                     const float* nrm, int* wst, Hit* hit_out = nullptr, float* lit_out = nullptr) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
-  const Hit h = trace<false, COUNT>(T, o, d, 0.0f, 1e9f, wst);
+  const Hit h = trace<false, COUNT, WT>(T, o, d, 0.0f, 1e9f, wst);
   if (hit_out) *hit_out = h;
   if (h.gtri == NO_HIT) return {0, 0, 0};
   if (MODE == 1) return {1, 1, 1};
@@ -488,7 +572,7 @@ __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCa
     if (MODE == 3) return diffuse;
   }
   float shadow, ndl;
-  lambert_terms<COUNT>(T, L, o, d, h, A, B, C, nrm, wst, shadow, ndl);
+  lambert_terms<COUNT, WT>(T, L, o, d, h, A, B, C, nrm, wst, shadow, ndl);
   if (MODE == MODE_LAMBERT_VCOL && lit_out) *lit_out = lit_encode(shadow, ndl);
   return mulv(lambert_factor(L, shadow, ndl), diffuse);
 }

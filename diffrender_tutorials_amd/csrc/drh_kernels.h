@@ -209,6 +209,7 @@ struct EdgeArgs {
   unsigned long long* dacc;  // deterministic mode: 32.32 fixed-point sums, G for grad then 12 per mesh for trscr
   int dG;                 // G: where trscr's entries start in dacc
   int full_jacobian;      // 1: edge_grad with the full screen Jacobian (cross terms kept)
+  int watertight;         // 1: side rays traced with the watertight ray/triangle test
 };
 
 // Edge samples binned by the EDGE_BIN x EDGE_BIN pixel tile of their sample point.  Only the
@@ -316,6 +317,8 @@ void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s)
 // The interior pass of a single 1x1 albedo with all texture coordinates 0 (a.uniform and
 // a.coverage): one wave per pixel tile, partial sums into part (16 B per tile), then a fixed-
 // order sum added to grad[uniform_off .. +3).
+// LAMBERT (mode 4) / LAMBERT_VCOL interior position term (drh_set_interior_pos_grad)
+void launch_interior_pos(int mode, const InteriorArgs& a, int tiles, hipStream_t s);
 void launch_coverage(const InteriorArgs& a, int tiles, float4* part, hipStream_t s);
 void launch_project(const ProjectArgs& a, hipStream_t s);
 void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s);

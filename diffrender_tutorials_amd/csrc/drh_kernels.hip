@@ -859,6 +859,106 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
 }
 
 // grad[0..2] += the sum of n partial float3 (part[i].xyz), in a fixed order, in double.
+// Interior position term of LAMBERT / LAMBERT_VCOL (the opt-in counterpart of
+// k_interior<VERTEX_COLOR, true>; the reference has no interior position gradient): a sample
+// whose ray hits triangle (A, B, C) at barycentrics (u, v) sees the shade
+//   F(u, v) = val . (albedo(u, v) * ((shadow * Lc) * max(0, n(u, v).(-L)) + ambient)),
+// n = nA (1-u-v) + nB v + nC u the interpolated (prepared) normal and albedo the interpolated
+// vertex colour (LAMBERT_VCOL) or the bilinear texel of the interpolated texture coordinate
+// (LAMBERT, sample_bilinear_clamp_3f, src/shade_common.cpp).  Moving a vertex moves (u, v):
+// dF/dV = dF/du du/dV + dF/dv dv/dV, with the closed-form du/dV, dv/dV of the ray's
+// barycentrics (BarU_grad / BarV_grad, src/functions.cpp:226-470) and mesh 0's inverse
+// transform as in edge_grad (src/drender.cpp:60-71).  The shadow is held fixed (it changes
+// only where a triangle's facing to the light flips).  One wave per pixel tile as the render.
+template <int MODE, bool COUNT = false>
+__global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
+  __shared__ int wst[64];
+  const Tracer& T = a.tr;
+  const DevScene& S = a.tr.s;
+  const DevCam& c = a.cams[a.view];
+  const int lane = threadIdx.x;
+  const int sq = a.sq;
+  const TileMap tm = tile_map(sq, c.w, blockIdx.x, lane);
+  if (!(tm.x < c.w && tm.y < c.h)) return;
+  const F3 val = divs(ld3(a.adj + c.img_off + 3ll * ((long long)tm.y * c.w + tm.x)), (float)a.nsamp);
+  const F3 negL = mul(-1.0f, a.light.L);
+  const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
+  for (int si = s_begin; si < s_end; si++) {
+    float xo = a.hamm[2 * si], yo = a.hamm[2 * si + 1];
+    if (a.aligned) sample_offset(a.jit, a.view, tm.x, tm.y, si, sq, xo, yo);
+    F3 ro, rd;
+    eye_ray(c, tm.x + xo, tm.y + yo, ro, rd);
+    Hit h;
+    if (a.hits) {
+      const float4 r = a.hits[(c.img_off / 3 + (long long)tm.y * c.w + tm.x) * a.nsamp + si];
+      h = Hit{__float_as_uint(r.x), r.w, r.y, r.z};
+    } else {
+      h = trace<false, COUNT>(T, ro, rd, 0.0f, 1e9f, wst);
+    }
+    if (h.gtri == NO_HIT) continue;
+    const int m = S.tri_mesh[h.gtri];
+    const int goff = a.layout[4 * m + 0];
+    if (goff < 0) continue;
+    const uint32_t* I = S.idx + 3 * h.gtri;
+    float shadow, ndl;
+    if (a.hits && MODE == MODE_LAMBERT_VCOL) lit_decode(h.t, shadow, ndl);   // stored hits keep the light terms
+    else lambert_terms<COUNT>(T, a.light, ro, rd, h, I[0], I[1], I[2], S.nrm, wst, shadow, ndl);
+    const F3 light = lambert_factor(a.light, shadow, ndl);
+    // d(light)/du, d(light)/dv: zero where max(0, n.(-L)) is clamped or the point is in shadow
+    const F3 nA = ld3(S.nrm + 3 * I[0]), nB = ld3(S.nrm + 3 * I[1]), nC = ld3(S.nrm + 3 * I[2]);
+    const bool on = shadow != 0.0f && ndl > 0.0f;
+    const F3 dLu = on ? mul(dot(sub(nC, nA), negL) * shadow, a.light.Lc) : F3{0, 0, 0};
+    const F3 dLv = on ? mul(dot(sub(nB, nA), negL) * shadow, a.light.Lc) : F3{0, 0, 0};
+    F3 alb, dAu, dAv;
+    if (MODE == MODE_LAMBERT_VCOL) {
+      const F3 cA = ld3(S.col + 3 * I[0]), cB = ld3(S.col + 3 * I[1]), cC = ld3(S.col + 3 * I[2]);
+      alb = blend3(cA, cB, cC, h.u, h.v);
+      dAu = sub(cC, cA);
+      dAv = sub(cB, cA);
+    } else {
+      const int w = S.tex_w[m], hh = S.tex_h[m];
+      const F2 t0 = blend2(S.tc, I[0], I[1], I[2], h.u, h.v);
+      alb = bilinear(S, m, t0);
+      // gradient of the clamped bilinear lookup in tc, then tc's in (u, v)
+      F3 gx = {0, 0, 0}, gy = {0, 0, 0};
+      if (w > 0 && hh > 0) {
+        const float tx = clampf(t0.x, 0.0f, 1.0f) * (float)w, ty = clampf(t0.y, 0.0f, 1.0f) * (float)hh;
+        const int ix = (int)tx, iy = (int)ty;
+        const int x0 = clampi(ix, 0, w - 1), y0 = clampi(iy, 0, hh - 1);
+        const int x1 = clampi(ix + 1, 0, w - 1), y1 = clampi(iy + 1, 0, hh - 1);
+        const float dx = tx - (float)x0, dy = ty - (float)y0;
+        const F3 p00 = tex_fetch(S, m, x0, y0), p01 = tex_fetch(S, m, x0, y1);
+        const F3 p10 = tex_fetch(S, m, x1, y0), p11 = tex_fetch(S, m, x1, y1);
+        const bool inx = t0.x > 0.0f && t0.x < 1.0f, iny = t0.y > 0.0f && t0.y < 1.0f;
+        if (inx) gx = mul((float)w, add(mul(1.0f - dy, sub(p10, p00)), mul(dy, sub(p11, p01))));
+        if (iny) gy = mul((float)hh, add(mul(1.0f - dx, sub(p01, p00)), mul(dx, sub(p11, p10))));
+      }
+      const float tAx = S.tc[2 * I[0]], tAy = S.tc[2 * I[0] + 1];
+      const float tBx = S.tc[2 * I[1]], tBy = S.tc[2 * I[1] + 1];
+      const float tCx = S.tc[2 * I[2]], tCy = S.tc[2 * I[2] + 1];
+      dAu = add(mul(tCx - tAx, gx), mul(tCy - tAy, gy));
+      dAv = add(mul(tBx - tAx, gx), mul(tBy - tAy, gy));
+    }
+    const float dFu = dot(val, add(mulv(dAu, light), mulv(alb, dLu)));
+    const float dFv = dot(val, add(mulv(dAv, light), mulv(alb, dLv)));
+    F3 du[3], dv[3];
+    if (!bary_grad(ro, rd, f4xyz(T.bf_a[h.gtri]), f4xyz(T.bf_e1[h.gtri]), f4xyz(T.bf_e2[h.gtri]), h.u, h.v, du, dv))
+      continue;
+    const float* Ti = a.tr_inv0;
+    const int vo = S.mesh_vo[m];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const F3 wgt = add(mul(dFu, du[k]), mul(dFv, dv[k]));
+      const float gxk = Ti[0] * wgt.x + Ti[4] * wgt.y + Ti[8] * wgt.z;
+      const float gyk = Ti[1] * wgt.x + Ti[5] * wgt.y + Ti[9] * wgt.z;
+      const float gzk = Ti[2] * wgt.x + Ti[6] * wgt.y + Ti[10] * wgt.z;
+      const int o = goff + 3 * ((int)I[k] - vo);
+      if (a.dacc) { dadd(a.dacc, o, gxk); dadd(a.dacc, o + 1, gyk); dadd(a.dacc, o + 2, gzk); }
+      else { atomicAdd(a.grad + o, gxk); atomicAdd(a.grad + o + 1, gyk); atomicAdd(a.grad + o + 2, gzk); }
+    }
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_coverage_sum(const float4* __restrict__ part, int n, float* grad) {
   __shared__ double s[3][1024];
   double x = 0.0, y = 0.0, z = 0.0;
@@ -1297,7 +1397,7 @@ __global__ void k_edge_scatter(EdgeBinArgs a) {
 // 32 samples with both sides in adjacent lanes, one ray (and shadow ray) per lane, so its
 // packet covers 32 nearby points and the kernel stays small enough for 8 waves per SIMD.
 // ecol[i] = (colour inside - colour outside, edge id bits; -1 for a rejected sample).
-template <int MODE, bool COUNT = false>
+template <int MODE, bool COUNT = false, bool WT = false>
 __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
   __shared__ int wstack[4][64];
   const Tracer& T = a.tr;
@@ -1332,7 +1432,7 @@ __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
       const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
       const float qx = sd == 0 ? p.px - 1e-3f * nx : p.px + 1e-3f * nx;
       const float qy = sd == 0 ? p.py - 1e-3f * ny : p.py + 1e-3f * ny;
-      col = shade<MODE, COUNT>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
+      col = shade<MODE, COUNT, WT>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
     }
     // both lanes of a sample are here together (same item, same `live`)
     const float ox = __shfl_xor(col.x, 1, 64), oy = __shfl_xor(col.y, 1, 64), oz = __shfl_xor(col.z, 1, 64);
@@ -1695,6 +1795,16 @@ void launch_coverage(const InteriorArgs& a, int tiles, float4* part, hipStream_t
   else hipLaunchKernelGGL((k_coverage<false>), dim3(tiles), dim3(64), 0, s, a, part);
   hipLaunchKernelGGL(k_coverage_sum, dim3(1), dim3(1024), 0, s, (const float4*)part, tiles, a.grad + a.uniform_off);
 }
+void launch_interior_pos(int mode, const InteriorArgs& a, int tiles, hipStream_t s) {
+  if (tiles <= 0) return;
+  if (mode == MODE_LAMBERT_VCOL) {
+    if (a.tr.rays) hipLaunchKernelGGL((k_interior_pos<MODE_LAMBERT_VCOL, true>), dim3(tiles), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_interior_pos<MODE_LAMBERT_VCOL, false>), dim3(tiles), dim3(64), 0, s, a);
+  } else {
+    if (a.tr.rays) hipLaunchKernelGGL((k_interior_pos<4, true>), dim3(tiles), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_interior_pos<4, false>), dim3(tiles), dim3(64), 0, s, a);
+  }
+}
 void launch_project(const ProjectArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_project, dim3((a.V + 255) / 256), dim3(256), 0, s, a);
 }
@@ -1760,8 +1870,14 @@ void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s) {
 }
 void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s) {
   const int blocks = (int)std::max<long long>(1, std::min<long long>((2 * a.N + 255) / 256, 8192));
-  if (a.tr.rays) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, true>), dim3(blocks), dim3(256), 0, s, a))
-  else DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M>), dim3(blocks), dim3(256), 0, s, a))
+  if (a.watertight) {
+    if (a.tr.rays) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, true, true>), dim3(blocks), dim3(256), 0, s, a))
+    else DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, false, true>), dim3(blocks), dim3(256), 0, s, a))
+  } else if (a.tr.rays) {
+    DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, true>), dim3(blocks), dim3(256), 0, s, a))
+  } else {
+    DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M>), dim3(blocks), dim3(256), 0, s, a))
+  }
 }
 void launch_edge_grad(const EdgeArgs& a, hipStream_t s) {
   const int blocks = (int)std::max<long long>(1, std::min<long long>((a.N + 255) / 256, 2048));

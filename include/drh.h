@@ -182,14 +182,17 @@ drh_status drh_get_transforms(drh_handle h, float* transforms);
  * reference's serial sum of its per-thread gradient copies (src/drender.h:229-232, 380-425),
  * which is ordered; no counterpart in the reference's interface. */
 drh_status drh_set_deterministic(drh_handle h, int on);
-/* VERTEX_COLOR: also the interior term's vertex position gradient (off by default, as in the
- * reference, whose shade_grad<VERTEX_COLOR> has it written out but disabled,
- * src/shade_simple.cpp:75-108): each interior sample adds dF/du du/dV + dF/dv dv/dV for the
- * three vertices of its hit, with dF/du = (c_C - c_A).val and dF/dv = (c_B - c_A).val and the
- * closed-form derivatives of the ray's barycentrics (BarU_grad / BarV_grad,
- * src/functions.cpp:226-470), through mesh 0's inverse transform like the edge term.  With it
- * the position gradient of a vertex-coloured mesh is the derivative of the image away from
- * silhouettes too.  Other modes: DRH_E_UNSUPPORTED. */
+/* The interior term's vertex position gradient (off by default, as in the reference, whose
+ * shade_grad<VERTEX_COLOR> has it written out but disabled, src/shade_simple.cpp:75-108, and
+ * whose shade_grad<LAMBERT> has none): each interior sample adds dF/du du/dV + dF/dv dv/dV
+ * for the three vertices of its hit, with the closed-form derivatives of the ray's
+ * barycentrics (BarU_grad / BarV_grad, src/functions.cpp:226-470), through mesh 0's inverse
+ * transform like the edge term.  VERTEX_COLOR: dF/du = (c_C - c_A).val, dF/dv = (c_B - c_A).val.
+ * LAMBERT / LAMBERT_VERTEX_COLOR: F = val.(albedo * ((shadow Lc) max(0, n.(-L)) + ambient))
+ * with the interpolated normal n and albedo (texel of the interpolated texture coordinate, or
+ * the interpolated vertex colour) differentiated in u and v, the shadow held fixed; a pass of
+ * its own (k_interior_pos).  With it the position gradient is the derivative of the image away
+ * from silhouettes too.  Other modes: DRH_E_UNSUPPORTED. */
 drh_status drh_set_interior_pos_grad(drh_handle h, int on);
 /* Download the current (optimised) restricted-transform parameters: 7 floats per mesh
  * (translate, rotate, scale); zeros for meshes that are not RESTRICTED_TRANSFORM meshes. */
@@ -251,6 +254,19 @@ drh_status drh_set_bvh_rebuild(drh_handle h, int every);
  * point (the reference has only the diagonal form). */
 enum { DRH_EDGE_GRAD_REFERENCE_DIAGONAL = 0, DRH_EDGE_GRAD_FULL_JACOBIAN = 1 };
 drh_status drh_set_edge_grad_mode(drh_handle h, int mode);
+/* Ray/triangle test of the edge samples' two side rays (and their shadow rays), which shade
+ * the points 1e-3 px either side of a sampled edge (src/drender.h:345-355).
+ *   DRH_SIDE_TEST_REFERENCE (default): the reference's Moller-Trumbore test
+ *     (raytrace3d_bf.cpp:123-150), the parity mode.  Its divide by the determinant lets a
+ *     nearly edge-on triangle accept a side ray that misses it by far more than the 1e-3 px
+ *     offset, which drops that sample's colour difference.
+ *   DRH_SIDE_TEST_WATERTIGHT: the watertight test of Woop, Benthin and Wald (JCGT 2013) on the
+ *     triangles' shared vertices: such rays are not accepted, and none slips between two
+ *     triangles that share an edge.
+ * Camera, interior and render rays keep the reference's test in both modes.  No counterpart
+ * in the reference's interface. */
+enum { DRH_SIDE_TEST_REFERENCE = 0, DRH_SIDE_TEST_WATERTIGHT = 1 };
+drh_status drh_set_edge_side_test(drh_handle h, int test);
 /* How many handles share the device (1 by default).  A handle alone runs view 0's edge
  * preparation on a second stream beside the render; when several share the device (the
  * server sets its slot count) that overlap comes from the other handles instead and the

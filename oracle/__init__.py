@@ -26,6 +26,10 @@ Parity status of each function (how it is pinned):
   * orc_silhouette_functional / orc_fd_silhouette (FP64 box-filtered silhouette and its
     central differences): pinned by the closed-form area of a cube's front face and by
     analytic derivatives of that area (tests/test_oracle.py).
+  * lambert_functional / fd_lambert (FP64 pixel-area functional of the Lambert image and its
+    central differences): pinned by the silhouette functional (ambient-only light: the
+    functional is ambient x albedo x covered area), by Gauss-Legendre order convergence and by
+    the oracle's own render at 256 samples per pixel (tests/test_oracle_lambert_fd.py).
   * mt_barycentrics / barycentric_grad (the MT barycentrics of a ray and their derivatives
     with respect to the triangle's vertices, the reference's BarU_grad / BarV_grad): pinned
     against FP64 central differences of mt_barycentrics (tests/test_oracle.py).
@@ -75,6 +79,7 @@ def lib():
         _lib.orc_loss.restype = ctypes.c_float
         _lib.orc_qmc_rnd.restype = ctypes.c_float
         _lib.orc_silhouette_functional.restype = ctypes.c_double
+        _lib.orc_lambert_functional.restype = ctypes.c_double
     return _lib
 
 
@@ -190,6 +195,23 @@ class Scene:
         out = np.empty(len(co), np.float64)
         lib().orc_fd_silhouette(self.h, ctypes.byref(c), _p(adj), len(co), _p(co, ctypes.c_int), ctypes.c_double(h),
                                 _p(out, ctypes.c_double))
+        return out
+
+    def lambert_functional(self, mode, cam, adjoint, lam=None, order=6):
+        """FP64 pixel-area functional sum_p adj_p . integral of the LAMBERT / LAMBERT_VERTEX_COLOR
+        shade over pixel p (closed convex mesh; front-facing triangles)."""
+        c = cam_struct(cam)
+        adj = np.ascontiguousarray(np.asarray(adjoint, np.float32).reshape(-1))
+        return lib().orc_lambert_functional(self.h, mode, lam_struct(lam), ctypes.byref(c), _p(adj), order)
+
+    def fd_lambert(self, mode, cam, adjoint, coords, h=1e-4, lam=None, order=6):
+        """FP64 central differences of lambert_functional for coordinates 3*vertex+axis."""
+        c = cam_struct(cam)
+        adj = np.ascontiguousarray(np.asarray(adjoint, np.float32).reshape(-1))
+        co = np.ascontiguousarray(np.asarray(coords, np.int32))
+        out = np.empty(len(co), np.float64)
+        lib().orc_fd_lambert(self.h, mode, lam_struct(lam), ctypes.byref(c), _p(adj), len(co), _p(co, ctypes.c_int),
+                             ctypes.c_double(h), order, _p(out, ctypes.c_double))
         return out
 
     def edge_sampler(self, cam):

@@ -1276,3 +1276,285 @@ void orc_fd_silhouette(void* sp, const orc_camera* cam, const float* adjoint, in
 }
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// FP64 pixel-area functional of the LAMBERT / LAMBERT_VERTEX_COLOR image, the finite-difference
+// pin of the position gradient with its interior term (src/fin_diff.cpp's check, made exact up
+// to quadrature): L(x) = sum_p adj_p . (integral over pixel p of the shade), pixels of area 1.
+// For a closed convex mesh seen from outside the front-facing triangles tile the silhouette
+// and nothing else is visible (background 0).  On triangle T the shade at screen point q is
+//   albedo(u, v) * ((lit_T Lc) max(0, n(u, v).(-L)) + ambient)
+// (shade<LAMBERT>, src/shade_lambert.cpp:6-35), (u, v) the barycentrics of the camera ray
+// through q (eye_ray in double), n the interpolated prepared normal, albedo the bilinear texel
+// of the interpolated texture coordinate (LAMBERT) or the interpolated vertex colour; lit_T:
+// the reference's shadow ray from a convex mesh's surface meets the mesh iff the face turns
+// away from the light.  Each triangle-pixel polygon (Sutherland-Hodgman) is fanned into
+// triangles, each integrated with an n x n Gauss-Legendre rule on the collapsed square
+// (Duffy's transform); the nodes come from Newton's method on P_n.
+// ---------------------------------------------------------------------------
+namespace {
+struct D3 { double x, y, z; };
+D3 d3(f3 a) { return D3{a.x, a.y, a.z}; }
+D3 dadd3(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+D3 dsub3(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+D3 dmul3(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+double ddot3(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+D3 dcross3(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// 4x4 inverse by Gauss-Jordan with partial pivoting, column-major in and out
+bool inv44d(const double* m, double* out) {
+  double a[4][8];
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) { a[r][c] = m[c * 4 + r]; a[r][c + 4] = r == c ? 1.0 : 0.0; }
+  for (int c = 0; c < 4; c++) {
+    int piv = c;
+    for (int r = c + 1; r < 4; r++) if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+    if (a[piv][c] == 0.0) return false;
+    if (piv != c) for (int k = 0; k < 8; k++) std::swap(a[c][k], a[piv][k]);
+    const double d = a[c][c];
+    for (int k = 0; k < 8; k++) a[c][k] /= d;
+    for (int r = 0; r < 4; r++) {
+      if (r == c) continue;
+      const double f = a[r][c];
+      for (int k = 0; k < 8; k++) a[r][k] -= f * a[c][k];
+    }
+  }
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) out[c * 4 + r] = a[r][c + 4];
+  return true;
+}
+
+// Sutherland-Hodgman of triangle t against pixel [x0, x0+1] x [y0, y0+1]; returns the count
+int clip_poly(const P2* tri, double x0, double y0, P2* out) {
+  P2 a[8], b[8];
+  int n = 3;
+  for (int i = 0; i < 3; i++) a[i] = tri[i];
+  const double lo[2] = {x0, y0}, hi[2] = {x0 + 1.0, y0 + 1.0};
+  for (int e = 0; e < 4 && n > 0; e++) {
+    const int ax = e & 1;
+    const bool upper = e >= 2;
+    const double c = upper ? hi[ax] : lo[ax];
+    auto inside = [&](const P2& p) { const double v = ax ? p.y : p.x; return upper ? v <= c : v >= c; };
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+      const P2 &P = a[i], &Q = a[(i + 1) % n];
+      const bool ip = inside(P), iq = inside(Q);
+      if (ip) b[m++] = P;
+      if (ip != iq) {
+        const double pv = ax ? P.y : P.x, qv = ax ? Q.y : Q.x;
+        const double t = (c - pv) / (qv - pv);
+        b[m++] = P2{P.x + t * (Q.x - P.x), P.y + t * (Q.y - P.y)};
+      }
+    }
+    n = m;
+    for (int i = 0; i < n; i++) a[i] = b[i];
+  }
+  for (int i = 0; i < n; i++) out[i] = a[i];
+  return n;
+}
+
+// Gauss-Legendre nodes and weights of order n on [0, 1] (Newton on P_n from Chebyshev guesses)
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < n; i++) {
+    double z = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; k++) { const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k; p0 = p1; p1 = p2; }
+      const double pn = n == 1 ? z : p1, pm = n == 1 ? 1.0 : p0;
+      dp = n * (z * pn - pm) / (z * z - 1.0);
+      const double dz = pn / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = 0.5 * (1.0 - z);
+    w[i] = 1.0 / ((1.0 - z * z) * dp * dp);   // 2/((1-z^2) P'^2) on [-1, 1], halved for [0, 1]
+  }
+}
+
+struct LambertFd {
+  const Scene* s;
+  int mode;
+  D3 negL, Lc, amb;
+  FdCam fc;
+  double proj_inv[16], wv_inv[16];
+  D3 eye;
+  const float* adj;   // H x W x 3
+  std::vector<double> gx, gw;   // Gauss-Legendre nodes on [0, 1] and weights
+
+  void ray(double x, double y, D3& o, D3& d) const {
+    const double xn = x / fc.W, yn = y / fc.H;
+    const double q[4] = {2.0 * xn - 1.0, -2.0 * yn + 1.0, 0.0, 1.0};
+    double p[4];
+    for (int r = 0; r < 4; r++) p[r] = proj_inv[r] * q[0] + proj_inv[4 + r] * q[1] + proj_inv[8 + r] * q[2] + proj_inv[12 + r] * q[3];
+    D3 dir{p[0] / p[3], p[1] / p[3], p[2] / p[3]};
+    dir = dmul3(dir, 1.0 / std::sqrt(ddot3(dir, dir)));
+    const D3 far = dmul3(dir, 100.0);
+    D3 b{wv_inv[0] * far.x + wv_inv[4] * far.y + wv_inv[8] * far.z + wv_inv[12],
+         wv_inv[1] * far.x + wv_inv[5] * far.y + wv_inv[9] * far.z + wv_inv[13],
+         wv_inv[2] * far.x + wv_inv[6] * far.y + wv_inv[10] * far.z + wv_inv[14]};
+    o = eye;
+    d = dsub3(b, o);
+    d = dmul3(d, 1.0 / std::sqrt(ddot3(d, d)));
+  }
+  // shade of global triangle g (positions V[3] in world space) at screen point (x, y)
+  D3 shade(int g, const D3* V, bool lit, double x, double y) const {
+    D3 o, d;
+    ray(x, y, o, d);
+    const D3 e1 = dsub3(V[1], V[0]), e2 = dsub3(V[2], V[0]);
+    const D3 pv = dcross3(d, e2);
+    const double det = ddot3(e1, pv);
+    if (det == 0.0) return D3{0, 0, 0};
+    const D3 tv = dsub3(o, V[0]);
+    const double bb = ddot3(tv, pv) / det;          // weight of B
+    const D3 qv = dcross3(tv, e1);
+    const double cc = ddot3(d, qv) / det;           // weight of C
+    const double aa = 1.0 - bb - cc;
+    const uint32_t A = s->idx[3 * g], B = s->idx[3 * g + 1], C = s->idx[3 * g + 2];
+    const D3 n = dadd3(dadd3(dmul3(d3(s->nrm[A]), aa), dmul3(d3(s->nrm[B]), bb)), dmul3(d3(s->nrm[C]), cc));
+    const double ndl = std::max(0.0, ddot3(n, negL));
+    D3 alb;
+    if (mode == ORC_LAMBERT_VERTEX_COLOR) {
+      alb = dadd3(dadd3(dmul3(d3(s->col[A]), aa), dmul3(d3(s->col[B]), bb)), dmul3(d3(s->col[C]), cc));
+    } else {
+      const int m = s->tri_mesh[g];
+      const int w = s->tex_w[m], h = s->tex_h[m];
+      if (w * h == 0) return D3{0, 0, 0};
+      double tx = s->tc[A].x * aa + s->tc[B].x * bb + s->tc[C].x * cc;
+      double ty = s->tc[A].y * aa + s->tc[B].y * bb + s->tc[C].y * cc;
+      tx = std::min(std::max(tx, 0.0), 1.0) * w;
+      ty = std::min(std::max(ty, 0.0), 1.0) * h;
+      const int ix = (int)tx, iy = (int)ty;
+      const int x0 = std::min(std::max(ix, 0), w - 1), y0 = std::min(std::max(iy, 0), h - 1);
+      const int x1 = std::min(std::max(ix + 1, 0), w - 1), y1 = std::min(std::max(iy + 1, 0), h - 1);
+      const double fx = tx - x0, fy = ty - y0;
+      auto T = [&](int xx, int yy) { return d3(tex_fetch(*s, m, xx, yy)); };
+      alb = dadd3(dadd3(dmul3(T(x0, y0), (1 - fx) * (1 - fy)), dmul3(T(x0, y1), (1 - fx) * fy)),
+                  dadd3(dmul3(T(x1, y0), fx * (1 - fy)), dmul3(T(x1, y1), fx * fy)));
+    }
+    const double sh = lit ? 1.0 : 0.0;
+    return D3{alb.x * (sh * Lc.x * ndl + amb.x), alb.y * (sh * Lc.y * ndl + amb.y), alb.z * (sh * Lc.z * ndl + amb.z)};
+  }
+  // sum over pixels of adj . integral of the shade over triangle g's projection
+  double tri(int g, const D3* V, bool lit) const {
+    double w3[3][3] = {{V[0].x, V[0].y, V[0].z}, {V[1].x, V[1].y, V[1].z}, {V[2].x, V[2].y, V[2].z}};
+    const P2 t[3] = {project_d(fc, w3[0]), project_d(fc, w3[1]), project_d(fc, w3[2])};
+    const double xmin = std::min(t[0].x, std::min(t[1].x, t[2].x)), xmax = std::max(t[0].x, std::max(t[1].x, t[2].x));
+    const double ymin = std::min(t[0].y, std::min(t[1].y, t[2].y)), ymax = std::max(t[0].y, std::max(t[1].y, t[2].y));
+    const int x0 = std::max(0, (int)std::floor(xmin)), x1 = std::min(fc.w - 1, (int)std::floor(xmax));
+    const int y0 = std::max(0, (int)std::floor(ymin)), y1 = std::min(fc.h - 1, (int)std::floor(ymax));
+    double L = 0.0;
+    const int ng = (int)gx.size();
+    for (int y = y0; y <= y1; y++)
+      for (int x = x0; x <= x1; x++) {
+        const float* ap = adj + 3 * ((size_t)y * fc.w + x);
+        if (ap[0] == 0.0f && ap[1] == 0.0f && ap[2] == 0.0f) continue;
+        P2 poly[8];
+        const int n = clip_poly(t, x, y, poly);
+        if (n < 3) continue;
+        D3 acc{0, 0, 0};
+        for (int k = 1; k + 1 < n; k++) {
+          const P2 a = poly[0], b = poly[k], c = poly[k + 1];
+          const double J = std::fabs((b.x - a.x) * (c.y - b.y) - (b.y - a.y) * (c.x - b.x));
+          if (J == 0.0) continue;
+          // Duffy: (s, r) in [0, 1]^2 -> a + s (b - a) + s r (c - b), area element s J
+          for (int i = 0; i < ng; i++)
+            for (int j = 0; j < ng; j++) {
+              const double ss = gx[i], rr = gx[j];
+              const double qx = a.x + ss * (b.x - a.x) + ss * rr * (c.x - b.x);
+              const double qy = a.y + ss * (b.y - a.y) + ss * rr * (c.y - b.y);
+              const double wgt = gw[i] * gw[j] * ss * J;
+              acc = dadd3(acc, dmul3(shade(g, V, lit, qx, qy), wgt));
+            }
+        }
+        L += acc.x * ap[0] + acc.y * ap[1] + acc.z * ap[2];
+      }
+    return L;
+  }
+};
+
+void lambert_fd_setup(LambertFd& F, const Scene& s, int mode, const orc_lambert* lam, const orc_camera* cam,
+                      const float* adj, int order) {
+  F.s = &s;
+  F.mode = mode;
+  const Lam L = make_lam(lam);
+  F.negL = D3{-(double)L.L.x, -(double)L.L.y, -(double)L.L.z};
+  F.Lc = d3(L.Lc);
+  F.amb = d3(L.amb);
+  const M44 wvp = mm(load44(cam->proj), load44(cam->world_view));
+  for (int i = 0; i < 16; i++) F.fc.m[i] = wvp.m[i];
+  F.fc.W = cam->width; F.fc.H = cam->height; F.fc.w = (int)cam->width; F.fc.h = (int)cam->height;
+  double pr[16], wv[16];
+  for (int i = 0; i < 16; i++) { pr[i] = cam->proj[i]; wv[i] = cam->world_view[i]; }
+  inv44d(pr, F.proj_inv);
+  inv44d(wv, F.wv_inv);
+  F.eye = D3{F.wv_inv[12], F.wv_inv[13], F.wv_inv[14]};
+  F.adj = adj;
+  gauss_legendre(order, F.gx, F.gw);
+}
+
+// facing (front to the camera, lit by the light) of triangle g with world positions V
+void facing(const LambertFd& F, const D3* V, bool& front, bool& lit) {
+  const D3 n = dcross3(dsub3(V[1], V[0]), dsub3(V[2], V[0]));
+  front = ddot3(n, dsub3(V[0], F.eye)) < 0.0;
+  lit = ddot3(n, F.negL) > 0.0;
+}
+}  // namespace
+
+extern "C" {
+// The functional itself, every front-facing triangle (for checks on small scenes).
+double orc_lambert_functional(void* sp, int mode, const orc_lambert* lam, const orc_camera* cam, const float* adjoint,
+                              int order) {
+  Scene& s = *(Scene*)sp;
+  LambertFd F;
+  lambert_fd_setup(F, s, mode, lam, cam, adjoint, order);
+  double L = 0.0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : L)
+  for (int g = 0; g < s.F; g++) {
+    D3 V[3];
+    for (int k = 0; k < 3; k++) V[k] = d3(s.pos[s.idx[3 * g + k]]);
+    bool front, lit;
+    facing(F, V, front, lit);
+    if (front) L += F.tri(g, V, lit);
+  }
+  return L;
+}
+
+// Central differences (L(x + h) - L(x - h)) / 2h of the Lambert functional for `n` coordinates
+// (3 * global vertex + axis, world space); only the triangles around the vertex change, each
+// keeping its facings at x.
+void orc_fd_lambert(void* sp, int mode, const orc_lambert* lam, const orc_camera* cam, const float* adjoint, int n,
+                    const int* coords, double h, int order, double* out) {
+  Scene& s = *(Scene*)sp;
+  LambertFd F;
+  lambert_fd_setup(F, s, mode, lam, cam, adjoint, order);
+  std::vector<std::vector<int>> ring(s.V);
+  for (int g = 0; g < s.F; g++)
+    for (int k = 0; k < 3; k++) ring[s.idx[3 * g + k]].push_back(g);
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < n; i++) {
+    const int vtx = coords[i] / 3, ax = coords[i] % 3;
+    double acc[2] = {0.0, 0.0};
+    for (int g : ring[vtx]) {
+      D3 V[3];
+      for (int k = 0; k < 3; k++) V[k] = d3(s.pos[s.idx[3 * g + k]]);
+      bool front, lit;
+      facing(F, V, front, lit);
+      if (!front) continue;
+      for (int sgn = 0; sgn < 2; sgn++) {
+        D3 W[3] = {V[0], V[1], V[2]};
+        for (int k = 0; k < 3; k++)
+          if ((int)s.idx[3 * g + k] == vtx) {
+            double* c = ax == 0 ? &W[k].x : ax == 1 ? &W[k].y : &W[k].z;
+            *c += sgn == 0 ? h : -h;
+          }
+        acc[sgn] += F.tri(g, W, lit);
+      }
+    }
+    out[i] = (acc[0] - acc[1]) / (2.0 * h);
+  }
+}
+}  // extern "C"
+
+

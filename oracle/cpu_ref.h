@@ -106,6 +106,15 @@ void  orc_optsimple_step(int n, float* grad, float* m, float* G, int nblocks, co
 double orc_silhouette_functional(void* s, const orc_camera* cam, const float* adjoint);
 void  orc_fd_silhouette(void* s, const orc_camera* cam, const float* adjoint, int n, const int* coords, double h,
                         double* out);
+/* FP64 pixel-area functional of the LAMBERT (4) / LAMBERT_VERTEX_COLOR (8) image
+ * L = sum_p adj_p . integral over pixel p of the shade (front-facing triangles of a closed
+ * convex mesh, the shadow of a face = it turns away from the light; n x n Gauss-Legendre on
+ * each triangle-pixel polygon), and its central differences for coordinates 3*vertex+axis
+ * (world space).  Pins the position gradient with its interior term. */
+double orc_lambert_functional(void* s, int mode, const orc_lambert* lam, const orc_camera* cam, const float* adjoint,
+                              int order);
+void  orc_fd_lambert(void* s, int mode, const orc_lambert* lam, const orc_camera* cam, const float* adjoint, int n,
+                     const int* coords, double h, int order, double* out);
 void  orc_adam_step(int n, float* param, const float* grad, float* m, float* v, int step, float lr,
                     float beta1, float beta2, float eps);
 

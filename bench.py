@@ -20,6 +20,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -556,21 +557,39 @@ def main():
         "cpu_baseline": None,
     }
 
+    # The serving legs run under a watchdog: should one not finish within its time limit, the
+    # line is printed with what was measured (that leg marked as timed out) and the process
+    # ends, so a stuck leg cannot cost the headline.
+    def guarded(name, limit_s, fn):
+        def expire():
+            result[name] = {"error": f"timed out after {limit_s} s"}
+            print(json.dumps(result), flush=True)
+            os._exit(0)
+        timer = threading.Timer(limit_s, expire)
+        timer.daemon = True
+        timer.start()
+        try:
+            return fn()
+        finally:
+            timer.cancel()
+
     if not args.no_serve:
         if args.serve_jobs > 0:
-            plain = plain_leg(drh, meshes, cam, target, N, spp, lam, args.plain_iters)
-            served = served_leg(drh, S, meshes, cam, N, spp, lam, args.serve_jobs, args.serve_iters)
+            plain = guarded("plain_one_job_iters_per_s", 300,
+                            lambda: plain_leg(drh, meshes, cam, target, N, spp, lam, args.plain_iters))
+            served = guarded("served", 300, lambda: served_leg(drh, S, meshes, cam, N, spp, lam, args.serve_jobs, args.serve_iters))
             result["plain_one_job_iters_per_s"] = round(plain, 3)
             result["served"] = served
             result["served_over_plain"] = round(served["iters_per_s"] / plain, 3)
             result["served_over_single_fused_step"] = round(served["iters_per_s"] / result["frozen_albedo_step"]["iters_per_s"], 3)
             result["p50_ms"], result["p99_ms"] = served["p50_ms"], served["p99_ms"]
         if args.mix_jobs > 0:
-            result["mix"] = mix_legs(drh, S, lam, args.mix_jobs, args.mix_iters, args.mix_active)
+            result["mix"] = guarded("mix", 300, lambda: mix_legs(drh, S, lam, args.mix_jobs, args.mix_iters, args.mix_active))
         if args.multiview_steps > 0:
-            result["multiview"] = multiview_leg(drh, S, lam, args.multiview_steps, 1 << 18)
+            result["multiview"] = guarded("multiview", 300, lambda: multiview_leg(drh, S, lam, args.multiview_steps, 1 << 18))
         if args.stream_jobs > 0:
-            result["stream"] = poisson_leg(drh, S, lam, args.stream_jobs, args.stream_rate, 8, args.stream_iter_scale)
+            result["stream"] = guarded("stream", 300, lambda: poisson_leg(drh, S, lam, args.stream_jobs, args.stream_rate, 8,
+                                                                          args.stream_iter_scale))
     if args.config1_steps > 0:
         result["config1"] = config1_leg(drh, S, args.config1_steps, not args.no_cpu)
 

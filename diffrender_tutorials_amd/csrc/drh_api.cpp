@@ -257,6 +257,9 @@ struct drh_renderer : BufOwner {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool prep0_forked = false;
+  // the interior pass runs on the second stream beside the edge pass (fork after the
+  // gradient is cleared, join before its final conversions)
+  hipEvent_t ev_ifork = nullptr, ev_ijoin = nullptr;
   int sharing = 1;          // handles sharing the device (drh_set_device_sharing)
   // interior samples: Hammersley (reference) or the render's grid (aligned); store: the render
   // keeps each sample's closest hit and the interior pass reads it back
@@ -457,12 +460,9 @@ struct drh_renderer : BufOwner {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<char> graph_sig[2];
   unsigned long long graph_epoch[2] = {0, 0};
-  // A dropped form's executable is kept (idle) and updated in place from the next capture of
-  // that form when the topology matches (hipGraphExecUpdate), which costs far less host time
-  // than instantiating again: a handle re-used for another scene of the same shape, or
-  // re-captured after new arguments, pays only the capture.  Dropping does not wait for the
-  // stream: an event recorded behind the form's last launch is waited for only when the idle
-  // executable is updated or destroyed (by then long passed).
+  // Dropping a form does not wait for the stream: its executable and graph are kept, with an
+  // event recorded behind the form's last launch, and destroyed once that event has passed
+  // (at the next drop or capture of the form, long after in practice).
   hipGraphExec_t stale[2] = {nullptr, nullptr};
   hipGraph_t stale_graph[2] = {nullptr, nullptr};
   hipEvent_t stale_ev[2] = {nullptr, nullptr};
@@ -502,22 +502,8 @@ struct drh_renderer : BufOwner {
       stale_ev[k] = nullptr;
     }
   }
-  // executable for captured graph g of form k: the idle one updated in place, else a new one
+  // executable for captured graph g of form k (the dropped one of that form destroyed first)
   bool instantiate(int k, hipGraph_t g) {
-    if (stale[k]) {
-      (void)hipEventSynchronize(stale_ev[k]);   // its last launch has finished
-      hipGraphNode_t bad = nullptr;
-      hipGraphExecUpdateResult res = hipGraphExecUpdateError;
-      const hipError_t e = hipGraphExecUpdate(stale[k], g, &bad, &res);
-      if (e == hipSuccess && res == hipGraphExecUpdateSuccess) {
-        gexec[k] = stale[k];
-        stale[k] = nullptr;
-        if (stale_graph[k]) (void)hipGraphDestroy(stale_graph[k]);
-        stale_graph[k] = nullptr;
-        return true;
-      }
-      (void)hipGetLastError();
-    }
     retire(k);
     return hipGraphInstantiate(&gexec[k], g, nullptr, nullptr, 0) == hipSuccess;
   }
@@ -1075,6 +1061,18 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         (s = hip(bin_rec.reserve(16 * (size_t)N), "alloc")) || (s = hip(bin_order.reserve(8 * (size_t)N), "alloc")))
       return s;
   }
+  // The interior pass reads the adjoint and the scene and adds to other gradient entries than
+  // the edge pass (or to the same ones by atomics): with the device to itself the handle runs
+  // it on the second stream beside the edge pass.  Not while profiling (per-kernel events time
+  // the main stream) or when other handles share the device (their work overlaps instead).
+  const bool fork_int = !prof && sharing == 1 && st.mode != DRH_SILHOUETTE && G > 0 &&
+                        (!skip_tex_interior || (interior_pos && (st.mode == DRH_LAMBERT || st.mode == DRH_LAMBERT_VERTEX_COLOR)));
+  hipStream_t iq = stream;
+  if (fork_int) {
+    if ((s = hip(hipEventRecord(ev_ifork, stream), "fork")) || (s = hip(hipStreamWaitEvent(side, ev_ifork, 0), "fork")))
+      return s;
+    iq = side;
+  }
   for (int v = 0; v < views; v++) {
     const DevCam& c = cam_cache[v];
     const int tiles = num_tiles(sq, c.w, c.h);   // interior samples: nsamp = sq*sq Hammersley points
@@ -1114,9 +1112,9 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
         if (!ia.uniform) blocks = tiles;
       }
       if (ia.uniform && ia.coverage)
-        timed("k_interior", [&] { launch_coverage(ia, tiles, cov_part.as<float4>(), stream); });
+        timed("k_interior", [&] { launch_coverage(ia, tiles, cov_part.as<float4>(), iq); });
       else
-        timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, stream); });
+        timed("k_interior", [&] { launch_interior(st.mode, ia, blocks, iq); });
     }
     if (interior_pos && (st.mode == DRH_LAMBERT || st.mode == DRH_LAMBERT_VERTEX_COLOR) && G > 0) {
       // the Lambert interior position term: a pass of its own over the same samples
@@ -1126,7 +1124,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       pa.layout = dlayout.as<int>(); pa.aligned = interior_aligned; pa.jit = jitter(); pa.light = light;
       pa.hits = (interior_store && hits_stored) ? hitbuf.as<float4>() : nullptr;
       pa.tr_inv0 = dtr_inv0.as<float>(); pa.dacc = dacc_p;
-      timed("k_interior_pos", [&] { launch_interior_pos(st.mode, pa, tiles, stream); });
+      timed("k_interior_pos", [&] { launch_interior_pos(st.mode, pa, tiles, iq); });
     }
     if (E > 0 && N > 0) {
       if (v == 0 && prep0_forked) {
@@ -1170,6 +1168,9 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       timed("k_edge_grad", [&] { launch_edge_grad(ea, stream); });
     }
   }
+  if (fork_int &&
+      ((s = hip(hipEventRecord(ev_ijoin, side), "join")) || (s = hip(hipStreamWaitEvent(stream, ev_ijoin, 0), "join"))))
+    return s;
   if (dacc_p)
     timed("k_det_finish", [&] {
       launch_det_finish(dacc_p, G, d_grad, tr_apart ? trscr.as<float>() : nullptr, tr_apart ? 12 * nm : 0, stream);
@@ -1205,7 +1206,9 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   r->pool_stream = r->stream;
   if (hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&r->ev_ifork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&r->ev_ijoin, hipEventDisableTiming) != hipSuccess)
     return DRH_E_HIP;
   r->sq = (int)std::sqrt((float)settings->spp);
   r->nsamp = r->sq * r->sq;
@@ -1244,6 +1247,8 @@ void drh_destroy(drh_handle h) {
   (void)hipStreamSynchronize(h->side);
   (void)hipEventDestroy(h->ev_fork);
   (void)hipEventDestroy(h->ev_join);
+  (void)hipEventDestroy(h->ev_ifork);
+  (void)hipEventDestroy(h->ev_ijoin);
   hipStream_t side = h->side, main = h->stream;
   delete h;   // pooled buffers are freed on the main stream, which still exists
   (void)hipStreamDestroy(side);

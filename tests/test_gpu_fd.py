@@ -225,3 +225,4 @@ def test_vertex_colour_interior_position_gradient_matches_fd():
             r2.set_interior_pos_grad(1)
         finally:
             r2.close()
+

@@ -413,6 +413,8 @@ def main():
     ap.add_argument("--stream-rate", type=float, default=20.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--stream-iter-scale", type=float, default=0.1, help="scale of the 50-500 iteration counts")
     ap.add_argument("--config1-steps", type=int, default=200, help="timed config-1 steps (0: skip)")
+    ap.add_argument("--eager", action="store_true",
+                    help="config-2 steps without hipGraph replay or second-stream overlap (for kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (k_copy_f32), the known-size reference "
                          "for FETCH_SIZE / WRITE_SIZE counters")
@@ -451,6 +453,11 @@ def main():
         r = drh.DiffRender(mode=drh.LAMBERT, spp=spp, light=lam)
         r.commit(meshes)
         r.set_pixel_jitter(True, JITTER_SEED)   # 4 spp jittered: a fresh stratified pattern per step
+        if args.eager:
+            # every step launched kernel by kernel, one stream: a kernel trace's durations are then
+            # those the per-kernel HIP events measure
+            drh.load().drh_opt_use_graph(r.h, 0)
+            r.set_device_sharing(2)
         r.opt_init(drh.OPT_ADAM, 1e-3, 1e-3)
         r.opt_set_texture_lr(tex_lr)
         return r

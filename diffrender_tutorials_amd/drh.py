@@ -243,6 +243,11 @@ class DiffRender:
         self._check(load().drh_get_hit_buffer_bytes(self.h, ctypes.byref(n)))
         return n.value
 
+    def set_device_sharing(self, handles):
+        """How many handles share the device (1: this handle overlaps its own passes on a second
+        stream; more: it leaves that to the other handles)."""
+        self._check(load().drh_set_device_sharing(self.h, int(handles)))
+
     def set_edge_order(self, order):
         """EDGE_ORDER_AUTO / _SAMPLE (rnd0 slot order) / _BINNED (kept samples by 8x8 pixel bin)."""
         self._check(load().drh_set_edge_order(self.h, int(order)))

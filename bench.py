@@ -152,9 +152,9 @@ def plain_fit(drh, meshes, cams, targets, mode, spp, N, lam, iters):
 
 
 def sequential_fused(drh, jobs, targets, lam, iters):
-    """Each job's fused device step (graph replays, no host round trips) run job after job on
-    one handle at a time: handle, commit, Adam set-up, `iters` steps, synchronise, tear down.
-    Returns the wall seconds of the whole batch."""
+    """Each job's fused device step (graph replays of four steps, as the server launches them; no
+    host round trips) run job after job on one handle at a time: handle, commit, Adam set-up,
+    `iters` steps, synchronise, tear down.  Returns the wall seconds of the whole batch."""
     t0 = time.perf_counter()
     for j, tg in zip(jobs, targets):
         r = drh.DiffRender(mode=j["mode"], spp=j["spp"], light=lam)
@@ -164,8 +164,11 @@ def sequential_fused(drh, jobs, targets, lam, iters):
         dt = r.device_array(np.concatenate([x.reshape(-1) for x in tg]))
         dl = r.device_array(np.zeros(1, np.float32))
         cs = drh.cameras_array(j["cams"])
-        for _ in range(iters):
-            r.opt_step_dev(cs, len(j["cams"]), dt.ptr, j["edge_samples"], dl.ptr)
+        done = 0
+        while done < iters:   # as the server steps: the first step alone, then graphs of 4 steps
+            k = 1 if done == 0 else min(4, iters - done)
+            r.opt_steps_dev(cs, len(j["cams"]), dt.ptr, j["edge_samples"], dl.ptr, k)
+            done += k
         r.sync()
         dt.free()
         dl.free()

@@ -493,7 +493,44 @@ struct drh_renderer : BufOwner {
     gexec[k] = nullptr;
     graph[k] = nullptr;
   }
-  void drop_graph() { drop_form(0); drop_form(1); }
+  // k-step graphs (drh_opt_steps): k consecutive fused steps captured as one graph, keyed by k,
+  // the steps' LBVH forms (bit s: full build at step s), the arguments and the epoch.  Dropped
+  // ones wait, as the forms above, for an event behind their last launch before destruction.
+  struct MultiGraph {
+    int k;
+    unsigned forms;
+    std::vector<char> sig;
+    unsigned long long epoch;
+    hipGraph_t g;
+    hipGraphExec_t e;
+  };
+  std::vector<MultiGraph> multi;
+  std::vector<MultiGraph> multi_dropped;
+  hipEvent_t multi_ev = nullptr;
+  static constexpr size_t MULTI_KEEP = 6;
+  void retire_multi() {
+    if (multi_dropped.empty()) return;
+    (void)hipEventSynchronize(multi_ev);
+    for (auto& m : multi_dropped) { (void)hipGraphExecDestroy(m.e); (void)hipGraphDestroy(m.g); }
+    multi_dropped.clear();
+  }
+  // drop the oldest `n` entries (all with n < 0)
+  void drop_multi(long n = -1) {
+    if (multi.empty()) return;
+    retire_multi();
+    const size_t cnt = n < 0 ? multi.size() : std::min(multi.size(), (size_t)n);
+    if (!multi_ev && hipEventCreateWithFlags(&multi_ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(stream);
+      for (size_t i = 0; i < cnt; i++) { (void)hipGraphExecDestroy(multi[i].e); (void)hipGraphDestroy(multi[i].g); }
+      multi.erase(multi.begin(), multi.begin() + cnt);
+      return;
+    }
+    (void)hipEventRecord(multi_ev, stream);
+    multi_dropped.assign(multi.begin(), multi.begin() + cnt);
+    multi.erase(multi.begin(), multi.begin() + cnt);
+  }
+  void drop_graph() { drop_form(0); drop_form(1); drop_multi(); }
   void destroy_graphs() {
     drop_graph();
     for (int k = 0; k < 2; k++) {
@@ -501,6 +538,9 @@ struct drh_renderer : BufOwner {
       if (stale_ev[k]) (void)hipEventDestroy(stale_ev[k]);
       stale_ev[k] = nullptr;
     }
+    retire_multi();
+    if (multi_ev) (void)hipEventDestroy(multi_ev);
+    multi_ev = nullptr;
   }
   // executable for captured graph g of form k (the dropped one of that form destroyed first)
   bool instantiate(int k, hipGraph_t g) {
@@ -1187,14 +1227,76 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
 // ===========================================================================
 extern "C" {
 
+static bool settings_valid(const drh_settings* s) {
+  return s && s->spp >= 1 && (s->mode >= DRH_SILHOUETTE && (s->mode <= DRH_LAMBERT || s->mode == DRH_LAMBERT_VERTEX_COLOR)) &&
+         (s->rng_mode == DRH_RNG_REF_PRNG14 || s->rng_mode == DRH_RNG_QMC) &&
+         (s->tracer == DRH_TRACER_BVH || s->tracer == DRH_TRACER_BF);
+}
+
+// settings, sample grid and light (src/shade_lambert.cpp:10-12 unless a custom one is given)
+static void apply_settings(drh_renderer& r, const drh_settings& settings) {
+  r.st = settings;
+  r.sq = (int)std::sqrt((float)settings.spp);
+  r.nsamp = r.sq * r.sq;
+  if (settings.light_custom) {
+    r.light.L = {settings.light_dir[0], settings.light_dir[1], settings.light_dir[2]};
+    r.light.Lc = {settings.light_color[0], settings.light_color[1], settings.light_color[2]};
+    r.light.amb = {settings.ambient[0], settings.ambient[1], settings.ambient[2]};
+  } else {
+    const float x = -1, y = -1, z = 0;
+    const float li = 1.0f / std::sqrt(x * x + y * y + z * z);
+    r.light.L = {x * li, y * li, z * li};
+    r.light.Lc = {1.0f, 0.5f, 0.2f};
+    r.light.amb = {0.1f * 1.0f, 0.1f * 1.0f, 0.1f * 1.0f};
+  }
+}
+
+// A handle's streams, events and buffers kept, its settings replaced: what drh_create would
+// give for `settings` on the same device, once the caller commits a scene (the server hands
+// an idle handle to a job with other settings this way).
+drh_status drh_reset_settings(drh_handle h, const drh_settings* settings) {
+  if (!h || !settings_valid(settings) || settings->device != h->st.device) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  const bool spp_changed = settings->spp != h->st.spp;
+  const bool rng_changed = settings->rng_mode != h->st.rng_mode;
+  h->drop_graph();
+  h->committed = false;
+  h->hits_stored = false;
+  // every option back to drh_create's value
+  h->opt_kind = -1;
+  h->opt_iter = 0;
+  h->pos_lr = 0.1f, h->col_lr = 0.1f, h->tex_lr = 0.0f, h->tr_lr = 0.0f;
+  h->skip_tex_interior = false;
+  h->sharing = 1;
+  h->interior_aligned = 0, h->interior_store = 0;
+  h->hit_limit = ~(size_t)0;
+  h->jit_on = 0, h->jit_seed = 0, h->jit_host_step = 0;
+  h->edge_order = DRH_EDGE_ORDER_AUTO;
+  h->edge_grad_mode = DRH_EDGE_GRAD_REFERENCE_DIAGONAL;
+  h->side_test = DRH_SIDE_TEST_REFERENCE;
+  h->deterministic = false;
+  h->interior_pos = false;
+  h->use_graph = true;
+  h->prof = false;
+  h->count_on = false;
+  h->rebuild_every = 8;
+  apply_settings(*h, *settings);
+  if (rng_changed) h->rnd_n = -1;
+  if (spp_changed) {
+    std::vector<float> hm;
+    hammersley(settings->spp, hm);
+    drh_status s;
+    if ((s = h->hip(h->hamm.reserve(4 * hm.size()), "alloc")) || (s = h->stage_begin(drh_renderer::stage_round(4 * hm.size()))) ||
+        (s = h->stage_send(h->hamm.p, hm.data(), 4 * hm.size())) || (s = h->stage_end()))
+      return s;
+  }
+  return DRH_OK;
+}
+
 drh_status drh_create(const drh_settings* settings, drh_handle* out) {
   if (!settings || !out) return DRH_E_INVALID;
   *out = nullptr;
-  if (settings->mode < DRH_SILHOUETTE || (settings->mode > DRH_LAMBERT && settings->mode != DRH_LAMBERT_VERTEX_COLOR) ||
-      settings->spp < 1 ||
-      (settings->rng_mode != DRH_RNG_REF_PRNG14 && settings->rng_mode != DRH_RNG_QMC) ||
-      (settings->tracer != DRH_TRACER_BVH && settings->tracer != DRH_TRACER_BF))
-    return DRH_E_INVALID;
+  if (!settings_valid(settings)) return DRH_E_INVALID;
   std::unique_ptr<drh_renderer> r(new drh_renderer());
   t_buf_owner = nullptr;   // later buffers of this thread belong to no handle
   r->st = *settings;
@@ -1210,20 +1312,7 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
       hipEventCreateWithFlags(&r->ev_ifork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev_ijoin, hipEventDisableTiming) != hipSuccess)
     return DRH_E_HIP;
-  r->sq = (int)std::sqrt((float)settings->spp);
-  r->nsamp = r->sq * r->sq;
-  // light (src/shade_lambert.cpp:10-12 unless a custom one is given)
-  if (settings->light_custom) {
-    r->light.L = {settings->light_dir[0], settings->light_dir[1], settings->light_dir[2]};
-    r->light.Lc = {settings->light_color[0], settings->light_color[1], settings->light_color[2]};
-    r->light.amb = {settings->ambient[0], settings->ambient[1], settings->ambient[2]};
-  } else {
-    const float x = -1, y = -1, z = 0;
-    const float li = 1.0f / std::sqrt(x * x + y * y + z * z);
-    r->light.L = {x * li, y * li, z * li};
-    r->light.Lc = {1.0f, 0.5f, 0.2f};
-    r->light.amb = {0.1f * 1.0f, 0.1f * 1.0f, 0.1f * 1.0f};
-  }
+  apply_settings(*r, *settings);
   std::vector<float> hm;
   hammersley(settings->spp, hm);
   if (r->hamm.reserve(4 * hm.size()) != hipSuccess) return DRH_E_OOM;
@@ -1566,10 +1655,89 @@ drh_status drh_set_edge_side_test(drh_handle h, int test) {
   return DRH_OK;
 }
 
+drh_status drh_opt_steps(drh_handle h, const drh_camera* cams, int views, const float* d_targets, int edge_samples,
+                         float* d_loss, int steps) {
+  if (!h) return DRH_E_INVALID;
+  if (steps < 1) return h->fail(DRH_E_INVALID, "opt_steps: steps must be at least 1");
+  if (steps == 1) return drh_opt_step(h, cams, views, d_targets, edge_samples, d_loss);
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "opt_steps: commit first");
+  if (h->opt_kind < 0) return h->fail(DRH_E_INVALID, "opt_steps: call drh_opt_init first");
+  if (!cams || views <= 0 || !d_targets || edge_samples < 0) return h->fail(DRH_E_INVALID, "opt_steps: bad arguments");
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  // the first step of a fresh optimisation sizes every buffer eagerly; without graphs, step by step
+  if (h->opt_iter == 0) {
+    if ((s = drh_opt_step(h, cams, views, d_targets, edge_samples, d_loss))) return s;
+    return drh_opt_steps(h, cams, views, d_targets, edge_samples, d_loss, steps - 1);
+  }
+  const bool graphable = h->use_graph && h->opt_kind == DRH_OPT_ADAM && !h->prof && !h->count_on;
+  if (!graphable) {
+    for (int k = 0; k < steps; k++)
+      if ((s = drh_opt_step(h, cams, views, d_targets, edge_samples, d_loss))) return s;
+    return DRH_OK;
+  }
+  std::vector<char> sig(sizeof(drh_camera) * views + 4 * sizeof(void*));
+  std::memcpy(sig.data(), cams, sizeof(drh_camera) * views);
+  const void* ptrs[4] = {d_targets, d_loss, (const void*)(intptr_t)edge_samples, (const void*)(intptr_t)views};
+  std::memcpy(sig.data() + sizeof(drh_camera) * views, ptrs, sizeof ptrs);
+  long long total;
+  if ((s = h->set_cams(cams, views, &total)) || (s = h->ensure_rnd(edge_samples))) return s;
+  const int it0 = h->opt_iter;
+  unsigned forms = 0;
+  for (int k = 0; k < steps && k < 32; k++)
+    if ((it0 + k) % h->rebuild_every == 0 || (k == 0 && !h->topo_valid)) forms |= 1u << k;
+  const unsigned long long epoch = h->realloc_epoch.load();
+  for (auto& m : h->multi)
+    if (m.k == steps && m.forms == forms && m.epoch == epoch && m.sig == sig) {
+      if ((s = h->hip(hipGraphLaunch(m.e, h->stream), "graph launch"))) return s;
+      h->opt_iter = it0 + steps;
+      return DRH_OK;
+    }
+  if (h->multi.size() >= drh_renderer::MULTI_KEEP) h->drop_multi(1);
+  h->retire_multi();
+  if ((s = h->hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return s;
+  drh_status body = DRH_OK;
+  for (int k = 0; k < steps && body == DRH_OK; k++) {
+    h->opt_iter = it0 + k;
+    body = opt_step_body(h, cams, views, d_targets, edge_samples, d_loss);
+  }
+  h->opt_iter = it0;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+  if (body == DRH_OK && ec == hipSuccess && g && hipGraphInstantiate(&e, g, nullptr, nullptr, 0) == hipSuccess) {
+    h->multi.push_back(drh_renderer::MultiGraph{steps, forms, sig, h->realloc_epoch.load(), g, e});
+    if ((s = h->hip(hipGraphLaunch(e, h->stream), "graph launch"))) return s;
+    h->opt_iter = it0 + steps;
+    return DRH_OK;
+  }
+  // capture failed: the host state of the captured steps never ran; forget it and step eagerly
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  h->topo_valid = false;
+  for (int k = 0; k < steps; k++)
+    if ((s = drh_opt_step(h, cams, views, d_targets, edge_samples, d_loss))) return s;
+  return DRH_OK;
+}
+
 drh_status drh_opt_use_graph(drh_handle h, int on) {
   if (!h) return DRH_E_INVALID;
   h->use_graph = on != 0;
   if (!on) h->drop_graph();
+  return DRH_OK;
+}
+
+drh_status drh_export_params(drh_handle h, float* pos, float* col, const void* d_loss, float* loss, uint32_t* fault) {
+  if (!h || !pos || !col || !loss || !d_loss || !fault) return DRH_E_INVALID;
+  if (!h->committed) return h->fail(DRH_E_NOT_COMMITTED, "export_params: commit first");
+  (void)hipSetDevice(h->st.device);
+  const size_t b = 12 * (size_t)h->V;
+  drh_status s;
+  if ((s = h->hip(hipMemcpyAsync(pos, h->pos_orig.p, b, hipMemcpyDeviceToHost, h->stream), "export")) ||
+      (s = h->hip(hipMemcpyAsync(col, h->col.p, b, hipMemcpyDeviceToHost, h->stream), "export")) ||
+      (s = h->hip(hipMemcpyAsync(loss, d_loss, 4, hipMemcpyDeviceToHost, h->stream), "export")) ||
+      (s = h->hip(hipMemcpyAsync(fault, h->fault.p, 4, hipMemcpyDeviceToHost, h->stream), "export")))
+    return s;
   return DRH_OK;
 }
 

@@ -4,14 +4,14 @@
 // drh_server_submit (inputs copied).  Threads serve them: admitters set queued jobs up while
 // a slot and the HBM budget allow (handle, commit with its LBVH build, uploads); the reaper
 // tears finished ones down (downloads, frees), which admits the next.
-// A finished job's handle is kept idle, with its device buffers, streams and graph
-// executables, for the next job with the same settings (its set-up then re-commits and
-// re-captures instead of creating and destroying HIP objects).
-// The stepper steps the admitted jobs round-robin: every job keeps up to four fused
-// fwd+bwd+Adam steps in flight on its own HIP stream (hipGraph replays after its first
-// step), so the device overlaps the jobs' kernels while set-up and teardown of others go on
-// beside them.  Per-iteration latency = HIP event time from a step's first to last launch on
-// its stream, contention included.
+// A finished job's handle is kept idle, with its device buffers, streams and events, for the
+// next job (one with the same settings first; another's settings are reset in place), whose
+// set-up then re-commits and re-captures instead of creating and destroying HIP objects.
+// The stepper steps the admitted jobs round-robin: every job keeps up to two launches of
+// four fused fwd+bwd+Adam steps each in flight on its own HIP stream (one graph launch per
+// four steps after its first, eager, step), so the device overlaps the jobs' kernels while
+// set-up and teardown of others go on beside them.  Per-iteration latency = HIP event time
+// of a launch on its stream over its steps, contention included.
 //
 // The reference has no serving layer (its optimiser runs one scene at a time,
 // src/optimizer_simple.cpp:272-309); this is the "serving" pattern of the path.
@@ -62,11 +62,19 @@ struct Job {
   }
   // run state
   drh_handle h = nullptr;
+  bool reset_handle = false;   // h is an idle handle of other settings, reset at admission
+  // pinned host memory the last launch's downloads land in: positions, colours (3V floats
+  // each), the loss, the fault word; `exported` once they are queued
+  float* host_out = nullptr;
+  size_t host_out_cap = 0;
+  bool exported = false;
+  size_t host_out_bytes() const { return 4 * (6 * (size_t)V() + 2); }
   void* d_targets = nullptr;
   size_t targets_cap = 0;   // bytes allocated at d_targets (an idle handle's may be larger)
   void* d_loss = nullptr;
   int issued = 0, completed = 0;
-  std::deque<std::pair<hipEvent_t, hipEvent_t>> inflight;
+  struct Launch { hipEvent_t a, b; int steps; };
+  std::deque<Launch> inflight;
   // results
   int status = 0;  // 0 queued / running, 1 done, <0 failed (drh_status)
   std::string error;
@@ -85,9 +93,13 @@ struct Job {
   }
 };
 
-// Steps a job keeps queued on its stream (graph replays back to back; more than two keeps
-// the device fed while the stepping thread serves the other jobs).
-constexpr size_t MAX_INFLIGHT = 4;
+// Launches a job keeps queued on its stream (graph replays back to back, so the device stays
+// fed while the stepping thread serves the other jobs), and steps per launch: after its first
+// (eager) step a job runs STEPS_PER_LAUNCH fused steps per graph launch (drh_opt_steps), which
+// cuts the HIP calls per step -- the calls of the stepping, admitting and teardown threads
+// serialise in the runtime.
+constexpr size_t MAX_INFLIGHT = 2;
+constexpr int STEPS_PER_LAUNCH = 4;
 // Threads setting jobs up (each job's set-up is serial host work: handle, uploads, edge set,
 // LBVH build); DRH_SERVER_ADMITTERS (1..16) overrides the default when the server starts.
 constexpr int ADMITTERS = 1;
@@ -105,7 +117,7 @@ bool same_settings(const drh_settings& a, const drh_settings& b) {
          std::memcmp(a.ambient, b.ambient, sizeof a.ambient) == 0;
 }
 
-// A finished job's handle waiting for the next job with the same settings.
+// A finished job's handle waiting for the next job (settings reset if they differ).
 struct Idle {
   drh_handle h;
   drh_settings st;
@@ -169,6 +181,9 @@ struct drh_server_s {
     spare_events.pop_back();
     return e;
   }
+  // pinned result buffers of finished jobs, by capacity, for the next jobs (freed at destroy:
+  // hipHostFree may wait for the device)
+  std::multimap<size_t, float*> pinned_free;   // under mu
   std::map<int, std::unique_ptr<Job>> jobs;
   int next_id = 1;
   bool stop = false, stepper_done = false;
@@ -184,6 +199,7 @@ struct drh_server_s {
   // admission by phase (handle and options, targets, cameras + sample table, commit,
   // optimiser set-up); printed at destroy when DRH_SERVER_TRACE is set
   double phase_s[5] = {0, 0, 0, 0, 0};
+  int handles_created = 0, handles_reused = 0, handles_reset = 0, handles_evicted = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   void run();         // stepping thread
@@ -202,21 +218,35 @@ struct drh_server_s {
 
 void drh_server_s::finish(Job* j, int status, const std::string& err) {
   bool kept = false;
+  std::string fault_err;
   if (j->h) {
     if (status == 1) {
-      const int V = j->V();
-      j->out_pos.resize(3 * (size_t)V);
-      j->out_col.resize(3 * (size_t)V);
-      if (drh_get_params(j->h, j->out_pos.data(), j->out_col.data()) != DRH_OK) { status = DRH_E_HIP; }
-      if (drh_memcpy_dtoh(j->h, &j->final_loss, j->d_loss, 4) != DRH_OK) status = DRH_E_HIP;
+      const size_t V = (size_t)j->V();
+      j->out_pos.resize(3 * V);
+      j->out_col.resize(3 * V);
+      if (j->exported) {
+        // the last launch's event has passed, and with it the downloads queued before it
+        std::memcpy(j->out_pos.data(), j->host_out, 12 * V);
+        std::memcpy(j->out_col.data(), j->host_out + 3 * V, 12 * V);
+        std::memcpy(&j->final_loss, j->host_out + 6 * V, 4);
+        uint32_t f;
+        std::memcpy(&f, j->host_out + 6 * V + 1, 4);
+        if (f) {
+          status = DRH_E_DEVICE;
+          fault_err = "device check failed (fault word set)";
+        }
+      } else {
+        if (drh_get_params(j->h, j->out_pos.data(), j->out_col.data()) != DRH_OK) { status = DRH_E_HIP; }
+        if (drh_memcpy_dtoh(j->h, &j->final_loss, j->d_loss, 4) != DRH_OK) status = DRH_E_HIP;
+      }
     }
-    for (auto& ev : j->inflight) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    for (auto& ev : j->inflight) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     j->inflight.clear();
     // a healthy handle (its last step completed: the downloads above waited for it) goes idle
-    // for the next job with the same settings; the oldest idle one makes room if need be
+    // for the next job; the oldest idle one makes room if need be
     const Idle e{j->h, j->st, j->d_targets, j->targets_cap, j->d_loss, j->bytes + j->hit_bytes};
     std::vector<Idle> drop;
-    if (status == 1 && drh_sync(j->h) == DRH_OK) {
+    if (status == 1 && (j->exported || drh_sync(j->h) == DRH_OK)) {
       std::lock_guard<std::mutex> lk(mu);
       if (!stop) {
         if ((int)idle.size() >= max_active) {
@@ -231,14 +261,18 @@ void drh_server_s::finish(Job* j, int status, const std::string& err) {
       }
     }
     if (!kept) drop.push_back(e);
-    for (const Idle& d : drop) destroy_idle(d);
+    for (const Idle& d : drop) destroy_idle(d);   // waits for the handle's queued copies
     j->h = nullptr;
     j->d_targets = j->d_loss = nullptr;
   }
   std::lock_guard<std::mutex> lk(mu);
+  if (j->host_out) {   // nothing is queued into it any more (kept: drained; else destroyed)
+    pinned_free.emplace(j->host_out_cap, j->host_out);
+    j->host_out = nullptr;
+  }
   if (!kept) used -= std::min(used, j->bytes + j->hit_bytes);   // a kept handle's charge stays
   j->status = status;
-  j->error = err;
+  j->error = fault_err.empty() ? err : fault_err;
   j->t_done = std::chrono::steady_clock::now();
   if (status == 1) jobs_done++; else jobs_failed++;
   cv_done.notify_all();
@@ -252,6 +286,9 @@ bool drh_server_s::admit_one(Job* j, double phase[5]) {
   if (!h) {
     if ((s = drh_create(&j->st, &h)) != DRH_OK) { finish(j, s, "drh_create failed"); return false; }
     j->h = h;
+  } else if (j->reset_handle && (s = drh_reset_settings(h, &j->st)) != DRH_OK) {
+    finish(j, s, drh_last_error(h));
+    return false;
   }
   // every option the server sets, set for this job (an idle handle keeps its last job's)
   (void)drh_set_device_sharing(h, max_active);
@@ -278,6 +315,28 @@ bool drh_server_s::admit_one(Job* j, double phase[5]) {
     j->targets_cap = tbytes;
   }
   if (!j->d_loss && (s = drh_malloc(h, 16, &j->d_loss)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
+  {
+    const size_t need = j->host_out_bytes();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = pinned_free.lower_bound(need);
+      if (it != pinned_free.end() && it->first <= 4 * need + (64 << 10)) {   // no far larger one
+        j->host_out_cap = it->first;
+        j->host_out = it->second;
+        pinned_free.erase(it);
+      }
+    }
+    if (!j->host_out) {
+      size_t cap = 64 << 10;
+      while (cap < need) cap *= 2;
+      if (hipHostMalloc((void**)&j->host_out, cap, hipHostMallocDefault) != hipSuccess) {
+        j->host_out = nullptr;
+        finish(j, DRH_E_OOM, "pinned host memory for the results");
+        return false;
+      }
+      j->host_out_cap = cap;
+    }
+  }
   // every upload first, while the handle's stream is still empty (their synchronisations then
   // wait for nothing): targets, cameras, edge-sample table; the commit's LBVH build and the
   // optimiser set-up are left queued behind them, so neither admission nor the job's first
@@ -321,12 +380,19 @@ void drh_server_s::admit_run() {
         cv_done.notify_all();
         continue;
       }
-      // take over an idle handle with the same settings if its charge fits, then evict idle
-      // handles (oldest first) until the job fits the budget
+      // take over an idle handle whose charge fits, one with the same settings first, else
+      // any (its settings are reset: cheaper than a new handle's streams, events and first
+      // allocations); then evict idle handles (oldest first) until the job fits the budget
+      size_t take = idle.size();
       for (size_t k = 0; k < idle.size(); k++) {
-        if (!same_settings(idle[k].st, j->st)) continue;
         if (used - idle_bytes + std::max(j->bytes, idle[k].bytes) > budget) continue;
+        if (same_settings(idle[k].st, j->st)) { take = k; break; }
+        if (take == idle.size()) take = k;
+      }
+      if (take < idle.size()) {
+        const size_t k = take;
         const Idle e = idle[k];
+        j->reset_handle = !same_settings(e.st, j->st);
         idle.erase(idle.begin() + k);
         idle_bytes -= e.bytes;
         used -= std::min(used, e.bytes);
@@ -335,10 +401,13 @@ void drh_server_s::admit_run() {
         j->targets_cap = e.targets_cap;
         j->d_loss = e.d_loss;
         j->bytes = std::max(j->bytes, e.bytes);   // its buffers are at least the last job's
-        break;
+        handles_reused++;
+        if (j->reset_handle) handles_reset++;
       }
       used += j->bytes;
+      if (!j->h) handles_created++;
       while (used > budget && !idle.empty()) {
+        handles_evicted++;
         evict.push_back(idle.front());
         idle_bytes -= idle.front().bytes;
         used -= std::min(used, idle.front().bytes);
@@ -410,23 +479,25 @@ void drh_server_s::run() {
       active_sum += active.size();
       active_samples++;
     }
-    // stepping: keep up to two steps in flight per job; harvest completed ones
+    // stepping: keep up to MAX_INFLIGHT launches in flight per job; harvest completed ones
     bool progressed = false;
     for (size_t k = 0; k < active.size();) {
       Job* j = active[k];
       // harvest
-      while (!j->inflight.empty() && hipEventQuery(j->inflight.front().second) == hipSuccess) {
+      while (!j->inflight.empty() && hipEventQuery(j->inflight.front().b) == hipSuccess) {
+        const Job::Launch l = j->inflight.front();
         float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, j->inflight.front().first, j->inflight.front().second);
-        spare_events.push_back(j->inflight.front().first);
-        spare_events.push_back(j->inflight.front().second);
+        (void)hipEventElapsedTime(&ms, l.a, l.b);
+        spare_events.push_back(l.a);
+        spare_events.push_back(l.b);
         j->inflight.pop_front();
-        j->completed++;
-        j->latencies_ms.push_back(ms);
+        j->completed += l.steps;
+        // per-iteration latency: the launch's device time over its steps
+        for (int q = 0; q < l.steps; q++) j->latencies_ms.push_back(ms / l.steps);
         {
           std::lock_guard<std::mutex> lk(mu);
-          lat_ms.push_back(ms);
-          iters_done++;
+          for (int q = 0; q < l.steps; q++) lat_ms.push_back(ms / l.steps);
+          iters_done += l.steps;
         }
         progressed = true;
       }
@@ -440,9 +511,17 @@ void drh_server_s::run() {
         const auto ts = std::chrono::steady_clock::now();
         hipEvent_t a = take_event(), b = take_event();
         hipStream_t st = (hipStream_t)drh_stream(j->h);
+        const int steps = j->issued == 0 ? 1 : std::min(STEPS_PER_LAUNCH, j->iterations - j->issued);
         (void)hipEventRecord(a, st);
-        const drh_status s = drh_opt_step(j->h, j->cams.data(), (int)j->cams.size(), (const float*)j->d_targets,
-                                          j->edge_samples, (float*)j->d_loss);
+        drh_status s = drh_opt_steps(j->h, j->cams.data(), (int)j->cams.size(), (const float*)j->d_targets,
+                                     j->edge_samples, (float*)j->d_loss, steps);
+        if (s == DRH_OK && j->issued + steps == j->iterations) {
+          // the last launch: its results go to pinned memory behind it, read at teardown
+          const size_t V = (size_t)j->V();
+          s = drh_export_params(j->h, j->host_out, j->host_out + 3 * V, j->d_loss, j->host_out + 6 * V,
+                                (uint32_t*)(j->host_out + 6 * V + 1));
+          j->exported = s == DRH_OK;
+        }
         (void)hipEventRecord(b, st);
         if (s != DRH_OK) {
           spare_events.push_back(a);
@@ -450,8 +529,8 @@ void drh_server_s::run() {
           hand_over(k, s, drh_last_error(j->h));
           continue;
         }
-        j->inflight.push_back({a, b});
-        j->issued++;
+        j->inflight.push_back(Job::Launch{a, b, steps});
+        j->issued += steps;
         std::lock_guard<std::mutex> lk(mu);
         step_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
         progressed = true;
@@ -613,11 +692,15 @@ void drh_server_destroy(drh_server s) {
   s->worker.join();     // hands its jobs (and those admitted meanwhile) to the reaper
   s->reaper.join();     // finishes them, then exits
   if (std::getenv("DRH_SERVER_TRACE"))
-    std::fprintf(stderr, "drh_server admission s: handle %.4f targets %.4f cameras+table %.4f commit %.4f opt_init %.4f\n",
-                 s->phase_s[0], s->phase_s[1], s->phase_s[2], s->phase_s[3], s->phase_s[4]);
+    std::fprintf(stderr, "drh_server admission s: handle %.4f targets %.4f cameras+table %.4f commit %.4f opt_init %.4f; "
+                 "handles created %d reused %d (of which reset %d) evicted %d\n",
+                 s->phase_s[0], s->phase_s[1], s->phase_s[2], s->phase_s[3], s->phase_s[4], s->handles_created,
+                 s->handles_reused, s->handles_reset, s->handles_evicted);
   (void)hipSetDevice(s->device);
   for (const Idle& e : s->idle) destroy_idle(e);   // no thread is left to take them
   s->idle.clear();
+  for (auto& p : s->pinned_free) (void)hipHostFree(p.second);
+  s->pinned_free.clear();
   delete s;
 }
 

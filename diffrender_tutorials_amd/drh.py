@@ -38,7 +38,7 @@ STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params",
-           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_restricted", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples", "drh_set_pixel_jitter", "drh_set_edge_side_test",
+           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_restricted", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples", "drh_set_pixel_jitter", "drh_set_edge_side_test", "drh_opt_steps",
            "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
@@ -220,6 +220,11 @@ class DiffRender:
     def opt_step_dev(self, cams_c, views, d_targets_ptr, edge_samples, d_loss_ptr=None):
         self._check(load().drh_opt_step(self.h, cams_c, views, ctypes.c_void_p(d_targets_ptr), int(edge_samples),
                                         ctypes.c_void_p(d_loss_ptr) if d_loss_ptr else None))
+
+    def opt_steps_dev(self, cams_c, views, d_targets_ptr, edge_samples, d_loss_ptr, steps):
+        """`steps` fused steps, captured together as one graph (drh_opt_steps)."""
+        self._check(load().drh_opt_steps(self.h, cams_c, views, ctypes.c_void_p(d_targets_ptr), int(edge_samples),
+                                         ctypes.c_void_p(d_loss_ptr) if d_loss_ptr else None, int(steps)))
 
     def set_interior_samples(self, aligned, store=False):
         """aligned: render-grid interior samples; store: HITS_RECOMPUTE / HITS_STORE / HITS_AUTO."""

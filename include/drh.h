@@ -153,6 +153,13 @@ drh_status drh_opt_step(drh_handle h, const drh_camera* cams, int views, const f
 /* DRH_OPT_ADAM steps after the first are captured once as a hipGraph and replayed (on by
  * default); the graph is re-captured when the arguments or any device buffer change. */
 drh_status drh_opt_use_graph(drh_handle h, int on);
+/* `steps` consecutive drh_opt_step calls with the same arguments, captured (Adam, graphs on)
+ * as one hipGraph and replayed with one launch: the same kernels in the same order, fewer host
+ * calls per step.  Graphs are kept per step count and per pattern of LBVH builds and refits
+ * (drh_set_bvh_rebuild); the first step after drh_opt_init always runs eagerly.  d_loss
+ * receives the loss of the last step.  No counterpart in the reference. */
+drh_status drh_opt_steps(drh_handle h, const drh_camera* cams, int views, const float* d_targets,
+                         int edge_samples, float* d_loss, int steps);
 /* Download the current (optimised) vertex positions (3V) and colours (3V, may be NULL). */
 drh_status drh_get_params(drh_handle h, float* pos, float* col);
 /* Learning rate of the texture block (OptimizerParameters::textures_lr with _OPT_TEX,

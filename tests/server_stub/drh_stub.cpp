@@ -26,6 +26,7 @@ struct drh_renderer {
 };
 
 std::atomic<int> g_live_handles{0};
+std::atomic<int> g_created_handles{0}, g_reset_handles{0};
 std::atomic<long long> g_steps{0};
 
 static drh_renderer* live(drh_handle h, const char* what) {
@@ -40,6 +41,7 @@ drh_status drh_create(const drh_settings* s, drh_handle* out) {
   if (!s || !out) return DRH_E_INVALID;
   *out = new drh_renderer();
   g_live_handles++;
+  g_created_handles++;
   return DRH_OK;
 }
 void drh_destroy(drh_handle h) {
@@ -101,7 +103,30 @@ drh_status drh_opt_step(drh_handle h, const drh_camera*, int, const float* d_tar
   if (d_loss) *d_loss = (float)h->steps + 0.0f * d_targets[0];
   return DRH_OK;
 }
+drh_status drh_opt_steps(drh_handle h, const drh_camera* c, int v, const float* d_targets, int edge_samples, float* d_loss,
+                         int steps) {
+  for (int k = 0; k < steps; k++) {
+    const drh_status s = drh_opt_step(h, c, v, d_targets, edge_samples, d_loss);
+    if (s != DRH_OK) return s;
+  }
+  return DRH_OK;
+}
 void* drh_stream(drh_handle h) { return &live(h, "drh_stream")->stream; }
+drh_status drh_reset_settings(drh_handle h, const drh_settings* s) {
+  live(h, "drh_reset_settings");
+  if (!s) return DRH_E_INVALID;
+  g_reset_handles++;
+  return DRH_OK;
+}
+// the downloads land when the stream passes them: written here at once, with the values the
+// queued steps leave (the server reads them only after the last launch's event)
+drh_status drh_export_params(drh_handle h, float* pos, float* col, const void* d_loss, float* loss, uint32_t* fault) {
+  live(h, "drh_export_params");
+  for (int k = 0; k < 3 * h->V; k++) { pos[k] = (float)h->steps; col[k] = 0.0f; }
+  std::memcpy(loss, d_loss, 4);
+  *fault = 0;
+  return DRH_OK;
+}
 drh_status drh_sync(drh_handle h) {
   live(h, "drh_sync");
   while (stub_now_ns() < h->stream.busy_until.load()) host_work(20);   // hipStreamSynchronize

@@ -59,4 +59,10 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
   return hipSuccess;
 }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+#define hipHostMallocDefault 0u
+inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
+  *p = std::malloc(bytes);
+  return *p ? hipSuccess : hipErrorInvalidValue;
+}
+inline hipError_t hipHostFree(void* p) { std::free(p); return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }

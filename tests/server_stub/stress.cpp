@@ -10,7 +10,7 @@
 
 #include "../../include/drh.h"
 
-extern std::atomic<int> g_live_handles;
+extern std::atomic<int> g_live_handles, g_created_handles, g_reset_handles;
 
 #define CHECK(c) do { if (!(c)) { fprintf(stderr, "check failed line %d: %s\n", __LINE__, #c); return 1; } } while (0)
 
@@ -100,11 +100,12 @@ int main() {
     drh_server_destroy(s);
     CHECK(g_live_handles.load() == 0);   // every admitted job was torn down
   }
-  // 4. idle handles: jobs of one setting take the last job's handle over; alternating
-  // settings under a budget of one and a half jobs evict the idle handle to fit
+  // 4. idle handles: under a budget of one and a half jobs every job takes the last job's
+  // handle over, those of other settings (alternating spp) resetting it: one handle in all
   {
     drh_job_desc d = tj.desc(4, 64);
     const size_t need = drh_server_job_bytes(&d);
+    const int created0 = g_created_handles.load(), reset0 = g_reset_handles.load();
     drh_server s;
     CHECK(drh_server_create(0, need + need / 2, 4, &s) == DRH_OK);
     std::vector<int> ids;
@@ -126,6 +127,30 @@ int main() {
     CHECK(drh_server_stats(s, &stt) == DRH_OK);
     CHECK(stt.hbm_used <= need + need / 2);
     CHECK(g_live_handles.load() == 1);   // the last job's handle, idle
+    CHECK(g_created_handles.load() - created0 == 1);
+    CHECK(g_reset_handles.load() - reset0 >= 7);   // jobs 8..15 alternate spp 4 / 1
+    drh_server_destroy(s);
+    CHECK(g_live_handles.load() == 0);
+  }
+  // 5. two idle handles of different settings: a job takes the one with its own settings
+  // (no reset) although the other went idle first
+  {
+    drh_job_desc d = tj.desc(4, 64);
+    const size_t need = drh_server_job_bytes(&d);
+    drh_server s;
+    CHECK(drh_server_create(0, 3 * need, 2, &s) == DRH_OK);
+    drh_job_desc a = tj.desc(3, 64), b = tj.desc(3, 64);
+    b.settings.spp = 4;
+    int ia, ib;
+    CHECK(drh_server_submit(s, &a, &ia) == DRH_OK);
+    CHECK(drh_server_submit(s, &b, &ib) == DRH_OK);
+    CHECK(drh_server_wait(s, ia, 60.0, nullptr) == DRH_OK);
+    CHECK(drh_server_wait(s, ib, 60.0, nullptr) == DRH_OK);
+    const int created0 = g_created_handles.load(), reset0 = g_reset_handles.load();
+    int ic;
+    CHECK(drh_server_submit(s, &b, &ic) == DRH_OK);
+    CHECK(drh_server_wait(s, ic, 60.0, nullptr) == DRH_OK);
+    CHECK(g_created_handles.load() == created0 && g_reset_handles.load() == reset0);
     drh_server_destroy(s);
     CHECK(g_live_handles.load() == 0);
   }

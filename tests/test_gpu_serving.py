@@ -255,3 +255,42 @@ def test_texture_learning_rate_fits_albedo():
     assert np.array_equal(pos, m.pos)
     assert losses[-1] < 0.2 * losses[0]
     assert np.all(np.abs(tex - 0.5) < np.abs(0.8 - 0.5) * 0.5)
+
+
+def test_reset_handle_gives_the_fresh_handle_result():
+    """One slot: each job takes the last job's idle handle over, resetting its settings when
+    they differ (mode, spp, light, options).  Deterministic jobs must then give, bit for bit,
+    what the same job gives on a fresh handle in a server of its own."""
+    lam = S.lambert_config2()
+    m, t, cams = _job(3, 96, 51)
+    tg4 = _targets(t, cams, 4, lam)
+    vc = S.with_random_colors(S.icosphere(3), 5)
+    tvc = S.with_random_colors(S.radial_displace(S.icosphere(3), 0.1, seed=52), 5)
+    r = drh.DiffRender(mode=drh.VERTEX_COLOR, spp=1)
+    r.commit([tvc])
+    tg_vc = r.render(cams)
+    r.close()
+    det = drh.JOB_DETERMINISTIC
+    lam_job = dict(meshes=[m], cams=cams, targets=tg4, iterations=6, edge_samples=1 << 13, mode=drh.LAMBERT,
+                   spp=4, light=lam, flags=det)
+    vc_job = dict(meshes=[vc], cams=cams, targets=tg_vc, iterations=5, edge_samples=1 << 12,
+                  mode=drh.VERTEX_COLOR, spp=1, flags=det | drh.JOB_FULL_JACOBIAN)
+
+    def run(jobs):
+        srv = drh.Server(hbm_budget=8 << 30, max_active=1)
+        ids = [srv.submit(**j) for j in jobs]
+        res = [srv.wait(i) for i in ids]
+        srv.close()
+        assert all(x["status"] == 0 for x in res), res
+        return res
+
+    say("fresh")
+    fresh_lam, = run([lam_job])
+    fresh_vc, = run([vc_job])
+    say("reset: VERTEX_COLOR spp 1 -> LAMBERT spp 4 -> VERTEX_COLOR spp 1")
+    a, b, c = run([vc_job, lam_job, vc_job])
+    for got, want in ((a, fresh_vc), (b, fresh_lam), (c, fresh_vc)):
+        assert np.array_equal(got["pos"], want["pos"])
+        assert np.array_equal(got["col"], want["col"])
+        assert got["final_loss"] == want["final_loss"]
+    assert np.abs(b["pos"].reshape(-1, 3) - m.pos).max() > 1e-4   # it fitted

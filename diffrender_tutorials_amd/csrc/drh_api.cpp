@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <tuple>
 #include <memory>
@@ -244,6 +245,53 @@ void keep_pool_memory(int device) {
   done.push_back(device);
 }
 
+// Pinned host blocks (the handles' staging buffers).  hipHostFree waits for the whole
+// device, every stream's queued work included, which stalls a process serving many handles
+// for milliseconds per free; freed blocks are therefore cached by size (powers of two from
+// 64 KiB) for the next handle that needs one, up to PINNED_CACHE_MAX bytes in all.
+constexpr size_t PINNED_CACHE_MAX = 512ull << 20;
+std::mutex g_pinned_mu;
+std::multimap<size_t, char*> g_pinned_free;
+size_t g_pinned_cached = 0;
+
+size_t pinned_class(size_t need) {
+  size_t cap = 64 << 10;
+  while (cap < need) cap *= 2;
+  return cap;
+}
+char* pinned_take(size_t need, size_t* cap) {
+  *cap = pinned_class(need);
+  {
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    auto it = g_pinned_free.find(*cap);
+    if (it != g_pinned_free.end()) {
+      char* p = it->second;
+      g_pinned_free.erase(it);
+      g_pinned_cached -= *cap;
+      return p;
+    }
+  }
+  char* p = nullptr;
+  if (hipHostMalloc((void**)&p, *cap, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+// p must have no transfer in flight
+void pinned_give(char* p, size_t cap) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    if (g_pinned_cached + cap <= PINNED_CACHE_MAX) {
+      g_pinned_free.emplace(cap, p);
+      g_pinned_cached += cap;
+      return;
+    }
+  }
+  (void)hipHostFree(p);
+}
+
 struct ProfEvent { std::string name; hipEvent_t a, b; };
 
 }  // namespace
@@ -332,31 +380,51 @@ struct drh_renderer : BufOwner {
   // per call
   DBuf cams, img, adj, tgt, loss_part, loss, grad;
   DBuf fault;   // the kernels' fault word (DevScene::fault), zero while all is well
+  DBuf job_tgt, job_loss;   // a served job's target images and loss word (drh_job_upload)
   // Host staging: what a commit, a camera change or a handle's own edge-sample table sends to
   // the device is first copied into a pinned buffer of the handle and sent from there by DMA
   // on the stream; the call returns without waiting for the transfer (the caller's arrays
-  // are free at once).  Two buffers alternate; a staging waits (an event) only for the
-  // transfers of the staging before the previous one, long done in practice.
-  struct Stage { char* p = nullptr; size_t cap = 0; hipEvent_t done = nullptr; bool pending = false; };
-  Stage stg[2];
+  // are free at once).  The buffers form a small ring: a staging takes one whose transfers
+  // have passed (asked without waiting), else adds one, and waits (an event) only when
+  // STAGE_MAX are in flight -- in a busy process a handle's transfers queue behind the
+  // kernels that share its stream's hardware queue, so waiting on them would stall the caller.
+  struct Stage { char* p = nullptr; size_t cap = 0; hipEvent_t done = nullptr; bool pending = false; unsigned long long seq = 0; };
+  static constexpr size_t STAGE_MAX = 6;
+  std::vector<Stage> stg;
   int stg_cur = 0;
   size_t stg_used = 0;
+  unsigned long long stg_seq = 0;
   drh_status stage_begin(size_t need) {
-    stg_cur ^= 1;
-    Stage& b = stg[stg_cur];
-    if (b.pending) {
-      if (hipEventSynchronize(b.done) != hipSuccess) return hip(hipGetLastError(), "staging");
-      b.pending = false;
+    int pick = -1, oldest = -1;
+    for (size_t i = 0; i < stg.size(); i++) {
+      Stage& b = stg[i];
+      if (b.pending) {
+        if (hipEventQuery(b.done) != hipSuccess) {
+          (void)hipGetLastError();   // not ready is no error of the next call
+          if (oldest < 0 || b.seq < stg[oldest].seq) oldest = (int)i;
+          continue;
+        }
+        b.pending = false;
+      }
+      if (pick < 0 || (b.cap >= need && stg[pick].cap < need)) pick = (int)i;
     }
+    if (pick < 0 && stg.size() < STAGE_MAX) {
+      stg.emplace_back();
+      pick = (int)stg.size() - 1;
+    }
+    if (pick < 0) {
+      pick = oldest;
+      if (hipEventSynchronize(stg[pick].done) != hipSuccess) return hip(hipGetLastError(), "staging");
+      stg[pick].pending = false;
+    }
+    stg_cur = pick;
+    Stage& b = stg[stg_cur];
     if (need > b.cap) {
-      if (b.p) (void)hipHostFree(b.p);
+      pinned_give(b.p, b.cap);   // its transfers are done
       b.p = nullptr;
       b.cap = 0;
-      const size_t cap = std::max<size_t>(need + need / 4, 1 << 16);
-      if (hipHostMalloc((void**)&b.p, cap, hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(DRH_E_OOM, "pinned staging buffer of %zu bytes", cap);
-      }
+      size_t cap = 0;
+      if (!(b.p = pinned_take(need, &cap))) return fail(DRH_E_OOM, "pinned staging buffer of %zu bytes", pinned_class(need));
       b.cap = cap;
     }
     if (!b.done && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) return hip(hipGetLastError(), "staging");
@@ -415,15 +483,16 @@ struct drh_renderer : BufOwner {
     Stage& b = stg[stg_cur];
     if (hipEventRecord(b.done, stream) != hipSuccess) return hip(hipGetLastError(), "staging");
     b.pending = true;
+    b.seq = ++stg_seq;
     return DRH_OK;
   }
   void stage_free() {
     for (Stage& b : stg) {
       if (b.pending) (void)hipEventSynchronize(b.done);
       if (b.done) (void)hipEventDestroy(b.done);
-      if (b.p) (void)hipHostFree(b.p);
-      b = Stage{};
+      pinned_give(b.p, b.cap);
     }
+    stg.clear();
   }
   const float2* rnd_ptr = nullptr;   // the edge-sample table (shared, or this handle's rnd)
   // Ray counting (drh_count_rays): the traversal kernels' counting instantiations add their
@@ -460,42 +529,8 @@ struct drh_renderer : BufOwner {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<char> graph_sig[2];
   unsigned long long graph_epoch[2] = {0, 0};
-  // Dropping a form does not wait for the stream: its executable and graph are kept, with an
-  // event recorded behind the form's last launch, and destroyed once that event has passed
-  // (at the next drop or capture of the form, long after in practice).
-  hipGraphExec_t stale[2] = {nullptr, nullptr};
-  hipGraph_t stale_graph[2] = {nullptr, nullptr};
-  hipEvent_t stale_ev[2] = {nullptr, nullptr};
-  void retire(int k) {
-    if (!stale[k] && !stale_graph[k]) return;
-    (void)hipEventSynchronize(stale_ev[k]);
-    if (stale[k]) (void)hipGraphExecDestroy(stale[k]);
-    if (stale_graph[k]) (void)hipGraphDestroy(stale_graph[k]);
-    stale[k] = nullptr;
-    stale_graph[k] = nullptr;
-  }
-  void drop_form(int k) {
-    graph_sig[k].clear();
-    if (!gexec[k] && !graph[k]) return;
-    retire(k);
-    if (!stale_ev[k] && hipEventCreateWithFlags(&stale_ev[k], hipEventDisableTiming) != hipSuccess) {
-      (void)hipGetLastError();
-      (void)hipStreamSynchronize(stream);   // no event: wait here instead
-      if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
-      if (graph[k]) (void)hipGraphDestroy(graph[k]);
-      gexec[k] = nullptr;
-      graph[k] = nullptr;
-      return;
-    }
-    (void)hipEventRecord(stale_ev[k], stream);   // behind every launch of this form so far
-    stale[k] = gexec[k];
-    stale_graph[k] = graph[k];
-    gexec[k] = nullptr;
-    graph[k] = nullptr;
-  }
   // k-step graphs (drh_opt_steps): k consecutive fused steps captured as one graph, keyed by k,
-  // the steps' LBVH forms (bit s: full build at step s), the arguments and the epoch.  Dropped
-  // ones wait, as the forms above, for an event behind their last launch before destruction.
+  // the steps' LBVH forms (bit s: full build at step s), the arguments and the epoch.
   struct MultiGraph {
     int k;
     unsigned forms;
@@ -505,46 +540,84 @@ struct drh_renderer : BufOwner {
     hipGraphExec_t e;
   };
   std::vector<MultiGraph> multi;
-  std::vector<MultiGraph> multi_dropped;
-  hipEvent_t multi_ev = nullptr;
   static constexpr size_t MULTI_KEEP = 6;
-  void retire_multi() {
-    if (multi_dropped.empty()) return;
-    (void)hipEventSynchronize(multi_ev);
-    for (auto& m : multi_dropped) { (void)hipGraphExecDestroy(m.e); (void)hipGraphDestroy(m.g); }
-    multi_dropped.clear();
-  }
-  // drop the oldest `n` entries (all with n < 0)
-  void drop_multi(long n = -1) {
-    if (multi.empty()) return;
-    retire_multi();
-    const size_t cnt = n < 0 ? multi.size() : std::min(multi.size(), (size_t)n);
-    if (!multi_ev && hipEventCreateWithFlags(&multi_ev, hipEventDisableTiming) != hipSuccess) {
+  // Dropping graphs does not wait for the stream: the executables and graphs wait, with an
+  // event recorded behind their last launch, and are destroyed once that event has passed
+  // (checked without waiting at every later drop; beyond GRAVES_MAX waiting entries the
+  // oldest is waited for).  A server thread that steps many handles then never blocks on a
+  // handle whose kernels queue behind the other handles' work.
+  struct Grave { hipEvent_t ev; std::vector<hipGraphExec_t> execs; std::vector<hipGraph_t> graphs; };
+  std::deque<Grave> graves;
+  std::vector<hipEvent_t> grave_events;   // recycled
+  static constexpr size_t GRAVES_MAX = 8;
+  void bury(std::vector<hipGraphExec_t>&& execs, std::vector<hipGraph_t>&& graphs) {
+    if (execs.empty() && graphs.empty()) return;
+    hipEvent_t ev = nullptr;
+    if (!grave_events.empty()) {
+      ev = grave_events.back();
+      grave_events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      ev = nullptr;
+    }
+    if (!ev || hipEventRecord(ev, stream) != hipSuccess) {
       (void)hipGetLastError();
-      (void)hipStreamSynchronize(stream);
-      for (size_t i = 0; i < cnt; i++) { (void)hipGraphExecDestroy(multi[i].e); (void)hipGraphDestroy(multi[i].g); }
-      multi.erase(multi.begin(), multi.begin() + cnt);
+      (void)hipStreamSynchronize(stream);   // no event: wait here instead
+      for (auto e : execs) (void)hipGraphExecDestroy(e);
+      for (auto g : graphs) (void)hipGraphDestroy(g);
+      if (ev) grave_events.push_back(ev);
       return;
     }
-    (void)hipEventRecord(multi_ev, stream);
-    multi_dropped.assign(multi.begin(), multi.begin() + cnt);
+    graves.push_back(Grave{ev, std::move(execs), std::move(graphs)});
+    reap(false);
+  }
+  // destroy the waiting entries whose event has passed (with wait: all, waiting for them)
+  void reap(bool wait) {
+    while (!graves.empty()) {
+      Grave& g = graves.front();
+      if (wait || graves.size() > GRAVES_MAX) {
+        (void)hipEventSynchronize(g.ev);
+      } else if (hipEventQuery(g.ev) != hipSuccess) {
+        (void)hipGetLastError();   // not ready is no error of the next call
+        break;
+      }
+      for (auto e : g.execs) (void)hipGraphExecDestroy(e);
+      for (auto x : g.graphs) (void)hipGraphDestroy(x);
+      grave_events.push_back(g.ev);
+      graves.pop_front();
+    }
+  }
+  void drop_form(int k) {
+    graph_sig[k].clear();
+    std::vector<hipGraphExec_t> e;
+    std::vector<hipGraph_t> g;
+    if (gexec[k]) e.push_back(gexec[k]);
+    if (graph[k]) g.push_back(graph[k]);
+    gexec[k] = nullptr;
+    graph[k] = nullptr;
+    bury(std::move(e), std::move(g));
+  }
+  // drop the oldest `n` k-step graphs (all with n < 0)
+  void drop_multi(long n = -1) {
+    const size_t cnt = n < 0 ? multi.size() : std::min(multi.size(), (size_t)n);
+    std::vector<hipGraphExec_t> e;
+    std::vector<hipGraph_t> g;
+    for (size_t i = 0; i < cnt; i++) {
+      e.push_back(multi[i].e);
+      g.push_back(multi[i].g);
+    }
     multi.erase(multi.begin(), multi.begin() + cnt);
+    bury(std::move(e), std::move(g));
   }
   void drop_graph() { drop_form(0); drop_form(1); drop_multi(); }
   void destroy_graphs() {
     drop_graph();
-    for (int k = 0; k < 2; k++) {
-      retire(k);
-      if (stale_ev[k]) (void)hipEventDestroy(stale_ev[k]);
-      stale_ev[k] = nullptr;
-    }
-    retire_multi();
-    if (multi_ev) (void)hipEventDestroy(multi_ev);
-    multi_ev = nullptr;
+    reap(true);
+    for (auto ev : grave_events) (void)hipEventDestroy(ev);
+    grave_events.clear();
   }
-  // executable for captured graph g of form k (the dropped one of that form destroyed first)
+  // executable for captured graph g of form k
   bool instantiate(int k, hipGraph_t g) {
-    retire(k);
+    reap(false);
     return hipGraphInstantiate(&gexec[k], g, nullptr, nullptr, 0) == hipSuccess;
   }
   // profiling
@@ -1313,13 +1386,15 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
       hipEventCreateWithFlags(&r->ev_ijoin, hipEventDisableTiming) != hipSuccess)
     return DRH_E_HIP;
   apply_settings(*r, *settings);
+  // queued on the new stream, no wait: in a busy process a synchronisation here waits for
+  // whatever shares the stream's hardware queue
   std::vector<float> hm;
   hammersley(settings->spp, hm);
-  if (r->hamm.reserve(4 * hm.size()) != hipSuccess) return DRH_E_OOM;
-  if (hipMemcpyAsync(r->hamm.p, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice, r->stream) != hipSuccess ||
-      hipStreamSynchronize(r->stream) != hipSuccess || r->loss.reserve(16) != hipSuccess ||
-      r->fault.reserve(16) != hipSuccess || hipMemsetAsync(r->fault.p, 0, 16, r->stream) != hipSuccess ||
-      hipStreamSynchronize(r->stream) != hipSuccess)
+  if (r->hamm.reserve(4 * hm.size()) != hipSuccess || r->loss.reserve(16) != hipSuccess ||
+      r->fault.reserve(16) != hipSuccess)
+    return DRH_E_OOM;
+  if (r->stage_begin(drh_renderer::stage_round(4 * hm.size())) || r->stage_send(r->hamm.p, hm.data(), 4 * hm.size()) ||
+      r->stage_end() || hipMemsetAsync(r->fault.p, 0, 16, r->stream) != hipSuccess)
     return DRH_E_HIP;
   *out = r.release();
   return DRH_OK;
@@ -1694,7 +1769,6 @@ drh_status drh_opt_steps(drh_handle h, const drh_camera* cams, int views, const 
       return DRH_OK;
     }
   if (h->multi.size() >= drh_renderer::MULTI_KEEP) h->drop_multi(1);
-  h->retire_multi();
   if ((s = h->hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return s;
   drh_status body = DRH_OK;
   for (int k = 0; k < steps && body == DRH_OK; k++) {
@@ -1738,6 +1812,19 @@ drh_status drh_export_params(drh_handle h, float* pos, float* col, const void* d
       (s = h->hip(hipMemcpyAsync(loss, d_loss, 4, hipMemcpyDeviceToHost, h->stream), "export")) ||
       (s = h->hip(hipMemcpyAsync(fault, h->fault.p, 4, hipMemcpyDeviceToHost, h->stream), "export")))
     return s;
+  return DRH_OK;
+}
+
+drh_status drh_job_upload(drh_handle h, const void* targets, size_t bytes, void** d_targets, void** d_loss) {
+  if (!h || (bytes && !targets) || !d_targets || !d_loss) return DRH_E_INVALID;
+  (void)hipSetDevice(h->st.device);
+  drh_status s;
+  if ((s = h->hip(h->job_tgt.reserve(std::max<size_t>(bytes, 4)), "alloc")) || (s = h->hip(h->job_loss.reserve(16), "alloc")) ||
+      (s = h->stage_begin(drh_renderer::stage_round(bytes))) || (s = h->stage_send(h->job_tgt.p, targets, bytes)) ||
+      (s = h->stage_end()))
+    return s;
+  *d_targets = h->job_tgt.p;
+  *d_loss = h->job_loss.p;
   return DRH_OK;
 }
 

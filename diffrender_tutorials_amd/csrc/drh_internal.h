@@ -21,3 +21,10 @@ extern "C" __attribute__((visibility("hidden"))) drh_status drh_reset_settings(d
 extern "C" __attribute__((visibility("hidden"))) drh_status drh_export_params(drh_handle h, float* pos, float* col,
                                                                               const void* d_loss, float* loss,
                                                                               uint32_t* fault);
+
+// A served job's target images (`bytes` from `targets`) and loss word in buffers the handle
+// owns (kept and regrown across jobs, freed with the handle): allocation ordered on the
+// handle's stream and the upload staged, without waiting.
+extern "C" __attribute__((visibility("hidden"))) drh_status drh_job_upload(drh_handle h, const void* targets,
+                                                                           size_t bytes, void** d_targets,
+                                                                           void** d_loss);

@@ -69,8 +69,7 @@ struct Job {
   size_t host_out_cap = 0;
   bool exported = false;
   size_t host_out_bytes() const { return 4 * (6 * (size_t)V() + 2); }
-  void* d_targets = nullptr;
-  size_t targets_cap = 0;   // bytes allocated at d_targets (an idle handle's may be larger)
+  void* d_targets = nullptr;   // the handle's job buffers (drh_job_upload)
   void* d_loss = nullptr;
   int issued = 0, completed = 0;
   struct Launch { hipEvent_t a, b; int steps; };
@@ -121,15 +120,10 @@ bool same_settings(const drh_settings& a, const drh_settings& b) {
 struct Idle {
   drh_handle h;
   drh_settings st;
-  void* d_targets;
-  size_t targets_cap;
-  void* d_loss;
   size_t bytes;   // charged to the budget while idle: the footprint of its last job
 };
 
 void destroy_idle(const Idle& e) {
-  (void)drh_free(e.h, e.d_targets);
-  (void)drh_free(e.h, e.d_loss);
   drh_destroy(e.h);
 }
 
@@ -244,7 +238,7 @@ void drh_server_s::finish(Job* j, int status, const std::string& err) {
     j->inflight.clear();
     // a healthy handle (its last step completed: the downloads above waited for it) goes idle
     // for the next job; the oldest idle one makes room if need be
-    const Idle e{j->h, j->st, j->d_targets, j->targets_cap, j->d_loss, j->bytes + j->hit_bytes};
+    const Idle e{j->h, j->st, j->bytes + j->hit_bytes};
     std::vector<Idle> drop;
     if (status == 1 && (j->exported || drh_sync(j->h) == DRH_OK)) {
       std::lock_guard<std::mutex> lk(mu);
@@ -308,13 +302,6 @@ bool drh_server_s::admit_one(Job* j, double phase[5]) {
     (void)drh_set_hit_buffer_limit(h, ~(size_t)0);
   }
   const drh_scene_desc d = j->desc();
-  const size_t tbytes = 4 * j->targets.size();
-  if (j->d_targets && j->targets_cap < tbytes) { (void)drh_free(h, j->d_targets); j->d_targets = nullptr; }
-  if (!j->d_targets) {
-    if ((s = drh_malloc(h, tbytes, &j->d_targets)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
-    j->targets_cap = tbytes;
-  }
-  if (!j->d_loss && (s = drh_malloc(h, 16, &j->d_loss)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
   {
     const size_t need = j->host_out_bytes();
     {
@@ -337,17 +324,19 @@ bool drh_server_s::admit_one(Job* j, double phase[5]) {
       j->host_out_cap = cap;
     }
   }
-  // every upload first, while the handle's stream is still empty (their synchronisations then
-  // wait for nothing): targets, cameras, edge-sample table; the commit's LBVH build and the
-  // optimiser set-up are left queued behind them, so neither admission nor the job's first
-  // step waits for kernels that queue behind the other jobs' steps
+  // uploads (targets, cameras, edge-sample table), the commit's LBVH build and the optimiser
+  // set-up are all queued on the handle's stream without a synchronisation: in a busy server
+  // one would wait for whatever shares the stream's hardware queue
   auto lap = [&](int k) {
     const auto now = std::chrono::steady_clock::now();
     phase[k] += std::chrono::duration<double>(now - phase_t).count();
     phase_t = now;
   };
   lap(0);
-  if ((s = drh_memcpy_htod(h, j->d_targets, j->targets.data(), tbytes)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
+  if ((s = drh_job_upload(h, j->targets.data(), 4 * j->targets.size(), &j->d_targets, &j->d_loss)) != DRH_OK) {
+    finish(j, s, drh_last_error(h));
+    return false;
+  }
   lap(1);
   if ((s = drh_prepare_steps(h, j->cams.data(), (int)j->cams.size(), j->edge_samples)) != DRH_OK) { finish(j, s, drh_last_error(h)); return false; }
   lap(2);
@@ -397,9 +386,6 @@ void drh_server_s::admit_run() {
         idle_bytes -= e.bytes;
         used -= std::min(used, e.bytes);
         j->h = e.h;
-        j->d_targets = e.d_targets;
-        j->targets_cap = e.targets_cap;
-        j->d_loss = e.d_loss;
         j->bytes = std::max(j->bytes, e.bytes);   // its buffers are at least the last job's
         handles_reused++;
         if (j->reset_handle) handles_reset++;

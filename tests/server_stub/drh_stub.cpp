@@ -10,6 +10,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include <hip/hip_runtime.h>
 
@@ -23,6 +24,8 @@ struct drh_renderer {
   int steps = 0;
   int calls = 0;           // written by every call (unsynchronised on purpose)
   std::string err;
+  std::vector<float> job_tgt;
+  float job_loss[4] = {0, 0, 0, 0};
 };
 
 std::atomic<int> g_live_handles{0};
@@ -125,6 +128,14 @@ drh_status drh_export_params(drh_handle h, float* pos, float* col, const void* d
   for (int k = 0; k < 3 * h->V; k++) { pos[k] = (float)h->steps; col[k] = 0.0f; }
   std::memcpy(loss, d_loss, 4);
   *fault = 0;
+  return DRH_OK;
+}
+drh_status drh_job_upload(drh_handle h, const void* targets, size_t bytes, void** d_targets, void** d_loss) {
+  live(h, "drh_job_upload");
+  h->job_tgt.resize(bytes / 4 + 1);
+  std::memcpy(h->job_tgt.data(), targets, bytes);
+  *d_targets = h->job_tgt.data();
+  *d_loss = h->job_loss;
   return DRH_OK;
 }
 drh_status drh_sync(drh_handle h) {

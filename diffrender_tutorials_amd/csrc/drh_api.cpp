@@ -692,7 +692,22 @@ struct drh_renderer : BufOwner {
   Tracer tracer_view(int v) const {
     Tracer t = tracer();
     if (!t.use_bf && F > 0) t.cam = {sus_view.as<float4>() + (size_t)v * F, sus_view_cnt.as<int>() + v};
+    t.s.lane_walk = lane_walk_cam(v);
     return t;
+  }
+  // LBVH walk per launch (drh_set_traversal).  A wave's camera rays cover a tile of 64
+  // pixels: with many triangles per pixel the nodes the packet opens together are many times
+  // those of one ray, and each lane walking alone is faster.  The edge pass's side rays are
+  // as coherent as its samples are dense on the edges.
+  int traversal = DRH_TRAVERSAL_AUTO;
+  static constexpr double LANE_TRIS_PER_PIXEL = 0.5, LANE_EDGES_PER_SAMPLE = 0.5;
+  int lane_walk_cam(int v) const {
+    if (traversal != DRH_TRAVERSAL_AUTO) return traversal == DRH_TRAVERSAL_LANE;
+    return (double)F > LANE_TRIS_PER_PIXEL * (double)cam_cache[v].w * (double)cam_cache[v].h;
+  }
+  int lane_walk_edge(long long n) const {
+    if (traversal != DRH_TRAVERSAL_AUTO) return traversal == DRH_TRAVERSAL_LANE;
+    return (double)E > LANE_EDGES_PER_SAMPLE * (double)n;
   }
   // Suspect triangles of every view's camera origin (CastSingleRay's origin,
   // WorldViewInv * (0,0,0,1), raytrace.h:67-76).
@@ -1247,6 +1262,7 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
       }
       EdgeArgs ea{};
       ea.tr = tracer_view(v); ea.tr.rays = ray_slot(2); ea.light = light; ea.cams = cams.as<DevCam>(); ea.view = v; ea.N = N; ea.E = E;
+      ea.tr.s.lane_walk = lane_walk_edge(N);
       ea.edges = edges.as<int2>(); ea.edge_mesh = edge_mesh.as<int>(); ea.pmf = pmf.as<float>(); ea.cdf = cdf.as<float>();
       ea.proj = proj.as<float4>();
       ea.rnd = rnd_ptr;
@@ -1347,6 +1363,7 @@ drh_status drh_reset_settings(drh_handle h, const drh_settings* settings) {
   h->edge_order = DRH_EDGE_ORDER_AUTO;
   h->edge_grad_mode = DRH_EDGE_GRAD_REFERENCE_DIAGONAL;
   h->side_test = DRH_SIDE_TEST_REFERENCE;
+  h->traversal = DRH_TRAVERSAL_AUTO;
   h->deterministic = false;
   h->interior_pos = false;
   h->use_graph = true;
@@ -1720,6 +1737,15 @@ drh_status drh_set_pixel_jitter(drh_handle h, int on, unsigned seed) {
   if ((on != 0) != (h->jit_on != 0) || seed != h->jit_seed) h->drop_graph();   // baked into the captured step
   h->jit_on = on != 0;
   h->jit_seed = seed;
+  return DRH_OK;
+}
+
+drh_status drh_set_traversal(drh_handle h, int mode) {
+  if (!h) return DRH_E_INVALID;
+  if (mode != DRH_TRAVERSAL_AUTO && mode != DRH_TRAVERSAL_WAVE && mode != DRH_TRAVERSAL_LANE)
+    return h->fail(DRH_E_INVALID, "set_traversal: unknown mode %d", mode);
+  if (mode != h->traversal) h->drop_graph();   // baked into the captured step's arguments
+  h->traversal = mode;
   return DRH_OK;
 }
 

@@ -135,6 +135,10 @@ struct DevScene {
   const int* __restrict__ tex_h;
   const int* __restrict__ tex_off;
   int n_meshes;
+  // 1: every lane walks the LBVH on its own (trace_bvh_lane) -- for incoherent rays, where a
+  // wave's rays open many more nodes together than each of them alone; 0: the wave walks it
+  // as one packet (trace_bvh).  Chosen by the host per launch.
+  int lane_walk;
   // Device-side fault word (bit 0: a traversal stack overflowed and a subtree was skipped);
   // read back by the host at its next synchronisation and reported as DRH_E_DEVICE.
   int* fault;
@@ -243,6 +247,80 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
   return tmin <= tmx;
 }
 
+// LBVH walk, one ray per lane: each lane keeps its own stack (private memory, 64 entries for
+// the same depth bound as below) and opens only the boxes its own ray passes, nearer child
+// first; a popped subtree whose entry distance is beyond the lane's closest hit so far is
+// skipped.  Same hits as the wavefront walk below: the (t, triangle id) rule picks the same
+// triangle from any superset of the triangles the lane's own boxes open.
+template <bool ANY, bool WT>
+__device__ __forceinline__ Hit trace_bvh_lane(const DevScene& s, F3 o, F3 d, float tNear, float tFar, bool retired) {
+  Hit h{NO_HIT, tFar, 0.0f, 0.0f};
+  if (retired) return h;
+  const F3 id = {slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z)};
+  const F3 nod = {-(o.x * id.x), -(o.y * id.y), -(o.z * id.z)};
+  const WtRay wr = WT ? wt_ray(d) : WtRay{};
+  float lim = tFar * 1.00002f + 1e-6f;
+  auto leaf = [&](int slot) -> bool {   // true: an any-hit ray is done
+    const float4 A = s.tri_a[slot];
+    float t, u, v;
+    bool hit;
+    if (WT) {
+      const uint32_t gg = __float_as_uint(A.w);
+      hit = wt_test(wr, o, vertex_of(s, gg, 0), vertex_of(s, gg, 1), vertex_of(s, gg, 2), tNear, tFar, t, u, v);
+    } else {
+      hit = mt_test(A, s.tri_e1[slot], s.tri_e2[slot], o, d, tNear, t, u, v);
+    }
+    if (!hit) return false;
+    const uint32_t g = __float_as_uint(A.w);
+    if (ANY) {
+      if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; return true; }
+    } else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) {
+      h.gtri = g; h.t = t; h.u = u; h.v = v;
+      lim = t * 1.00002f + 1e-6f;
+    }
+    return false;
+  };
+  if (s.F == 1) { leaf(0); return h; }
+  int stk[64];
+  float stt[64];
+  int sp = 0;
+  int node = 0;
+  for (int visits = 0; visits < s.F; visits++) {
+    const float4 n0 = s.nodes[4 * node], n1 = s.nodes[4 * node + 1];
+    const float4 n2 = s.nodes[4 * node + 2], n3 = s.nodes[4 * node + 3];
+    const int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
+    float tl, tr;
+    bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, id, nod, lim, tl);
+    bool hr = slab(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, id, nod, lim, tr);
+    if (hl && cl < 0) { if (leaf(~cl)) break; hl = false; }
+    if (hr && cr < 0) { if (leaf(~cr)) break; hr = false; }
+    if (hl && hr) {
+      const bool lfirst = tl <= tr;
+      if (sp < 64) {
+        stk[sp] = lfirst ? cr : cl;
+        stt[sp] = lfirst ? tr : tl;
+        sp++;
+      } else if (s.fault) {
+        atomicOr(s.fault, FAULT_STACK_OVERFLOW);
+      }
+      node = lfirst ? cl : cr;
+    } else if (hl) {
+      node = cl;
+    } else if (hr) {
+      node = cr;
+    } else {
+      // pop the next subtree still in front of the closest hit
+      bool found = false;
+      while (sp > 0) {
+        sp--;
+        if (stt[sp] <= lim) { node = stk[sp]; found = true; break; }
+      }
+      if (!found) break;
+    }
+  }
+  return h;
+}
+
 // LBVH walk, wavefront-coherent: the active lanes of a wave traverse together.  The node
 // index is wave-uniform, so a node's two child boxes and links arrive by one scalar load
 // for all 64 rays; each lane tests its own ray (per-lane closest-hit culling through `lim`),
@@ -266,6 +344,7 @@ template <bool ANY, bool WT = false>
 __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar, int* wst) {
   if (s.F == 0) return Hit{NO_HIT, tFar, 0.0f, 0.0f};
   const bool retired = o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z;
+  if (s.lane_walk) return trace_bvh_lane<ANY, WT>(s, o, d, tNear, tFar, retired);
   Hit h{NO_HIT, tFar, 0.0f, 0.0f};
   const F3 id = {slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z)};
   const F3 nod = {-(o.x * id.x), -(o.y * id.y), -(o.z * id.z)};

@@ -194,6 +194,12 @@ struct drh_server_s {
   // optimiser set-up); printed at destroy when DRH_SERVER_TRACE is set
   double phase_s[5] = {0, 0, 0, 0, 0};
   int handles_created = 0, handles_reused = 0, handles_reset = 0, handles_evicted = 0;
+  // DRH_SERVER_TIMELINE=<path>: every launch's device interval (HIP events against one
+  // reference event recorded at start), written to <path> as CSV at destroy
+  struct Span { int job, steps; float t0, t1; };
+  std::string timeline_path;
+  hipEvent_t t_ref = nullptr;
+  std::vector<Span> timeline;   // stepper only
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   void run();         // stepping thread
@@ -474,6 +480,10 @@ void drh_server_s::run() {
         const Job::Launch l = j->inflight.front();
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, l.a, l.b);
+        if (t_ref) {
+          float t0 = 0.0f;
+          if (hipEventElapsedTime(&t0, t_ref, l.a) == hipSuccess) timeline.push_back(Span{j->id, l.steps, t0, t0 + ms});
+        }
         spare_events.push_back(l.a);
         spare_events.push_back(l.b);
         j->inflight.pop_front();
@@ -554,6 +564,16 @@ drh_status drh_server_create(int device, size_t hbm_budget_bytes, int max_active
   s->device = device;
   s->budget = hbm_budget_bytes;
   s->max_active = max_active;
+  if (const char* tl = std::getenv("DRH_SERVER_TIMELINE")) {
+    (void)hipSetDevice(device);
+    if (hipEventCreate(&s->t_ref) == hipSuccess && hipEventRecord(s->t_ref, nullptr) == hipSuccess) {
+      s->timeline_path = tl;
+    } else {
+      (void)hipGetLastError();
+      if (s->t_ref) (void)hipEventDestroy(s->t_ref);
+      s->t_ref = nullptr;
+    }
+  }
   s->worker = std::thread([s] { s->run(); });
   for (int k = 0, n = admitter_threads(); k < n; k++) s->admitters.emplace_back([s] { s->admit_run(); });
   s->reaper = std::thread([s] { s->reap_run(); });
@@ -687,6 +707,14 @@ void drh_server_destroy(drh_server s) {
   s->idle.clear();
   for (auto& p : s->pinned_free) (void)hipHostFree(p.second);
   s->pinned_free.clear();
+  if (s->t_ref) {
+    if (FILE* f = std::fopen(s->timeline_path.c_str(), "w")) {
+      std::fprintf(f, "job,steps,start_ms,end_ms\n");
+      for (const auto& sp : s->timeline) std::fprintf(f, "%d,%d,%.4f,%.4f\n", sp.job, sp.steps, sp.t0, sp.t1);
+      std::fclose(f);
+    }
+    (void)hipEventDestroy(s->t_ref);
+  }
   delete s;
 }
 

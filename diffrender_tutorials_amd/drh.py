@@ -29,6 +29,7 @@ EDGE_GRAD_REFERENCE_DIAGONAL, EDGE_GRAD_FULL_JACOBIAN = 0, 1
 OPT_SIMPLE, OPT_ADAM = 0, 1
 HITS_RECOMPUTE, HITS_STORE, HITS_AUTO = 0, 1, 2
 SIDE_TEST_REFERENCE, SIDE_TEST_WATERTIGHT = 0, 1
+TRAVERSAL_AUTO, TRAVERSAL_WAVE, TRAVERSAL_LANE = 0, 1, 2
 JOB_DETERMINISTIC, JOB_ALIGNED_INTERIOR, JOB_FULL_JACOBIAN = 1, 2, 4
 
 STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_E_HIP", -4: "DRH_E_OOM",
@@ -38,7 +39,7 @@ STATUS = {0: "DRH_OK", -1: "DRH_E_INVALID", -2: "DRH_E_NOT_COMMITTED", -3: "DRH_
 EXPORTS = ["drh_create", "drh_destroy", "drh_last_error", "drh_commit", "drh_grad_size", "drh_render",
            "drh_d_render", "drh_d_render_and_compare", "drh_render_dev", "drh_d_render_dev",
            "drh_d_render_and_compare_dev", "drh_opt_init", "drh_opt_step", "drh_get_params",
-           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_restricted", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples", "drh_set_pixel_jitter", "drh_set_edge_side_test", "drh_opt_steps",
+           "drh_opt_set_texture_lr", "drh_opt_set_transform_lr", "drh_get_transforms", "drh_get_restricted", "drh_get_texels", "drh_set_device_sharing", "drh_set_interior_samples", "drh_set_pixel_jitter", "drh_set_edge_side_test", "drh_opt_steps", "drh_set_traversal",
            "drh_stream",
            "drh_sync", "drh_profile_enable", "drh_profile_read", "drh_debug_hits", "drh_debug_bvh",
            "drh_debug_edges", "drh_malloc", "drh_free", "drh_memcpy_htod", "drh_memcpy_dtoh",
@@ -234,6 +235,11 @@ class DiffRender:
         """SIDE_TEST_REFERENCE (Moller-Trumbore, the default) or SIDE_TEST_WATERTIGHT for the edge
         samples' side rays."""
         self._check(load().drh_set_edge_side_test(self.h, int(test)))
+
+    def set_traversal(self, mode):
+        """TRAVERSAL_AUTO (default), TRAVERSAL_WAVE (a wave walks the LBVH as one packet) or
+        TRAVERSAL_LANE (each ray walks alone); the hits are the same in every mode."""
+        self._check(load().drh_set_traversal(self.h, int(mode)))
 
     def set_pixel_jitter(self, on, seed=0):
         """Stratified jitter of the pixel samples with pattern `seed` (the fused step's t-th step

@@ -274,6 +274,19 @@ drh_status drh_set_edge_grad_mode(drh_handle h, int mode);
  * in the reference's interface. */
 enum { DRH_SIDE_TEST_REFERENCE = 0, DRH_SIDE_TEST_WATERTIGHT = 1 };
 drh_status drh_set_edge_side_test(drh_handle h, int test);
+/* How rays walk the LBVH.
+ *   DRH_TRAVERSAL_AUTO (default): per launch, from the rays' expected coherence -- camera
+ *     rays walk per lane when the scene has more than 0.5 triangles per pixel of the view,
+ *     the edge pass's side rays when there are more than 0.5 edges per edge sample;
+ *   DRH_TRAVERSAL_WAVE: a wave's 64 rays walk the tree as one packet (shared node loads and
+ *     stack), fastest for coherent rays;
+ *   DRH_TRAVERSAL_LANE: each ray walks alone (own stack), fastest when a packet's rays open
+ *     many different subtrees.
+ * Every mode returns the same hits bit for bit (the closest hit with the (t, triangle id)
+ * rule does not depend on the order the triangles are tested in).  No counterpart in the
+ * reference's interface (its tracer is brute force, raytrace3d_bf.cpp). */
+enum { DRH_TRAVERSAL_AUTO = 0, DRH_TRAVERSAL_WAVE = 1, DRH_TRAVERSAL_LANE = 2 };
+drh_status drh_set_traversal(drh_handle h, int mode);
 /* How many handles share the device (1 by default).  A handle alone runs view 0's edge
  * preparation on a second stream beside the render; when several share the device (the
  * server sets its slot count) that overlap comes from the other handles instead and the

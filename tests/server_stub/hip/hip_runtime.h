@@ -59,6 +59,7 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
   return hipSuccess;
 }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
 #define hipHostMallocDefault 0u
 inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
   *p = std::malloc(bytes);

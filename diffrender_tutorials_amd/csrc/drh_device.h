@@ -137,13 +137,18 @@ struct DevScene {
   int n_meshes;
   // 1: every lane walks the LBVH on its own (trace_bvh_lane) -- for incoherent rays, where a
   // wave's rays open many more nodes together than each of them alone; 0: the wave walks it
-  // as one packet (trace_bvh).  Chosen by the host per launch.
+  // as one packet (trace_bvh).  Chosen by the host per launch; the launchers pick the
+  // kernels' TF_LANE instantiation by it (the walk is a compile-time choice on the device, so
+  // the packet walk's code is what it is without the other).
   int lane_walk;
   // Device-side fault word (bit 0: a traversal stack overflowed and a subtree was skipped);
   // read back by the host at its next synchronisation and reported as DRH_E_DEVICE.
   int* fault;
 };
 constexpr int FAULT_STACK_OVERFLOW = 1;
+// Trace flags, a template parameter of trace / shade and the kernels that cast rays:
+// TF_WT the watertight triangle test, TF_LANE the per-lane LBVH walk.
+constexpr int TF_WT = 1, TF_LANE = 2;
 
 struct Hit { uint32_t gtri; float t, u, v; };
 
@@ -344,7 +349,6 @@ template <bool ANY, bool WT = false>
 __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tNear, float tFar, int* wst) {
   if (s.F == 0) return Hit{NO_HIT, tFar, 0.0f, 0.0f};
   const bool retired = o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z;
-  if (s.lane_walk) return trace_bvh_lane<ANY, WT>(s, o, d, tNear, tFar, retired);
   Hit h{NO_HIT, tFar, 0.0f, 0.0f};
   const F3 id = {slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z)};
   const F3 nod = {-(o.x * id.x), -(o.y * id.y), -(o.z * id.z)};
@@ -501,16 +505,28 @@ __device__ __forceinline__ Hit trace_bf_wt(const DevScene& s, F3 o, F3 d, float 
   return h;
 }
 
-template <bool ANY, bool COUNT = false, bool WT = false>
+// LBVH walk of trace flags TF (packet or per lane, MT or watertight test)
+template <bool ANY, int TF>
+__device__ __forceinline__ Hit walk(const DevScene& s, F3 o, F3 d, float tNear, float tFar, int* wst) {
+  constexpr bool WT = (TF & TF_WT) != 0;
+  if (TF & TF_LANE) {
+    if (s.F == 0) return Hit{NO_HIT, tFar, 0.0f, 0.0f};
+    const bool retired = o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z;
+    return trace_bvh_lane<ANY, WT>(s, o, d, tNear, tFar, retired);
+  }
+  return trace_bvh<ANY, WT>(s, o, d, tNear, tFar, wst);
+}
+
+template <bool ANY, bool COUNT = false, int TF = 0>
 __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* wst) {
   count_rays<COUNT>(T.rays + (ANY ? 1 : 0), !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z));
-  if (WT) {
+  if (TF & TF_WT) {
     // no clamped determinants to emulate: no suspect lists
     if (T.use_bf) return trace_bf_wt<ANY>(T.s, o, d, tNear, tFar);
-    return trace_bvh<ANY, true>(T.s, o, d, tNear, tFar, wst);
+    return walk<ANY, TF>(T.s, o, d, tNear, tFar, wst);
   }
   if (T.use_bf) return trace_bf<ANY>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, tNear, tFar);
-  Hit h = trace_bvh<ANY>(T.s, o, d, tNear, tFar, wst);
+  Hit h = walk<ANY, TF>(T.s, o, d, tNear, tFar, wst);
   if (ANY && h.gtri != NO_HIT) return h;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
   test_suspects<ANY>(T, ANY ? T.light : T.cam, o, d, tNear, tFar, h);
@@ -520,11 +536,11 @@ __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, f
 // Does a camera ray hit anything?  Any-hit traversal with the view's suspect list: the same
 // yes / no as the closest-hit query (both accept the first triangle whose MT test passes with
 // 0 < t < tFar; the closest-hit query only keeps looking for a nearer one).
-template <bool COUNT = false>
+template <bool COUNT = false, int TF = 0>
 __device__ __forceinline__ bool covered(const Tracer& T, F3 o, F3 d, int* wst) {
   count_rays<COUNT>(T.rays + 1, !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z));
   if (T.use_bf) return trace_bf<true>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, 0.0f, 1e9f).gtri != NO_HIT;
-  Hit h = trace_bvh<true>(T.s, o, d, 0.0f, 1e9f, wst);
+  Hit h = walk<true, TF & TF_LANE>(T.s, o, d, 0.0f, 1e9f, wst);
   if (h.gtri != NO_HIT) return true;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return false;
   test_suspects<true>(T, T.cam, o, d, 0.0f, 1e9f, h);
@@ -598,7 +614,7 @@ constexpr int MODE_LAMBERT_VCOL = 8;
 // pos + (t - 1e-5) dir toward -L hits anything.  The lit colour is (shadow * Lc) * ndl.  With
 // ndl == +0 and a finite light colour the shadow ray cannot change that product (bit for bit),
 // so it is only cast where ndl > 0 (shadow stays 1 otherwise).
-template <bool COUNT = false, bool WT = false>
+template <bool COUNT = false, int TF = 0>
 __device__ __forceinline__ void lambert_terms(const Tracer& T, const Light& L, F3 o, F3 d, const Hit& h,
                                               uint32_t A, uint32_t B, uint32_t C, const float* nrm, int* wst,
                                               float& shadow, float& ndl) {
@@ -609,7 +625,7 @@ __device__ __forceinline__ void lambert_terms(const Tracer& T, const Light& L, F
   shadow = 1.0f;
   if (ndl != 0.0f || !(fabsf(L.Lc.x) <= 3.4e38f && fabsf(L.Lc.y) <= 3.4e38f && fabsf(L.Lc.z) <= 3.4e38f)) {
     const F3 surf = add(o, mul(h.t - BIAS, d));
-    const Hit sh = trace<true, COUNT, WT>(T, surf, negL, 0.0f, 1e9f, wst);
+    const Hit sh = trace<true, COUNT, TF>(T, surf, negL, 0.0f, 1e9f, wst);
     shadow = sh.gtri == NO_HIT ? 1.0f : 0.0f;
   }
 }
@@ -629,12 +645,12 @@ __device__ __forceinline__ void lit_decode(float w, float& shadow, float& ndl) {
 // shade<m>(screen_pos) for SIL / VCOL / TEXTURE / LAMBERT / LAMBERT_VCOL.  `nrm` is the 3D
 // scene's normals for render and the projected scene's for edge samples (src/drender.h:353-354).
 // lit_out (LAMBERT_VCOL, may be NULL) receives the hit's light terms, lit_encode'd.
-template <int MODE, bool COUNT = false, bool WT = false>
+template <int MODE, bool COUNT = false, int TF = 0>
 __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCam& c, float sx, float sy,
                                     const float* nrm, int* wst, Hit* hit_out = nullptr, float* lit_out = nullptr) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
-  const Hit h = trace<false, COUNT, WT>(T, o, d, 0.0f, 1e9f, wst);
+  const Hit h = trace<false, COUNT, TF>(T, o, d, 0.0f, 1e9f, wst);
   if (hit_out) *hit_out = h;
   if (h.gtri == NO_HIT) return {0, 0, 0};
   if (MODE == 1) return {1, 1, 1};
@@ -651,7 +667,7 @@ __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCa
     if (MODE == 3) return diffuse;
   }
   float shadow, ndl;
-  lambert_terms<COUNT, WT>(T, L, o, d, h, A, B, C, nrm, wst, shadow, ndl);
+  lambert_terms<COUNT, TF>(T, L, o, d, h, A, B, C, nrm, wst, shadow, ndl);
   if (MODE == MODE_LAMBERT_VCOL && lit_out) *lit_out = lit_encode(shadow, ndl);
   return mulv(lambert_factor(L, shadow, ndl), diffuse);
 }

@@ -455,7 +455,7 @@ __device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
   return tile_map_xy(sq, tile % tiles_x, tile / tiles_x, lane);
 }
 
-template <int MODE, bool COUNT = false>
+template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   __shared__ int wst[64];   // the wave's traversal stack
   const Tracer& T = a.tr;
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
       sample_offset(a.jit, a.view, x, y, tm.s, sq, xo, yo);
       Hit h;
       float lit = 0.0f;
-      col = shade<MODE, COUNT>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
+      col = shade<MODE, COUNT, TF>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
       if (a.hits)
         a.hits[(c.img_off / 3 + (long long)y * c.w + x) * tm.S + tm.s] =
             make_float4(__uint_as_float(h.gtri), h.u, h.v, MODE == MODE_LAMBERT_VCOL ? lit : h.t);
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
         sample_offset(a.jit, a.view, x, y, dy * sq + dx, sq, xo, yo);
         Hit h;
         float lit = 0.0f;
-        const F3 col = shade<MODE, COUNT>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
+        const F3 col = shade<MODE, COUNT, TF>(T, a.light, c, x + xo, y + yo, a.tr.s.nrm, wst, &h, &lit);
         if (a.hits)
           a.hits[(c.img_off / 3 + (long long)y * c.w + x) * (sq * sq) + dy * sq + dx] =
               make_float4(__uint_as_float(h.gtri), h.u, h.v, MODE == MODE_LAMBERT_VCOL ? lit : h.t);
@@ -611,7 +611,7 @@ void launch_det_finish(const unsigned long long* dacc, int G, float* grad, float
 // LAMBERT_VCOL: the colour gradient of VERTEX_COLOR (shade_simple.cpp:39-74) times the light
 // factor (shadow * Lc) * ndl + ambient of the hit (the derivative of its shade), which needs the
 // hit's normal and shadow ray (traced again, or decoded from the render's stored hit).
-template <int MODE, bool POS = false, bool COUNT = false>
+template <int MODE, bool POS = false, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   constexpr bool VCOL_LIKE = MODE == 2 || MODE == MODE_LAMBERT_VCOL;
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
@@ -676,10 +676,10 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
             if (a.coverage) {
               // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
               // hit, so the sample adds exactly `val` if the ray hits anything
-              if (covered<COUNT>(T, o, d, wst)) uni = add(uni, val);
+              if (covered<COUNT, TF>(T, o, d, wst)) uni = add(uni, val);
               break;
             }
-            h = trace<false, COUNT>(T, o, d, 0.0f, 1e9f, wst);
+            h = trace<false, COUNT, TF>(T, o, d, 0.0f, 1e9f, wst);
           }
           if (h.gtri == NO_HIT) break;
           const int m = a.tr.s.tri_mesh[h.gtri];
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
               const uint32_t* I3 = a.tr.s.idx + 3 * h.gtri;
               float shadow, ndl;
               if (a.hits) lit_decode(lit_stored, shadow, ndl);
-              else lambert_terms<COUNT>(T, a.light, ro, rd, h, I3[0], I3[1], I3[2], a.tr.s.nrm, wst, shadow, ndl);
+              else lambert_terms<COUNT, TF>(T, a.light, ro, rd, h, I3[0], I3[1], I3[2], a.tr.s.nrm, wst, shadow, ndl);
               vs = mulv(val, lambert_factor(a.light, shadow, ndl));
             }
             const float w = 1.0f - h.u - h.v;
@@ -827,7 +827,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
 // the render (one sample per lane), the wave's sum of its covered samples into part[tile];
 // k_coverage_sum adds the partial sums in a fixed order, so the result does not depend on
 // scheduling.  Small in registers, it keeps as many waves resident as the render.
-template <bool COUNT = false>
+template <bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restrict__ part) {
   __shared__ int wst[64];
   const Tracer& T = a.tr;
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
         if (a.aligned) sample_offset(a.jit, a.view, tm.x, tm.y, si, sq, xo, yo);
         F3 o, d;
         eye_ray(c, tm.x + xo, tm.y + yo, o, d);
-        hit = covered<COUNT>(T, o, d, wst);
+        hit = covered<COUNT, TF>(T, o, d, wst);
       }
       if (hit) acc = add(acc, val);
     }
@@ -870,7 +870,7 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
 // barycentrics (BarU_grad / BarV_grad, src/functions.cpp:226-470) and mesh 0's inverse
 // transform as in edge_grad (src/drender.cpp:60-71).  The shadow is held fixed (it changes
 // only where a triangle's facing to the light flips).  One wave per pixel tile as the render.
-template <int MODE, bool COUNT = false>
+template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
   __shared__ int wst[64];
   const Tracer& T = a.tr;
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
       const float4 r = a.hits[(c.img_off / 3 + (long long)tm.y * c.w + tm.x) * a.nsamp + si];
       h = Hit{__float_as_uint(r.x), r.w, r.y, r.z};
     } else {
-      h = trace<false, COUNT>(T, ro, rd, 0.0f, 1e9f, wst);
+      h = trace<false, COUNT, TF>(T, ro, rd, 0.0f, 1e9f, wst);
     }
     if (h.gtri == NO_HIT) continue;
     const int m = S.tri_mesh[h.gtri];
@@ -902,7 +902,7 @@ __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
     const uint32_t* I = S.idx + 3 * h.gtri;
     float shadow, ndl;
     if (a.hits && MODE == MODE_LAMBERT_VCOL) lit_decode(h.t, shadow, ndl);   // stored hits keep the light terms
-    else lambert_terms<COUNT>(T, a.light, ro, rd, h, I[0], I[1], I[2], S.nrm, wst, shadow, ndl);
+    else lambert_terms<COUNT, TF>(T, a.light, ro, rd, h, I[0], I[1], I[2], S.nrm, wst, shadow, ndl);
     const F3 light = lambert_factor(a.light, shadow, ndl);
     // d(light)/du, d(light)/dv: zero where max(0, n.(-L)) is clamped or the point is in shadow
     const F3 nA = ld3(S.nrm + 3 * I[0]), nB = ld3(S.nrm + 3 * I[1]), nC = ld3(S.nrm + 3 * I[2]);
@@ -1397,7 +1397,7 @@ __global__ void k_edge_scatter(EdgeBinArgs a) {
 // 32 samples with both sides in adjacent lanes, one ray (and shadow ray) per lane, so its
 // packet covers 32 nearby points and the kernel stays small enough for 8 waves per SIMD.
 // ecol[i] = (colour inside - colour outside, edge id bits; -1 for a rejected sample).
-template <int MODE, bool COUNT = false, bool WT = false>
+template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
   __shared__ int wstack[4][64];
   const Tracer& T = a.tr;
@@ -1432,7 +1432,7 @@ __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
       const float nx = -ndy, ny = ndx;  // normal2D (src/utils.h:29)
       const float qx = sd == 0 ? p.px - 1e-3f * nx : p.px + 1e-3f * nx;
       const float qy = sd == 0 ? p.py - 1e-3f * ny : p.py + 1e-3f * ny;
-      col = shade<MODE, COUNT, WT>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
+      col = shade<MODE, COUNT, TF>(T, a.light, c, qx, qy, a.tr.s.nrm2d, wst);
     }
     // both lanes of a sample are here together (same item, same `live`)
     const float ox = __shfl_xor(col.x, 1, 64), oy = __shfl_xor(col.y, 1, 64), oz = __shfl_xor(col.z, 1, 64);
@@ -1765,8 +1765,11 @@ void launch_suspects(const SuspectArgs& a, hipStream_t s) {
 void launch_reorder(const ReorderArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
 }
+// The ray-counting (COUNT) and watertight instantiations walk the LBVH as packets only: a
+// walk changes no hit and no ray count.
 void launch_render(int mode, const RenderArgs& a, int tiles, hipStream_t s) {
   if (a.tr.rays) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_render<M, true>), dim3(tiles), dim3(64), 0, s, a))
+  else if (a.tr.s.lane_walk) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_render<M, false, TF_LANE>), dim3(tiles), dim3(64), 0, s, a))
   else DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_render<M>), dim3(tiles), dim3(64), 0, s, a))
 }
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s) {
@@ -1775,23 +1778,29 @@ void launch_loss_final(const double* part, int n, float* out, double denom, hipS
 void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s) {
   const size_t lds = sizeof(float) * (size_t)std::max(a.lds_n, 4 * a.hash);
   const bool cnt = a.tr.rays != nullptr;   // the ray-counting instantiations
+  const bool lane = !cnt && a.tr.s.lane_walk;
   if (mode == 2 && a.pos_grad) {
     if (cnt) hipLaunchKernelGGL((k_interior<2, true, true>), dim3(blocks), dim3(64), lds, s, a);
+    else if (lane) hipLaunchKernelGGL((k_interior<2, true, false, TF_LANE>), dim3(blocks), dim3(64), lds, s, a);
     else hipLaunchKernelGGL((k_interior<2, true>), dim3(blocks), dim3(64), lds, s, a);
   } else if (mode == 2) {
     if (cnt) hipLaunchKernelGGL((k_interior<2, false, true>), dim3(blocks), dim3(64), lds, s, a);
+    else if (lane) hipLaunchKernelGGL((k_interior<2, false, false, TF_LANE>), dim3(blocks), dim3(64), lds, s, a);
     else hipLaunchKernelGGL((k_interior<2>), dim3(blocks), dim3(64), lds, s, a);
   } else if (mode == MODE_LAMBERT_VCOL) {
     if (cnt) hipLaunchKernelGGL((k_interior<MODE_LAMBERT_VCOL, false, true>), dim3(blocks), dim3(64), lds, s, a);
+    else if (lane) hipLaunchKernelGGL((k_interior<MODE_LAMBERT_VCOL, false, false, TF_LANE>), dim3(blocks), dim3(64), lds, s, a);
     else hipLaunchKernelGGL((k_interior<MODE_LAMBERT_VCOL>), dim3(blocks), dim3(64), lds, s, a);
   } else {
     if (cnt) hipLaunchKernelGGL((k_interior<3, false, true>), dim3(blocks), dim3(64), lds, s, a);
+    else if (lane) hipLaunchKernelGGL((k_interior<3, false, false, TF_LANE>), dim3(blocks), dim3(64), lds, s, a);
     else hipLaunchKernelGGL((k_interior<3>), dim3(blocks), dim3(64), lds, s, a);
   }
 }
 void launch_coverage(const InteriorArgs& a, int tiles, float4* part, hipStream_t s) {
   if (tiles <= 0) return;
   if (a.tr.rays) hipLaunchKernelGGL((k_coverage<true>), dim3(tiles), dim3(64), 0, s, a, part);
+  else if (a.tr.s.lane_walk) hipLaunchKernelGGL((k_coverage<false, TF_LANE>), dim3(tiles), dim3(64), 0, s, a, part);
   else hipLaunchKernelGGL((k_coverage<false>), dim3(tiles), dim3(64), 0, s, a, part);
   hipLaunchKernelGGL(k_coverage_sum, dim3(1), dim3(1024), 0, s, (const float4*)part, tiles, a.grad + a.uniform_off);
 }
@@ -1799,9 +1808,11 @@ void launch_interior_pos(int mode, const InteriorArgs& a, int tiles, hipStream_t
   if (tiles <= 0) return;
   if (mode == MODE_LAMBERT_VCOL) {
     if (a.tr.rays) hipLaunchKernelGGL((k_interior_pos<MODE_LAMBERT_VCOL, true>), dim3(tiles), dim3(64), 0, s, a);
+    else if (a.tr.s.lane_walk) hipLaunchKernelGGL((k_interior_pos<MODE_LAMBERT_VCOL, false, TF_LANE>), dim3(tiles), dim3(64), 0, s, a);
     else hipLaunchKernelGGL((k_interior_pos<MODE_LAMBERT_VCOL, false>), dim3(tiles), dim3(64), 0, s, a);
   } else {
     if (a.tr.rays) hipLaunchKernelGGL((k_interior_pos<4, true>), dim3(tiles), dim3(64), 0, s, a);
+    else if (a.tr.s.lane_walk) hipLaunchKernelGGL((k_interior_pos<4, false, TF_LANE>), dim3(tiles), dim3(64), 0, s, a);
     else hipLaunchKernelGGL((k_interior_pos<4, false>), dim3(tiles), dim3(64), 0, s, a);
   }
 }
@@ -1871,10 +1882,12 @@ void launch_edge_bin(const EdgeBinArgs& a, hipStream_t s) {
 void launch_edge_trace(int mode, const EdgeArgs& a, hipStream_t s) {
   const int blocks = (int)std::max<long long>(1, std::min<long long>((2 * a.N + 255) / 256, 8192));
   if (a.watertight) {
-    if (a.tr.rays) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, true, true>), dim3(blocks), dim3(256), 0, s, a))
-    else DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, false, true>), dim3(blocks), dim3(256), 0, s, a))
+    if (a.tr.rays) DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, true, TF_WT>), dim3(blocks), dim3(256), 0, s, a))
+    else DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, false, TF_WT>), dim3(blocks), dim3(256), 0, s, a))
   } else if (a.tr.rays) {
     DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, true>), dim3(blocks), dim3(256), 0, s, a))
+  } else if (a.tr.s.lane_walk) {
+    DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M, false, TF_LANE>), dim3(blocks), dim3(256), 0, s, a))
   } else {
     DRH_DISPATCH_MODE(mode, hipLaunchKernelGGL((k_edge_trace<M>), dim3(blocks), dim3(256), 0, s, a))
   }

@@ -442,12 +442,39 @@ __device__ __forceinline__ Hit trace_bf(const DevScene& s, const float4* tri_a_o
 // so only the triangles meeting that bound for the view's origin can be accepted this way;
 // shadow rays share d = -L, so only triangles with |L.N| < 1e-6 can.  Those "suspect"
 // triangles are found per view / per light (k_suspects) and tested by brute force after the
-// LBVH traversal, which makes the LBVH answer the BF answer.  Suspect entries are
-// (N.xyz, triangle id); rays with |d.N| >= 1.5e-6 skip an entry (their det is not clamped).
+// LBVH traversal, which makes the LBVH answer the BF answer.
+// View entries are one float4 (N.xyz, triangle id); rays with |d.N| >= 1.5e-6 skip an entry
+// (their det is not clamped).  A shadow ray's determinant depends on the triangle alone, so
+// k_suspects computes it as mt_test would and lists a triangle only when 0 < |det| < 1e-6,
+// in two float4: (A.xyz, id) and (pvec.xyz, invDet) of that test; a ray then skips an entry
+// unless its v = (o - A).pvec * invDet -- mt_test's own v -- lies in (0, 1), which
+// acceptance needs (v > 0, u > 0 and u + v < 1).
+// Both lists are also binned (k_sus_bin, k_light_bin): per cell, the records of the suspects
+// that some ray of the cell could be accepted by, in a pool (cell[c] = (offset, n), two float4
+// per record; n < 0: the cell overflowed, its rays use the whole list).  View lists use screen
+// cells of SUS_CELL_PX pixels and records (N.xyz, id), (qvec.xyz, ub): qvec = (o - A) x e1 is
+// mt_test's own qvec (every camera ray of a view starts at the same o) and acceptance needs
+// |qvec.d| <= ub.  Light lists use a grid over the plane across the light, located by
+// grid = (a1.xyz, u0), (a2.xyz, v0), (1/du, 1/dv, -, -), and the light records.  A ray outside
+// its grid uses the whole list.
+// The records are read in chunks staged in the wave's LDS (every active lane copies a share,
+// the records are then read as broadcasts): one memory round trip per chunk, not two per
+// record as with scalar loads of each record and its triangle.
 struct Suspects {
-  const float4* list;  // (N.x, N.y, N.z, id bits) in no particular order
+  const float4* list;   // view: (N.x, N.y, N.z, id bits); light: (A, id bits), (pvec, invDet); any order
   const int* count;
+  const float4* pool;   // binned records, or NULL
+  const int2* cell;     // (offset, n) per cell
+  int cells_x, cells_y;
+  const float4* grid;   // light lists: the grid's placement (3 float4)
 };
+constexpr int SUS_CELL_PX = 8;
+constexpr int SUS_STASH = 256;   // records a cell can hold (more: the cell uses the whole list)
+// LDS per wave for the walks: the traversal stack (WST_STACK ints), then the suspect stage
+// (WST_STAGE ints = 64 float4).
+constexpr int WST_STACK = 64;
+constexpr int WST_STAGE = 256;
+constexpr int WST_INTS = WST_STACK + WST_STAGE;
 
 struct Tracer {
   DevScene s;
@@ -471,22 +498,124 @@ __device__ __forceinline__ void count_rays(unsigned long long* c, bool cast) {
   if ((int)(threadIdx.x & 63) == leader && b) atomicAdd(c, (unsigned long long)__popcll(b));
 }
 
+// Copies records [base, base + m) of a list of `per` float4 each into the wave's LDS stage
+// (64 float4): the active lanes share the copy; LDS operations of one wave complete in order,
+// so the reads that follow see it.  Returns m.
+__device__ __forceinline__ int stage_records(const float4* __restrict__ list, int base, int n, int per, float4* stage) {
+  const int m = min(64 / per, n - base);
+  const uint64_t act = __ballot(1);
+  const int nact = __popcll(act);
+  const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+  for (int i = rank; i < m * per; i += nact) stage[i] = list[(long long)base * per + i];
+  return m;
+}
+
+// Stages the records [0, n) of `rec` (two float4 each) chunk by chunk and calls f(r0, r1) for
+// each; true as soon as f does (the lane is done; the others go on).
+template <class Fn>
+__device__ __forceinline__ bool walk_records(const float4* rec, int n, float4* stage, Fn f) {
+  for (int base = 0; base < n; base += 32) {
+    const int m = stage_records(rec, base, n, 2, stage);
+    for (int k = 0; k < m; k++)
+      if (f(stage[2 * k], stage[2 * k + 1])) return true;
+  }
+  return false;
+}
+__device__ __forceinline__ int2 uload2(const int2* p, int i) {
+  const int* q = (const int*)(p + i);
+  return make_int2(uload(q), uload(q + 1));
+}
+// Walks the pooled records of every distinct cell of the active lanes (a lane testing another
+// lane's cell as well only tests a superset); true when this lane is done.
+template <class Fn>
+__device__ __forceinline__ bool walk_cells(const Suspects& S, int cell, float4* stage, Fn f) {
+  bool todo = true;
+  for (;;) {
+    const uint64_t p = __ballot(todo);
+    if (!p) break;
+    const int c = __builtin_amdgcn_readlane(cell, __ffsll((long long)p) - 1);
+    const int2 cc = uload2(S.cell, c);
+    if (walk_records(S.pool + 2ll * cc.x, cc.y, stage, f)) return true;
+    if (cell == c) todo = false;
+  }
+  return false;
+}
+
+// The view's suspects for camera rays (closest hit, or any hit for `covered`); (sx, sy) is
+// the ray's screen position (its cell), negative when unknown.
 template <bool ANY>
-__device__ __forceinline__ void test_suspects(const Tracer& T, const Suspects& S, F3 o, F3 d, float tNear, float tFar,
-                                              Hit& h) {
+__device__ __forceinline__ void test_cam_suspects(const Tracer& T, F3 o, F3 d, float tNear, float tFar, Hit& h,
+                                                  int* wst, float sx, float sy) {
+  const Suspects& S = T.cam;
   if (!S.count) return;
-  const int n = uload(S.count);
-  for (int k = 0; k < n; k++) {
-    const float4 e = uload(S.list, k);
-    const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
-    if (!(dn < 1.5e-6f)) continue;
-    const uint32_t g = __float_as_uint(e.w);
+  float4* stage = (float4*)(wst + WST_STACK);
+  // one record: true when an any-hit lane has its hit
+  auto test = [&](float4 e) -> bool {
+    const int g = __builtin_amdgcn_readfirstlane(__float_as_int(e.w));   // the same in every lane
     float t, u, v;
-    if (mt_test(uload(T.bf_a, (int)g), uload(T.bf_e1, (int)g), uload(T.bf_e2, (int)g), o, d, tNear, t, u, v)) {
-      if (ANY) { if (t < tFar) { h.gtri = g; h.t = t; h.u = u; h.v = v; return; } }
-      else if (t < h.t || (t == h.t && h.gtri != NO_HIT && g < h.gtri)) { h.gtri = g; h.t = t; h.u = u; h.v = v; }
+    if (mt_test(uload(T.bf_a, g), uload(T.bf_e1, g), uload(T.bf_e2, g), o, d, tNear, t, u, v)) {
+      if (ANY) { if (t < tFar) { h.gtri = (uint32_t)g; h.t = t; h.u = u; h.v = v; return true; } }
+      else if (t < h.t || (t == h.t && h.gtri != NO_HIT && (uint32_t)g < h.gtri)) {
+        h.gtri = (uint32_t)g; h.t = t; h.u = u; h.v = v;
+      }
+    }
+    return false;
+  };
+  if (S.pool) {
+    const float fx = floorf(sx * (1.0f / SUS_CELL_PX)), fy = floorf(sy * (1.0f / SUS_CELL_PX));
+    int cell = -1;
+    if (fx >= 0.0f && fy >= 0.0f && fx < (float)S.cells_x && fy < (float)S.cells_y) cell = (int)fy * S.cells_x + (int)fx;
+    const int n = cell >= 0 ? S.cell[cell].y : -1;
+    if (!__ballot(n < 0)) {
+      walk_cells(S, cell, stage, [&](float4 e, float4 q) -> bool {
+        const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
+        if (!(dn < 1.5e-6f) || !(fabsf(dot(f4xyz(q), d)) <= q.w)) return false;
+        return test(e);
+      });
+      return;
     }
   }
+  const int n = uload(S.count);
+  for (int base = 0; base < n; base += 64) {
+    const int m = stage_records(S.list, base, n, 1, stage);
+    for (int k = 0; k < m; k++) {
+      const float4 e = stage[k];
+      const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
+      if (!(dn < 1.5e-6f)) continue;
+      if (test(e)) return;
+    }
+  }
+}
+
+// The light's suspects for shadow rays (any hit, d = -L exactly as k_suspects used it).
+__device__ __forceinline__ void test_light_suspects(const Tracer& T, F3 o, F3 d, float tNear, float tFar, Hit& h,
+                                                    int* wst) {
+  const Suspects& S = T.light;
+  if (!S.count) return;
+  float4* stage = (float4*)(wst + WST_STACK);
+  auto rec = [&](float4 r0, float4 r1) -> bool {
+    const float v0 = dot(sub(o, f4xyz(r0)), f4xyz(r1)) * r1.w;
+    if (!(v0 > 0.0f && v0 < 1.0f)) return false;
+    const int g = __builtin_amdgcn_readfirstlane(__float_as_int(r0.w));
+    float t, u, v;
+    if (mt_test(uload(T.bf_a, g), uload(T.bf_e1, g), uload(T.bf_e2, g), o, d, tNear, t, u, v) && t < tFar) {
+      h.gtri = (uint32_t)g; h.t = t; h.u = u; h.v = v;
+      return true;
+    }
+    return false;
+  };
+  if (S.pool) {
+    const float4 g0 = uload(S.grid, 0), g1 = uload(S.grid, 1), g2 = uload(S.grid, 2);
+    const float fu = floorf((dot(o, f4xyz(g0)) - g0.w) * g2.x), fv = floorf((dot(o, f4xyz(g1)) - g1.w) * g2.y);
+    int cell = -1;
+    if (fu >= 0.0f && fv >= 0.0f && fu < (float)S.cells_x && fv < (float)S.cells_y) cell = (int)fv * S.cells_x + (int)fu;
+    const int n = cell >= 0 ? S.cell[cell].y : -1;
+    if (!__ballot(n < 0)) {
+      walk_cells(S, cell, stage, rec);
+      return;
+    }
+  }
+  walk_records(S.list, uload(S.count), stage, rec);
 }
 
 // Brute force with the watertight test (the WT form of trace_bf, original triangle order).
@@ -517,8 +646,10 @@ __device__ __forceinline__ Hit walk(const DevScene& s, F3 o, F3 d, float tNear, 
   return trace_bvh<ANY, WT>(s, o, d, tNear, tFar, wst);
 }
 
+// (sx, sy): a camera ray's screen position (selects its suspect cell); negative if unknown.
 template <bool ANY, bool COUNT = false, int TF = 0>
-__device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* wst) {
+__device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, float tFar, int* wst,
+                                     float sx = -1.0f, float sy = -1.0f) {
   count_rays<COUNT>(T.rays + (ANY ? 1 : 0), !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z));
   if (TF & TF_WT) {
     // no clamped determinants to emulate: no suspect lists
@@ -529,7 +660,9 @@ __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, f
   Hit h = walk<ANY, TF>(T.s, o, d, tNear, tFar, wst);
   if (ANY && h.gtri != NO_HIT) return h;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return h;
-  test_suspects<ANY>(T, ANY ? T.light : T.cam, o, d, tNear, tFar, h);
+  // any-hit rays here are the shadow rays (direction -L); camera any-hit queries use covered()
+  if (ANY) test_light_suspects(T, o, d, tNear, tFar, h, wst);
+  else test_cam_suspects<false>(T, o, d, tNear, tFar, h, wst, sx, sy);
   return h;
 }
 
@@ -537,13 +670,13 @@ __device__ __forceinline__ Hit trace(const Tracer& T, F3 o, F3 d, float tNear, f
 // yes / no as the closest-hit query (both accept the first triangle whose MT test passes with
 // 0 < t < tFar; the closest-hit query only keeps looking for a nearer one).
 template <bool COUNT = false, int TF = 0>
-__device__ __forceinline__ bool covered(const Tracer& T, F3 o, F3 d, int* wst) {
+__device__ __forceinline__ bool covered(const Tracer& T, F3 o, F3 d, int* wst, float sx = -1.0f, float sy = -1.0f) {
   count_rays<COUNT>(T.rays + 1, !(o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z));
   if (T.use_bf) return trace_bf<true>(T.s, T.bf_a, T.bf_e1, T.bf_e2, o, d, 0.0f, 1e9f).gtri != NO_HIT;
   Hit h = walk<true, TF & TF_LANE>(T.s, o, d, 0.0f, 1e9f, wst);
   if (h.gtri != NO_HIT) return true;
   if (o.x != o.x || o.y != o.y || o.z != o.z || d.x != d.x || d.y != d.y || d.z != d.z) return false;
-  test_suspects<true>(T, T.cam, o, d, 0.0f, 1e9f, h);
+  test_cam_suspects<true>(T, o, d, 0.0f, 1e9f, h, wst, sx, sy);
   return h.gtri != NO_HIT;
 }
 
@@ -650,7 +783,7 @@ __device__ __forceinline__ F3 shade(const Tracer& T, const Light& L, const DevCa
                                     const float* nrm, int* wst, Hit* hit_out = nullptr, float* lit_out = nullptr) {
   F3 o, d;
   eye_ray(c, sx, sy, o, d);
-  const Hit h = trace<false, COUNT, TF>(T, o, d, 0.0f, 1e9f, wst);
+  const Hit h = trace<false, COUNT, TF>(T, o, d, 0.0f, 1e9f, wst, sx, sy);
   if (hit_out) *hit_out = h;
   if (h.gtri == NO_HIT) return {0, 0, 0};
   if (MODE == 1) return {1, 1, 1};

@@ -48,7 +48,7 @@ struct PrepareArgs {
   float* nrm;
   float* nrm2d;
   uint32_t* bounds;       // optional: set to the empty key range (6 values) by thread 0
-  uint32_t* zero;         // optional: one counter cleared by thread 0
+  uint32_t* zero;         // optional: two counters cleared by thread 0
 };
 
 struct TriSetupArgs {
@@ -130,8 +130,37 @@ struct SuspectArgs {
   const float4* bf_a;
   const float4* bf_e1;
   const float4* bf_e2;
-  float4* list;           // capacity F
+  float4* list;           // capacity F entries (view: one float4 each; light: two)
   int* count;             // zeroed by the caller
+};
+
+// Binning of suspect lists into per-cell records in a pool (see Suspects in drh_device.h).
+struct SusBinArgs {        // a view's list by screen cell (one workgroup per cell)
+  const DevCam* cams;
+  int view;
+  const float4* list;     // the view's suspects (N.xyz, id)
+  const int* count;
+  const float4* bf_a;
+  const float4* bf_e1;
+  const float4* bf_e2;
+  float4* pool;           // two float4 per record
+  int pool_cap;           // records
+  int* pool_used;         // zeroed by the caller
+  int2* cell;             // (offset, n) per cell
+  int cells_x, cells_y;
+};
+struct LightBinArgs {      // the light's list over a grid across the light (one workgroup per cell)
+  const float4* list;     // the light's suspects (A, id), (pvec, invDet)
+  const int* count;
+  const float4* bf_e1;
+  const float4* nodes;    // LBVH (the root's child boxes give the scene's extent)
+  float Lx, Ly, Lz;       // the light direction L (shadow rays go along -L)
+  float4* grid;           // out: the grid's placement (3 float4)
+  float4* pool;
+  int pool_cap;
+  int* pool_used;         // zeroed by the caller
+  int2* cell;
+  int cells_x, cells_y;
 };
 
 struct RenderArgs {
@@ -311,6 +340,8 @@ void launch_refit(const RefitArgs& a, hipStream_t s);
 void launch_pack(const PackArgs& a, hipStream_t s);
 void launch_reorder(const ReorderArgs& a, hipStream_t s);
 void launch_suspects(const SuspectArgs& a, hipStream_t s);
+void launch_sus_bin(const SusBinArgs& a, hipStream_t s);
+void launch_light_bin(const LightBinArgs& a, hipStream_t s);
 void launch_render(int mode, const RenderArgs& a, int tiles, hipStream_t s);
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s);
 void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s);

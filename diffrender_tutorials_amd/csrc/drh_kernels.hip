@@ -73,7 +73,7 @@ __global__ void k_prepare(PrepareArgs a) {
       a.bounds[0] = a.bounds[1] = a.bounds[2] = 0xFFFFFFFFu;
       a.bounds[3] = a.bounds[4] = a.bounds[5] = 0u;
     }
-    if (a.zero) *a.zero = 0u;
+    if (a.zero) { a.zero[0] = 0u; a.zero[1] = 0u; }
   }
   if (k >= a.V) return;
   int m = 0;
@@ -402,28 +402,244 @@ __global__ void k_pack_nodes(PackArgs a) {
 }
 
 // Suspect triangles (see Tracer in drh_device.h): light == 0 tests the view origin bound
-// |(o-A).N| <= 2e-6 (|e1| + |e2| + |o-A|), light == 1 tests |L.N| <= 2e-6; both in double
-// with a 2x margin over the exact bound.
+// |(o-A).N| <= 2e-6 (|e1| + |e2| + |o-A|) in double with a 2x margin over the exact bound;
+// light == 1 lists the triangles whose shadow-ray determinant, computed as mt_test computes it
+// for d = -L, is clamped (0 < |det| < 1e-6), with that test's pvec and invDet.  det = 0, NaN
+// or infinite gives an invDet that rejects every ray; |det| >= 1e-6 is the unclamped test the
+// LBVH walk makes.
 __global__ void k_suspects(SuspectArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.F) return;
   const float4 A = a.bf_a[g], E1 = a.bf_e1[g], E2 = a.bf_e2[g];
+  if (a.light) {
+    const F3 negL = mul(-1.0f, F3{a.ox, a.oy, a.oz});
+    const F3 pvec = cross(negL, f4xyz(E2));
+    const float det = dot(f4xyz(E1), pvec);
+    const float sv = sgn(det);
+    if (!(sv != 0.0f && sv * det < 1e-6f)) return;
+    const float invDet = sv / max_std(sv * det, 1e-6f);
+    const int k = atomicAdd(a.count, 1);
+    a.list[2 * k] = make_float4(A.x, A.y, A.z, __uint_as_float((uint32_t)g));
+    a.list[2 * k + 1] = make_float4(pvec.x, pvec.y, pvec.z, invDet);
+    return;
+  }
   const double e1x = E1.x, e1y = E1.y, e1z = E1.z, e2x = E2.x, e2y = E2.y, e2z = E2.z;
   const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-  bool hit;
-  if (a.light) {
-    hit = fabs(a.ox * nx + a.oy * ny + a.oz * nz) <= 2e-6;
-  } else {
-    const double tx = (double)a.ox - A.x, ty = (double)a.oy - A.y, tz = (double)a.oz - A.z;
-    const double V = fabs(tx * nx + ty * ny + tz * nz);
-    const double bound = sqrt(e1x * e1x + e1y * e1y + e1z * e1z) + sqrt(e2x * e2x + e2y * e2y + e2z * e2z) +
-                         sqrt(tx * tx + ty * ty + tz * tz);
-    hit = V <= 2e-6 * bound + 1e-12;
-  }
+  const double tx = (double)a.ox - A.x, ty = (double)a.oy - A.y, tz = (double)a.oz - A.z;
+  const double V = fabs(tx * nx + ty * ny + tz * nz);
+  const double bound = sqrt(e1x * e1x + e1y * e1y + e1z * e1z) + sqrt(e2x * e2x + e2y * e2y + e2z * e2z) +
+                       sqrt(tx * tx + ty * ty + tz * tz);
+  const bool hit = V <= 2e-6 * bound + 1e-12;
   if (hit || !(nx == nx && ny == ny && nz == nz)) {
     const int k = atomicAdd(a.count, 1);
     a.list[k] = make_float4((float)nx, (float)ny, (float)nz, __uint_as_float((uint32_t)g));
   }
+}
+
+// A cell's stashed records (cnt_ref of them, read after the barrier; SUS_STASH at most kept) into the pool: one offset per
+// cell from the pool counter; a cell with more records than the stash, or past the pool's
+// capacity, gets n = -1 (its rays use the whole list).  Called by every thread of the block.
+__device__ __forceinline__ void pool_cell(const float4* stash, const int& cnt_ref, int& off, float4* pool, int cap,
+                                          int* used, int2* cell) {
+  __syncthreads();   // every record is stashed and counted
+  const int cnt = cnt_ref;
+  if (threadIdx.x == 0) {
+    int o = -1;
+    if (cnt <= SUS_STASH) {
+      o = atomicAdd(used, cnt);
+      if (o + cnt > cap) o = -1;
+    }
+    off = o;
+    *cell = o < 0 ? make_int2(0, -1) : make_int2(o, cnt);
+  }
+  __syncthreads();
+  if (off >= 0)
+    for (int i = threadIdx.x; i < 2 * cnt; i += blockDim.x) pool[2ll * off + i] = stash[i];
+}
+
+// Bins the light's suspect list over a grid across the light.  Shadow rays share d = -L, so a
+// record's determinant and invDet are fixed and its acceptance confines the ray's origin o:
+// with s = sign(invDet), l1 = s (o - A).pvec (mt_test's v before the scale |invDet| = 1e6)
+// and l2 = s (o - A).(e1 x d) (its u), acceptance needs l1 > 0, l2 > 0 and l1 + l2 < 1e-6
+// (1 + 2.4e-7).  l1, l2 are affine in the origin's coordinates (u, v) = (o.a1, o.a2) across
+// the light (a1, a2 orthonormal, perpendicular to L; the component along L enters only
+// through pvec's rounding), so the origins lie in the triangle of the (u, v) plane where
+// l1 >= -m1, l2 >= -m2, l1 + l2 <= 1e-6 (1 + 2.4e-7) + m1 + m2, with m1, m2 bounds on the
+// float tvec / dot rounding (3e-7 and 5e-7 of |o - A| |pvec| and |o - A| |e1|, |o - A| at most
+// the scene box's diagonal) and on the component along L.  A record is kept for a cell when
+// that triangle's box, widened by one cell on every side, overlaps the cell (a ray's own
+// (u, v) in float differs from the exact by far less than a cell).  The grid covers the
+// scene box (the LBVH root's child boxes) with a margin.  Every block computes the same
+// placement; block 0 stores it for the rays.
+__global__ __launch_bounds__(256) void k_light_bin(LightBinArgs a) {
+  __shared__ float4 stash[2 * SUS_STASH];
+  __shared__ int cnt, off;
+  const int cell = blockIdx.x;
+  const int gx = cell % a.cells_x, gy = cell / a.cells_x;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  // orthonormal frame across L
+  const double ll = sqrt((double)a.Lx * a.Lx + (double)a.Ly * a.Ly + (double)a.Lz * a.Lz);
+  const double Lx = a.Lx / ll, Ly = a.Ly / ll, Lz = a.Lz / ll;
+  const double ax = fabs(Lx), ay = fabs(Ly), az = fabs(Lz);
+  // a1 = L x (the axis least aligned with L), normalised; a2 = L x a1
+  double ex = 0, ey = 0, ez = 0;
+  if (ax <= ay && ax <= az) ex = 1; else if (ay <= az) ey = 1; else ez = 1;
+  double b1x = Ly * ez - Lz * ey, b1y = Lz * ex - Lx * ez, b1z = Lx * ey - Ly * ex;
+  const double bl = sqrt(b1x * b1x + b1y * b1y + b1z * b1z);
+  b1x /= bl; b1y /= bl; b1z /= bl;
+  const double b2x = Ly * b1z - Lz * b1y, b2y = Lz * b1x - Lx * b1z, b2z = Lx * b1y - Ly * b1x;
+  // scene box: the root's two child boxes
+  const float4 n0 = a.nodes[0], n1 = a.nodes[1], n2 = a.nodes[2];
+  const double lo[3] = {fmin((double)n0.x, (double)n1.z), fmin((double)n0.y, (double)n1.w), fmin((double)n0.z, (double)n2.x)};
+  const double hi[3] = {fmax((double)n0.w, (double)n2.y), fmax((double)n1.x, (double)n2.z), fmax((double)n1.y, (double)n2.w)};
+  double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300, wmax = 0.0;
+  for (int k = 0; k < 8; k++) {
+    const double x = (k & 1) ? hi[0] : lo[0], y = (k & 2) ? hi[1] : lo[1], z = (k & 4) ? hi[2] : lo[2];
+    const double u = x * b1x + y * b1y + z * b1z, v = x * b2x + y * b2y + z * b2z, w = x * Lx + y * Ly + z * Lz;
+    umin = fmin(umin, u); umax = fmax(umax, u); vmin = fmin(vmin, v); vmax = fmax(vmax, v);
+    wmax = fmax(wmax, fabs(w));
+  }
+  const double diag = sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  const double pad = 0.01 * (umax - umin + vmax - vmin) + 1e-3 * (1.0 + fmax(fmax(fabs(umin), fabs(umax)), fmax(fabs(vmin), fabs(vmax))));
+  const float u0 = (float)(umin - pad), v0 = (float)(vmin - pad);
+  const float iu = (float)(a.cells_x / (umax - umin + 2 * pad)), iv = (float)(a.cells_y / (vmax - vmin + 2 * pad));
+  if (cell == 0 && threadIdx.x == 0) {
+    a.grid[0] = make_float4((float)b1x, (float)b1y, (float)b1z, u0);
+    a.grid[1] = make_float4((float)b2x, (float)b2y, (float)b2z, v0);
+    a.grid[2] = make_float4(iu, iv, 0.0f, 0.0f);
+  }
+  const bool ok = diag == diag && diag < 1e30 && iu > 0.0f && iv > 0.0f && iu < 3.0e38f && iv < 3.0e38f;
+  const F3 d = mul(-1.0f, F3{a.Lx, a.Ly, a.Lz});   // the shadow rays' direction, as lambert_terms forms it
+  const int n = *a.count;
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    const float4 r0 = a.list[2 * s], r1 = a.list[2 * s + 1];
+    bool keep = true;
+    if (ok) {
+      const float4 E1 = a.bf_e1[__float_as_int(r0.w)];
+      const double sg = r1.w > 0.0f ? 1.0 : -1.0;
+      const double px = r1.x, py = r1.y, pz = r1.z;                       // pvec
+      const double qx = (double)E1.y * d.z - (double)E1.z * d.y;          // e1 x d
+      const double qy = (double)E1.z * d.x - (double)E1.x * d.z;
+      const double qz = (double)E1.x * d.y - (double)E1.y * d.x;
+      const double lp = sqrt(px * px + py * py + pz * pz), lq = sqrt(qx * qx + qy * qy + qz * qz);
+      const double le1 = sqrt((double)E1.x * E1.x + (double)E1.y * E1.y + (double)E1.z * E1.z);
+      // l1 = j11 u + j12 v + c1 (+ the L component), l2 likewise
+      const double j11 = sg * (b1x * px + b1y * py + b1z * pz), j12 = sg * (b2x * px + b2y * py + b2z * pz);
+      const double j21 = sg * (b1x * qx + b1y * qy + b1z * qz), j22 = sg * (b2x * qx + b2y * qy + b2z * qz);
+      const double c1 = -sg * (r0.x * px + r0.y * py + r0.z * pz), c2 = -sg * (r0.x * qx + r0.y * qy + r0.z * qz);
+      const double w1 = fabs(Lx * px + Ly * py + Lz * pz) * wmax + 1e-7 * lp * wmax;
+      const double w2 = fabs(Lx * qx + Ly * qy + Lz * qz) * wmax + 1e-7 * lq * wmax;
+      const double m1 = 2.0 * (3e-7 * diag * lp + w1) + 1e-12, m2 = 2.0 * (5e-7 * diag * le1 + w2) + 1e-12;
+      const double T = 1e-6 * (1.0 + 2.4e-7) + m1 + m2;
+      const double det = j11 * j22 - j12 * j21;
+      if (det != 0.0 && det == det) {
+        const double L1[3] = {-m1, T + m2, -m1}, L2[3] = {-m2, -m2, T + m1};
+        double ulo = 1e300, uhi = -1e300, vlo = 1e300, vhi = -1e300;
+        for (int k = 0; k < 3; k++) {
+          const double r1_ = L1[k] - c1, r2_ = L2[k] - c2;
+          const double u = (r1_ * j22 - r2_ * j12) / det, v = (j11 * r2_ - j21 * r1_) / det;
+          ulo = fmin(ulo, u); uhi = fmax(uhi, u); vlo = fmin(vlo, v); vhi = fmax(vhi, v);
+        }
+        const double cu0 = floor((ulo - u0) * iu) - 1.0, cu1 = floor((uhi - u0) * iu) + 1.0;
+        const double cv0 = floor((vlo - v0) * iv) - 1.0, cv1 = floor((vhi - v0) * iv) + 1.0;
+        keep = !(gx < cu0 || gx > cu1 || gy < cv0 || gy > cv1);   // NaN bounds keep
+      }
+    }
+    if (!keep) continue;
+    const int k = atomicAdd(&cnt, 1);
+    if (k < SUS_STASH) { stash[2 * k] = r0; stash[2 * k + 1] = r1; }
+  }
+  pool_cell(stash, cnt, off, a.pool, a.pool_cap, a.pool_used, a.cell + cell);
+}
+
+// Bins a view's suspect list by screen cell.  A record is kept for a cell unless a necessary
+// condition of its clamped acceptance fails for every direction d through the cell:
+//   C: |d.N| < 1.5e-6 (the per-lane filter the rays apply, N from the list);
+//   U: |qvec.d| <= ub, with qvec = (o - A) x e1 as mt_test computes it for this view's rays
+//      and ub >= (1 + 2^-23) max(|det|, 1e-6) for every det the C filter lets through (u in
+//      (0, 1) needs |qvec.d| |invDet| < 1);
+//   V: |w.d| <= ub + ev, w = e2 x (o - A): mt_test's v needs |(o - A).(d x e2)| < ub as well;
+//   U and V combined: |(a qvec - b w).d| <= |a| (ub + eu) + |b| (ub + ev) for the coordinates
+//      (a, b) of o - A along (e1, e2) in the basis (e1, e2, N/|N|).  Both qvec and w are nearly
+//      parallel to N (the triangle's plane passes close to o), so the U and V bands are two
+//      almost coincident lines across the image; in a qvec - b w the N parts cancel, and its
+//      band crosses them, which confines the record to a short stretch of the line.
+// eu, ev bound the float dot products' rounding and the ray direction's error (2e-5 + 1e-7 |o|
+// relative), scaled by |qvec| and |o - A| |e2|.  The exact direction through screen point
+// (x, y) is +-R p(x, y) / |R p| (R: the rotation of the view's inverse world-view matrix, p:
+// proj_inv (ndc, 0, 1), both affine in x, y, the sign that of p.w), so n.D is affine over the
+// cell and |D| convex: if n.D has one sign at the four corners and min |n.D| > tau max |D|,
+// then |n.d| > tau everywhere in the cell.  Corners in double.
+__global__ __launch_bounds__(256) void k_sus_bin(SusBinArgs a) {
+  __shared__ float4 stash[2 * SUS_STASH];
+  __shared__ int cnt, off;
+  const int cell = blockIdx.x;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const DevCam& c = a.cams[a.view];
+  const int cx = cell % a.cells_x, cy = cell / a.cells_x;
+  double D[4][3], dmax = 0.0;
+  for (int k = 0; k < 4; k++) {
+    const double x = (double)((cx + (k & 1)) * SUS_CELL_PX), y = (double)((cy + (k >> 1)) * SUS_CELL_PX);
+    const double nx = 2.0 * (x / (double)c.W) - 1.0, ny = -2.0 * (y / (double)c.H) + 1.0;
+    double p[3];
+    for (int r = 0; r < 3; r++) p[r] = nx * (double)c.proj_inv[r] + ny * (double)c.proj_inv[4 + r] + (double)c.proj_inv[12 + r];
+    for (int r = 0; r < 3; r++)
+      D[k][r] = p[0] * (double)c.wv_inv[r] + p[1] * (double)c.wv_inv[4 + r] + p[2] * (double)c.wv_inv[8 + r];
+    dmax = fmax(dmax, sqrt(D[k][0] * D[k][0] + D[k][1] * D[k][1] + D[k][2] * D[k][2]));
+  }
+  float oa[4];
+  mv4(c.wv_inv, 0.0f, 0.0f, 0.0f, 1.0f, oa);   // eye_ray's origin, bit for bit
+  const F3 o = {oa[0], oa[1], oa[2]};
+  const double delta = 2e-5 + 1e-7 * sqrt((double)o.x * o.x + (double)o.y * o.y + (double)o.z * o.z);
+  // false when |n.d| > tau for every direction of the cell
+  auto possible = [&](double n0, double n1, double n2, double tau) {
+    double f[4];
+    for (int k = 0; k < 4; k++) f[k] = D[k][0] * n0 + D[k][1] * n1 + D[k][2] * n2;
+    const bool pos = f[0] > 0.0 && f[1] > 0.0 && f[2] > 0.0 && f[3] > 0.0;
+    const bool neg = f[0] < 0.0 && f[1] < 0.0 && f[2] < 0.0 && f[3] < 0.0;
+    if (!pos && !neg) return true;
+    const double lo = fmin(fmin(fabs(f[0]), fabs(f[1])), fmin(fabs(f[2]), fabs(f[3])));
+    return !(lo > tau * dmax);
+  };
+  const int n = *a.count;
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    const float4 e = a.list[s];
+    const int g = __float_as_int(e.w);
+    const float4 A = a.bf_a[g], E1 = a.bf_e1[g], E2 = a.bf_e2[g];
+    const F3 qvec = cross(sub(o, f4xyz(A)), f4xyz(E1));
+    const double l1 = sqrt((double)E1.x * E1.x + (double)E1.y * E1.y + (double)E1.z * E1.z);
+    const double l2 = sqrt((double)E2.x * E2.x + (double)E2.y * E2.y + (double)E2.z * E2.z);
+    const float ub = (float)(1.6e-6 + 1e-6 * l1 * l2);
+    const double ln = sqrt((double)e.x * e.x + (double)e.y * e.y + (double)e.z * e.z);
+    const double lq = sqrt((double)qvec.x * qvec.x + (double)qvec.y * qvec.y + (double)qvec.z * qvec.z);
+    if (!possible(e.x, e.y, e.z, 1.5e-6 + (delta + 1e-6) * ln)) continue;
+    const double eu = (delta + 1e-6) * lq;
+    if (!possible(qvec.x, qvec.y, qvec.z, (double)ub + eu)) continue;
+    const F3 tv = sub(o, f4xyz(A));   // mt_test's tvec
+    const double tx = tv.x, ty = tv.y, tz = tv.z, ax = E1.x, ay = E1.y, az = E1.z, bx = E2.x, by = E2.y, bz = E2.z;
+    const double wx = by * tz - bz * ty, wy = bz * tx - bx * tz, wz = bx * ty - by * tx;   // e2 x tvec
+    const double ev = (delta + 1e-6) * sqrt(tx * tx + ty * ty + tz * tz) * l2;
+    if (!possible(wx, wy, wz, (double)ub + ev)) continue;
+    // (a, b): tvec = a e1 + b e2 + c N/|N|, by Cramer's rule with det = |N|
+    const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+    const double nn = sqrt(nx * nx + ny * ny + nz * nz);
+    if (nn > 0.0) {
+      const double hx = nx / nn, hy = ny / nn, hz = nz / nn;
+      const double ca = tx * (by * hz - bz * hy) + ty * (bz * hx - bx * hz) + tz * (bx * hy - by * hx);   // tvec.(e2 x n)
+      const double cb = tx * (hy * az - hz * ay) + ty * (hz * ax - hx * az) + tz * (hx * ay - hy * ax);   // tvec.(n x e1)
+      const double ka = ca / nn, kb = cb / nn;
+      const double tc = (fabs(ka) * ((double)ub + eu) + fabs(kb) * ((double)ub + ev)) * (1.0 + 1e-9);
+      if (!possible(ka * qvec.x - kb * wx, ka * qvec.y - kb * wy, ka * qvec.z - kb * wz, tc)) continue;
+    }
+    const int k = atomicAdd(&cnt, 1);
+    if (k < SUS_STASH) {
+      stash[2 * k] = e;
+      stash[2 * k + 1] = make_float4(qvec.x, qvec.y, qvec.z, ub);
+    }
+  }
+  pool_cell(stash, cnt, off, a.pool, a.pool_cap, a.pool_used, a.cell + cell);
 }
 
 __global__ void k_reorder(ReorderArgs a) {
@@ -457,7 +673,7 @@ __device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
 
 template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
-  __shared__ int wst[64];   // the wave's traversal stack
+  __shared__ __align__(16) int wst[WST_INTS];   // the wave's traversal stack and suspect stage
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
@@ -615,7 +831,7 @@ template <int MODE, bool POS = false, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   constexpr bool VCOL_LIKE = MODE == 2 || MODE == MODE_LAMBERT_VCOL;
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
-  __shared__ int wst[64];
+  __shared__ __align__(16) int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
@@ -676,10 +892,10 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
             if (a.coverage) {
               // 1x1 texture and tc == 0 everywhere: the bilinear weights are (1, 0, 0, 0) at any
               // hit, so the sample adds exactly `val` if the ray hits anything
-              if (covered<COUNT, TF>(T, o, d, wst)) uni = add(uni, val);
+              if (covered<COUNT, TF>(T, o, d, wst, x + xo, y + yo)) uni = add(uni, val);
               break;
             }
-            h = trace<false, COUNT, TF>(T, o, d, 0.0f, 1e9f, wst);
+            h = trace<false, COUNT, TF>(T, o, d, 0.0f, 1e9f, wst, x + xo, y + yo);
           }
           if (h.gtri == NO_HIT) break;
           const int m = a.tr.s.tri_mesh[h.gtri];
@@ -829,7 +1045,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
 // scheduling.  Small in registers, it keeps as many waves resident as the render.
 template <bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restrict__ part) {
-  __shared__ int wst[64];
+  __shared__ __align__(16) int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
@@ -849,7 +1065,7 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
         if (a.aligned) sample_offset(a.jit, a.view, tm.x, tm.y, si, sq, xo, yo);
         F3 o, d;
         eye_ray(c, tm.x + xo, tm.y + yo, o, d);
-        hit = covered<COUNT, TF>(T, o, d, wst);
+        hit = covered<COUNT, TF>(T, o, d, wst, tm.x + xo, tm.y + yo);
       }
       if (hit) acc = add(acc, val);
     }
@@ -872,7 +1088,7 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
 // only where a triangle's facing to the light flips).  One wave per pixel tile as the render.
 template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
-  __shared__ int wst[64];
+  __shared__ __align__(16) int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevScene& S = a.tr.s;
   const DevCam& c = a.cams[a.view];
@@ -893,7 +1109,7 @@ __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
       const float4 r = a.hits[(c.img_off / 3 + (long long)tm.y * c.w + tm.x) * a.nsamp + si];
       h = Hit{__float_as_uint(r.x), r.w, r.y, r.z};
     } else {
-      h = trace<false, COUNT, TF>(T, ro, rd, 0.0f, 1e9f, wst);
+      h = trace<false, COUNT, TF>(T, ro, rd, 0.0f, 1e9f, wst, tm.x + xo, tm.y + yo);
     }
     if (h.gtri == NO_HIT) continue;
     const int m = S.tri_mesh[h.gtri];
@@ -1399,7 +1615,7 @@ __global__ void k_edge_scatter(EdgeBinArgs a) {
 // ecol[i] = (colour inside - colour outside, edge id bits; -1 for a rejected sample).
 template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
-  __shared__ int wstack[4][64];
+  __shared__ __align__(16) int wstack[4][WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x & 63;
@@ -1680,7 +1896,7 @@ void launch_step_inc(int* step, hipStream_t s) { hipLaunchKernelGGL(k_step_inc, 
 // debug: closest hit at pixel centres
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
-  __shared__ int wst[64];
+  __shared__ __align__(16) int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[0];
   const int lane = threadIdx.x;
@@ -1690,7 +1906,7 @@ __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
   if (x >= c.w || y >= c.h) return;
   F3 o, d;
   eye_ray(c, x + 0.5f, y + 0.5f, o, d);
-  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst);
+  const Hit h = trace<false>(T, o, d, 0.0f, 1e9f, wst, x + 0.5f, y + 0.5f);
   const int p = y * c.w + x;
   uint32_t prim = NO_HIT, geom = NO_HIT;
   if (h.gtri != NO_HIT) { geom = (uint32_t)a.tr.s.tri_mesh[h.gtri]; prim = h.gtri - (uint32_t)a.tr.s.mesh_to[geom]; }
@@ -1761,6 +1977,12 @@ void launch_pack(const PackArgs& a, hipStream_t s) {
 }
 void launch_suspects(const SuspectArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_suspects, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_sus_bin(const SusBinArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_sus_bin, dim3(a.cells_x * a.cells_y), dim3(256), 0, s, a);
+}
+void launch_light_bin(const LightBinArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_light_bin, dim3(a.cells_x * a.cells_y), dim3(256), 0, s, a);
 }
 void launch_reorder(const ReorderArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);

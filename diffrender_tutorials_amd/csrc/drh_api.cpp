@@ -370,8 +370,9 @@ struct drh_renderer : BufOwner {
       leaf_parent, counter, nrange, frontier, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
   // clamped-determinant suspects (per view for camera rays, per light for shadow rays)
   DBuf sus_view, sus_view_cnt, sus_light, sus_light_cnt;
-  // the suspect lists binned per cell (k_sus_bin per view, k_light_bin): records in a pool
-  DBuf sus_pool, sus_cell, sus_light_pool, sus_light_cell, sus_light_grid;
+  // the suspect lists binned per cell (k_sus_bin, k_light_bin) when F >= SUS_BIN_MIN_F:
+  // records in a pool, (offset, count) and a counter per cell
+  DBuf sus_pool, sus_cell, sus_cell_cnt, sus_light_pool, sus_light_cell, sus_light_cell_cnt, sus_light_grid;
   long long sus_cells_stride = 0, sus_pool_cap = 0;   // cells and pool records per view
   static constexpr int LIGHT_GRID = 64;                // light grid: LIGHT_GRID^2 cells
   static constexpr int LIGHT_POOL = 16 * LIGHT_GRID * LIGHT_GRID;
@@ -691,7 +692,7 @@ struct drh_renderer : BufOwner {
     t.use_bf = st.tracer == DRH_TRACER_BF;
     t.cam = {nullptr, nullptr};
     t.light = {light_sus ? sus_light.as<float4>() : nullptr, light_sus ? sus_light_cnt.as<int>() : nullptr};
-    if (light_sus && F >= 2)
+    if (light_sus && F >= SUS_BIN_MIN_F)
       t.light = {sus_light.as<float4>(), sus_light_cnt.as<int>(), sus_light_pool.as<float4>(), sus_light_cell.as<int2>(),
                  LIGHT_GRID, LIGHT_GRID, sus_light_grid.as<float4>()};
     return t;
@@ -701,9 +702,13 @@ struct drh_renderer : BufOwner {
     Tracer t = tracer();
     if (!t.use_bf && F > 0) {
       const DevCam& c = cam_cache[v];
-      t.cam = {sus_view.as<float4>() + (size_t)v * F, sus_view_cnt.as<int>() + v,
-               sus_pool.as<float4>() + 2 * sus_pool_cap * v, sus_cell.as<int2>() + sus_cells_stride * v,
-               (c.w + SUS_CELL_PX - 1) / SUS_CELL_PX, (c.h + SUS_CELL_PX - 1) / SUS_CELL_PX, nullptr};
+      t.cam = {sus_view.as<float4>() + (size_t)v * F, sus_view_cnt.as<int>() + v, nullptr, nullptr, 0, 0, nullptr};
+      if (F >= SUS_BIN_MIN_F) {
+        t.cam.pool = sus_pool.as<float4>() + 2 * sus_pool_cap * v;
+        t.cam.cell = sus_cell.as<int2>() + sus_cells_stride * v;
+        t.cam.cells_x = (c.w + SUS_CELL_PX - 1) / SUS_CELL_PX;
+        t.cam.cells_y = (c.h + SUS_CELL_PX - 1) / SUS_CELL_PX;
+      }
     }
     t.s.lane_walk = lane_walk_cam(v);
     return t;
@@ -733,16 +738,19 @@ struct drh_renderer : BufOwner {
     for (int v = 0; v < views; v++)
       cells = std::max<long long>(cells, (long long)((cam_cache[v].w + SUS_CELL_PX - 1) / SUS_CELL_PX) *
                                              ((cam_cache[v].h + SUS_CELL_PX - 1) / SUS_CELL_PX));
+    const bool bin = F >= SUS_BIN_MIN_F;
     const long long cap = std::max<long long>(4096, 8 * cells);   // pool records per view
     if ((s = hip(sus_view.reserve(16 * (size_t)F * views), "alloc")) ||
-        (s = hip(sus_view_cnt.reserve(8 * (size_t)views), "alloc")) ||
-        (s = hip(sus_pool.reserve(32 * (size_t)cap * views), "alloc")) ||
-        (s = hip(sus_cell.reserve(8 * (size_t)cells * views), "alloc")))
+        (s = hip(sus_view_cnt.reserve(4 * (size_t)views), "alloc")))
+      return s;
+    if (bin && ((s = hip(sus_pool.reserve(32 * (size_t)cap * views), "alloc")) ||
+                (s = hip(sus_cell.reserve(8 * (size_t)cells * views), "alloc")) ||
+                (s = hip(sus_cell_cnt.reserve(4 * (size_t)cells * views), "alloc"))))
       return s;
     sus_cells_stride = cells;
     sus_pool_cap = cap;
-    // each view's suspect count, then each view's pool counter
-    timed("k_fill_u32", [&] { launch_fill_u32(sus_view_cnt.as<uint32_t>(), 2 * views, 0u, stream); });
+    timed("k_fill_u32", [&] { launch_fill_u32(sus_view_cnt.as<uint32_t>(), views, 0u, stream); });
+    if (bin) timed("k_fill_u32", [&] { launch_fill_u32(sus_cell_cnt.as<uint32_t>(), cells * views, 0u, stream); });
     for (int v = 0; v < views; v++) {
       const M4 wi = m4_load(cam_cache[v].wv_inv);
       const float z4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
@@ -751,11 +759,12 @@ struct drh_renderer : BufOwner {
       SuspectArgs a{F, 0, o[0], o[1], o[2], bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
                     sus_view.as<float4>() + (size_t)v * F, sus_view_cnt.as<int>() + v};
       timed("k_suspects", [&] { launch_suspects(a, stream); });
-      const int cx = (cam_cache[v].w + SUS_CELL_PX - 1) / SUS_CELL_PX, cy = (cam_cache[v].h + SUS_CELL_PX - 1) / SUS_CELL_PX;
-      SusBinArgs b{cams.as<DevCam>(), v, a.list, a.count, bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
-                   sus_pool.as<float4>() + 2 * cap * v, (int)cap, sus_view_cnt.as<int>() + views + v,
-                   sus_cell.as<int2>() + cells * v, cx, cy};
-      timed("k_sus_bin", [&] { launch_sus_bin(b, stream); });
+    }
+    if (bin) {
+      SusBinArgs b{cams.as<DevCam>(), sus_view.as<float4>(), sus_view_cnt.as<int>(), F, bf_a.as<float4>(),
+                   bf_e1.as<float4>(), bf_e2.as<float4>(),
+                   BinPool{sus_pool.as<float4>(), (int)cap, sus_cell.as<int2>(), sus_cell_cnt.as<int>(), 0, 0}, cap, cells};
+      timed("k_sus_bin", [&] { launch_sus_bin(b, views, stream); });
     }
     views_built = views;
     return hip(hipGetLastError(), "suspects");
@@ -1008,9 +1017,11 @@ drh_status drh_renderer::build_bvh(bool full) {
       (s = R(tri_a, f16)) || (s = R(tri_e1, f16)) || (s = R(tri_e2, f16)))
     return s;
   light_sus = (st.mode == DRH_LAMBERT || st.mode == DRH_LAMBERT_VERTEX_COLOR) && st.tracer != DRH_TRACER_BF;
-  if (light_sus && ((s = R(sus_light, 2 * f16)) || (s = R(sus_light_cnt, 16)) ||   // two float4 per entry
-                    (s = R(sus_light_pool, 32 * (size_t)LIGHT_POOL)) ||
-                    (s = R(sus_light_cell, 8 * (size_t)LIGHT_GRID * LIGHT_GRID)) || (s = R(sus_light_grid, 48))))
+  if (light_sus && ((s = R(sus_light, 2 * f16)) || (s = R(sus_light_cnt, 16))))   // two float4 per entry
+    return s;
+  if (light_sus && F >= SUS_BIN_MIN_F &&
+      ((s = R(sus_light_pool, 32 * (size_t)LIGHT_POOL)) || (s = R(sus_light_cell, 8 * (size_t)LIGHT_GRID * LIGHT_GRID)) ||
+       (s = R(sus_light_cell_cnt, 4 * (size_t)LIGHT_GRID * LIGHT_GRID)) || (s = R(sus_light_grid, 48))))
     return s;
   // k_prepare also empties the centroid bounds and the light's suspect count, and
   // k_tri_setup clears the refit counters
@@ -1058,12 +1069,13 @@ drh_status drh_renderer::build_bvh(bool full) {
   ReorderArgs ro{F, vals0.as<uint32_t>(), bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
                  tri_a.as<float4>(), tri_e1.as<float4>(), tri_e2.as<float4>()};
   timed("k_reorder", [&] { launch_reorder(ro, stream); });
-  if (light_sus && F >= 2) {
-    // the light's suspects over a grid across the light (needs the tree's root boxes);
-    // k_prepare cleared the pool counter (sus_light_cnt[1])
+  if (light_sus && F >= SUS_BIN_MIN_F) {
+    // the light's suspects over a grid across the light (needs the tree's root boxes)
+    timed("k_fill_u32", [&] { launch_fill_u32(sus_light_cell_cnt.as<uint32_t>(), LIGHT_GRID * LIGHT_GRID, 0u, stream); });
     LightBinArgs lb{sus_light.as<float4>(), sus_light_cnt.as<int>(), bf_e1.as<float4>(), nodes.as<float4>(),
-                    light.L.x, light.L.y, light.L.z, sus_light_grid.as<float4>(), sus_light_pool.as<float4>(),
-                    LIGHT_POOL, sus_light_cnt.as<int>() + 1, sus_light_cell.as<int2>(), LIGHT_GRID, LIGHT_GRID};
+                    light.L.x, light.L.y, light.L.z, sus_light_grid.as<float4>(),
+                    BinPool{sus_light_pool.as<float4>(), LIGHT_POOL, sus_light_cell.as<int2>(), sus_light_cell_cnt.as<int>(),
+                            LIGHT_GRID, LIGHT_GRID}};
     timed("k_light_bin", [&] { launch_light_bin(lb, stream); });
   }
   return hip(hipGetLastError(), "commit kernels");

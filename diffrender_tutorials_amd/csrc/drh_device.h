@@ -469,7 +469,7 @@ struct Suspects {
   const float4* grid;   // light lists: the grid's placement (3 float4)
 };
 constexpr int SUS_CELL_PX = 8;
-constexpr int SUS_STASH = 256;   // records a cell can hold (more: the cell uses the whole list)
+constexpr int SUS_BIN_MIN_F = 32768;   // lists of smaller scenes are not binned (short lists cost less than binning)
 // LDS per wave for the walks: the traversal stack (WST_STACK ints), then the suspect stage
 // (WST_STAGE ints = 64 float4).
 constexpr int WST_STACK = 64;

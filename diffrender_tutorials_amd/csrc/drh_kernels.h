@@ -48,7 +48,7 @@ struct PrepareArgs {
   float* nrm;
   float* nrm2d;
   uint32_t* bounds;       // optional: set to the empty key range (6 values) by thread 0
-  uint32_t* zero;         // optional: two counters cleared by thread 0
+  uint32_t* zero;         // optional: one counter cleared by thread 0
 };
 
 struct TriSetupArgs {
@@ -134,33 +134,35 @@ struct SuspectArgs {
   int* count;             // zeroed by the caller
 };
 
-// Binning of suspect lists into per-cell records in a pool (see Suspects in drh_device.h).
-struct SusBinArgs {        // a view's list by screen cell (one workgroup per cell)
+// Binning of suspect lists into per-cell records in a pool (see Suspects in drh_device.h), in
+// three passes over the suspects -- count the kept (suspect, cell) pairs per cell, scan the
+// counts into pool offsets, fill -- one wavefront per suspect walking its candidate cells.
+struct BinPool {
+  float4* pool;           // two float4 per record
+  int cap;                // records
+  int2* cell;             // (offset, n) per cell (n = -1: overflowed, the rays use the list)
+  int* cnt;               // per cell: kept pairs, then (after the scan) the fill cursor; zeroed
+  int cells_x, cells_y;
+};
+struct SusBinArgs {        // the views' lists by screen cell (blockIdx.y = view)
   const DevCam* cams;
-  int view;
-  const float4* list;     // the view's suspects (N.xyz, id)
-  const int* count;
+  const float4* list;     // view v's suspects (N.xyz, id) at list + v * F
+  const int* count;       // per view
+  int F;
   const float4* bf_a;
   const float4* bf_e1;
   const float4* bf_e2;
-  float4* pool;           // two float4 per record
-  int pool_cap;           // records
-  int* pool_used;         // zeroed by the caller
-  int2* cell;             // (offset, n) per cell
-  int cells_x, cells_y;
+  BinPool bp;             // view v's at pool + 2 v pool_stride, cell / cnt + v cell_stride (its own cells_x / y)
+  long long pool_stride, cell_stride;
 };
-struct LightBinArgs {      // the light's list over a grid across the light (one workgroup per cell)
+struct LightBinArgs {      // the light's list over a grid across the light
   const float4* list;     // the light's suspects (A, id), (pvec, invDet)
   const int* count;
   const float4* bf_e1;
   const float4* nodes;    // LBVH (the root's child boxes give the scene's extent)
   float Lx, Ly, Lz;       // the light direction L (shadow rays go along -L)
   float4* grid;           // out: the grid's placement (3 float4)
-  float4* pool;
-  int pool_cap;
-  int* pool_used;         // zeroed by the caller
-  int2* cell;
-  int cells_x, cells_y;
+  BinPool bp;
 };
 
 struct RenderArgs {
@@ -340,7 +342,7 @@ void launch_refit(const RefitArgs& a, hipStream_t s);
 void launch_pack(const PackArgs& a, hipStream_t s);
 void launch_reorder(const ReorderArgs& a, hipStream_t s);
 void launch_suspects(const SuspectArgs& a, hipStream_t s);
-void launch_sus_bin(const SusBinArgs& a, hipStream_t s);
+void launch_sus_bin(const SusBinArgs& a, int views, hipStream_t s);   // count, scan and fill
 void launch_light_bin(const LightBinArgs& a, hipStream_t s);
 void launch_render(int mode, const RenderArgs& a, int tiles, hipStream_t s);
 void launch_loss_final(const double* part, int n, float* out, double denom, hipStream_t s);

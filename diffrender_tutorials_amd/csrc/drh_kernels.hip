@@ -73,7 +73,7 @@ __global__ void k_prepare(PrepareArgs a) {
       a.bounds[0] = a.bounds[1] = a.bounds[2] = 0xFFFFFFFFu;
       a.bounds[3] = a.bounds[4] = a.bounds[5] = 0u;
     }
-    if (a.zero) { a.zero[0] = 0u; a.zero[1] = 0u; }
+    if (a.zero) *a.zero = 0u;
   }
   if (k >= a.V) return;
   int m = 0;
@@ -436,51 +436,177 @@ __global__ void k_suspects(SuspectArgs a) {
   }
 }
 
-// A cell's stashed records (cnt_ref of them, read after the barrier; SUS_STASH at most kept) into the pool: one offset per
-// cell from the pool counter; a cell with more records than the stash, or past the pool's
-// capacity, gets n = -1 (its rays use the whole list).  Called by every thread of the block.
-__device__ __forceinline__ void pool_cell(const float4* stash, const int& cnt_ref, int& off, float4* pool, int cap,
-                                          int* used, int2* cell) {
-  __syncthreads();   // every record is stashed and counted
-  const int cnt = cnt_ref;
-  if (threadIdx.x == 0) {
-    int o = -1;
-    if (cnt <= SUS_STASH) {
-      o = atomicAdd(used, cnt);
-      if (o + cnt > cap) o = -1;
-    }
-    off = o;
-    *cell = o < 0 ? make_int2(0, -1) : make_int2(o, cnt);
+// ---------------------------------------------------------------------------
+// Suspect binning.  Every pass runs one wavefront per suspect: its lanes walk the suspect's
+// candidate cells (a box that bounds where it can be accepted) and keep a cell when an exact
+// test says some ray of the cell could be accepted; the count pass adds 1 per kept cell, the
+// scan turns the counts into pool offsets (a cell past the pool's capacity gets n = -1) and
+// the fill pass writes each kept pair's record at its cell's next slot.  Count and fill compute
+// the same cells (the same arithmetic).
+// ---------------------------------------------------------------------------
+
+// View binning.  A record is kept for a screen cell unless a necessary condition of its
+// clamped acceptance fails for every direction d through the cell:
+//   C: |d.N| < 1.5e-6 (the per-lane filter the rays apply, N from the list);
+//   U: |qvec.d| <= ub, with qvec = (o - A) x e1 as mt_test computes it for this view's rays
+//      and ub >= (1 + 2^-23) max(|det|, 1e-6) for every det the C filter lets through (u in
+//      (0, 1) needs |qvec.d| |invDet| < 1);
+//   V: |w.d| <= ub + ev, w = e2 x (o - A): mt_test's v needs |(o - A).(d x e2)| < ub as well;
+//   K: |(a qvec - b w).d| <= |a| (ub + eu) + |b| (ub + ev), U and V combined with the
+//      coordinates (a, b) of o - A along (e1, e2) in the basis (e1, e2, N/|N|).  qvec and w
+//      are nearly parallel to N (the triangle's plane passes close to o), so the U and V
+//      bands are two almost coincident lines across the image; in a qvec - b w the N parts
+//      cancel, and its band crosses them, which confines the record to a short stretch.
+// eu, ev bound the float dot products' rounding and the ray direction's error (2e-5 + 1e-7 |o|
+// relative), scaled by |qvec| and |o - A| |e2|.  The exact direction through screen point
+// (x, y) is +-D / |D| with D = R p (R: the rotation of the view's inverse world-view matrix,
+// p: proj_inv (ndc, 0, 1), both affine in x and y; the sign that of p.w), so n.D is affine
+// over a cell and |D| convex: if n.D has one sign at the cell's four corners and
+// min |n.D| > tau max |D|, then |n.d| > tau everywhere in the cell.  Candidate cells: the box
+// shared by the parallelograms where two of the conditions hold with |D| at its largest over
+// the grid, over every pair of non-parallel bands.  Double precision.
+struct CamBin {
+  double D0[3], Dx[3], Dy[3];   // D(x, y) = D0 + x Dx + y Dy
+  double dmax;                  // max |D| over the grid
+  double n[4][3], tau[4];       // conditions C, U, V, K (K: tau < 0 when absent)
+  double f[4][3];               // n_i.D(x, y) = f[i][0] + x f[i][1] + y f[i][2]
+  double t2[4];                 // tau_i^2
+  double g[6];                  // |D(x, y)|^2 = g0 + x (g1 + x g3 + y g4) + y (g2 + y g5)
+  float4 e, q;                  // the record
+};
+__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ void cam_bin_setup(const SusBinArgs& a, int view, int s, int cells_x, int cells_y, CamBin& b) {
+  const DevCam& c = a.cams[view];
+  const float* B = c.proj_inv;
+  const float* M = c.wv_inv;
+  double P0[3], Px[3], Py[3];
+  for (int r = 0; r < 3; r++) {
+    P0[r] = -(double)B[r] + (double)B[4 + r] + (double)B[12 + r];
+    Px[r] = 2.0 / (double)c.W * (double)B[r];
+    Py[r] = -2.0 / (double)c.H * (double)B[4 + r];
   }
-  __syncthreads();
-  if (off >= 0)
-    for (int i = threadIdx.x; i < 2 * cnt; i += blockDim.x) pool[2ll * off + i] = stash[i];
+  for (int r = 0; r < 3; r++) {
+    b.D0[r] = P0[0] * M[r] + P0[1] * M[4 + r] + P0[2] * M[8 + r];
+    b.Dx[r] = Px[0] * M[r] + Px[1] * M[4 + r] + Px[2] * M[8 + r];
+    b.Dy[r] = Py[0] * M[r] + Py[1] * M[4 + r] + Py[2] * M[8 + r];
+  }
+  const double gw = (double)(cells_x * SUS_CELL_PX), gh = (double)(cells_y * SUS_CELL_PX);
+  b.dmax = 0.0;
+  for (int k = 0; k < 4; k++) {
+    const double x = (k & 1) ? gw : 0.0, y = (k & 2) ? gh : 0.0;
+    double D[3];
+    for (int r = 0; r < 3; r++) D[r] = b.D0[r] + x * b.Dx[r] + y * b.Dy[r];
+    b.dmax = fmax(b.dmax, sqrt(dot3(D, D)));
+  }
+  float oa[4];
+  mv4(c.wv_inv, 0.0f, 0.0f, 0.0f, 1.0f, oa);   // eye_ray's origin, bit for bit
+  const F3 o = {oa[0], oa[1], oa[2]};
+  const double delta = 2e-5 + 1e-7 * sqrt((double)o.x * o.x + (double)o.y * o.y + (double)o.z * o.z);
+  const float4 e = a.list[(long long)view * a.F + s];
+  const int g = __float_as_int(e.w);
+  const float4 A = a.bf_a[g], E1 = a.bf_e1[g], E2 = a.bf_e2[g];
+  const F3 tv = sub(o, f4xyz(A));   // mt_test's tvec
+  const F3 qvec = cross(tv, f4xyz(E1));
+  const double l1 = sqrt((double)E1.x * E1.x + (double)E1.y * E1.y + (double)E1.z * E1.z);
+  const double l2 = sqrt((double)E2.x * E2.x + (double)E2.y * E2.y + (double)E2.z * E2.z);
+  const float ub = (float)(1.6e-6 + 1e-6 * l1 * l2);
+  b.e = e;
+  b.q = make_float4(qvec.x, qvec.y, qvec.z, ub);
+  b.n[0][0] = e.x; b.n[0][1] = e.y; b.n[0][2] = e.z;
+  b.tau[0] = 1.5e-6 + (delta + 1e-6) * sqrt(dot3(b.n[0], b.n[0]));
+  b.n[1][0] = qvec.x; b.n[1][1] = qvec.y; b.n[1][2] = qvec.z;
+  const double eu = (delta + 1e-6) * sqrt(dot3(b.n[1], b.n[1]));
+  b.tau[1] = (double)ub + eu;
+  const double tx = tv.x, ty = tv.y, tz = tv.z, ax = E1.x, ay = E1.y, az = E1.z, bx = E2.x, by = E2.y, bz = E2.z;
+  b.n[2][0] = by * tz - bz * ty; b.n[2][1] = bz * tx - bx * tz; b.n[2][2] = bx * ty - by * tx;   // e2 x tvec
+  const double ev = (delta + 1e-6) * sqrt(tx * tx + ty * ty + tz * tz) * l2;
+  b.tau[2] = (double)ub + ev;
+  // (ka, kb): tvec = ka e1 + kb e2 + c N/|N|, by Cramer's rule with det = |N|
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double nn = sqrt(nx * nx + ny * ny + nz * nz);
+  b.tau[3] = -1.0;
+  if (nn > 0.0) {
+    const double hx = nx / nn, hy = ny / nn, hz = nz / nn;
+    const double ka = (tx * (by * hz - bz * hy) + ty * (bz * hx - bx * hz) + tz * (bx * hy - by * hx)) / nn;
+    const double kb = (tx * (hy * az - hz * ay) + ty * (hz * ax - hx * az) + tz * (hx * ay - hy * ax)) / nn;
+    for (int r = 0; r < 3; r++) b.n[3][r] = ka * b.n[1][r] - kb * b.n[2][r];
+    b.tau[3] = (fabs(ka) * ((double)ub + eu) + fabs(kb) * ((double)ub + ev)) * (1.0 + 1e-9);
+  }
+  for (int i = 0; i < 4; i++) {
+    b.f[i][0] = dot3(b.D0, b.n[i]); b.f[i][1] = dot3(b.Dx, b.n[i]); b.f[i][2] = dot3(b.Dy, b.n[i]);
+    b.t2[i] = b.tau[i] * b.tau[i];
+  }
+  b.g[0] = dot3(b.D0, b.D0); b.g[1] = 2.0 * dot3(b.D0, b.Dx); b.g[2] = 2.0 * dot3(b.D0, b.Dy);
+  b.g[3] = dot3(b.Dx, b.Dx); b.g[4] = 2.0 * dot3(b.Dx, b.Dy); b.g[5] = dot3(b.Dy, b.Dy);
+}
+// false when |n_i.d| > tau_i for every direction through the cell, for some condition i
+__device__ __forceinline__ bool cam_cell_kept(const CamBin& b, int cx, int cy) {
+  double f[4][4], dm2 = 0.0;   // dm2: max |D|^2 over the corners
+  for (int k = 0; k < 4; k++) {
+    const double x = (double)((cx + (k & 1)) * SUS_CELL_PX), y = (double)((cy + (k >> 1)) * SUS_CELL_PX);
+    dm2 = fmax(dm2, b.g[0] + x * (b.g[1] + x * b.g[3] + y * b.g[4]) + y * (b.g[2] + y * b.g[5]));
+    for (int i = 0; i < 4; i++) f[i][k] = b.f[i][0] + x * b.f[i][1] + y * b.f[i][2];
+  }
+  for (int i = 0; i < 4; i++) {
+    if (!(b.tau[i] >= 0.0)) continue;
+    const bool pos = f[i][0] > 0.0 && f[i][1] > 0.0 && f[i][2] > 0.0 && f[i][3] > 0.0;
+    const bool neg = f[i][0] < 0.0 && f[i][1] < 0.0 && f[i][2] < 0.0 && f[i][3] < 0.0;
+    if (!pos && !neg) continue;
+    const double lo = fmin(fmin(fabs(f[i][0]), fabs(f[i][1])), fmin(fabs(f[i][2]), fabs(f[i][3])));
+    if (lo * lo > b.t2[i] * dm2) return false;   // min |n.D| > tau max |D|, squared
+  }
+  return true;
+}
+// candidate cells: [cx0, cx1] x [cy0, cy1], the box shared by the parallelograms of every pair
+// of non-parallel conditions (|f_i| <= tau_i max |D| for both)
+__device__ __forceinline__ void cam_candidates(const CamBin& b, int cells_x, int cells_y, int& cx0, int& cx1, int& cy0,
+                                               int& cy1) {
+  double x0 = 0.0, x1 = (double)(cells_x * SUS_CELL_PX), y0 = 0.0, y1 = (double)(cells_y * SUS_CELL_PX);
+  for (int i = 0; i < 4; i++)
+    for (int j = i + 1; j < 4; j++) {
+      if (!(b.tau[i] >= 0.0) || !(b.tau[j] >= 0.0)) continue;
+      const double ti = b.tau[i] * b.dmax, tj = b.tau[j] * b.dmax;
+      const double det = b.f[i][1] * b.f[j][2] - b.f[i][2] * b.f[j][1];
+      const double scale = (fabs(b.f[i][1]) + fabs(b.f[i][2])) * (fabs(b.f[j][1]) + fabs(b.f[j][2]));
+      if (!(fabs(det) > 1e-9 * scale) || !(scale < 1e300)) continue;
+      double px0 = 1e300, px1 = -1e300, py0 = 1e300, py1 = -1e300;
+      for (int k = 0; k < 4; k++) {
+        const double ri = ((k & 1) ? ti : -ti) - b.f[i][0], rj = ((k & 2) ? tj : -tj) - b.f[j][0];
+        const double x = (ri * b.f[j][2] - rj * b.f[i][2]) / det, y = (b.f[i][1] * rj - b.f[j][1] * ri) / det;
+        px0 = fmin(px0, x); px1 = fmax(px1, x); py0 = fmin(py0, y); py1 = fmax(py1, y);
+      }
+      if (!(px0 == px0 && px1 == px1 && py0 == py0 && py1 == py1)) continue;
+      x0 = fmax(x0, px0); x1 = fmin(x1, px1); y0 = fmax(y0, py0); y1 = fmin(y1, py1);
+    }
+  const double g0 = floor(x0 / SUS_CELL_PX) - 1.0, g1 = floor(x1 / SUS_CELL_PX) + 1.0;
+  const double h0 = floor(y0 / SUS_CELL_PX) - 1.0, h1 = floor(y1 / SUS_CELL_PX) + 1.0;
+  cx0 = (int)fmax(0.0, fmin(g0, (double)cells_x)); cx1 = (int)fmin((double)(cells_x - 1), fmax(g1, -1.0));
+  cy0 = (int)fmax(0.0, fmin(h0, (double)cells_y)); cy1 = (int)fmin((double)(cells_y - 1), fmax(h1, -1.0));
 }
 
-// Bins the light's suspect list over a grid across the light.  Shadow rays share d = -L, so a
-// record's determinant and invDet are fixed and its acceptance confines the ray's origin o:
-// with s = sign(invDet), l1 = s (o - A).pvec (mt_test's v before the scale |invDet| = 1e6)
-// and l2 = s (o - A).(e1 x d) (its u), acceptance needs l1 > 0, l2 > 0 and l1 + l2 < 1e-6
-// (1 + 2.4e-7).  l1, l2 are affine in the origin's coordinates (u, v) = (o.a1, o.a2) across
-// the light (a1, a2 orthonormal, perpendicular to L; the component along L enters only
-// through pvec's rounding), so the origins lie in the triangle of the (u, v) plane where
-// l1 >= -m1, l2 >= -m2, l1 + l2 <= 1e-6 (1 + 2.4e-7) + m1 + m2, with m1, m2 bounds on the
-// float tvec / dot rounding (3e-7 and 5e-7 of |o - A| |pvec| and |o - A| |e1|, |o - A| at most
-// the scene box's diagonal) and on the component along L.  A record is kept for a cell when
-// that triangle's box, widened by one cell on every side, overlaps the cell (a ray's own
-// (u, v) in float differs from the exact by far less than a cell).  The grid covers the
-// scene box (the LBVH root's child boxes) with a margin.  Every block computes the same
-// placement; block 0 stores it for the rays.
-__global__ __launch_bounds__(256) void k_light_bin(LightBinArgs a) {
-  __shared__ float4 stash[2 * SUS_STASH];
-  __shared__ int cnt, off;
-  const int cell = blockIdx.x;
-  const int gx = cell % a.cells_x, gy = cell / a.cells_x;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
-  // orthonormal frame across L
+// Light binning over a grid across the light.  Shadow rays share d = -L, so a record's
+// determinant and invDet are fixed and its acceptance confines the ray's origin o: with
+// s = sign(invDet), l1 = s (o - A).pvec (mt_test's v before the scale |invDet| = 1e6) and
+// l2 = s (o - A).(e1 x d) (its u), acceptance needs l1 > 0, l2 > 0 and
+// l1 + l2 < 1e-6 (1 + 2.4e-7).  l1, l2 are affine in the origin's coordinates (u, v) =
+// (o.a1, o.a2) across the light (a1, a2 orthonormal, perpendicular to L; the component along
+// L enters only through pvec's rounding), so the origins lie in the triangle of the (u, v)
+// plane where l1 >= -m1, l2 >= -m2, l1 + l2 <= 1e-6 (1 + 2.4e-7) + m1 + m2, with m1, m2
+// bounds on the float tvec / dot rounding (3e-7 and 5e-7 of |o - A| |pvec| and |o - A| |e1|,
+// |o - A| at most the scene box's diagonal) and on the component along L.  Candidate cells: the
+// triangle's box; a cell is kept when no triangle edge has all four (widened) corners outside
+// (separating axes).  A cell is widened by 2% of its size plus the rounding of a ray's own
+// float (u, v).  The grid covers the scene box (the LBVH root's child boxes) with a margin.
+struct LightGrid {
+  double b1[3], b2[3], L[3];
+  float u0, v0, iu, iv;
+  double wmax, diag, slack;
+  bool ok;
+};
+__device__ __forceinline__ void light_grid(const LightBinArgs& a, LightGrid& G) {
   const double ll = sqrt((double)a.Lx * a.Lx + (double)a.Ly * a.Ly + (double)a.Lz * a.Lz);
   const double Lx = a.Lx / ll, Ly = a.Ly / ll, Lz = a.Lz / ll;
+  G.L[0] = Lx; G.L[1] = Ly; G.L[2] = Lz;
   const double ax = fabs(Lx), ay = fabs(Ly), az = fabs(Lz);
   // a1 = L x (the axis least aligned with L), normalised; a2 = L x a1
   double ex = 0, ey = 0, ez = 0;
@@ -488,158 +614,237 @@ __global__ __launch_bounds__(256) void k_light_bin(LightBinArgs a) {
   double b1x = Ly * ez - Lz * ey, b1y = Lz * ex - Lx * ez, b1z = Lx * ey - Ly * ex;
   const double bl = sqrt(b1x * b1x + b1y * b1y + b1z * b1z);
   b1x /= bl; b1y /= bl; b1z /= bl;
-  const double b2x = Ly * b1z - Lz * b1y, b2y = Lz * b1x - Lx * b1z, b2z = Lx * b1y - Ly * b1x;
-  // scene box: the root's two child boxes
+  G.b1[0] = b1x; G.b1[1] = b1y; G.b1[2] = b1z;
+  G.b2[0] = Ly * b1z - Lz * b1y; G.b2[1] = Lz * b1x - Lx * b1z; G.b2[2] = Lx * b1y - Ly * b1x;
   const float4 n0 = a.nodes[0], n1 = a.nodes[1], n2 = a.nodes[2];
   const double lo[3] = {fmin((double)n0.x, (double)n1.z), fmin((double)n0.y, (double)n1.w), fmin((double)n0.z, (double)n2.x)};
   const double hi[3] = {fmax((double)n0.w, (double)n2.y), fmax((double)n1.x, (double)n2.z), fmax((double)n1.y, (double)n2.w)};
-  double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300, wmax = 0.0;
+  double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300, cmax = 0.0;
+  G.wmax = 0.0;
   for (int k = 0; k < 8; k++) {
-    const double x = (k & 1) ? hi[0] : lo[0], y = (k & 2) ? hi[1] : lo[1], z = (k & 4) ? hi[2] : lo[2];
-    const double u = x * b1x + y * b1y + z * b1z, v = x * b2x + y * b2y + z * b2z, w = x * Lx + y * Ly + z * Lz;
+    const double p[3] = {(k & 1) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1], (k & 4) ? hi[2] : lo[2]};
+    const double u = dot3(p, G.b1), v = dot3(p, G.b2), w = dot3(p, G.L);
     umin = fmin(umin, u); umax = fmax(umax, u); vmin = fmin(vmin, v); vmax = fmax(vmax, v);
-    wmax = fmax(wmax, fabs(w));
+    G.wmax = fmax(G.wmax, fabs(w));
+    cmax = fmax(cmax, sqrt(dot3(p, p)));
   }
-  const double diag = sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  G.diag = sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]));
   const double pad = 0.01 * (umax - umin + vmax - vmin) + 1e-3 * (1.0 + fmax(fmax(fabs(umin), fabs(umax)), fmax(fabs(vmin), fabs(vmax))));
-  const float u0 = (float)(umin - pad), v0 = (float)(vmin - pad);
-  const float iu = (float)(a.cells_x / (umax - umin + 2 * pad)), iv = (float)(a.cells_y / (vmax - vmin + 2 * pad));
-  if (cell == 0 && threadIdx.x == 0) {
-    a.grid[0] = make_float4((float)b1x, (float)b1y, (float)b1z, u0);
-    a.grid[1] = make_float4((float)b2x, (float)b2y, (float)b2z, v0);
-    a.grid[2] = make_float4(iu, iv, 0.0f, 0.0f);
-  }
-  const bool ok = diag == diag && diag < 1e30 && iu > 0.0f && iv > 0.0f && iu < 3.0e38f && iv < 3.0e38f;
+  G.u0 = (float)(umin - pad);
+  G.v0 = (float)(vmin - pad);
+  G.iu = (float)(a.bp.cells_x / (umax - umin + 2 * pad));
+  G.iv = (float)(a.bp.cells_y / (vmax - vmin + 2 * pad));
+  G.slack = 0.02 / fmin((double)G.iu, (double)G.iv) + 1e-5 * (1.0 + cmax);
+  G.ok = G.diag == G.diag && G.diag < 1e30 && G.iu > 0.0f && G.iv > 0.0f && G.iu < 3.0e38f && G.iv < 3.0e38f;
+}
+struct LightRec {
+  double j11, j12, j21, j22, c1, c2, m1, m2, T;
+  float4 r0, r1;
+};
+__device__ __forceinline__ void light_rec(const LightBinArgs& a, const LightGrid& G, int s, LightRec& R) {
+  R.r0 = a.list[2 * s];
+  R.r1 = a.list[2 * s + 1];
+  const float4 E1 = a.bf_e1[__float_as_int(R.r0.w)];
   const F3 d = mul(-1.0f, F3{a.Lx, a.Ly, a.Lz});   // the shadow rays' direction, as lambert_terms forms it
-  const int n = *a.count;
-  for (int s = threadIdx.x; s < n; s += blockDim.x) {
-    const float4 r0 = a.list[2 * s], r1 = a.list[2 * s + 1];
-    bool keep = true;
-    if (ok) {
-      const float4 E1 = a.bf_e1[__float_as_int(r0.w)];
-      const double sg = r1.w > 0.0f ? 1.0 : -1.0;
-      const double px = r1.x, py = r1.y, pz = r1.z;                       // pvec
-      const double qx = (double)E1.y * d.z - (double)E1.z * d.y;          // e1 x d
-      const double qy = (double)E1.z * d.x - (double)E1.x * d.z;
-      const double qz = (double)E1.x * d.y - (double)E1.y * d.x;
-      const double lp = sqrt(px * px + py * py + pz * pz), lq = sqrt(qx * qx + qy * qy + qz * qz);
-      const double le1 = sqrt((double)E1.x * E1.x + (double)E1.y * E1.y + (double)E1.z * E1.z);
-      // l1 = j11 u + j12 v + c1 (+ the L component), l2 likewise
-      const double j11 = sg * (b1x * px + b1y * py + b1z * pz), j12 = sg * (b2x * px + b2y * py + b2z * pz);
-      const double j21 = sg * (b1x * qx + b1y * qy + b1z * qz), j22 = sg * (b2x * qx + b2y * qy + b2z * qz);
-      const double c1 = -sg * (r0.x * px + r0.y * py + r0.z * pz), c2 = -sg * (r0.x * qx + r0.y * qy + r0.z * qz);
-      const double w1 = fabs(Lx * px + Ly * py + Lz * pz) * wmax + 1e-7 * lp * wmax;
-      const double w2 = fabs(Lx * qx + Ly * qy + Lz * qz) * wmax + 1e-7 * lq * wmax;
-      const double m1 = 2.0 * (3e-7 * diag * lp + w1) + 1e-12, m2 = 2.0 * (5e-7 * diag * le1 + w2) + 1e-12;
-      const double T = 1e-6 * (1.0 + 2.4e-7) + m1 + m2;
-      const double det = j11 * j22 - j12 * j21;
-      if (det != 0.0 && det == det) {
-        const double L1[3] = {-m1, T + m2, -m1}, L2[3] = {-m2, -m2, T + m1};
-        double ulo = 1e300, uhi = -1e300, vlo = 1e300, vhi = -1e300;
-        for (int k = 0; k < 3; k++) {
-          const double r1_ = L1[k] - c1, r2_ = L2[k] - c2;
-          const double u = (r1_ * j22 - r2_ * j12) / det, v = (j11 * r2_ - j21 * r1_) / det;
-          ulo = fmin(ulo, u); uhi = fmax(uhi, u); vlo = fmin(vlo, v); vhi = fmax(vhi, v);
-        }
-        const double cu0 = floor((ulo - u0) * iu) - 1.0, cu1 = floor((uhi - u0) * iu) + 1.0;
-        const double cv0 = floor((vlo - v0) * iv) - 1.0, cv1 = floor((vhi - v0) * iv) + 1.0;
-        keep = !(gx < cu0 || gx > cu1 || gy < cv0 || gy > cv1);   // NaN bounds keep
-      }
-    }
-    if (!keep) continue;
-    const int k = atomicAdd(&cnt, 1);
-    if (k < SUS_STASH) { stash[2 * k] = r0; stash[2 * k + 1] = r1; }
+  const double sg = R.r1.w > 0.0f ? 1.0 : -1.0;
+  const double p[3] = {R.r1.x, R.r1.y, R.r1.z};                                               // pvec
+  const double q[3] = {(double)E1.y * d.z - (double)E1.z * d.y, (double)E1.z * d.x - (double)E1.x * d.z,
+                       (double)E1.x * d.y - (double)E1.y * d.x};                                 // e1 x d
+  const double A[3] = {R.r0.x, R.r0.y, R.r0.z};
+  const double lp = sqrt(dot3(p, p)), lq = sqrt(dot3(q, q));
+  const double le1 = sqrt((double)E1.x * E1.x + (double)E1.y * E1.y + (double)E1.z * E1.z);
+  R.j11 = sg * dot3(G.b1, p); R.j12 = sg * dot3(G.b2, p);
+  R.j21 = sg * dot3(G.b1, q); R.j22 = sg * dot3(G.b2, q);
+  R.c1 = -sg * dot3(A, p); R.c2 = -sg * dot3(A, q);
+  const double w1 = fabs(dot3(G.L, p)) * G.wmax + 1e-7 * lp * G.wmax;
+  const double w2 = fabs(dot3(G.L, q)) * G.wmax + 1e-7 * lq * G.wmax;
+  R.m1 = 2.0 * (3e-7 * G.diag * lp + w1) + 1e-12;
+  R.m2 = 2.0 * (5e-7 * G.diag * le1 + w2) + 1e-12;
+  R.T = 1e-6 * (1.0 + 2.4e-7) + R.m1 + R.m2;
+}
+__device__ __forceinline__ void light_candidates(const LightGrid& G, const LightRec& R, int cells_x, int cells_y,
+                                                 int& cx0, int& cx1, int& cy0, int& cy1) {
+  cx0 = 0; cx1 = cells_x - 1; cy0 = 0; cy1 = cells_y - 1;
+  const double det = R.j11 * R.j22 - R.j12 * R.j21;
+  if (!(det != 0.0) || det != det) return;
+  const double L1[3] = {-R.m1, R.T + R.m2, -R.m1}, L2[3] = {-R.m2, -R.m2, R.T + R.m1};
+  double ulo = 1e300, uhi = -1e300, vlo = 1e300, vhi = -1e300;
+  for (int k = 0; k < 3; k++) {
+    const double r1 = L1[k] - R.c1, r2 = L2[k] - R.c2;
+    const double u = (r1 * R.j22 - r2 * R.j12) / det, v = (R.j11 * r2 - R.j21 * r1) / det;
+    ulo = fmin(ulo, u); uhi = fmax(uhi, u); vlo = fmin(vlo, v); vhi = fmax(vhi, v);
   }
-  pool_cell(stash, cnt, off, a.pool, a.pool_cap, a.pool_used, a.cell + cell);
+  if (!(ulo == ulo && uhi == uhi && vlo == vlo && vhi == vhi)) return;
+  const double g0 = floor((ulo - G.slack - G.u0) * G.iu), g1 = floor((uhi + G.slack - G.u0) * G.iu);
+  const double h0 = floor((vlo - G.slack - G.v0) * G.iv), h1 = floor((vhi + G.slack - G.v0) * G.iv);
+  cx0 = (int)fmax(0.0, fmin(g0, (double)cells_x)); cx1 = (int)fmin((double)(cells_x - 1), fmax(g1, -1.0));
+  cy0 = (int)fmax(0.0, fmin(h0, (double)cells_y)); cy1 = (int)fmin((double)(cells_y - 1), fmax(h1, -1.0));
+}
+__device__ __forceinline__ bool light_cell_kept(const LightGrid& G, const LightRec& R, int gx, int gy) {
+  const double ua = G.u0 + gx / (double)G.iu - G.slack, ub = G.u0 + (gx + 1) / (double)G.iu + G.slack;
+  const double va = G.v0 + gy / (double)G.iv - G.slack, vb = G.v0 + (gy + 1) / (double)G.iv + G.slack;
+  bool out1 = true, out2 = true, out3 = true;
+  for (int k = 0; k < 4; k++) {
+    const double u = (k & 1) ? ub : ua, v = (k & 2) ? vb : va;
+    const double l1 = R.j11 * u + R.j12 * v + R.c1, l2 = R.j21 * u + R.j22 * v + R.c2;
+    out1 = out1 && l1 < -R.m1;
+    out2 = out2 && l2 < -R.m2;
+    out3 = out3 && l1 + l2 > R.T;
+  }
+  return !(out1 || out2 || out3);
 }
 
-// Bins a view's suspect list by screen cell.  A record is kept for a cell unless a necessary
-// condition of its clamped acceptance fails for every direction d through the cell:
-//   C: |d.N| < 1.5e-6 (the per-lane filter the rays apply, N from the list);
-//   U: |qvec.d| <= ub, with qvec = (o - A) x e1 as mt_test computes it for this view's rays
-//      and ub >= (1 + 2^-23) max(|det|, 1e-6) for every det the C filter lets through (u in
-//      (0, 1) needs |qvec.d| |invDet| < 1);
-//   V: |w.d| <= ub + ev, w = e2 x (o - A): mt_test's v needs |(o - A).(d x e2)| < ub as well;
-//   U and V combined: |(a qvec - b w).d| <= |a| (ub + eu) + |b| (ub + ev) for the coordinates
-//      (a, b) of o - A along (e1, e2) in the basis (e1, e2, N/|N|).  Both qvec and w are nearly
-//      parallel to N (the triangle's plane passes close to o), so the U and V bands are two
-//      almost coincident lines across the image; in a qvec - b w the N parts cancel, and its
-//      band crosses them, which confines the record to a short stretch of the line.
-// eu, ev bound the float dot products' rounding and the ray direction's error (2e-5 + 1e-7 |o|
-// relative), scaled by |qvec| and |o - A| |e2|.  The exact direction through screen point
-// (x, y) is +-R p(x, y) / |R p| (R: the rotation of the view's inverse world-view matrix, p:
-// proj_inv (ndc, 0, 1), both affine in x, y, the sign that of p.w), so n.D is affine over the
-// cell and |D| convex: if n.D has one sign at the four corners and min |n.D| > tau max |D|,
-// then |n.d| > tau everywhere in the cell.  Corners in double.
+// kept pairs into the pool: PASS 0 counts, PASS 1 fills (after the scan)
+template <int PASS>
+__device__ __forceinline__ void bin_emit(const BinPool& bp, int cell, float4 r0, float4 r1) {
+  if (PASS == 0) {
+    atomicAdd(bp.cnt + cell, 1);
+  } else {
+    const int2 ci = bp.cell[cell];
+    if (ci.y < 0) return;
+    const int k = atomicAdd(bp.cnt + cell, 1);
+    bp.pool[2ll * (ci.x + k)] = r0;
+    bp.pool[2ll * (ci.x + k) + 1] = r1;
+  }
+}
+
+// Calls f(X, Y) for the cells of [cx0, cx1] x [cy0, cy1] (in cell units: cell (X, Y) is the unit
+// square at (X, Y)) that the band |A + BX x + BY y| <= t meets, widened by one cell; the lanes
+// take the rows (or columns, for a band closer to horizontal) in turn.  Without a usable
+// direction every cell of the box is visited.
+template <class Fn>
+__device__ __forceinline__ void band_walk(double A, double BX, double BY, double t, int cx0, int cx1, int cy0, int cy1,
+                                          int lane, Fn f) {
+  const int wx = cx1 - cx0 + 1, wy = cy1 - cy0 + 1;
+  if (wx <= 0 || wy <= 0) return;   // an empty range has no cell
+  const bool dir = fabs(BX) + fabs(BY) > 0.0 && fabs(A) + fabs(BX) + fabs(BY) + t < 1e300 && t == t;
+  if (!dir) {
+    for (long long k = lane; k < (long long)wx * wy; k += 64) f(cx0 + (int)(k % wx), cy0 + (int)(k / wx));
+    return;
+  }
+  const bool rows = fabs(BX) >= fabs(BY);
+  const double P = rows ? BX : BY, Q = rows ? BY : BX;   // |A + P a + Q b| <= t along a row (b fixed) or column
+  const int o0 = rows ? cy0 : cx0, o1 = rows ? cy1 : cx1, i0 = rows ? cx0 : cy0, i1 = rows ? cx1 : cy1;
+  for (int r = o0 + lane; r <= o1; r += 64) {
+    double lo = 1e300, hi = -1e300;
+    for (int k = 0; k < 4; k++) {
+      const double b = (double)(r + (k & 1)), sg = (k & 2) ? t : -t;
+      const double a = (sg - A - Q * b) / P;
+      lo = fmin(lo, a); hi = fmax(hi, a);
+    }
+    int a0 = i0, a1 = i1;
+    if (lo == lo && hi == hi) {
+      a0 = (int)fmax((double)i0, fmin(floor(lo) - 1.0, (double)i1 + 1.0));
+      a1 = (int)fmin((double)i1, fmax(floor(hi) + 1.0, (double)i0 - 1.0));
+    }
+    for (int c = a0; c <= a1; c++) {
+      if (rows) f(c, r); else f(r, c);
+    }
+  }
+}
+
+template <int PASS>
 __global__ __launch_bounds__(256) void k_sus_bin(SusBinArgs a) {
-  __shared__ float4 stash[2 * SUS_STASH];
-  __shared__ int cnt, off;
-  const int cell = blockIdx.x;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
-  const DevCam& c = a.cams[a.view];
-  const int cx = cell % a.cells_x, cy = cell / a.cells_x;
-  double D[4][3], dmax = 0.0;
-  for (int k = 0; k < 4; k++) {
-    const double x = (double)((cx + (k & 1)) * SUS_CELL_PX), y = (double)((cy + (k >> 1)) * SUS_CELL_PX);
-    const double nx = 2.0 * (x / (double)c.W) - 1.0, ny = -2.0 * (y / (double)c.H) + 1.0;
-    double p[3];
-    for (int r = 0; r < 3; r++) p[r] = nx * (double)c.proj_inv[r] + ny * (double)c.proj_inv[4 + r] + (double)c.proj_inv[12 + r];
-    for (int r = 0; r < 3; r++)
-      D[k][r] = p[0] * (double)c.wv_inv[r] + p[1] * (double)c.wv_inv[4 + r] + p[2] * (double)c.wv_inv[8 + r];
-    dmax = fmax(dmax, sqrt(D[k][0] * D[k][0] + D[k][1] * D[k][1] + D[k][2] * D[k][2]));
+  const int view = blockIdx.y;
+  BinPool bp = a.bp;
+  bp.pool += 2 * a.pool_stride * view;
+  bp.cell += a.cell_stride * view;
+  bp.cnt += a.cell_stride * view;
+  bp.cells_x = (a.cams[view].w + SUS_CELL_PX - 1) / SUS_CELL_PX;
+  bp.cells_y = (a.cams[view].h + SUS_CELL_PX - 1) / SUS_CELL_PX;
+  const int lane = threadIdx.x & 63;
+  const int n = a.count[view];
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += waves) {
+    CamBin b;
+    cam_bin_setup(a, view, s, bp.cells_x, bp.cells_y, b);
+    int cx0, cx1, cy0, cy1;
+    cam_candidates(b, bp.cells_x, bp.cells_y, cx0, cx1, cy0, cy1);
+    // walk along the thinnest band (in cell units: x = SUS_CELL_PX X)
+    int best = -1;
+    double bw = 1e300;
+    for (int i = 0; i < 4; i++) {
+      if (!(b.tau[i] >= 0.0)) continue;
+      const double g = sqrt(b.f[i][1] * b.f[i][1] + b.f[i][2] * b.f[i][2]);
+      const double w = b.tau[i] * b.dmax / g;
+      if (w < bw) { bw = w; best = i; }
+    }
+    const double A = best >= 0 ? b.f[best][0] : 0.0;
+    const double BX = best >= 0 ? SUS_CELL_PX * b.f[best][1] : 0.0, BY = best >= 0 ? SUS_CELL_PX * b.f[best][2] : 0.0;
+    const double t = best >= 0 ? b.tau[best] * b.dmax : 0.0;
+    band_walk(A, BX, BY, t, cx0, cx1, cy0, cy1, lane, [&](int cx, int cy) {
+      if (cam_cell_kept(b, cx, cy)) bin_emit<PASS>(bp, cy * bp.cells_x + cx, b.e, b.q);
+    });
   }
-  float oa[4];
-  mv4(c.wv_inv, 0.0f, 0.0f, 0.0f, 1.0f, oa);   // eye_ray's origin, bit for bit
-  const F3 o = {oa[0], oa[1], oa[2]};
-  const double delta = 2e-5 + 1e-7 * sqrt((double)o.x * o.x + (double)o.y * o.y + (double)o.z * o.z);
-  // false when |n.d| > tau for every direction of the cell
-  auto possible = [&](double n0, double n1, double n2, double tau) {
-    double f[4];
-    for (int k = 0; k < 4; k++) f[k] = D[k][0] * n0 + D[k][1] * n1 + D[k][2] * n2;
-    const bool pos = f[0] > 0.0 && f[1] > 0.0 && f[2] > 0.0 && f[3] > 0.0;
-    const bool neg = f[0] < 0.0 && f[1] < 0.0 && f[2] < 0.0 && f[3] < 0.0;
-    if (!pos && !neg) return true;
-    const double lo = fmin(fmin(fabs(f[0]), fabs(f[1])), fmin(fabs(f[2]), fabs(f[3])));
-    return !(lo > tau * dmax);
-  };
+}
+
+template <int PASS>
+__global__ __launch_bounds__(256) void k_light_bin(LightBinArgs a) {
+  LightGrid G;
+  light_grid(a, G);
+  if (PASS == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    a.grid[0] = make_float4((float)G.b1[0], (float)G.b1[1], (float)G.b1[2], G.u0);
+    a.grid[1] = make_float4((float)G.b2[0], (float)G.b2[1], (float)G.b2[2], G.v0);
+    a.grid[2] = make_float4(G.iu, G.iv, 0.0f, 0.0f);
+  }
+  const int lane = threadIdx.x & 63;
   const int n = *a.count;
-  for (int s = threadIdx.x; s < n; s += blockDim.x) {
-    const float4 e = a.list[s];
-    const int g = __float_as_int(e.w);
-    const float4 A = a.bf_a[g], E1 = a.bf_e1[g], E2 = a.bf_e2[g];
-    const F3 qvec = cross(sub(o, f4xyz(A)), f4xyz(E1));
-    const double l1 = sqrt((double)E1.x * E1.x + (double)E1.y * E1.y + (double)E1.z * E1.z);
-    const double l2 = sqrt((double)E2.x * E2.x + (double)E2.y * E2.y + (double)E2.z * E2.z);
-    const float ub = (float)(1.6e-6 + 1e-6 * l1 * l2);
-    const double ln = sqrt((double)e.x * e.x + (double)e.y * e.y + (double)e.z * e.z);
-    const double lq = sqrt((double)qvec.x * qvec.x + (double)qvec.y * qvec.y + (double)qvec.z * qvec.z);
-    if (!possible(e.x, e.y, e.z, 1.5e-6 + (delta + 1e-6) * ln)) continue;
-    const double eu = (delta + 1e-6) * lq;
-    if (!possible(qvec.x, qvec.y, qvec.z, (double)ub + eu)) continue;
-    const F3 tv = sub(o, f4xyz(A));   // mt_test's tvec
-    const double tx = tv.x, ty = tv.y, tz = tv.z, ax = E1.x, ay = E1.y, az = E1.z, bx = E2.x, by = E2.y, bz = E2.z;
-    const double wx = by * tz - bz * ty, wy = bz * tx - bx * tz, wz = bx * ty - by * tx;   // e2 x tvec
-    const double ev = (delta + 1e-6) * sqrt(tx * tx + ty * ty + tz * tz) * l2;
-    if (!possible(wx, wy, wz, (double)ub + ev)) continue;
-    // (a, b): tvec = a e1 + b e2 + c N/|N|, by Cramer's rule with det = |N|
-    const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-    const double nn = sqrt(nx * nx + ny * ny + nz * nz);
-    if (nn > 0.0) {
-      const double hx = nx / nn, hy = ny / nn, hz = nz / nn;
-      const double ca = tx * (by * hz - bz * hy) + ty * (bz * hx - bx * hz) + tz * (bx * hy - by * hx);   // tvec.(e2 x n)
-      const double cb = tx * (hy * az - hz * ay) + ty * (hz * ax - hx * az) + tz * (hx * ay - hy * ax);   // tvec.(n x e1)
-      const double ka = ca / nn, kb = cb / nn;
-      const double tc = (fabs(ka) * ((double)ub + eu) + fabs(kb) * ((double)ub + ev)) * (1.0 + 1e-9);
-      if (!possible(ka * qvec.x - kb * wx, ka * qvec.y - kb * wy, ka * qvec.z - kb * wz, tc)) continue;
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += waves) {
+    LightRec R;
+    light_rec(a, G, s, R);
+    int cx0 = 0, cx1 = a.bp.cells_x - 1, cy0 = 0, cy1 = a.bp.cells_y - 1;
+    if (!G.ok) {
+      band_walk(0.0, 0.0, 0.0, 0.0, cx0, cx1, cy0, cy1, lane, [&](int gx, int gy) {
+        bin_emit<PASS>(a.bp, gy * a.bp.cells_x + gx, R.r0, R.r1);
+      });
+      continue;
     }
-    const int k = atomicAdd(&cnt, 1);
-    if (k < SUS_STASH) {
-      stash[2 * k] = e;
-      stash[2 * k + 1] = make_float4(qvec.x, qvec.y, qvec.z, ub);
-    }
+    light_candidates(G, R, a.bp.cells_x, a.bp.cells_y, cx0, cx1, cy0, cy1);
+    // the thinner of the bands -m1 <= l1 <= T + m2 and -m2 <= l2 <= T + m1, in cell units
+    // (u = u0 + X / iu, v = v0 + Y / iv)
+    const double h1 = 0.5 * (R.T + R.m1 + R.m2), mid1 = 0.5 * (R.T + R.m2 - R.m1);
+    const double h2 = 0.5 * (R.T + R.m1 + R.m2), mid2 = 0.5 * (R.T + R.m1 - R.m2);
+    const double X1 = R.j11 / G.iu, Y1 = R.j12 / G.iv, X2 = R.j21 / G.iu, Y2 = R.j22 / G.iv;
+    const double A1 = R.c1 - mid1 + R.j11 * G.u0 + R.j12 * G.v0, A2 = R.c2 - mid2 + R.j21 * G.u0 + R.j22 * G.v0;
+    const bool first = h1 / sqrt(X1 * X1 + Y1 * Y1) <= h2 / sqrt(X2 * X2 + Y2 * Y2);
+    // the cell widening (slack) is far below a cell: the walk's one-cell margin covers it
+    band_walk(first ? A1 : A2, first ? X1 : X2, first ? Y1 : Y2, first ? h1 : h2, cx0, cx1, cy0, cy1, lane,
+              [&](int gx, int gy) {
+                if (light_cell_kept(G, R, gx, gy)) bin_emit<PASS>(a.bp, gy * a.bp.cells_x + gx, R.r0, R.r1);
+              });
   }
-  pool_cell(stash, cnt, off, a.pool, a.pool_cap, a.pool_used, a.cell + cell);
+}
+
+// Pool offsets from the counts of n cells (one workgroup per list, blockIdx.x = list): cell =
+// (offset, count), count = -1 past the capacity; the counts become the fill cursors (zero).
+__global__ __launch_bounds__(1024) void k_bin_offsets(BinPool bp, long long cell_stride, int n) {
+  __shared__ int part[1024];
+  __shared__ int carry;
+  int* cnt = bp.cnt + cell_stride * blockIdx.x;
+  int2* cell = bp.cell + cell_stride * blockIdx.x;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int c = i < n ? cnt[i] : 0;
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {   // inclusive scan
+      const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int start = carry + part[threadIdx.x] - c;
+    if (i < n) {
+      cell[i] = (start + c <= bp.cap) ? make_int2(start, c) : make_int2(0, -1);
+      cnt[i] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
 }
 
 __global__ void k_reorder(ReorderArgs a) {
@@ -1978,11 +2183,16 @@ void launch_pack(const PackArgs& a, hipStream_t s) {
 void launch_suspects(const SuspectArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_suspects, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
 }
-void launch_sus_bin(const SusBinArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_sus_bin, dim3(a.cells_x * a.cells_y), dim3(256), 0, s, a);
+void launch_sus_bin(const SusBinArgs& a, int views, hipStream_t s) {
+  const dim3 grid(std::max(1, std::min(1024, (a.F + 1023) / 1024)), views);   // waves loop over the suspects
+  hipLaunchKernelGGL(k_sus_bin<0>, grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_bin_offsets, dim3(views), dim3(1024), 0, s, a.bp, a.cell_stride, (int)a.cell_stride);
+  hipLaunchKernelGGL(k_sus_bin<1>, grid, dim3(256), 0, s, a);
 }
 void launch_light_bin(const LightBinArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_light_bin, dim3(a.cells_x * a.cells_y), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_light_bin<0>, dim3(256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(1024), 0, s, a.bp, 0ll, a.bp.cells_x * a.bp.cells_y);
+  hipLaunchKernelGGL(k_light_bin<1>, dim3(256), dim3(256), 0, s, a);
 }
 void launch_reorder(const ReorderArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);

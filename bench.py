@@ -496,8 +496,8 @@ def main():
     events_ms = 1e3 * (time.perf_counter() - g0) / args.steps
     kernels = ["k_render", "k_interior", "k_edge_trace", "k_edge_grad", "k_cdf_par", "k_edge_bin",
                "k_suspects", "k_radix_sort", "k_refit", "k_karras",
-               "k_tri_setup", "k_morton", "k_reorder", "k_pack_nodes", "k_prepare", "k_project", "k_edge_len",
-               "k_loss_final", "k_fill_f32", "k_fill_u32", "k_adam", "k_step_inc"]
+               "k_tri_setup", "k_morton", "k_pack_nodes", "k_prepare", "k_project", "k_sus_bin", "k_light_bin",
+               "k_loss_final", "k_fill_f32", "k_fill_u32", "k_adam"]
     prof = {}
     for k in kernels:
         tot, cnt = r.profile_read(k)

@@ -773,6 +773,7 @@ struct drh_renderer : BufOwner {
   // full: Morton sort + Karras topology + refit; otherwise (same triangle count, a topology
   // from an earlier full build) only the boxes are refitted over the moved geometry
   drh_status build_bvh(bool full = true);
+  int* prep_step = nullptr;   // set by a fused step: k_prepare advances this Adam step counter
   bool topo_valid = false;   // child / parent / leaf-order buffers hold a topology for F
   int rebuild_every = 8;     // fused steps: a full build every this many steps, refits between
   drh_status upload_scene(const drh_scene_desc* d);
@@ -783,9 +784,8 @@ struct drh_renderer : BufOwner {
   drh_status ensure_rnd(long long N);
   // projection, edge lengths and the exact CDF of view v on stream q
   void edge_prep(int v, hipStream_t q) {
-    ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>()};
-    timed("k_project", [&] { launch_project(pa, q); });
-    timed("k_edge_len", [&] { launch_edge_len(edges.as<int2>(), E, proj.as<float4>(), pmf_raw.as<float>(), q); });
+    ProjectArgs pa{V, cams.as<DevCam>(), v, pos.as<float>(), proj.as<float4>(), edges.as<int2>(), E, pmf_raw.as<float>()};
+    timed("k_project", [&] { launch_project(pa, q); });   // and the edge lengths
     timed("k_cdf_par", [&] { launch_cdf(pmf_raw.as<float>(), E, pmf.as<float>(), cdf.as<float>(), cdfw.p, q, true); });
   }
   // fork view 0's edge preparation onto the side stream (not while profiling: per-kernel
@@ -999,9 +999,10 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
 drh_status drh_renderer::build_bvh(bool full) {
   views_built = -1;   // the views' suspect lists belong to the old geometry
   PrepareArgs pa{V, nm, has_nrm ? 1 : 0, mesh_vo.as<int>(), dmats.as<float>(), pos_orig.as<float>(),
-                 nrm_in.as<float>(), pos.as<float>(), nrm.as<float>(), nrm2d.as<float>(), nullptr, nullptr};
+                 nrm_in.as<float>(), pos.as<float>(), nrm.as<float>(), nrm2d.as<float>(), nullptr, nullptr, prep_step};
   if (F == 0) {
     if (V > 0) timed("k_prepare", [&] { launch_prepare(pa, stream); });
+    else if (prep_step) timed("k_step_inc", [&] { launch_step_inc(prep_step, stream); });
     return hip(hipGetLastError(), "commit kernels");
   }
   auto R = [&](DBuf& b, size_t bytes) -> drh_status { return hip(b.reserve(std::max<size_t>(bytes, 16)), "alloc"); };
@@ -1062,13 +1063,13 @@ drh_status drh_renderer::build_bvh(bool full) {
                  frontier.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
                  node_hi.as<float4>()};
     timed("k_refit", [&] { launch_refit(ra, stream); });
-    PackArgs pk{F, bounds.as<uint32_t>(), child.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
-                node_hi.as<float4>(), nodes.as<float4>()};
-    timed("k_pack_nodes", [&] { launch_pack(pk, stream); });
   }
+  // node packing (F > 1) and the triangles in leaf order, one launch
+  PackArgs pk{F, bounds.as<uint32_t>(), child.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
+              node_hi.as<float4>(), nodes.as<float4>()};
   ReorderArgs ro{F, vals0.as<uint32_t>(), bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
                  tri_a.as<float4>(), tri_e1.as<float4>(), tri_e2.as<float4>()};
-  timed("k_reorder", [&] { launch_reorder(ro, stream); });
+  timed("k_pack_nodes", [&] { launch_pack_reorder(pk, ro, stream); });
   if (light_sus && F >= SUS_BIN_MIN_F) {
     // the light's suspects over a grid across the light (needs the tree's root boxes)
     timed("k_fill_u32", [&] { launch_fill_u32(sus_light_cell_cnt.as<uint32_t>(), LIGHT_GRID * LIGHT_GRID, 0u, stream); });
@@ -1649,7 +1650,14 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
                                 int edge_samples, float* d_loss) {
   drh_status s;
   // re-commit from the device parameters (the optimiser moved them)
-  if (h->opt_iter > 0 && (s = h->build_bvh(h->opt_iter % h->rebuild_every == 0))) return s;   // a fresh commit's tree is current
+  // (k_prepare then also advances Adam's device step counter: step t, from 1, is in place for
+  // this step's jitter pattern and Adam update; the first step uses the commit's t = 1)
+  if (h->opt_iter > 0) {
+    h->prep_step = h->opt_kind == DRH_OPT_ADAM ? h->opt_t.as<int>() : nullptr;
+    s = h->build_bvh(h->opt_iter % h->rebuild_every == 0);   // a fresh commit's tree is current
+    h->prep_step = nullptr;
+    if (s) return s;
+  }
   // TEXTURE / LAMBERT interior adjoints reach only the texture block (shade_grad,
   // src/shade_lambert.cpp:37-42): with the texture frozen they feed nothing
   h->skip_tex_interior = (h->st.mode == DRH_TEXTURE_COLOR || h->st.mode == DRH_LAMBERT) && h->tex_lr == 0.0f;
@@ -1722,7 +1730,6 @@ static drh_status opt_step_body(drh_handle h, const drh_camera* cams, int views,
                      h->drparams.as<float>()};
     h->timed("k_transform_update", [&] { launch_transform_update(ta, h->stream); });
   }
-  if (h->opt_kind == DRH_OPT_ADAM) h->timed("k_step_inc", [&] { launch_step_inc(h->opt_t.as<int>(), h->stream); });
   return h->hip(hipGetLastError(), "opt_step");
 }
 

@@ -457,9 +457,8 @@ __device__ __forceinline__ Hit trace_bf(const DevScene& s, const float4* tri_a_o
 // |qvec.d| <= ub.  Light lists use a grid over the plane across the light, located by
 // grid = (a1.xyz, u0), (a2.xyz, v0), (1/du, 1/dv, -, -), and the light records.  A ray outside
 // its grid uses the whole list.
-// The records are read in chunks staged in the wave's LDS (every active lane copies a share,
-// the records are then read as broadcasts): one memory round trip per chunk, not two per
-// record as with scalar loads of each record and its triangle.
+// Records are read by scalar loads (the index is wave-uniform) and carry what the per-lane
+// filters need, so only a record some lane passes loads its triangle.
 struct Suspects {
   const float4* list;   // view: (N.x, N.y, N.z, id bits); light: (A, id bits), (pvec, invDet); any order
   const int* count;
@@ -470,11 +469,8 @@ struct Suspects {
 };
 constexpr int SUS_CELL_PX = 8;
 constexpr int SUS_BIN_MIN_F = 32768;   // lists of smaller scenes are not binned (short lists cost less than binning)
-// LDS per wave for the walks: the traversal stack (WST_STACK ints), then the suspect stage
-// (WST_STAGE ints = 64 float4).
-constexpr int WST_STACK = 64;
-constexpr int WST_STAGE = 256;
-constexpr int WST_INTS = WST_STACK + WST_STAGE;
+// LDS per wave for the walks: the traversal stack.
+constexpr int WST_INTS = 64;
 
 struct Tracer {
   DevScene s;
@@ -498,27 +494,12 @@ __device__ __forceinline__ void count_rays(unsigned long long* c, bool cast) {
   if ((int)(threadIdx.x & 63) == leader && b) atomicAdd(c, (unsigned long long)__popcll(b));
 }
 
-// Copies records [base, base + m) of a list of `per` float4 each into the wave's LDS stage
-// (64 float4): the active lanes share the copy; LDS operations of one wave complete in order,
-// so the reads that follow see it.  Returns m.
-__device__ __forceinline__ int stage_records(const float4* __restrict__ list, int base, int n, int per, float4* stage) {
-  const int m = min(64 / per, n - base);
-  const uint64_t act = __ballot(1);
-  const int nact = __popcll(act);
-  const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-  for (int i = rank; i < m * per; i += nact) stage[i] = list[(long long)base * per + i];
-  return m;
-}
-
-// Stages the records [0, n) of `rec` (two float4 each) chunk by chunk and calls f(r0, r1) for
-// each; true as soon as f does (the lane is done; the others go on).
+// Calls f(r0, r1) for the records [0, n) of `rec` (two float4 each, scalar loads: the index
+// is wave-uniform); true as soon as f does (the lane is done; the others go on).
 template <class Fn>
-__device__ __forceinline__ bool walk_records(const float4* rec, int n, float4* stage, Fn f) {
-  for (int base = 0; base < n; base += 32) {
-    const int m = stage_records(rec, base, n, 2, stage);
-    for (int k = 0; k < m; k++)
-      if (f(stage[2 * k], stage[2 * k + 1])) return true;
-  }
+__device__ __forceinline__ bool walk_records(const float4* rec, int n, Fn f) {
+  for (int k = 0; k < n; k++)
+    if (f(uload(rec, 2 * k), uload(rec, 2 * k + 1))) return true;
   return false;
 }
 __device__ __forceinline__ int2 uload2(const int2* p, int i) {
@@ -528,14 +509,14 @@ __device__ __forceinline__ int2 uload2(const int2* p, int i) {
 // Walks the pooled records of every distinct cell of the active lanes (a lane testing another
 // lane's cell as well only tests a superset); true when this lane is done.
 template <class Fn>
-__device__ __forceinline__ bool walk_cells(const Suspects& S, int cell, float4* stage, Fn f) {
+__device__ __forceinline__ bool walk_cells(const Suspects& S, int cell, Fn f) {
   bool todo = true;
   for (;;) {
     const uint64_t p = __ballot(todo);
     if (!p) break;
     const int c = __builtin_amdgcn_readlane(cell, __ffsll((long long)p) - 1);
     const int2 cc = uload2(S.cell, c);
-    if (walk_records(S.pool + 2ll * cc.x, cc.y, stage, f)) return true;
+    if (walk_records(S.pool + 2ll * cc.x, cc.y, f)) return true;
     if (cell == c) todo = false;
   }
   return false;
@@ -548,7 +529,6 @@ __device__ __forceinline__ void test_cam_suspects(const Tracer& T, F3 o, F3 d, f
                                                   int* wst, float sx, float sy) {
   const Suspects& S = T.cam;
   if (!S.count) return;
-  float4* stage = (float4*)(wst + WST_STACK);
   // one record: true when an any-hit lane has its hit
   auto test = [&](float4 e) -> bool {
     const int g = __builtin_amdgcn_readfirstlane(__float_as_int(e.w));   // the same in every lane
@@ -567,7 +547,7 @@ __device__ __forceinline__ void test_cam_suspects(const Tracer& T, F3 o, F3 d, f
     if (fx >= 0.0f && fy >= 0.0f && fx < (float)S.cells_x && fy < (float)S.cells_y) cell = (int)fy * S.cells_x + (int)fx;
     const int n = cell >= 0 ? S.cell[cell].y : -1;
     if (!__ballot(n < 0)) {
-      walk_cells(S, cell, stage, [&](float4 e, float4 q) -> bool {
+      walk_cells(S, cell, [&](float4 e, float4 q) -> bool {
         const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
         if (!(dn < 1.5e-6f) || !(fabsf(dot(f4xyz(q), d)) <= q.w)) return false;
         return test(e);
@@ -576,14 +556,11 @@ __device__ __forceinline__ void test_cam_suspects(const Tracer& T, F3 o, F3 d, f
     }
   }
   const int n = uload(S.count);
-  for (int base = 0; base < n; base += 64) {
-    const int m = stage_records(S.list, base, n, 1, stage);
-    for (int k = 0; k < m; k++) {
-      const float4 e = stage[k];
-      const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
-      if (!(dn < 1.5e-6f)) continue;
-      if (test(e)) return;
-    }
+  for (int k = 0; k < n; k++) {
+    const float4 e = uload(S.list, k);
+    const float dn = fabsf(fmaf(d.x, e.x, fmaf(d.y, e.y, d.z * e.z)));
+    if (!(dn < 1.5e-6f)) continue;
+    if (test(e)) return;
   }
 }
 
@@ -592,7 +569,6 @@ __device__ __forceinline__ void test_light_suspects(const Tracer& T, F3 o, F3 d,
                                                     int* wst) {
   const Suspects& S = T.light;
   if (!S.count) return;
-  float4* stage = (float4*)(wst + WST_STACK);
   auto rec = [&](float4 r0, float4 r1) -> bool {
     const float v0 = dot(sub(o, f4xyz(r0)), f4xyz(r1)) * r1.w;
     if (!(v0 > 0.0f && v0 < 1.0f)) return false;
@@ -611,11 +587,11 @@ __device__ __forceinline__ void test_light_suspects(const Tracer& T, F3 o, F3 d,
     if (fu >= 0.0f && fv >= 0.0f && fu < (float)S.cells_x && fv < (float)S.cells_y) cell = (int)fv * S.cells_x + (int)fu;
     const int n = cell >= 0 ? S.cell[cell].y : -1;
     if (!__ballot(n < 0)) {
-      walk_cells(S, cell, stage, rec);
+      walk_cells(S, cell, rec);
       return;
     }
   }
-  walk_records(S.list, uload(S.count), stage, rec);
+  walk_records(S.list, uload(S.count), rec);
 }
 
 // Brute force with the watertight test (the WT form of trace_bf, original triangle order).

@@ -13,6 +13,7 @@ namespace drh {
 constexpr int RADIX_TILE = 2048;           // keys per radix-sort block (8 rounds of 256)
 constexpr int REFIT_CHUNK = 1024;          // sorted leaves refit per workgroup in LDS (one per thread)
 constexpr int CDF_CHUNK = 8192;            // pmf floats staged in LDS per CDF round (sequential CDF)
+constexpr int CDF_SERIAL_MAX = 2048;       // edge sets up to this size take the sequential CDF (one launch)
 constexpr int CDF_CH = 128;                // edges per chunk of the parallel exact CDF
 constexpr int CDF_CPB = 64;                // CDF chunks per workgroup (one per lane)
 constexpr int CDF_WALK_STAGE = 8192;       // floats of LDS for the chunks replayed serially
@@ -49,6 +50,7 @@ struct PrepareArgs {
   float* nrm2d;
   uint32_t* bounds;       // optional: set to the empty key range (6 values) by thread 0
   uint32_t* zero;         // optional: one counter cleared by thread 0
+  int* step;              // optional: the Adam step counter, advanced by thread 0 (a fused step's first kernel)
 };
 
 struct TriSetupArgs {
@@ -211,6 +213,10 @@ struct ProjectArgs {
   int view;
   const float* pos;
   float4* proj;
+  // edge lengths (pmf before normalisation) in the same launch
+  const int2* edges;
+  int E;
+  float* pmf_raw;
 };
 
 struct EdgeArgs {
@@ -340,7 +346,7 @@ void launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
 void launch_karras(const KarrasArgs& a, hipStream_t s);
 void launch_refit(const RefitArgs& a, hipStream_t s);
 void launch_pack(const PackArgs& a, hipStream_t s);
-void launch_reorder(const ReorderArgs& a, hipStream_t s);
+void launch_pack_reorder(const PackArgs& pk, const ReorderArgs& ro, hipStream_t s);   // F > 1; one launch
 void launch_suspects(const SuspectArgs& a, hipStream_t s);
 void launch_sus_bin(const SusBinArgs& a, int views, hipStream_t s);   // count, scan and fill
 void launch_light_bin(const LightBinArgs& a, hipStream_t s);
@@ -354,7 +360,6 @@ void launch_interior(int mode, const InteriorArgs& a, int blocks, hipStream_t s)
 void launch_interior_pos(int mode, const InteriorArgs& a, int tiles, hipStream_t s);
 void launch_coverage(const InteriorArgs& a, int tiles, float4* part, hipStream_t s);
 void launch_project(const ProjectArgs& a, hipStream_t s);
-void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s);
 // exact CDF; `work` holds cdf_work_bytes(E) of device memory (parallel form only)
 size_t cdf_work_bytes(int E);
 void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work, hipStream_t s, bool parallel = true);

@@ -74,6 +74,7 @@ __global__ void k_prepare(PrepareArgs a) {
       a.bounds[3] = a.bounds[4] = a.bounds[5] = 0u;
     }
     if (a.zero) *a.zero = 0u;
+    if (a.step) *a.step += 1;
   }
   if (k >= a.V) return;
   int m = 0;
@@ -378,9 +379,7 @@ __device__ __forceinline__ void pad_box(float4& lo, float4& hi, float pa) {
   lo.x -= px; lo.y -= py; lo.z -= pz;
   hi.x += px; hi.y += py; hi.z += pz;
 }
-__global__ void k_pack_nodes(PackArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.F - 1) return;
+__device__ __forceinline__ void pack_node(const PackArgs& a, int i) {
   const int c0 = a.child[2 * i], c1 = a.child[2 * i + 1];
   float4 l0 = c0 < 0 ? a.leaf_lo[~c0] : a.node_lo[c0];
   float4 h0 = c0 < 0 ? a.leaf_hi[~c0] : a.node_hi[c0];
@@ -847,8 +846,10 @@ __global__ __launch_bounds__(1024) void k_bin_offsets(BinPool bp, long long cell
   }
 }
 
-__global__ void k_reorder(ReorderArgs a) {
+// the triangles in leaf order, beside the node packing (one launch)
+__global__ void k_pack_nodes(PackArgs pk, ReorderArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < pk.F - 1) pack_node(pk, s);
   if (s >= a.F) return;
   const int g = (int)a.order[s];
   a.tri_a[s] = a.bf_a[g];
@@ -878,7 +879,7 @@ __device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
 
 template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
-  __shared__ __align__(16) int wst[WST_INTS];   // the wave's traversal stack and suspect stage
+  __shared__ int wst[WST_INTS];   // the wave's traversal stack
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
@@ -1036,7 +1037,7 @@ template <int MODE, bool POS = false, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   constexpr bool VCOL_LIKE = MODE == 2 || MODE == MODE_LAMBERT_VCOL;
   extern __shared__ float lgrad[];  // a.lds_n floats (dynamic)
-  __shared__ __align__(16) int wst[WST_INTS];
+  __shared__ int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
@@ -1250,7 +1251,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
 // scheduling.  Small in registers, it keeps as many waves resident as the render.
 template <bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restrict__ part) {
-  __shared__ __align__(16) int wst[WST_INTS];
+  __shared__ int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
@@ -1293,7 +1294,7 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
 // only where a triangle's facing to the light flips).  One wave per pixel tile as the render.
 template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
-  __shared__ __align__(16) int wst[WST_INTS];
+  __shared__ int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevScene& S = a.tr.s;
   const DevCam& c = a.cams[a.view];
@@ -1402,26 +1403,27 @@ __global__ __launch_bounds__(1024) void k_coverage_sum(const float4* __restrict_
 // ---------------------------------------------------------------------------
 // VertexShader of every vertex (transform() already applied: pos is T*v), then the
 // projected scene's identity prepare: I*(sx, sy, z, 1).
-__global__ void k_project(ProjectArgs a) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.V) return;
-  const DevCam& c = a.cams[a.view];
-  const F3 v = ld3(a.pos + 3 * k);
+__device__ __forceinline__ float4 project_vertex(const DevCam& c, const float* pos, int k) {
+  const F3 v = ld3(pos + 3 * k);
   float sx, sy;
   vertex_shader(c, v.x, v.y, v.z, sx, sy);
   const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   const F3 p = mp(I, F3{sx, sy, v.z});
-  a.proj[k] = make_float4(p.x, p.y, p.z, 0.0f);
+  return make_float4(p.x, p.y, p.z, 0.0f);
 }
-
-// pmf before normalisation: float3 length in (screen x, screen y, world z)
-// (build_edge_sampler, src/drender.cpp:13-15)
-__global__ void k_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  const int2 v = edges[e];
-  const float4 p0 = proj[v.x], p1 = proj[v.y];
-  pmf_raw[e] = length(F3{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z});
+// Projected vertices and, in the same launch, the edges' lengths -- the pmf before
+// normalisation: float3 length in (screen x, screen y, world z) (build_edge_sampler,
+// src/drender.cpp:13-15).  An edge projects its two endpoints itself (the same arithmetic as
+// the vertex threads, so the same values).
+__global__ void k_project(ProjectArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const DevCam& c = a.cams[a.view];
+  if (k < a.V) a.proj[k] = project_vertex(c, a.pos, k);
+  if (k < a.E) {
+    const int2 v = a.edges[k];
+    const float4 p0 = project_vertex(c, a.pos, v.x), p1 = project_vertex(c, a.pos, v.y);
+    a.pmf_raw[k] = length(F3{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z});
+  }
 }
 
 // The reference's CDF is a sequential float running sum (src/drender.cpp:12-17); its
@@ -1820,7 +1822,7 @@ __global__ void k_edge_scatter(EdgeBinArgs a) {
 // ecol[i] = (colour inside - colour outside, edge id bits; -1 for a rejected sample).
 template <int MODE, bool COUNT = false, int TF = 0>
 __global__ __launch_bounds__(256) void k_edge_trace(EdgeArgs a) {
-  __shared__ __align__(16) int wstack[4][WST_INTS];
+  __shared__ int wstack[4][WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x & 63;
@@ -2101,7 +2103,7 @@ void launch_step_inc(int* step, hipStream_t s) { hipLaunchKernelGGL(k_step_inc, 
 // debug: closest hit at pixel centres
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_debug_hits(HitsArgs a) {
-  __shared__ __align__(16) int wst[WST_INTS];
+  __shared__ int wst[WST_INTS];
   const Tracer& T = a.tr;
   const DevCam& c = a.cams[0];
   const int lane = threadIdx.x;
@@ -2175,10 +2177,11 @@ void launch_karras(const KarrasArgs& a, hipStream_t s) {
 }
 void launch_refit(const RefitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_refit_local, dim3((a.F + REFIT_CHUNK - 1) / REFIT_CHUNK), dim3(REFIT_CHUNK), 0, s, a);
-  hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, a);
+  // one chunk holds the whole tree up to its root: no node spans chunks, the frontier is empty
+  if (a.F > REFIT_CHUNK) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, a);
 }
-void launch_pack(const PackArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pack_nodes, dim3((a.F + 254) / 256), dim3(256), 0, s, a);
+void launch_pack_reorder(const PackArgs& pk, const ReorderArgs& ro, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_nodes, dim3((ro.F + 255) / 256), dim3(256), 0, s, pk, ro);
 }
 void launch_suspects(const SuspectArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_suspects, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
@@ -2193,9 +2196,6 @@ void launch_light_bin(const LightBinArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_light_bin<0>, dim3(256), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(1024), 0, s, a.bp, 0ll, a.bp.cells_x * a.bp.cells_y);
   hipLaunchKernelGGL(k_light_bin<1>, dim3(256), dim3(256), 0, s, a);
-}
-void launch_reorder(const ReorderArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_reorder, dim3((a.F + 255) / 256), dim3(256), 0, s, a);
 }
 // The ray-counting (COUNT) and watertight instantiations walk the LBVH as packets only: a
 // walk changes no hit and no ray count.
@@ -2249,14 +2249,12 @@ void launch_interior_pos(int mode, const InteriorArgs& a, int tiles, hipStream_t
   }
 }
 void launch_project(const ProjectArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_project, dim3((a.V + 255) / 256), dim3(256), 0, s, a);
-}
-void launch_edge_len(const int2* edges, int E, const float4* proj, float* pmf_raw, hipStream_t s) {
-  hipLaunchKernelGGL(k_edge_len, dim3((E + 255) / 256), dim3(256), 0, s, edges, E, proj, pmf_raw);
+  const int n = std::max(a.V, a.E);
+  if (n > 0) hipLaunchKernelGGL(k_project, dim3((n + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_cdf(const float* pmf_raw, int E, float* pmf, float* cdf, void* work, hipStream_t s, bool parallel) {
   if (E <= 0) return;
-  if (!parallel) {
+  if (!parallel || E <= CDF_SERIAL_MAX) {   // a short sum: one workgroup, one launch
     hipLaunchKernelGGL(k_cdf, dim3(1), dim3(1024), 0, s, pmf_raw, E, pmf, cdf);
     return;
   }

@@ -50,6 +50,15 @@ __device__ __forceinline__ void mv4(const float* B, float x, float y, float z, f
   r[2] = x * B[2] + y * B[6] + z * B[10] + w * B[14];
   r[3] = x * B[3] + y * B[7] + z * B[11] + w * B[15];
 }
+// The same product with B read through the constant address space: a matrix the whole wave
+// shares (a camera's) then arrives by scalar loads, one copy per wave instead of one per lane.
+__device__ __forceinline__ float cload(const float* p) { return *(const __attribute__((address_space(4))) float*)p; }
+__device__ __forceinline__ void mv4c(const float* B, float x, float y, float z, float w, float r[4]) {
+  r[0] = x * cload(B) + y * cload(B + 4) + z * cload(B + 8) + w * cload(B + 12);
+  r[1] = x * cload(B + 1) + y * cload(B + 5) + z * cload(B + 9) + w * cload(B + 13);
+  r[2] = x * cload(B + 2) + y * cload(B + 6) + z * cload(B + 10) + w * cload(B + 14);
+  r[3] = x * cload(B + 3) + y * cload(B + 7) + z * cload(B + 11) + w * cload(B + 15);
+}
 __device__ __forceinline__ F3 mp(const float* B, F3 v) { float r[4]; mv4(B, v.x, v.y, v.z, 1.0f, r); return {r[0], r[1], r[2]}; }
 
 // Per-view camera constants (host computes inverse4x4 and mProj*mWorldView exactly as
@@ -64,15 +73,25 @@ struct DevCam {
 };
 
 // CastSingleRay: EyeRayDirNormalized + transform_ray3f (raytrace.h:49-76, raytrace3d_bf.cpp:158-174)
+// q / s.  When s is a power of two (2^-126 < 1/s < 2^127) its reciprocal is exact and q * (1/s)
+// is the same correctly rounded quotient without the divide sequence.  The callers' s is the
+// same in every lane (an image size, a sample count), so this is a branch, not a select.
+__device__ __forceinline__ float div_pow2(float q, float s) {
+  const uint32_t b = __float_as_uint(s), e = b >> 23;
+  if ((b & 0x007FFFFFu) == 0u && e >= 1u && e <= 253u) return q * __uint_as_float((254u - e) << 23);
+  return q / s;
+}
+__device__ __forceinline__ F3 divs_pow2(F3 a, float s) { return {div_pow2(a.x, s), div_pow2(a.y, s), div_pow2(a.z, s)}; }
+
 __device__ __forceinline__ void eye_ray(const DevCam& c, float x, float y, F3& o, F3& d) {
-  const float xn = x / c.W, yn = y / c.H;
+  const float xn = div_pow2(x, cload(&c.W)), yn = div_pow2(y, cload(&c.H));
   float p[4];
-  mv4(c.proj_inv, 2.0f * xn - 1.0f, -2.0f * yn + 1.0f, 0.0f, 1.0f, p);
+  mv4c(c.proj_inv, 2.0f * xn - 1.0f, -2.0f * yn + 1.0f, 0.0f, 1.0f, p);
   const float w = p[3];
   const F3 dir = normalize(F3{p[0] / w, p[1] / w, p[2] / w});
   float a[4], b[4];
-  mv4(c.wv_inv, 0.0f, 0.0f, 0.0f, 1.0f, a);
-  mv4(c.wv_inv, 0.0f + 100.0f * dir.x, 0.0f + 100.0f * dir.y, 0.0f + 100.0f * dir.z, 1.0f, b);
+  mv4c(c.wv_inv, 0.0f, 0.0f, 0.0f, 1.0f, a);
+  mv4c(c.wv_inv, 0.0f + 100.0f * dir.x, 0.0f + 100.0f * dir.y, 0.0f + 100.0f * dir.z, 1.0f, b);
   o = {a[0], a[1], a[2]};
   d = normalize(F3{b[0] - a[0], b[1] - a[1], b[2] - a[2]});
 }
@@ -101,8 +120,8 @@ __device__ __forceinline__ void sample_offset(const Jitter& j, int view, int x, 
     ux = (float)(k >> 8) * (1.0f / 16777216.0f);
     uy = (float)(jitter_hash(k) >> 8) * (1.0f / 16777216.0f);
   }
-  xo = ((si % sq) + ux) / float(sq);
-  yo = ((si / sq) + uy) / float(sq);
+  xo = div_pow2((si % sq) + ux, float(sq));
+  yo = div_pow2((si / sq) + uy, float(sq));
 }
 
 // ---------------------------------------------------------------------------
@@ -161,10 +180,15 @@ __device__ __forceinline__ bool mt_test(float4 A4, float4 E14, float4 E24, F3 o,
   const F3 qvec = cross(tvec, e1);
   const float det = dot(e1, pvec);
   const float sv = sgn(det);
+  const float nv = dot(tvec, pvec), nu = dot(qvec, d), nt = dot(e2, qvec);
+  // invDet has det's sign and is 0 only when det is, so v > 0, u > 0 and t > tNear >= 0 need
+  // numerators of det's sign: a ray that fails there fails the test below too, and skips the
+  // divide (t, u, v are then left unset, as for any rejected ray)
+  if (!(nv * sv > 0.0f && nu * sv > 0.0f && (tNear < 0.0f || nt * sv > 0.0f))) return false;
   const float invDet = sv / max_std(sv * det, 1e-6f);
-  v = dot(tvec, pvec) * invDet;
-  u = dot(qvec, d) * invDet;
-  t = dot(e2, qvec) * invDet;
+  v = nv * invDet;
+  u = nu * invDet;
+  t = nt * invDet;
   return v > 0.0f && u > 0.0f && (u + v < 1.0f) && t > tNear;
 }
 

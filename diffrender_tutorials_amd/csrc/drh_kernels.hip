@@ -906,7 +906,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
     const int base = lane - tm.s;
     for (int k = 0; k < tm.S; k++) {
       const F3 ck = {__shfl(col.x, base + k, 64), __shfl(col.y, base + k, 64), __shfl(col.z, base + k, 64)};
-      acc = add(acc, divs(ck, ns));
+      acc = add(acc, divs_pow2(ck, ns));
     }
   } else if (inside) {
     for (int dy = 0; dy < sq; dy++)
@@ -919,7 +919,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
         if (a.hits)
           a.hits[(c.img_off / 3 + (long long)y * c.w + x) * (sq * sq) + dy * sq + dx] =
               make_float4(__uint_as_float(h.gtri), h.u, h.v, MODE == MODE_LAMBERT_VCOL ? lit : h.t);
-        acc = add(acc, divs(col, ns));
+        acc = add(acc, divs_pow2(col, ns));
       }
   }
   double l2 = 0.0;
@@ -1071,7 +1071,7 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
       // trip count is the same in every lane)
       const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
       for (int si = s_begin; si < s_end; si++) {
-        const F3 val = divs(adj, (float)a.nsamp);
+        const F3 val = divs_pow2(adj, (float)a.nsamp);
         long long key = -1 - lane;   // lanes without a contribution form runs of their own
         constexpr int NG = POS ? 18 : 12;
         float g[NG];
@@ -1260,7 +1260,7 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
   const bool inside = tm.x < c.w && tm.y < c.h;
   F3 acc = {0, 0, 0};
   if (inside) {
-    const F3 val = divs(ld3(a.adj + c.img_off + 3ll * ((long long)tm.y * c.w + tm.x)), (float)a.nsamp);
+    const F3 val = divs_pow2(ld3(a.adj + c.img_off + 3ll * ((long long)tm.y * c.w + tm.x)), (float)a.nsamp);
     const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
     for (int si = s_begin; si < s_end; si++) {
       bool hit;
@@ -1302,7 +1302,7 @@ __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
   const int sq = a.sq;
   const TileMap tm = tile_map(sq, c.w, blockIdx.x, lane);
   if (!(tm.x < c.w && tm.y < c.h)) return;
-  const F3 val = divs(ld3(a.adj + c.img_off + 3ll * ((long long)tm.y * c.w + tm.x)), (float)a.nsamp);
+  const F3 val = divs_pow2(ld3(a.adj + c.img_off + 3ll * ((long long)tm.y * c.w + tm.x)), (float)a.nsamp);
   const F3 negL = mul(-1.0f, a.light.L);
   const int s_begin = tm.S > 1 ? tm.s : 0, s_end = tm.S > 1 ? tm.s + 1 : a.nsamp;
   for (int si = s_begin; si < s_end; si++) {

@@ -872,6 +872,20 @@ __device__ __forceinline__ TileMap tile_map_xy(int sq, int tx, int ty, int lane)
   const int p = lane / S;                     // pixel within the tile
   return TileMap{tx * ts + p % ts, ty * ts + p / ts, S, lane % S};
 }
+// The tile of workgroup b of a grid of n (one per tile).  The dispatcher deals workgroups to
+// the 8 XCDs round robin (b goes to XCD b % 8); this hands each XCD runs of 256 consecutive
+// tiles instead (run j to XCD j % 8), so neighbouring tiles, which open the same LBVH nodes
+// and triangles, share one XCD's L2 and scalar caches, while every XCD still gets runs from
+// all over the image (one contiguous band per XCD left some XCDs with mostly empty
+// background: config-2 render 18% slower; 64-tile runs 12% slower; 256-tile runs as fast as
+// round robin there and 10% faster on the 16-view 200k-triangle scene).  A bijection on
+// [0, n); the last n mod 2048 workgroups keep their own tile.
+constexpr int XCD_RUN = 256;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  if (b >= (n / (8 * XCD_RUN)) * (8 * XCD_RUN)) return b;
+  const int x = b & 7, k = b >> 3;
+  return ((k / XCD_RUN) * 8 + x) * XCD_RUN + k % XCD_RUN;
+}
 __device__ __forceinline__ TileMap tile_map(int sq, int w, int tile, int lane) {
   const int tiles_x = (w + tile_side(sq) - 1) / tile_side(sq);
   return tile_map_xy(sq, tile % tiles_x, tile / tiles_x, lane);
@@ -885,7 +899,8 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   const int lane = threadIdx.x;
   const int sq = a.sq;
   const float ns = (float)(sq * sq);
-  const TileMap tm = tile_map(sq, c.w, blockIdx.x, lane);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const TileMap tm = tile_map(sq, c.w, tile, lane);
   const int x = tm.x, y = tm.y;
   const bool inside = x < c.w && y < c.h;
   F3 acc = {0, 0, 0};
@@ -936,7 +951,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
   }
   if (a.target) {
     l2 = wave_sum_d(l2);
-    if (lane == 0) a.loss_part[a.part_off + blockIdx.x] = l2;
+    if (lane == 0) a.loss_part[a.part_off + tile] = l2;
   }
 }
 
@@ -1256,7 +1271,8 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
   const int sq = a.sq;
-  const TileMap tm = tile_map(sq, c.w, blockIdx.x, lane);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const TileMap tm = tile_map(sq, c.w, tile, lane);
   const bool inside = tm.x < c.w && tm.y < c.h;
   F3 acc = {0, 0, 0};
   if (inside) {
@@ -1277,7 +1293,7 @@ __global__ __launch_bounds__(64) void k_coverage(InteriorArgs a, float4* __restr
     }
   }
   acc.x = wave_sum(acc.x); acc.y = wave_sum(acc.y); acc.z = wave_sum(acc.z);
-  if (lane == 0) part[blockIdx.x] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+  if (lane == 0) part[tile] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }
 
 // grad[0..2] += the sum of n partial float3 (part[i].xyz), in a fixed order, in double.
@@ -1300,7 +1316,7 @@ __global__ __launch_bounds__(64) void k_interior_pos(InteriorArgs a) {
   const DevCam& c = a.cams[a.view];
   const int lane = threadIdx.x;
   const int sq = a.sq;
-  const TileMap tm = tile_map(sq, c.w, blockIdx.x, lane);
+  const TileMap tm = tile_map(sq, c.w, xcd_tile(blockIdx.x, gridDim.x), lane);
   if (!(tm.x < c.w && tm.y < c.h)) return;
   const F3 val = divs_pow2(ld3(a.adj + c.img_off + 3ll * ((long long)tm.y * c.w + tm.x)), (float)a.nsamp);
   const F3 negL = mul(-1.0f, a.light.L);

@@ -878,8 +878,9 @@ __device__ __forceinline__ TileMap tile_map_xy(int sq, int tx, int ty, int lane)
 // and triangles, share one XCD's L2 and scalar caches, while every XCD still gets runs from
 // all over the image (one contiguous band per XCD left some XCDs with mostly empty
 // background: config-2 render 18% slower; 64-tile runs 12% slower; 256-tile runs as fast as
-// round robin there and 10% faster on the 16-view 200k-triangle scene).  A bijection on
-// [0, n); the last n mod 2048 workgroups keep their own tile.
+// round robin there and 10% faster on the 16-view 200k-triangle scene; the edge pass's sample
+// chunks dealt the same way ran 24% slower at config 2, so it keeps round robin).  A bijection
+// on [0, n); the last n mod 2048 workgroups keep their own tile.
 constexpr int XCD_RUN = 256;
 __device__ __forceinline__ int xcd_tile(int b, int n) {
   if (b >= (n / (8 * XCD_RUN)) * (8 * XCD_RUN)) return b;
@@ -1074,7 +1075,8 @@ __global__ __launch_bounds__(64) void k_interior(InteriorArgs a) {
   // triangle (vertex colours) or texel quad (texture): the per-sample terms are summed over
   // runs of equal target in the wave and each run issues one set of atomics.
   const bool runs = !a.coverage && (VCOL_LIKE || !a.uniform);
-  for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
+  for (int unit0 = blockIdx.x; unit0 < nunits; unit0 += gridDim.x) {
+    const int unit = xcd_tile(unit0, nunits);   // runs of neighbouring units per XCD
     for (int j = 0; j < rt * rt; j++) {
       const int tx = (unit % units_x) * rt + j % rt, ty = (unit / units_x) * rt + j / rt;
       if (tx >= tiles_x || ty >= tiles_y) continue;   // whole tile outside the image

@@ -811,8 +811,14 @@ drh_status drh_renderer::upload_scene(const drh_scene_desc* d) {
     return fail(DRH_E_INVALID, "commit: scene arrays missing");
   const int n = d->n_meshes;
   std::vector<int> nvo(n + 1, 0), nto(n + 1, 0);
+  long long tv = 0, tf = 0;
   for (int i = 0; i < n; i++) {
     if (d->vcount[i] < 0 || d->fcount[i] < 0) return fail(DRH_E_INVALID, "commit: negative size in mesh %d", i);
+    tv += d->vcount[i];
+    tf += d->fcount[i];
+    // LBVH nodes and leaves are addressed by 32-bit byte offsets (node * 64, triangle * 16)
+    if (tf > MAX_TRIANGLES || tv > MAX_TRIANGLES * 3)
+      return fail(DRH_E_INVALID, "commit: more than %lld triangles or %lld vertices", MAX_TRIANGLES, 3 * MAX_TRIANGLES);
     nvo[i + 1] = nvo[i] + d->vcount[i];
     nto[i + 1] = nto[i] + d->fcount[i];
   }

@@ -254,6 +254,17 @@ __device__ __forceinline__ float4 uload(const float4* p, int i) {
   return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ int uload(const int* p) { return *(cint*)p; }
+// The same at a 32-bit byte offset (element index times 16, no wider than 32 bits): the
+// address is then the base register pair plus an offset register, one scalar shift instead of
+// a sign-extended 64-bit multiply-add per load.
+__device__ __forceinline__ float4 uload_at(const float4* p, uint32_t byte_off) {
+  const v4f v = *(cv4f*)((const __attribute__((address_space(4))) char*)p + byte_off);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+// Per-lane load at a 32-bit byte offset: base in scalar registers plus a 32-bit lane offset.
+__device__ __forceinline__ float4 vload_at(const float4* p, uint32_t byte_off) {
+  return *(const float4*)((const char*)p + byte_off);
+}
 
 // 1/d for the slab test; a zero (or denormal) component becomes +-1e30 so the fused form
 // below stays finite: that axis' slab is then all-pass or all-fail, as for an infinite 1/d.
@@ -290,14 +301,15 @@ __device__ __forceinline__ Hit trace_bvh_lane(const DevScene& s, F3 o, F3 d, flo
   const WtRay wr = WT ? wt_ray(d) : WtRay{};
   float lim = tFar * 1.00002f + 1e-6f;
   auto leaf = [&](int slot) -> bool {   // true: an any-hit ray is done
-    const float4 A = s.tri_a[slot];
+    const uint32_t off = (uint32_t)slot << 4;
+    const float4 A = vload_at(s.tri_a, off);
     float t, u, v;
     bool hit;
     if (WT) {
       const uint32_t gg = __float_as_uint(A.w);
       hit = wt_test(wr, o, vertex_of(s, gg, 0), vertex_of(s, gg, 1), vertex_of(s, gg, 2), tNear, tFar, t, u, v);
     } else {
-      hit = mt_test(A, s.tri_e1[slot], s.tri_e2[slot], o, d, tNear, t, u, v);
+      hit = mt_test(A, vload_at(s.tri_e1, off), vload_at(s.tri_e2, off), o, d, tNear, t, u, v);
     }
     if (!hit) return false;
     const uint32_t g = __float_as_uint(A.w);
@@ -315,8 +327,9 @@ __device__ __forceinline__ Hit trace_bvh_lane(const DevScene& s, F3 o, F3 d, flo
   int sp = 0;
   int node = 0;
   for (int visits = 0; visits < s.F; visits++) {
-    const float4 n0 = s.nodes[4 * node], n1 = s.nodes[4 * node + 1];
-    const float4 n2 = s.nodes[4 * node + 2], n3 = s.nodes[4 * node + 3];
+    const uint32_t off = (uint32_t)node << 6;
+    const float4 n0 = vload_at(s.nodes, off), n1 = vload_at(s.nodes, off + 16);
+    const float4 n2 = vload_at(s.nodes, off + 32), n3 = vload_at(s.nodes, off + 48);
     const int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
     float tl, tr;
     bool hl = slab(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, id, nod, lim, tl);
@@ -380,7 +393,8 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
   float lim = retired ? -1.0f : tFar * 1.00002f + 1e-6f;
   bool live = !retired;   // any-hit: still looking
   auto leaf = [&](int slot, bool mine) {
-    const float4 A = uload(s.tri_a, slot), E1 = uload(s.tri_e1, slot), E2 = uload(s.tri_e2, slot);
+    const uint32_t off = (uint32_t)slot << 4;
+    const float4 A = uload_at(s.tri_a, off), E1 = uload_at(s.tri_e1, off), E2 = uload_at(s.tri_e2, off);
     float t, u, v;
     bool hit = false;
     if (mine) {
@@ -406,8 +420,9 @@ __device__ __forceinline__ Hit trace_bvh(const DevScene& s, F3 o, F3 d, float tN
   // a tree over F leaves has F-1 internal nodes, each visited at most once per wave
   for (int visits = 0; visits < s.F; visits++) {
     node = __builtin_amdgcn_readfirstlane(node);   // wave-uniform by construction; keep it scalar
-    const float4 n0 = uload(s.nodes, 4 * node), n1 = uload(s.nodes, 4 * node + 1);
-    const float4 n2 = uload(s.nodes, 4 * node + 2), n3 = uload(s.nodes, 4 * node + 3);
+    const uint32_t off = (uint32_t)node << 6;
+    const float4 n0 = uload_at(s.nodes, off), n1 = uload_at(s.nodes, off + 16);
+    const float4 n2 = uload_at(s.nodes, off + 32), n3 = uload_at(s.nodes, off + 48);
     const int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
     float tl, tr;
     // any-hit lanes that are done skip the box tests (exec-masked); closest-hit lanes are

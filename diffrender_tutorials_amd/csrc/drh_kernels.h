@@ -23,6 +23,9 @@ constexpr int EDGE_LDS_FLOATS = 4096;      // largest position gradient accumula
 
 // pixel tiles per view, one wavefront each (matches tile_map in the kernels)
 __host__ __device__ inline int tile_side(int sq) { return (sq == 1 || sq == 2 || sq == 4 || sq == 8) ? 8 / sq : 8; }
+// Largest scene: the LBVH walks address nodes (64 B) and leaf triangles (16 B per array) by
+// 32-bit byte offsets.
+constexpr long long MAX_TRIANGLES = 1ll << 26;
 __host__ __device__ inline int num_tiles(int sq, int w, int h) {
   const int ts = tile_side(sq);
   return ((w + ts - 1) / ts) * ((h + ts - 1) / ts);

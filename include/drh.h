@@ -110,7 +110,8 @@ void       drh_destroy(drh_handle h);
 const char* drh_last_error(drh_handle h);
 
 /* IDiffRender::commit (src/drender.h:91-96): flattens the scene (Scene::prepare_for_render,
- * common_src/scene_utils.cpp:98-173) into device buffers and builds the LBVH. */
+ * common_src/scene_utils.cpp:98-173) into device buffers and builds the LBVH.  At most 2^26
+ * triangles and 3 * 2^26 vertices in all (DRH_E_INVALID beyond). */
 drh_status drh_commit(drh_handle h, const drh_scene_desc* scene);
 /* Number of floats of the flat gradient (DScene::full_size, src/dmesh.h:231-237). */
 int        drh_grad_size(drh_handle h);

@@ -1,13 +1,17 @@
 // drh_kernels.hip -- HIP kernels of the render / d_render path for gfx950 (MI355X).
 //
 //   commit   k_prepare, k_tri_setup, k_morton, k_radix_{hist,scan,scatter},
-//            k_karras, k_refit_{local,top}, k_pack_nodes, k_reorder   (replaces IRayTracer::Init)
+//            k_karras, k_refit_local (+ k_refit_top above one chunk), k_pack_nodes (with the
+//            leaf reorder), k_suspects, k_sus_bin / k_light_bin + k_bin_offsets
+//                                                                     (replaces IRayTracer::Init)
 //   render   k_render<MODE> (+ fused L2 loss / adjoint), k_loss_final
-//   interior k_interior<MODE>                                          (interior_derivatives)
-//   edges    k_project, k_edge_len, k_cdf_{sums,scan,spec,walk,final},
-//            k_edge_bin, k_bin_scan, k_edge_scatter (pixel-bin order),
-//            k_edge_trace<MODE> (side colours), k_edge_grad               (edge_derivatives)
-//   optimise k_optsimple_{moments,apply}, k_adam, k_transform_update, k_step_inc
+//   interior k_interior<MODE>, k_coverage, k_interior_pos              (interior_derivatives)
+//   edges    k_project (with edge lengths), k_cdf (one wave, small edge sets) or
+//            k_cdf_{sums,scan,spec,walk,final}, k_edge_bin, k_bin_scan, k_edge_scatter
+//            (pixel-bin order), k_edge_trace<MODE> (side colours), k_edge_grad
+//                                                                     (edge_derivatives)
+//   optimise k_optsimple_{moments,apply}, k_adam, k_transform_update (k_step_inc only for a
+//            scene without vertices; otherwise k_prepare advances the step counter)
 //
 // Wave64 everywhere.  Rays are traced a wavefront at a time (trace_bvh in drh_device.h): a
 // pixel tile or 32 edge samples (both sides) per wave, one wave-uniform LBVH walk per packet.

@@ -108,6 +108,7 @@ __global__ void k_tri_setup(TriSetupArgs a) {
   if (threadIdx.x < 3) { smin[threadIdx.x] = 0xFFFFFFFFu; smax[threadIdx.x] = 0u; }
   if (a.counter && g <= a.F) a.counter[g] = 0u;
   __syncthreads();
+  uint32_t kmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, kmax[3] = {0u, 0u, 0u};
   if (g < a.F) {
     const int m = a.tri_mesh[g];
     const float* T = a.mats + 48 * m;
@@ -126,9 +127,23 @@ __global__ void k_tri_setup(TriSetupArgs a) {
     const float cx = (lx + hx) * 0.5f, cy = (ly + hy) * 0.5f, cz = (lz + hz) * 0.5f;
     a.cen[g] = make_float4(cx, cy, cz, 0.0f);
     if (cx == cx && cy == cy && cz == cz) {
-      atomicMin(&smin[0], f2key(cx)); atomicMin(&smin[1], f2key(cy)); atomicMin(&smin[2], f2key(cz));
-      atomicMax(&smax[0], f2key(cx)); atomicMax(&smax[1], f2key(cy)); atomicMax(&smax[2], f2key(cz));
+      kmin[0] = f2key(cx); kmin[1] = f2key(cy); kmin[2] = f2key(cz);
+      kmax[0] = kmin[0]; kmax[1] = kmin[1]; kmax[2] = kmin[2];
     }
+  }
+  // the centroid key bounds over the wave first (min / max are order-free), then one LDS
+  // atomic per wave and key instead of one per lane on the same six words
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      kmin[k] = min(kmin[k], (uint32_t)__shfl_xor((int)kmin[k], off, 64));
+      kmax[k] = max(kmax[k], (uint32_t)__shfl_xor((int)kmax[k], off, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) { atomicMin(&smin[k], kmin[k]); atomicMax(&smax[k], kmax[k]); }
   }
   __syncthreads();
   if (threadIdx.x < 3) {

@@ -308,6 +308,12 @@ struct drh_renderer : BufOwner {
   // the interior pass runs on the second stream beside the edge pass (fork after the
   // gradient is cleared, join before its final conversions)
   hipEvent_t ev_ifork = nullptr, ev_ijoin = nullptr;
+  // with view 0's edge preparation forked, the fused step also clears the gradient on the
+  // second stream before the render (grad_zeroed) and reduces the loss there after it
+  // (loss_on_side: fork after the render, join at the end of d_render), so the render is
+  // followed directly by the edge pass on the main stream
+  hipEvent_t ev_lfork = nullptr, ev_ljoin = nullptr;
+  bool grad_zeroed = false, loss_on_side = false, loss_side_pending = false;
   int sharing = 1;          // handles sharing the device (drh_set_device_sharing)
   // interior samples: Hammersley (reference) or the render's grid (aligned); store: the render
   // keeps each sample's closest hit and the interior pass reads it back
@@ -790,13 +796,19 @@ struct drh_renderer : BufOwner {
   }
   // fork view 0's edge preparation onto the side stream (not while profiling: per-kernel
   // events time the main stream)
-  drh_status fork_prep0(long long N) {
+  // (zero_grad, n: the gradient the step's d_render will overwrite, cleared there as well)
+  drh_status fork_prep0(long long N, float* zero_grad = nullptr, long long n = 0) {
     prep0_forked = false;
+    grad_zeroed = false;
     if (prof || sharing > 1 || E <= 0 || N <= 0) return DRH_OK;
     drh_status s;
     if ((s = hip(hipEventRecord(ev_fork, stream), "fork")) || (s = hip(hipStreamWaitEvent(side, ev_fork, 0), "fork")))
       return s;
     edge_prep(0, side);
+    if (zero_grad && n > 0) {
+      launch_fill_f32(zero_grad, n, 0.0f, side);
+      grad_zeroed = true;
+    }
     if ((s = hip(hipEventRecord(ev_join, side), "fork"))) return s;
     prep0_forked = true;
     return hip(hipGetLastError(), "edge prep");
@@ -1152,7 +1164,15 @@ drh_status drh_renderer::run_render(int views, float* d_img, const float* d_targ
   if (d_target && d_loss) {
     double denom = 0;
     for (int v = 0; v < views; v++) denom += (double)cam_cache[0].w * cam_cache[0].h;
-    timed("k_loss_final", [&] { launch_loss_final(loss_part.as<double>(), part, d_loss, denom, stream); });
+    if (loss_on_side) {   // nothing on the main stream waits for the loss before d_render ends
+      drh_status s;
+      if ((s = hip(hipEventRecord(ev_lfork, stream), "fork")) || (s = hip(hipStreamWaitEvent(side, ev_lfork, 0), "fork")))
+        return s;
+      launch_loss_final(loss_part.as<double>(), part, d_loss, denom, side);
+      loss_side_pending = true;
+    } else {
+      timed("k_loss_final", [&] { launch_loss_final(loss_part.as<double>(), part, d_loss, denom, stream); });
+    }
   }
   hits_stored = store_hits;
   return hip(hipGetLastError(), "render kernels");
@@ -1206,7 +1226,9 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   drh_status s;
   if ((s = ensure_rnd(N))) return s;
   if ((s = build_view_suspects(views))) return s;
-  if (!accumulate) timed("k_fill_f32", [&] { launch_fill_f32(d_grad, G, 0.0f, stream); });
+  const bool zeroed_on_side = grad_zeroed && !accumulate;   // cleared beside the render (fork_prep0)
+  grad_zeroed = false;
+  if (!accumulate && !zeroed_on_side) timed("k_fill_f32", [&] { launch_fill_f32(d_grad, G, 0.0f, stream); });
   // with restricted meshes the edge pass gathers every mesh's 12 transform entries apart,
   // then k_restricted_grad maps them into the gradient blocks
   const bool tr_apart = any_restricted && E > 0 && N > 0;
@@ -1254,10 +1276,12 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   const bool fork_int = !prof && sharing == 1 && st.mode != DRH_SILHOUETTE && G > 0 &&
                         (!skip_tex_interior || (interior_pos && (st.mode == DRH_LAMBERT || st.mode == DRH_LAMBERT_VERTEX_COLOR)));
   hipStream_t iq = stream;
-  if (fork_int) {
+  if (fork_int) {   // the side stream cleared the gradient before this point in its order
     if ((s = hip(hipEventRecord(ev_ifork, stream), "fork")) || (s = hip(hipStreamWaitEvent(side, ev_ifork, 0), "fork")))
       return s;
     iq = side;
+  } else if (zeroed_on_side && (s = hip(hipStreamWaitEvent(stream, ev_join, 0), "join"))) {
+    return s;   // an interior pass on the main stream writes the gradient: the clear comes first
   }
   for (int v = 0; v < views; v++) {
     const DevCam& c = cam_cache[v];
@@ -1358,6 +1382,11 @@ drh_status drh_renderer::run_d_render(int views, const float* d_adj, long long N
   if (fork_int &&
       ((s = hip(hipEventRecord(ev_ijoin, side), "join")) || (s = hip(hipStreamWaitEvent(stream, ev_ijoin, 0), "join"))))
     return s;
+  if (loss_side_pending) {
+    loss_side_pending = false;
+    if ((s = hip(hipEventRecord(ev_ljoin, side), "join")) || (s = hip(hipStreamWaitEvent(stream, ev_ljoin, 0), "join")))
+      return s;
+  }
   if (dacc_p)
     timed("k_det_finish", [&] {
       launch_det_finish(dacc_p, G, d_grad, tr_apart ? trscr.as<float>() : nullptr, tr_apart ? 12 * nm : 0, stream);
@@ -1458,7 +1487,9 @@ drh_status drh_create(const drh_settings* settings, drh_handle* out) {
       hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&r->ev_ifork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&r->ev_ijoin, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&r->ev_ijoin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&r->ev_lfork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&r->ev_ljoin, hipEventDisableTiming) != hipSuccess)
     return DRH_E_HIP;
   apply_settings(*r, *settings);
   // queued on the new stream, no wait: in a busy process a synchronisation here waits for
@@ -1488,6 +1519,8 @@ void drh_destroy(drh_handle h) {
   (void)hipEventDestroy(h->ev_join);
   (void)hipEventDestroy(h->ev_ifork);
   (void)hipEventDestroy(h->ev_ijoin);
+  (void)hipEventDestroy(h->ev_lfork);
+  (void)hipEventDestroy(h->ev_ljoin);
   hipStream_t side = h->side, main = h->stream;
   delete h;   // pooled buffers are freed on the main stream, which still exists
   (void)hipStreamDestroy(side);
@@ -1588,15 +1621,26 @@ drh_status drh_d_render_and_compare_dev(drh_handle h, const drh_camera* cams, in
     img = h->img.as<float>();
   }
   float* loss = d_loss ? d_loss : h->loss.as<float>();
-  if ((s = h->ensure_rnd(edge_samples)) || (s = h->fork_prep0(edge_samples))) return s;
+  // the views' suspect lists (the render's first kernels) before the fork, so that the edge
+  // preparation's branch starts beside the render itself
+  if ((s = h->ensure_rnd(edge_samples)) || (s = h->build_view_suspects(views)) ||
+      (s = h->fork_prep0(edge_samples, d_grad, h->G)))
+    return s;
   bool store = h->interior_store && h->interior_aligned && !h->skip_tex_interior && h->st.mode != DRH_SILHOUETTE;
   if (store && h->interior_store == DRH_HITS_AUTO && h->hit_bytes(views) > h->hit_limit) {
     store = false;
     h->release_uncaptured(h->hitbuf);   // recompute: hold no hit buffer beyond the limit
   }
-  if ((s = h->run_render(views, img, d_targets, h->adj.as<float>(), loss, store))) return s;
+  h->loss_on_side = h->prep0_forked;
+  s = h->run_render(views, img, d_targets, h->adj.as<float>(), loss, store);
+  h->loss_on_side = false;
+  if (s) {
+    h->prep0_forked = h->grad_zeroed = h->loss_side_pending = false;
+    return s;
+  }
   s = h->run_d_render(views, h->adj.as<float>(), edge_samples, d_grad, false);
   h->prep0_forked = false;
+  h->grad_zeroed = false;
   h->hits_stored = false;
   return s;
 }

@@ -374,6 +374,8 @@ struct drh_renderer : BufOwner {
   // LBVH
   DBuf bf_a, bf_e1, bf_e2, box_lo, box_hi, cen, bounds, keys0, vals0, keys1, vals1, hist, child, node_parent,
       leaf_parent, counter, nrange, frontier, leaf_lo, leaf_hi, node_lo, node_hi, nodes, tri_a, tri_e1, tri_e2;
+  // the spanning nodes of the current topology (k_span_mark / k_span_link at each full build)
+  DBuf span, span_cid, span_of, span_p, span_n;
   // clamped-determinant suspects (per view for camera rays, per light for shadow rays)
   DBuf sus_view, sus_view_cnt, sus_light, sus_light_cnt;
   // the suspect lists binned per cell (k_sus_bin, k_light_bin) when F >= SUS_BIN_MIN_F:
@@ -1035,6 +1037,10 @@ drh_status drh_renderer::build_bvh(bool full) {
       (s = R(leaf_hi, f16)) || (s = R(node_lo, f16)) || (s = R(node_hi, f16)) || (s = R(nodes, 4 * f16)) ||
       (s = R(tri_a, f16)) || (s = R(tri_e1, f16)) || (s = R(tri_e2, f16)))
     return s;
+  if (F > REFIT_CHUNK &&
+      ((s = R(span, 16 * (size_t)SPAN_MAX)) || (s = R(span_cid, 8 * (size_t)SPAN_MAX)) || (s = R(span_p, 4 * (size_t)SPAN_MAX)) ||
+       (s = R(span_of, f4)) || (s = R(span_n, 16))))
+    return s;
   light_sus = (st.mode == DRH_LAMBERT || st.mode == DRH_LAMBERT_VERTEX_COLOR) && st.tracer != DRH_TRACER_BF;
   if (light_sus && ((s = R(sus_light, 2 * f16)) || (s = R(sus_light_cnt, 16))))   // two float4 per entry
     return s;
@@ -1072,6 +1078,11 @@ drh_status drh_renderer::build_bvh(bool full) {
   if (F > 1) {
     KarrasArgs ka{F, keys0.as<uint32_t>(), nrange.as<int2>(), child.as<int>(), node_parent.as<int>(), leaf_parent.as<int>()};
     timed("k_karras", [&] { launch_karras(ka, stream); });
+    if (F > REFIT_CHUNK) {
+      SpanArgs sa{F, nrange.as<int2>(), child.as<int>(), node_parent.as<int>(), span_of.as<int>(), span_p.as<int>(),
+                  span.as<int4>(), span_cid.as<int2>(), span_n.as<uint32_t>()};
+      timed("k_span", [&] { launch_span(sa, stream); });
+    }
   }
   }  // full
   topo_valid = true;
@@ -1079,7 +1090,8 @@ drh_status drh_renderer::build_bvh(bool full) {
     RefitArgs ra{F, vals0.as<uint32_t>(), box_lo.as<float4>(), box_hi.as<float4>(), child.as<int>(),
                  node_parent.as<int>(), leaf_parent.as<int>(), nrange.as<int2>(), counter.as<uint32_t>(),
                  frontier.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
-                 node_hi.as<float4>()};
+                 node_hi.as<float4>(), span.as<int4>(), span_cid.as<int2>(),
+                 F > REFIT_CHUNK ? span_n.as<uint32_t>() : nullptr};
     timed("k_refit", [&] { launch_refit(ra, stream); });
   }
   // node packing (F > 1) and the triangles in leaf order, one launch

@@ -104,6 +104,26 @@ struct RefitArgs {
   float4* leaf_hi;
   float4* node_lo;
   float4* node_hi;
+  // the spanning nodes listed at the last full build (k_span_mark / k_span_link); k_refit_top
+  // climbs them in LDS when there are at most SPAN_MAX, and through `frontier` otherwise
+  const int4* span;        // (node, parent's span index or -1, child 0 / 1: span index or -1)
+  const int2* span_cid;    // the two children's ids (~leaf for a leaf)
+  const uint32_t* span_n;  // spanning nodes counted (may exceed SPAN_MAX)
+};
+constexpr int SPAN_MAX = 1024;
+
+// Spanning-node list (one per full build): node i spans when its leaf range crosses a
+// REFIT_CHUNK boundary.
+struct SpanArgs {
+  int F;
+  const int2* range;
+  const int* child;
+  const int* node_parent;
+  int* span_of;            // F - 1: a spanning node's index in the list
+  int* span_p;             // SPAN_MAX: the list's nodes
+  int4* span;              // SPAN_MAX
+  int2* span_cid;          // SPAN_MAX
+  uint32_t* span_n;        // zeroed before k_span_mark
 };
 
 struct PackArgs {
@@ -348,6 +368,7 @@ void launch_radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                        uint32_t* hist, hipStream_t s);
 void launch_karras(const KarrasArgs& a, hipStream_t s);
 void launch_refit(const RefitArgs& a, hipStream_t s);
+void launch_span(const SpanArgs& a, hipStream_t s);   // F > REFIT_CHUNK, after k_karras
 void launch_pack(const PackArgs& a, hipStream_t s);
 void launch_pack_reorder(const PackArgs& pk, const ReorderArgs& ro, hipStream_t s);   // F > 1; one launch
 void launch_suspects(const SuspectArgs& a, hipStream_t s);

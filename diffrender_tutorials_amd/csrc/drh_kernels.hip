@@ -361,7 +361,84 @@ __global__ __launch_bounds__(REFIT_CHUNK) void k_refit_local(RefitArgs a) {
     p = lpar[pk];
   }
 }
+// The nodes whose leaf range crosses a refit chunk: listed once per full build (the topology
+// stays until the next one), with each one's parent and children as list indices, so that
+// k_refit_top can climb them in LDS.
+__device__ __forceinline__ bool spans_chunks(int2 r) { return r.x / REFIT_CHUNK != r.y / REFIT_CHUNK; }
+__global__ void k_span_mark(SpanArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.F - 1 || !spans_chunks(a.range[i])) return;
+  const int k = (int)atomicAdd(a.span_n, 1u);
+  a.span_of[i] = k;
+  if (k < SPAN_MAX) a.span_p[k] = i;
+}
+__global__ void k_span_link(SpanArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = min((int)*a.span_n, SPAN_MAX);
+  if (k >= n) return;
+  const int p = a.span_p[k];
+  const int par = a.node_parent[p];   // the parent's range holds this one's: it spans too
+  const int c0 = a.child[2 * p], c1 = a.child[2 * p + 1];
+  const int s0 = (c0 >= 0 && spans_chunks(a.range[c0])) ? a.span_of[c0] : -1;
+  const int s1 = (c1 >= 0 && spans_chunks(a.range[c1])) ? a.span_of[c1] : -1;
+  a.span[k] = make_int4(p, par >= 0 ? a.span_of[par] : -1, s0, s1);
+  a.span_cid[k] = make_int2(c0, c1);
+}
+
+// Refit of the spanning nodes in LDS (one workgroup, at most SPAN_MAX of them): each node's
+// children are spanning nodes of the list or chunk-local nodes / leaves whose boxes
+// k_refit_local has written; the latter are staged first and counted as arrived, then the
+// nodes with both children staged start climbs, and a climb continues into the parent when
+// it brings the parent's second child (LDS arrival counters).
+__device__ __forceinline__ void refit_top_lds(const RefitArgs& a, int ns) {
+  __shared__ float sbox[SPAN_MAX][6];      // the spanning nodes' boxes
+  __shared__ float cbox[SPAN_MAX][2][6];   // their chunk-local children's boxes
+  __shared__ int4 sinfo[SPAN_MAX];
+  __shared__ int scnt[SPAN_MAX];
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+    const int4 e = a.span[k];
+    const int2 cid = a.span_cid[k];
+    sinfo[k] = e;
+    int staged = 0;
+    for (int j = 0; j < 2; j++) {
+      if ((j ? e.w : e.z) >= 0) continue;
+      const int c = j ? cid.y : cid.x;
+      const float4 L = c < 0 ? a.leaf_lo[~c] : a.node_lo[c];
+      const float4 H = c < 0 ? a.leaf_hi[~c] : a.node_hi[c];
+      cbox[k][j][0] = L.x; cbox[k][j][1] = L.y; cbox[k][j][2] = L.z;
+      cbox[k][j][3] = H.x; cbox[k][j][4] = H.y; cbox[k][j][5] = H.z;
+      staged++;
+    }
+    scnt[k] = staged;
+  }
+  __syncthreads();
+  for (int k0 = threadIdx.x; k0 < ns; k0 += blockDim.x) {
+    if (scnt[k0] != 2) continue;
+    int k = k0;
+    for (int level = 0; level < 64; level++) {
+      const int4 e = sinfo[k];
+      const float* b0 = e.z >= 0 ? sbox[e.z] : cbox[k][0];
+      const float* b1 = e.w >= 0 ? sbox[e.w] : cbox[k][1];
+      float4 L = make_float4(b0[0], b0[1], b0[2], 0.0f), H = make_float4(b0[3], b0[4], b0[5], 0.0f);
+      box_union(L, H, make_float4(b1[0], b1[1], b1[2], 0.0f), make_float4(b1[3], b1[4], b1[5], 0.0f));
+      sbox[k][0] = L.x; sbox[k][1] = L.y; sbox[k][2] = L.z; sbox[k][3] = H.x; sbox[k][4] = H.y; sbox[k][5] = H.z;
+      a.node_lo[e.x] = L;
+      a.node_hi[e.x] = H;
+      if (e.y < 0) break;   // the root
+      __threadfence_block();
+      if (atomicAdd(&scnt[e.y], 1) != 1) break;   // the parent's other child is still on its way
+      __threadfence_block();
+      k = e.y;
+    }
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_refit_top(RefitArgs a) {
+  const int ns = a.span_n ? (int)*a.span_n : SPAN_MAX + 1;
+  if (ns <= SPAN_MAX) {
+    refit_top_lds(a, ns);
+    return;
+  }
   const int n = (int)a.counter[a.F];
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     int q = a.frontier[k];
@@ -2216,6 +2293,11 @@ void launch_refit(const RefitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_refit_local, dim3((a.F + REFIT_CHUNK - 1) / REFIT_CHUNK), dim3(REFIT_CHUNK), 0, s, a);
   // one chunk holds the whole tree up to its root: no node spans chunks, the frontier is empty
   if (a.F > REFIT_CHUNK) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, a);
+}
+void launch_span(const SpanArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, a.span_n, 1ll, 0u);
+  hipLaunchKernelGGL(k_span_mark, dim3((a.F + 254) / 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_span_link, dim3((SPAN_MAX + 255) / 256), dim3(256), 0, s, a);
 }
 void launch_pack_reorder(const PackArgs& pk, const ReorderArgs& ro, hipStream_t s) {
   hipLaunchKernelGGL(k_pack_nodes, dim3((ro.F + 255) / 256), dim3(256), 0, s, pk, ro);

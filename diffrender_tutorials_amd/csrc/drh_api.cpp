@@ -1053,15 +1053,13 @@ drh_status drh_renderer::build_bvh(bool full) {
   pa.bounds = bounds.as<uint32_t>();
   pa.zero = light_sus ? sus_light_cnt.as<uint32_t>() : nullptr;
   timed("k_prepare", [&] { launch_prepare(pa, stream); });
+  // with the light's suspect list (k_prepare zeroed its count)
+  SuspectArgs la{F, 1, light.L.x, light.L.y, light.L.z, bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
+                 light_sus ? sus_light.as<float4>() : nullptr, light_sus ? sus_light_cnt.as<int>() : nullptr};
   TriSetupArgs ta{F, tri_mesh.as<int>(), dmats.as<float>(), idx.as<uint32_t>(), pos.as<float>(), bf_a.as<float4>(),
                   bf_e1.as<float4>(), bf_e2.as<float4>(), box_lo.as<float4>(), box_hi.as<float4>(), cen.as<float4>(),
-                  bounds.as<uint32_t>(), F > 1 ? counter.as<uint32_t>() : nullptr};
+                  bounds.as<uint32_t>(), F > 1 ? counter.as<uint32_t>() : nullptr, la};
   timed("k_tri_setup", [&] { launch_tri_setup(ta, stream); });
-  if (light_sus) {
-    SuspectArgs la{F, 1, light.L.x, light.L.y, light.L.z, bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
-                   sus_light.as<float4>(), sus_light_cnt.as<int>()};
-    timed("k_suspects", [&] { launch_suspects(la, stream); });
-  }
   if (!topo_valid) full = true;
   if (full) {
   MortonArgs ma{F, cen.as<float4>(), bounds.as<uint32_t>(), keys0.as<uint32_t>(), vals0.as<uint32_t>()};

@@ -56,6 +56,17 @@ struct PrepareArgs {
   int* step;              // optional: the Adam step counter, advanced by thread 0 (a fused step's first kernel)
 };
 
+struct SuspectArgs {
+  int F;
+  int light;              // 0: view origin (ox, oy, oz); 1: light direction (ox, oy, oz)
+  float ox, oy, oz;
+  const float4* bf_a;
+  const float4* bf_e1;
+  const float4* bf_e2;
+  float4* list;           // capacity F entries (view: one float4 each; light: two)
+  int* count;             // zeroed by the caller
+};
+
 struct TriSetupArgs {
   int F;
   const int* tri_mesh;
@@ -70,6 +81,7 @@ struct TriSetupArgs {
   float4* cen;
   uint32_t* bounds;       // 6 ordered float keys: min xyz, max xyz
   uint32_t* counter;      // optional: F + 1 refit arrival counters cleared here
+  SuspectArgs light;      // the light's suspect list built here too when light.list is set
 };
 
 struct MortonArgs {
@@ -148,16 +160,6 @@ struct ReorderArgs {
   float4* tri_e2;
 };
 
-struct SuspectArgs {
-  int F;
-  int light;              // 0: view origin (ox, oy, oz); 1: light direction (ox, oy, oz)
-  float ox, oy, oz;
-  const float4* bf_a;
-  const float4* bf_e1;
-  const float4* bf_e2;
-  float4* list;           // capacity F entries (view: one float4 each; light: two)
-  int* count;             // zeroed by the caller
-};
 
 // Binning of suspect lists into per-cell records in a pool (see Suspects in drh_device.h), in
 // three passes over the suspects -- count the kept (suspect, cell) pairs per cell, scan the

@@ -102,6 +102,18 @@ __global__ void k_prepare(PrepareArgs a) {
 
 // Per triangle: BF's transformed corners T*pos (raytrace3d_bf.cpp:123-125), its edges, AABB
 // and centroid; scene centroid bounds by ordered-key atomics.
+// A triangle of the light's suspect list (see k_suspects): clamped shadow-ray determinant.
+__device__ __forceinline__ void light_suspect(const SuspectArgs& a, int g, float4 A, float4 E1, float4 E2) {
+  const F3 negL = mul(-1.0f, F3{a.ox, a.oy, a.oz});
+  const F3 pvec = cross(negL, f4xyz(E2));
+  const float det = dot(f4xyz(E1), pvec);
+  const float sv = sgn(det);
+  if (!(sv != 0.0f && sv * det < 1e-6f)) return;
+  const float invDet = sv / max_std(sv * det, 1e-6f);
+  const int k = atomicAdd(a.count, 1);
+  a.list[2 * k] = make_float4(A.x, A.y, A.z, __uint_as_float((uint32_t)g));
+  a.list[2 * k + 1] = make_float4(pvec.x, pvec.y, pvec.z, invDet);
+}
 __global__ void k_tri_setup(TriSetupArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ uint32_t smin[3], smax[3];
@@ -117,9 +129,12 @@ __global__ void k_tri_setup(TriSetupArgs a) {
     const F3 B = mp(T, ld3(a.pos + 3 * ib));
     const F3 C = mp(T, ld3(a.pos + 3 * ic));
     const F3 e1 = sub(B, A), e2 = sub(C, A);
-    a.bf_a[g] = make_float4(A.x, A.y, A.z, __uint_as_float((uint32_t)g));
-    a.bf_e1[g] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-    a.bf_e2[g] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    const float4 A4 = make_float4(A.x, A.y, A.z, __uint_as_float((uint32_t)g));
+    const float4 E14 = make_float4(e1.x, e1.y, e1.z, 0.0f), E24 = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    a.bf_a[g] = A4;
+    a.bf_e1[g] = E14;
+    a.bf_e2[g] = E24;
+    if (a.light.list) light_suspect(a.light, g, A4, E14, E24);   // the light's suspect list, k_suspects' light form
     const float lx = fminf(A.x, fminf(B.x, C.x)), ly = fminf(A.y, fminf(B.y, C.y)), lz = fminf(A.z, fminf(B.z, C.z));
     const float hx = fmaxf(A.x, fmaxf(B.x, C.x)), hy = fmaxf(A.y, fmaxf(B.y, C.y)), hz = fmaxf(A.z, fmaxf(B.z, C.z));
     a.box_lo[g] = make_float4(lx, ly, lz, 0.0f);
@@ -507,15 +522,7 @@ __global__ void k_suspects(SuspectArgs a) {
   if (g >= a.F) return;
   const float4 A = a.bf_a[g], E1 = a.bf_e1[g], E2 = a.bf_e2[g];
   if (a.light) {
-    const F3 negL = mul(-1.0f, F3{a.ox, a.oy, a.oz});
-    const F3 pvec = cross(negL, f4xyz(E2));
-    const float det = dot(f4xyz(E1), pvec);
-    const float sv = sgn(det);
-    if (!(sv != 0.0f && sv * det < 1e-6f)) return;
-    const float invDet = sv / max_std(sv * det, 1e-6f);
-    const int k = atomicAdd(a.count, 1);
-    a.list[2 * k] = make_float4(A.x, A.y, A.z, __uint_as_float((uint32_t)g));
-    a.list[2 * k + 1] = make_float4(pvec.x, pvec.y, pvec.z, invDet);
+    light_suspect(a, g, A, E1, E2);
     return;
   }
   const double e1x = E1.x, e1y = E1.y, e1z = E1.z, e2x = E2.x, e2y = E2.y, e2z = E2.z;

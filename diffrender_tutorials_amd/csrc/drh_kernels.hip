@@ -410,6 +410,10 @@ __device__ __forceinline__ void refit_top_lds(const RefitArgs& a, int ns) {
   __shared__ float cbox[SPAN_MAX][2][6];   // their chunk-local children's boxes
   __shared__ int4 sinfo[SPAN_MAX];
   __shared__ int scnt[SPAN_MAX];
+  // one entry per thread (SPAN_MAX <= blockDim); the climbs' start nodes are fixed here,
+  // before any climb can raise a counter to 2
+  static_assert(SPAN_MAX <= 1024, "one spanning node per thread of the 1024-thread workgroup");
+  int start = -1;
   for (int k = threadIdx.x; k < ns; k += blockDim.x) {
     const int4 e = a.span[k];
     const int2 cid = a.span_cid[k];
@@ -425,11 +429,11 @@ __device__ __forceinline__ void refit_top_lds(const RefitArgs& a, int ns) {
       staged++;
     }
     scnt[k] = staged;
+    if (staged == 2) start = k;
   }
   __syncthreads();
-  for (int k0 = threadIdx.x; k0 < ns; k0 += blockDim.x) {
-    if (scnt[k0] != 2) continue;
-    int k = k0;
+  if (start >= 0) {
+    int k = start;
     for (int level = 0; level < 64; level++) {
       const int4 e = sinfo[k];
       const float* b0 = e.z >= 0 ? sbox[e.z] : cbox[k][0];

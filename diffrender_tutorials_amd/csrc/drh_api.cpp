@@ -378,6 +378,7 @@ struct drh_renderer : BufOwner {
   DBuf span, span_cid, span_of, span_p, span_n;
   // clamped-determinant suspects (per view for camera rays, per light for shadow rays)
   DBuf sus_view, sus_view_cnt, sus_light, sus_light_cnt;
+  int view_cnt_zeroed = 0;   // view suspect counters the last BVH build cleared and no list build used
   // the suspect lists binned per cell (k_sus_bin, k_light_bin) when F >= SUS_BIN_MIN_F:
   // records in a pool, (offset, count) and a counter per cell
   DBuf sus_pool, sus_cell, sus_cell_cnt, sus_light_pool, sus_light_cell, sus_light_cell_cnt, sus_light_grid;
@@ -757,7 +758,8 @@ struct drh_renderer : BufOwner {
       return s;
     sus_cells_stride = cells;
     sus_pool_cap = cap;
-    timed("k_fill_u32", [&] { launch_fill_u32(sus_view_cnt.as<uint32_t>(), views, 0u, stream); });
+    if (view_cnt_zeroed < views) timed("k_fill_u32", [&] { launch_fill_u32(sus_view_cnt.as<uint32_t>(), views, 0u, stream); });
+    view_cnt_zeroed = 0;
     if (bin) timed("k_fill_u32", [&] { launch_fill_u32(sus_cell_cnt.as<uint32_t>(), cells * views, 0u, stream); });
     for (int v = 0; v < views; v++) {
       const M4 wi = m4_load(cam_cache[v].wv_inv);
@@ -1093,8 +1095,12 @@ drh_status drh_renderer::build_bvh(bool full) {
     timed("k_refit", [&] { launch_refit(ra, stream); });
   }
   // node packing (F > 1) and the triangles in leaf order, one launch
+  // the packing launch also clears the views' suspect counters for the next list build
+  // (k_pack_nodes has at least 256 threads)
+  const int nzero = (int)std::min<size_t>(sus_view_cnt.bytes / 4, 256);
   PackArgs pk{F, bounds.as<uint32_t>(), child.as<int>(), leaf_lo.as<float4>(), leaf_hi.as<float4>(), node_lo.as<float4>(),
-              node_hi.as<float4>(), nodes.as<float4>()};
+              node_hi.as<float4>(), nodes.as<float4>(), nzero > 0 ? sus_view_cnt.as<uint32_t>() : nullptr, nzero};
+  view_cnt_zeroed = nzero;
   ReorderArgs ro{F, vals0.as<uint32_t>(), bf_a.as<float4>(), bf_e1.as<float4>(), bf_e2.as<float4>(),
                  tri_a.as<float4>(), tri_e1.as<float4>(), tri_e2.as<float4>()};
   timed("k_pack_nodes", [&] { launch_pack_reorder(pk, ro, stream); });

@@ -147,6 +147,8 @@ struct PackArgs {
   const float4* node_lo;
   const float4* node_hi;
   float4* nodes;
+  uint32_t* zero;          // optional: nzero (<= 256) counters cleared for the next list build
+  int nzero;
 };
 
 struct ReorderArgs {

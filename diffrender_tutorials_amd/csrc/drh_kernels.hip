@@ -956,6 +956,7 @@ __global__ __launch_bounds__(1024) void k_bin_offsets(BinPool bp, long long cell
 // the triangles in leaf order, beside the node packing (one launch)
 __global__ void k_pack_nodes(PackArgs pk, ReorderArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pk.zero && s < pk.nzero) pk.zero[s] = 0u;
   if (s < pk.F - 1) pack_node(pk, s);
   if (s >= a.F) return;
   const int g = (int)a.order[s];

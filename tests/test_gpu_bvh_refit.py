@@ -97,3 +97,22 @@ def test_bvh_rebuild_rejects_non_positive_interval():
             r.set_bvh_rebuild(0)
     finally:
         r.close()
+
+
+def test_commit_refuses_more_triangles_than_the_walk_addresses():
+    """The LBVH walks address nodes and leaf triangles by 32-bit byte offsets: a scene of more
+    than 2^26 triangles is refused (DRH_E_INVALID) before any of its arrays is read, and the
+    handle stays usable."""
+    import ctypes
+    m = S.cube()
+    r = drh.DiffRender(mode=drh.VERTEX_COLOR, spp=1)
+    try:
+        d, keep = drh.scene_desc([m])
+        keep[1][0] = (1 << 26) + 1   # the mesh's triangle count, as the descriptor sees it
+        assert drh.load().drh_commit(r.h, ctypes.byref(d)) == -1
+        say("refused:", drh.load().drh_last_error(r.h))
+        r.commit([m])
+        img = r.render([S.camera_look((2.0, 2.0, 2.0), 64, 64)])[0]
+        assert np.isfinite(img).all() and img.max() > 0
+    finally:
+        r.close()
